@@ -1,0 +1,236 @@
+// C ABI (include/dqsim_sv.h): error mapping, handles and thread-local errors.
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "../../include/dqsim_sv.h"
+#include "engine.hpp"
+
+struct dq_sv {
+  dq::Options o;
+};
+struct dq_result {
+  std::unique_ptr<dq::Result> r;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F fn) {
+  g_err.clear();
+  try {
+    fn();
+    return DQ_OK;
+  } catch (const dq::ValueErr& e) {
+    g_err = e.msg;
+    return DQ_EVALUE;
+  } catch (const dq::MemErr& e) {
+    g_err = e.msg;
+    return DQ_EMEMORY;
+  } catch (const dq::RuntimeErr& e) {
+    g_err = e.msg;
+    return DQ_ERUNTIME;
+  } catch (const dq::DeviceErr& e) {
+    g_err = e.msg;
+    return DQ_EDEVICE;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return DQ_EMEMORY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return DQ_ERUNTIME;
+  }
+}
+
+int emit(const dq::Counts& c, dq_count_cb cb, void* user) {
+  if (!cb) return DQ_OK;
+  for (auto& kv : c)
+    if (cb(user, kv.first.data(), kv.first.size(), kv.second)) break;
+  return DQ_OK;
+}
+
+void check_query(const dq::Result& r, const uint64_t* qubits, uint64_t k) {
+  for (uint64_t i = 0; i < k; ++i) {
+    bool bad = qubits[i] >= r.n;
+    for (uint64_t j = 0; j < i && !bad; ++j) bad = qubits[j] == qubits[i];
+    if (bad) throw dq::ValueErr("Query qubits must be distinct and within the statevector's range");
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void dq_sv_opts_default(dq_sv_opts* o) {
+  std::memset(o, 0, sizeof *o);
+  o->max_memory_mb = 1024;
+  o->sample_terminal = 1;
+  o->precision = DQ_PREC_C128;
+  o->fusion = DQ_FUSE_NONE;
+  o->max_fused_qubits = 4;
+}
+
+int dq_sv_create(const dq_sv_opts* o, dq_sv** out) {
+  return guard([&] {
+    if (!o || !out) throw dq::ValueErr("null argument");
+    const uint64_t mib = 1024ull * 1024ull;
+    if (o->max_memory_mb == 0 || o->max_memory_mb > (uint64_t)INT64_MAX / mib)
+      throw dq::ValueErr("max_memory_mb must be positive and representable in bytes");
+    if (o->has_max_parallel_shots && o->max_parallel_shots == 0)
+      throw dq::ValueErr("max_parallel_shots must be positive or None");
+    if (o->precision != DQ_PREC_C128 && o->precision != DQ_PREC_C64)
+      throw dq::ValueErr("precision must be complex128 or complex64");
+    if (o->fusion < DQ_FUSE_NONE || o->fusion > DQ_FUSE_LEARNED) throw dq::ValueErr("unknown fusion mode");
+    if (o->max_fused_qubits < 1 || o->max_fused_qubits > 5) throw dq::ValueErr("max_fused_qubits must be in 1..5");
+    auto* s = new dq_sv;
+    s->o.has_seed = o->has_seed != 0;
+    s->o.seed = o->seed;
+    s->o.profile = o->profile != 0;
+    s->o.budget = o->max_memory_mb * mib;
+    s->o.has_max_parallel_shots = o->has_max_parallel_shots != 0;
+    s->o.max_parallel_shots = o->max_parallel_shots;
+    s->o.sample_terminal = o->sample_terminal != 0;
+    s->o.prec = o->precision;
+    s->o.fusion = o->fusion;
+    s->o.max_fused = o->max_fused_qubits;
+    *out = s;
+  });
+}
+
+void dq_sv_free(dq_sv* sim) { delete sim; }
+
+int dq_sv_simulate_shots(dq_sv* sim, const char* json, size_t len, uint64_t shots, int32_t profile, dq_count_cb cb,
+                         void* user) {
+  return guard([&] {
+    dq::Counts c = dq::simulate_shots(sim->o, json, len, shots, profile != 0);
+    emit(c, cb, user);
+  });
+}
+
+int dq_sv_simulate(dq_sv* sim, const char* json, size_t len, dq_result** out) {
+  return guard([&] {
+    auto* r = new dq_result;
+    try {
+      r->r = dq::simulate(sim->o, json, len);
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    *out = r;
+  });
+}
+
+uint32_t dq_result_num_qubits(const dq_result* r) { return r->r->n; }
+uint64_t dq_result_num_cbits(const dq_result* r) { return r->r->nc; }
+
+int dq_result_classical_bits(const dq_result* r, uint8_t* out, uint64_t len) {
+  return guard([&] {
+    if (len != r->r->nc) throw dq::ValueErr("length mismatch");
+    if (len) std::memcpy(out, r->r->cbits.data(), len);
+  });
+}
+
+int dq_result_profile(const dq_result* r, dq_sim_profile* out) {
+  if (!r->r->has_profile) return 0;
+  const dq::SimProfile& p = r->r->prof;
+  out->apply_one_qubit_calls = p.oq_calls;
+  out->apply_one_qubit_time = p.oq_time;
+  out->apply_n_qubit_calls = p.nq_calls;
+  out->apply_n_qubit_time = p.nq_time;
+  out->measure_qubit_calls = p.mq_calls;
+  out->measure_qubit_time = p.mq_time;
+  out->total_time = p.total_time;
+  return 1;
+}
+
+int dq_result_statevector_c128(const dq_result* r, void* out, uint64_t len) {
+  return guard([&] {
+    if (len != (1ull << r->r->n)) throw dq::ValueErr("length mismatch");
+    dq::result_statevector(*r->r, out);
+  });
+}
+
+int dq_result_probabilities(const dq_result* r, const uint64_t* qubits, uint64_t k, double* out) {
+  return guard([&] {
+    check_query(*r->r, qubits, k);
+    dq::result_probabilities(*r->r, std::vector<uint64_t>(qubits, qubits + k), out);
+  });
+}
+
+int dq_result_counts(const dq_result* r, uint64_t shots, const uint64_t* qubits, uint64_t k, int32_t has_seed,
+                     uint64_t seed, dq_count_cb cb, void* user) {
+  return guard([&] {
+    check_query(*r->r, qubits, k);
+    uint64_t s = seed;
+    if (!has_seed) {
+      std::random_device rd;
+      s = ((uint64_t)rd() << 32) ^ rd();
+    }
+    dq::Counts c = dq::result_counts(*r->r, shots, std::vector<uint64_t>(qubits, qubits + k), s);
+    emit(c, cb, user);
+  });
+}
+
+int dq_result_fidelity(const dq_result* r, const void* other, uint64_t len, double* out) {
+  return guard([&] { *out = dq::result_fidelity(*r->r, other, len); });
+}
+
+int dq_result_amplitude(const dq_result* r, uint64_t index, double* re_im) {
+  return guard([&] {
+    if (index >> r->r->n) throw dq::ValueErr("amplitude index out of range");
+    dq::result_amplitude(*r->r, index, re_im);
+  });
+}
+
+int dq_result_compare(const dq_result* a, const dq_result* b, double* max_abs, double* l2) {
+  return guard([&] { dq::result_compare(*a->r, *b->r, max_abs, l2); });
+}
+
+void dq_result_free(dq_result* r) { delete r; }
+
+const char* dq_last_error(void) { return g_err.c_str(); }
+
+int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused, char** out) {
+  return guard([&] {
+    std::string s = dq::plan_description(json, len, precision, fusion, max_fused);
+    char* p = (char*)std::malloc(s.size() + 1);
+    if (!p) throw std::bad_alloc();
+    std::memcpy(p, s.c_str(), s.size() + 1);
+    *out = p;
+  });
+}
+
+void dq_string_free(char* s) { std::free(s); }
+
+int dq_device_count(void) { return dq::device_count(); }
+
+int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats, dq_op_timing* out, uint64_t cap,
+                       uint64_t* nops, uint64_t* gates, double* total_ms, dq_result** keep) {
+  return guard([&] {
+    std::unique_ptr<dq::Result> kept;
+    dq::EvolveStats s = dq::evolve_timed(sim->o, json, len, repeats, out != nullptr, keep ? &kept : nullptr);
+    for (uint64_t i = 0; i < s.ops.size() && i < cap && out; ++i) {
+      out[i].kind = s.ops[i].kind;
+      out[i].k = s.ops[i].k;
+      out[i].low = s.ops[i].low;
+      out[i].gates = s.ops[i].gates;
+      out[i].ms = s.ops[i].ms;
+      out[i].bytes = s.ops[i].bytes;
+    }
+    if (nops) *nops = s.launches;
+    if (gates) *gates = s.gates;
+    if (total_ms) *total_ms = s.total_ms;
+    if (keep) {
+      auto* r = new dq_result;
+      r->r = std::move(kept);
+      *keep = r;
+    }
+  });
+}
+
+int dq_set_planner_weights(const float* w, uint64_t count) {
+  return guard([&] { dq::set_planner_weights(w, (size_t)count); });
+}
+
+}  // extern "C"

@@ -1,0 +1,1133 @@
+// Device statevector engine (host side).
+//
+// Reference semantics followed here:
+//   state_layout / zero_state / check_budget ... src/monolithic/statevector/shots.rs:22-48
+//   run_shots (terminal + trajectories) ........ shots.rs:58-260
+//   Operation::execute, conditionals ........... execution.rs:276-366
+//   simulate / SimulationResult queries ........ simulator.rs:85-202,319-371
+//   write_shots_profile ........................ src/profiling.rs:8-18
+//
+// All device work of one call runs on one HIP stream; measurement outcomes,
+// conditions and RNG draws stay on the device, so a trajectory never waits
+// for the host between operations.
+#include "engine.hpp"
+
+#include <hip/hip_runtime_api.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <random>
+#include <sstream>
+#include <unordered_map>
+
+#include "fuse.hpp"
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace dq {
+
+// ---------------------------------------------------------------------------
+// HIP helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceErr(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+struct Stream {
+  hipStream_t s = nullptr;
+  Stream() { HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~Stream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  void sync() { HIPCHK(hipStreamSynchronize(s)); }
+};
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { HIPCHK(hipEventCreate(&e)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  Event(Event&& o) noexcept : e(o.e) { o.e = nullptr; }
+};
+
+float elapsed_ms(const Event& a, const Event& b) {
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, a.e, b.e));
+  return ms;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+DeviceMem& DeviceMem::operator=(DeviceMem&& o) noexcept {
+  if (this != &o) {
+    reset();
+    p_ = o.p_;
+    n_ = o.n_;
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  return *this;
+}
+DeviceMem::~DeviceMem() { reset(); }
+void DeviceMem::reset() {
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  n_ = 0;
+}
+void DeviceMem::alloc(size_t bytes) {
+  reset();
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p_, bytes);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    p_ = nullptr;
+    throw MemErr("Unable to allocate statevector");
+  }
+  hip_check(e, "hipMalloc");
+  n_ = bytes;
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Budget (shots.rs:22-48)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Layout {
+  uint64_t len, bytes;
+};
+
+Layout state_layout(uint64_t n, int prec) {
+  if (n >= 64) throw MemErr("State dimension overflow");
+  uint64_t len = 1ull << n;
+  if (len > (uint64_t)INT64_MAX / (uint64_t)prec) throw MemErr("State byte size overflow");
+  return {len, len * (uint64_t)prec};
+}
+
+void check_budget(uint64_t required, uint64_t budget) {
+  if (required > budget)
+    throw MemErr("Simulation needs at least " + std::to_string(required) +
+                 " working bytes, exceeding max_memory_mb budget (" + std::to_string(budget) + " bytes)");
+}
+
+uint64_t random_seed() {
+  std::random_device rd;
+  return ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+}
+
+// ---------------------------------------------------------------------------
+// Device plan
+// ---------------------------------------------------------------------------
+enum DKind { D_ONE, D_X, D_SWAP, D_DIAG, D_DENSE, D_FUSED, D_MEAS, D_RESET, D_NOOP };
+
+struct DevOp {
+  DKind kind = D_NOOP;
+  int category = -1;  // profile bucket (execution.rs:297-347)
+  k::Expand e;
+  uint32_t target = 0, a = 0, b = 0, qubit = 0;
+  int64_t cbit = -1;
+  double u[8] = {};
+  std::vector<uint64_t> pat;
+  std::vector<double> dval;
+  int kq = 0;
+  uint64_t offsets[32] = {};
+  size_t mat_off = 0;
+  bool mfma = false;
+  size_t emb_off = 0;
+  std::vector<k::Cond> conds;
+  size_t cond_off = 0;
+  int gates = 1;
+  int low = 0;
+  int nctl = 0;
+};
+
+struct Plan {
+  std::vector<DevOp> ops;
+  std::vector<uint8_t> blob;  // matrices and condition chains
+  DeviceMem dev;
+  size_t push(const void* p, size_t bytes) {
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, p, bytes);
+    return off;
+  }
+  void upload(hipStream_t st) {
+    if (blob.empty()) return;
+    dev.alloc(blob.size());
+    HIPCHK(hipMemcpyAsync(dev.get(), blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+  }
+  const uint8_t* base() const { return dev.as<const uint8_t>(); }
+};
+
+k::Expand make_expand(std::vector<uint32_t> pos, uint64_t fixed) {
+  std::sort(pos.begin(), pos.end());
+  k::Expand e;
+  e.nbits = (uint32_t)pos.size();
+  for (size_t i = 0; i < pos.size(); ++i) e.pos[i] = pos[i];
+  e.fixed = fixed;
+  return e;
+}
+
+void set_dense_offsets(DevOp& d, const std::vector<uint32_t>& qubits) {
+  const int k = (int)qubits.size();
+  d.kq = k;
+  for (int j = 0; j < (1 << k); ++j) {
+    uint64_t off = 0;
+    for (int i = 0; i < k; ++i)
+      if ((j >> (k - 1 - i)) & 1) off |= 1ull << qubits[i];
+    d.offsets[j] = off;
+  }
+  d.e = make_expand(qubits, 0);
+  d.low = (int)*std::min_element(qubits.begin(), qubits.end());
+}
+
+void push_matrix(Plan& plan, DevOp& d, const std::vector<cplx>& mat, int prec) {
+  if (prec == 16) {
+    std::vector<double> v;
+    for (auto& c : mat) {
+      v.push_back(c.re);
+      v.push_back(c.im);
+    }
+    d.mat_off = plan.push(v.data(), v.size() * sizeof(double));
+  } else {
+    std::vector<float> v;
+    for (auto& c : mat) {
+      v.push_back((float)c.re);
+      v.push_back((float)c.im);
+    }
+    d.mat_off = plan.push(v.data(), v.size() * sizeof(float));
+  }
+}
+
+void set_diag(DevOp& d, const std::vector<uint32_t>& qubits, const std::vector<cplx>& diag) {
+  const int k = (int)qubits.size();
+  d.kind = D_DIAG;
+  d.e = make_expand(qubits, 0);
+  d.kq = k;
+  d.low = (int)*std::min_element(qubits.begin(), qubits.end());
+  for (int j = 0; j < (1 << k); ++j) {
+    if (diag[j].re == 1.0 && diag[j].im == 0.0) continue;
+    uint64_t pat = 0;
+    for (int i = 0; i < k; ++i)
+      if ((j >> (k - 1 - i)) & 1) pat |= 1ull << qubits[i];
+    d.pat.push_back(pat);
+    d.dval.push_back(diag[j].re);
+    d.dval.push_back(diag[j].im);
+  }
+}
+
+// Lower one compiled op (execution.rs:276-347) to a device op.
+DevOp lower(const Op& op, Plan& plan, int prec) {
+  DevOp d;
+  if (op.kind == OpKind::Conditional) {
+    const Op* cur = &op;
+    std::vector<k::Cond> conds;
+    while (cur->kind == OpKind::Conditional) {
+      conds.push_back({cur->cond.creg_base, cur->cond.creg_size, cur->cond.creg_value});
+      cur = cur->inner.get();
+    }
+    d = lower(*cur, plan, prec);
+    d.conds = conds;
+    d.cond_off = plan.push(conds.data(), conds.size() * sizeof(k::Cond));
+    return d;
+  }
+  std::vector<uint32_t> pos;
+  uint64_t fixed = 0;
+  for (auto& c : op.controls) {
+    pos.push_back(c.first);
+    if (c.second) fixed |= 1ull << c.first;
+  }
+  d.nctl = (int)op.controls.size();
+  switch (op.kind) {
+    case OpKind::One:
+      d.kind = D_ONE;
+      d.category = op.controls.empty() ? 0 : 1;
+      d.target = op.target;
+      pos.push_back(op.target);
+      d.e = make_expand(pos, fixed);
+      for (int i = 0; i < 4; ++i) {
+        d.u[2 * i] = op.mat[i].re;
+        d.u[2 * i + 1] = op.mat[i].im;
+      }
+      d.kq = 1 + d.nctl;
+      d.low = (int)*std::min_element(pos.begin(), pos.end());
+      break;
+    case OpKind::X:
+      d.kind = D_X;
+      d.category = op.controls.empty() ? 0 : 1;
+      d.target = op.target;
+      pos.push_back(op.target);
+      d.e = make_expand(pos, fixed);
+      d.kq = 1 + d.nctl;
+      d.low = (int)*std::min_element(pos.begin(), pos.end());
+      break;
+    case OpKind::Swap:
+      d.kind = D_SWAP;
+      d.category = 1;
+      d.a = op.a;
+      d.b = op.b;
+      pos.push_back(op.a);
+      pos.push_back(op.b);
+      d.e = make_expand(pos, fixed);
+      d.kq = 2 + d.nctl;
+      d.low = (int)*std::min_element(pos.begin(), pos.end());
+      break;
+    case OpKind::Diagonal:
+      set_diag(d, op.qubits, op.mat);
+      d.category = op.qubits.size() > 1 ? 1 : 0;
+      break;
+    case OpKind::Dense:
+      d.kind = D_DENSE;
+      d.category = 1;
+      set_dense_offsets(d, op.qubits);
+      push_matrix(plan, d, op.mat, prec);
+      break;
+    case OpKind::Measure:
+      d.kind = D_MEAS;
+      d.category = 2;
+      d.qubit = op.qubit;
+      d.cbit = (int64_t)op.cbit;
+      d.kq = 1;
+      d.low = (int)op.qubit;
+      break;
+    case OpKind::Reset:
+      d.kind = D_RESET;
+      d.category = 2;
+      d.qubit = op.qubit;
+      d.kq = 1;
+      d.low = (int)op.qubit;
+      break;
+    default:
+      d.kind = D_NOOP;
+      d.category = -1;
+      break;
+  }
+  return d;
+}
+
+// Real embedding of a complex block for the MFMA kernel: [[Ur,-Ui],[Ui,Ur]].
+std::vector<float> real_embedding(const std::vector<cplx>& m, int k) {
+  const int D = 1 << k, R = 2 * D;
+  std::vector<float> e((size_t)R * R);
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) {
+      const cplx v = m[(size_t)r * D + c];
+      e[(size_t)r * R + c] = (float)v.re;
+      e[(size_t)r * R + c + D] = (float)-v.im;
+      e[(size_t)(r + D) * R + c] = (float)v.im;
+      e[(size_t)(r + D) * R + c + D] = (float)v.re;
+    }
+  return e;
+}
+
+std::vector<float> g_planner;  // learned planner weights (empty = greedy)
+
+double planner_score(const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& ops, int n);
+
+Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n) {
+  Plan plan;
+  size_t i = begin;
+  while (i < end) {
+    if (!ops[i].is_unitary() || fusion == FUSE_NONE) {
+      plan.ops.push_back(lower(ops[i], plan, prec));
+      ++i;
+      continue;
+    }
+    size_t j = i;
+    while (j < end && ops[j].is_unitary()) ++j;
+    MergeScorer scorer = nullptr;
+    if (fusion == FUSE_LEARNED && !g_planner.empty())
+      scorer = [n](const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& o) {
+        return planner_score(a, b, o, n);
+      };
+    std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
+    for (FusedBlock& blk : blocks) {
+      if (blk.ops.size() == 1) {
+        plan.ops.push_back(lower(ops[blk.ops[0]], plan, prec));
+        continue;
+      }
+      build_block_matrix(blk, ops);
+      DevOp d;
+      d.category = 1;
+      d.gates = (int)blk.ops.size();
+      if (blk.diagonal) {
+        std::vector<cplx> dg;
+        const size_t dim = (size_t)1 << blk.qubits.size();
+        for (size_t r = 0; r < dim; ++r) dg.push_back(blk.mat[r * dim + r]);
+        set_diag(d, blk.qubits, dg);
+      } else {
+        d.kind = prec == 8 ? D_FUSED : D_DENSE;
+        set_dense_offsets(d, blk.qubits);
+        push_matrix(plan, d, blk.mat, prec);
+        if (prec == 8 && d.kq >= 4) {
+          std::vector<float> emb = real_embedding(blk.mat, d.kq);
+          d.emb_off = plan.push(emb.data(), emb.size() * sizeof(float));
+          d.mfma = true;
+        }
+      }
+      plan.ops.push_back(std::move(d));
+    }
+    i = j;
+  }
+  return plan;
+}
+
+// ---------------------------------------------------------------------------
+// Executor
+// ---------------------------------------------------------------------------
+struct Exec {
+  int prec, n;
+  uint64_t nc, ncs, stride, c1;
+  uint32_t batch;
+  hipStream_t st;
+  DeviceMem owned, cbits, rng, ms, flags, err, red_a, red_b;
+  void* sp = nullptr;  // state of slot 0; slots follow at `stride` amplitudes
+  const uint8_t* cbase = nullptr;
+  bool mfma_enabled = true;
+
+  // With `external` set, the executor works on a caller-owned state.
+  Exec(int prec_, int n_, uint64_t nc_, uint32_t batch_, hipStream_t st_, void* external = nullptr)
+      : prec(prec_), n(n_), nc(nc_), batch(batch_), st(st_) {
+    ncs = nc ? nc : 1;
+    stride = 1ull << n;
+    const int seg1 = n < 11 ? n : 11;
+    c1 = 1ull << (n - seg1);
+    if (external) {
+      sp = external;
+    } else {
+      owned.alloc(stride * (uint64_t)prec * batch);
+      sp = owned.get();
+    }
+    cbits.alloc(ncs * batch);
+    rng.alloc(sizeof(k::RngState) * batch);
+    ms.alloc(sizeof(k::MeasState) * batch);
+    flags.alloc(sizeof(int) * batch);
+    err.alloc(sizeof(int));
+    red_a.alloc(16 * c1 * batch);
+    red_b.alloc(16 * c1 * batch);
+    HIPCHK(hipMemsetAsync(cbits.get(), 0, ncs * batch, st));
+    HIPCHK(hipMemsetAsync(err.get(), 0, sizeof(int), st));
+    HIPCHK(hipMemsetAsync(ms.get(), 0, sizeof(k::MeasState) * batch, st));
+  }
+
+  void check_err() {
+    HIPCHK(hipGetLastError());
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, err.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (e & 1) throw RuntimeErr("Cannot measure a state with nonfinite or zero norm");
+    if (e & 2) throw RuntimeErr("Invalid state normalization when sampling");
+  }
+
+  // Total |a|^2 split by `branch` for each slot; returns the device pointer
+  // to the double2 totals (stride c1).
+  const void* reduce_prob(int branch, uint32_t nb) {
+    k::ProbSpec ps;
+    ps.branch = branch;
+    const int seg1 = n < 11 ? n : 11;
+    k::seg_reduce_prob(prec, sp, stride, nb, n, ps, seg1, red_a.get(), c1, st);
+    uint64_t count = 1ull << (n - seg1);
+    void* src = red_a.get();
+    void* dst = red_b.get();
+    while (count > 1) {
+      int lg = 0;
+      while ((1ull << lg) < count) ++lg;
+      int s = lg < 11 ? lg : 11;
+      k::seg_reduce_f64(src, c1, count, nb, s, dst, c1, st);
+      count >>= s;
+      std::swap(src, dst);
+    }
+    return src;
+  }
+
+  void measure(uint32_t q, int64_t cbit, k::Flag f, uint32_t nb) {
+    const void* tot = reduce_prob((int)q, nb);
+    k::meas_decide(tot, c1, nb, ms.as<k::MeasState>(), rng.as<k::RngState>(), cbits.as<uint8_t>(), ncs, cbit, f,
+                   err.as<int>(), st);
+    k::collapse(prec, sp, stride, nb, n, q, ms.as<k::MeasState>(), st);
+  }
+
+  void run(const DevOp& d, uint32_t nb) {
+    k::Flag f;
+    if (!d.conds.empty()) {
+      k::cond_eval(cbits.as<uint8_t>(), ncs, nb, (const k::Cond*)(cbase + d.cond_off), (int)d.conds.size(),
+                   flags.as<int>(), st);
+      f.ptr = flags.as<int>();
+      f.stride = 1;
+    }
+    void* s = sp;
+    switch (d.kind) {
+      case D_ONE: k::one(prec, s, stride, nb, n, d.e, d.target, d.u, f, st); break;
+      case D_X: k::xgate(prec, s, stride, nb, n, d.e, d.target, f, st); break;
+      case D_SWAP: k::swap(prec, s, stride, nb, n, d.e, d.a, d.b, f, st); break;
+      case D_DIAG:
+        k::diag(prec, s, stride, nb, n, d.e, (int)d.pat.size(), d.pat.data(), d.dval.data(), f, st);
+        break;
+      case D_DENSE: k::dense(prec, s, stride, nb, n, d.kq, d.e, cbase + d.mat_off, d.offsets, f, st); break;
+      case D_FUSED:
+        if (d.mfma && mfma_enabled &&
+            k::fused_block_mfma(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.emb_off), d.offsets, st))
+          break;
+        k::fused_block(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.mat_off), d.offsets, st);
+        break;
+      case D_MEAS: measure(d.qubit, d.cbit, f, nb); break;
+      case D_RESET: {
+        measure(d.qubit, -1, f, nb);
+        k::Expand e = make_expand({d.qubit}, 0);
+        k::Flag xf;
+        xf.ptr = (const int*)((const char*)ms.get() + offsetof(k::MeasState, xflag));
+        xf.stride = sizeof(k::MeasState) / sizeof(int);
+        k::xgate(prec, s, stride, nb, n, e, d.qubit, xf, st);
+        break;
+      }
+      case D_NOOP: break;
+    }
+  }
+};
+
+double op_bytes(const DevOp& d, int n, int prec) {
+  const double len = std::ldexp(1.0, n);
+  switch (d.kind) {
+    case D_ONE:
+    case D_X: return 2.0 * prec * len / std::ldexp(1.0, d.nctl);
+    case D_SWAP: return 2.0 * prec * len / std::ldexp(1.0, 1 + d.nctl);
+    case D_DIAG: return 2.0 * prec * len / std::ldexp(1.0, d.kq) * (double)d.pat.size();
+    case D_DENSE:
+    case D_FUSED: return 2.0 * prec * len;
+    case D_MEAS:
+    case D_RESET: return 3.0 * prec * len;
+    default: return 0.0;
+  }
+}
+
+int op_kind_code(const DevOp& d) {
+  switch (d.kind) {
+    case D_ONE: return 0;
+    case D_X: return 1;
+    case D_SWAP: return 2;
+    case D_DIAG: return 3;
+    case D_DENSE: return 4;
+    case D_FUSED: return d.mfma ? 8 : 5;
+    case D_MEAS: return 6;
+    case D_RESET: return 7;
+    default: return -1;
+  }
+}
+
+struct Parsed {
+  Circuit c;
+  std::vector<Op> ops;
+  uint64_t n = 0, nc = 0;
+};
+
+Parsed read_and_validate(const char* json, size_t len) {
+  Parsed p;
+  p.c = parse_circuit(json, len);
+  validate_circuit(p.c);
+  p.n = p.c.num_qubits();
+  p.nc = p.c.num_cbits();
+  return p;
+}
+
+size_t prefix_end_of(const std::vector<Op>& ops) {
+  size_t i = 0;
+  while (i < ops.size() && ops[i].is_unitary()) ++i;
+  return i;
+}
+
+uint64_t count_gates(const std::vector<Instruction>& insts) {
+  uint64_t g = 0;
+  for (auto& in : insts) {
+    switch (in.kind) {
+      case Kind::Measure:
+      case Kind::Reset:
+      case Kind::Barrier:
+      case Kind::Classical:
+      case Kind::Conditional: break;
+      case Kind::Gate:
+        if (in.name != "remote_barrier") ++g;
+        break;
+      default: ++g;
+    }
+  }
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Marginals, CDF and sampling
+// ---------------------------------------------------------------------------
+
+// Virtual-index map for the marginal over `qubits` (qubits[0] = MSB of the
+// output): low virtual bits walk the other qubits in ascending order.
+std::vector<uint8_t> marginal_map(int n, const std::vector<uint64_t>& qubits) {
+  std::vector<char> used(n, 0);
+  for (auto q : qubits) used[q] = 1;
+  std::vector<uint8_t> map;
+  for (int q = 0; q < n; ++q)
+    if (!used[q]) map.push_back((uint8_t)q);
+  for (size_t i = 0; i < qubits.size(); ++i) map.push_back((uint8_t)qubits[qubits.size() - 1 - i]);
+  return map;
+}
+
+// Reduce the state in virtual order over segments of 2^seg elements.
+// Returns a device buffer with 2^(n-seg) double2 values (channel x).
+DeviceMem reduce_virtual(Exec& ex, const std::vector<uint8_t>& map, int seg) {
+  const int n = ex.n;
+  const int seg1 = seg < 11 ? seg : 11;
+  const uint64_t count1 = 1ull << (n - seg1);
+  DeviceMem a(16 * count1), b;
+  k::ProbSpec ps;
+  ps.branch = -1;
+  ps.map = map.data();
+  k::seg_reduce_prob(ex.prec, ex.sp, ex.stride, 1, n, ps, seg1, a.get(), 0, ex.st);
+  int remaining = seg - seg1;
+  uint64_t count = count1;
+  while (remaining > 0) {
+    int s = remaining < 11 ? remaining : 11;
+    uint64_t next = count >> s;
+    if (next == 0) next = 1;
+    b.alloc(16 * next);
+    k::seg_reduce_f64(a.get(), 0, count, 1, s, b.get(), 0, ex.st);
+    count = next;
+    remaining -= s;
+    std::swap(a, b);
+  }
+  return a;
+}
+
+constexpr int kMatMax = 24;  // largest marginal width sampled from a materialised CDF
+
+uint64_t cdf_bytes(uint64_t m) {
+  if (m <= (uint64_t)kMatMax) return (1ull << m) * 8;
+  return (1ull << (m - 11)) * 8;
+}
+
+// Sample `shots` indices over the marginal of `qubits` (qubits[0] = MSB).
+std::vector<uint64_t> sample_indices(Exec& ex, const std::vector<uint64_t>& qubits, uint64_t shots, uint64_t seed) {
+  const int n = ex.n;
+  const int m = (int)qubits.size();
+  std::vector<uint8_t> map = marginal_map(n, qubits);
+  DeviceMem idx(8 * std::max<uint64_t>(shots, 1));
+  DeviceMem tot(8);
+  if (m <= kMatMax) {
+    DeviceMem bins = reduce_virtual(ex, map, n - m);
+    const uint64_t count = 1ull << m;
+    DeviceMem C(8 * count), gt(8 * ((count + 1023) / 1024));
+    k::cdf_build(bins.as<double>(), 2, count, C.as<double>(), gt.as<double>(), tot.as<double>(), ex.err.as<int>(),
+                 ex.st);
+    k::sample_cdf(C.as<double>(), count, tot.as<double>(), seed, 0, shots, idx.as<uint64_t>(), ex.st);
+    ex.check_err();
+  } else {
+    if (n - m > 11) throw ValueErr("marginal sampling over more than 2^11 amplitudes per outcome is not supported");
+    DeviceMem blk = reduce_virtual(ex, map, n - m + 11);
+    const uint64_t nblocks = 1ull << (m - 11);
+    DeviceMem C(8 * nblocks), gt(8 * ((nblocks + 1023) / 1024));
+    k::cdf_build(blk.as<double>(), 2, nblocks, C.as<double>(), gt.as<double>(), tot.as<double>(), ex.err.as<int>(),
+                 ex.st);
+    k::sample_blocked(ex.prec, ex.sp, n, map.data(), n - m, C.as<double>(), nblocks, tot.as<double>(), seed,
+                      0, shots, idx.as<uint64_t>(), ex.st);
+    ex.check_err();
+  }
+  std::vector<uint64_t> out(shots);
+  if (shots) HIPCHK(hipMemcpyAsync(out.data(), idx.get(), 8 * shots, hipMemcpyDeviceToHost, ex.st));
+  HIPCHK(hipStreamSynchronize(ex.st));
+  return out;
+}
+
+std::string format_bits(const uint8_t* cb, uint64_t nc) {
+  std::string s(nc, '0');
+  for (uint64_t i = 0; i < nc; ++i) s[nc - 1 - i] = (char)('0' + cb[i]);
+  return s;
+}
+
+std::string json_num(double x) {
+  if (!std::isfinite(x)) return "null";
+  char buf[40];
+  std::snprintf(buf, sizeof buf, "%.17g", x);
+  return buf;
+}
+
+void write_shots_profile(double pre, double fusion, double shots_total, double total, uint64_t nshots,
+                         const std::vector<double>& shot_times, const char* strategy, uint64_t parallel,
+                         uint64_t working, uint64_t prefix_ops) {
+  std::ostringstream js;
+  js << "{\n  \"preprocessing_time\": " << json_num(pre) << ",\n  \"fusion_time\": " << json_num(fusion)
+     << ",\n  \"shots_total_time\": " << json_num(shots_total) << ",\n  \"total_time\": " << json_num(total)
+     << ",\n  \"num_shots\": " << nshots << ",\n  \"shot_times\": [";
+  for (size_t i = 0; i < shot_times.size(); ++i) js << (i ? ",\n    " : "\n    ") << json_num(shot_times[i]);
+  js << (shot_times.empty() ? "]" : "\n  ]");
+  js << ",\n  \"execution_strategy\": \"" << strategy << "\",\n  \"parallel_shots\": " << parallel
+     << ",\n  \"working_bytes\": " << working << ",\n  \"deterministic_prefix_ops\": " << prefix_ops << "\n}";
+  if (mkdir("dqsim_profiles", 0777) != 0 && errno != EEXIST)
+    throw RuntimeErr(std::string("Failed to write shots profile: ") + std::strerror(errno));
+  auto ns = std::chrono::duration_cast<std::chrono::nanoseconds>(
+                std::chrono::system_clock::now().time_since_epoch())
+                .count();
+  std::string path = "dqsim_profiles/statevector_shots_profile_" + std::to_string(ns) + ".json";
+  std::ofstream f(path);
+  if (!f) throw RuntimeErr("Failed to write shots profile: cannot open " + path);
+  f << js.str();
+  if (!f) throw RuntimeErr("Failed to write shots profile: write failed");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// simulate() (simulator.rs:319-371)
+// ---------------------------------------------------------------------------
+std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len) {
+  const double t0 = now_s();
+  Parsed p = read_and_validate(json, len);
+  Layout lay = state_layout(p.n, o.prec);
+  if (lay.bytes > UINT64_MAX - p.nc) throw MemErr("Working memory size overflow");
+  check_budget(lay.bytes + p.nc, o.budget);
+  p.ops = compile(p.c.instructions);
+  const uint64_t seed = o.has_seed ? o.seed : random_seed();
+  Stream st;
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n);
+  plan.upload(st.s);
+  Exec ex(o.prec, (int)p.n, p.nc, 1, st.s);
+  ex.cbase = plan.base();
+  k::init_zero(o.prec, ex.sp, ex.stride, 1, (int)p.n, st.s);
+  k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
+  auto res = std::make_unique<Result>();
+  res->prec = o.prec;
+  res->n = (uint32_t)p.n;
+  res->nc = p.nc;
+  if (o.profile) {
+    SimProfile prof;
+    for (const DevOp& d : plan.ops) {
+      if (!d.conds.empty()) {
+        k::cond_eval(ex.cbits.as<uint8_t>(), ex.ncs, 1, (const k::Cond*)(plan.base() + d.cond_off),
+                     (int)d.conds.size(), ex.flags.as<int>(), st.s);
+        int flag = 0;
+        HIPCHK(hipMemcpyAsync(&flag, ex.flags.get(), sizeof(int), hipMemcpyDeviceToHost, st.s));
+        st.sync();
+        if (!flag) continue;
+        DevOp plain = d;
+        plain.conds.clear();
+        Event a, b;
+        HIPCHK(hipEventRecord(a.e, st.s));
+        ex.run(plain, 1);
+        HIPCHK(hipEventRecord(b.e, st.s));
+        st.sync();
+        double s = elapsed_ms(a, b) / 1e3;
+        if (plain.category == 0) prof.oq_calls++, prof.oq_time += s;
+        else if (plain.category == 1) prof.nq_calls++, prof.nq_time += s;
+        else if (plain.category == 2) prof.mq_calls++, prof.mq_time += s;
+        continue;
+      }
+      if (d.kind == D_NOOP) continue;
+      Event a, b;
+      HIPCHK(hipEventRecord(a.e, st.s));
+      ex.run(d, 1);
+      HIPCHK(hipEventRecord(b.e, st.s));
+      st.sync();
+      double s = elapsed_ms(a, b) / 1e3;
+      if (d.category == 0) prof.oq_calls++, prof.oq_time += s;
+      else if (d.category == 1) prof.nq_calls++, prof.nq_time += s;
+      else if (d.category == 2) prof.mq_calls++, prof.mq_time += s;
+    }
+    ex.check_err();
+    prof.total_time = now_s() - t0;
+    res->has_profile = true;
+    res->prof = prof;
+  } else {
+    for (const DevOp& d : plan.ops) ex.run(d, 1);
+    ex.check_err();
+  }
+  res->cbits.assign(p.nc, 0);
+  if (p.nc) HIPCHK(hipMemcpyAsync(res->cbits.data(), ex.cbits.get(), p.nc, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+  res->state = std::move(ex.owned);
+  return res;
+}
+
+// ---------------------------------------------------------------------------
+// simulate_shots() / run_shots (shots.rs:58-260, simulator.rs:256-317)
+// ---------------------------------------------------------------------------
+Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t shots, bool profile_call) {
+  const double total_t0 = now_s();
+  const bool profile = profile_call || o.profile;
+  Parsed p = read_and_validate(json, len);
+  const double preprocessing_time = now_s() - total_t0;
+  const double fusion_t0 = now_s();
+  p.ops = compile(p.c.instructions);
+  const double fusion_time = now_s() - fusion_t0;
+  const uint64_t seed = o.has_seed ? o.seed : random_seed();
+  const double shots_t0 = now_s();
+  const int n = (int)p.n;
+  const uint64_t nc = p.nc;
+
+  std::map<std::string, uint64_t> counts;
+  std::vector<double> shot_times;
+  const char* strategy = "empty";
+  uint64_t parallel_shots = 0, working_bytes = 0, prefix_ops = 0;
+
+  auto finish = [&]() {
+    Counts out(counts.begin(), counts.end());
+    if (profile)
+      write_shots_profile(preprocessing_time, fusion_time, now_s() - shots_t0, now_s() - total_t0, shots, shot_times,
+                          strategy, parallel_shots, working_bytes, prefix_ops);
+    return out;
+  };
+  if (shots == 0) return finish();
+
+  Layout lay = state_layout(p.n, o.prec);
+  if (lay.bytes > UINT64_MAX - nc) throw MemErr("Working memory size overflow");
+  const uint64_t worker_bytes = lay.bytes + nc;
+  check_budget(worker_bytes, o.budget);
+  const size_t prefix_end = prefix_end_of(p.ops);
+  bool terminal = true;
+  for (size_t i = prefix_end; i < p.ops.size(); ++i)
+    if (p.ops[i].kind != OpKind::Measure) terminal = false;
+
+  if (o.sample_terminal && terminal) {
+    std::map<uint64_t, uint64_t> mapping;  // cbit -> qubit, last write wins
+    for (size_t i = prefix_end; i < p.ops.size(); ++i) mapping[p.ops[i].cbit] = p.ops[i].qubit;
+    if (mapping.empty()) {
+      strategy = "terminal_sampling";
+      counts[std::string(nc, '0')] = shots;
+      return finish();
+    }
+    std::vector<uint64_t> qubits;
+    for (auto& kv : mapping) qubits.push_back(kv.second);
+    std::sort(qubits.begin(), qubits.end());
+    qubits.erase(std::unique(qubits.begin(), qubits.end()), qubits.end());
+    std::reverse(qubits.begin(), qubits.end());
+    const uint64_t m = qubits.size();
+    std::vector<std::pair<uint64_t, uint64_t>> shifts;  // (cbit, shift)
+    for (auto& kv : mapping) {
+      size_t pos = std::find(qubits.begin(), qubits.end(), kv.second) - qubits.begin();
+      shifts.push_back({kv.first, m - 1 - pos});
+    }
+    const uint64_t cb = cdf_bytes(m);
+    if (worker_bytes <= UINT64_MAX - cb && worker_bytes + cb <= o.budget) {
+      Stream st;
+      Plan plan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n);
+      plan.upload(st.s);
+      Exec ex(o.prec, n, nc, 1, st.s);
+      ex.cbase = plan.base();
+      k::init_zero(o.prec, ex.sp, ex.stride, 1, n, st.s);
+      k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
+      for (const DevOp& d : plan.ops) ex.run(d, 1);
+      std::vector<uint64_t> idx = sample_indices(ex, qubits, shots, seed);
+      std::unordered_map<uint64_t, uint64_t> sampled;
+      for (uint64_t x : idx) sampled[x]++;
+      std::vector<uint8_t> cbits(nc, 0);
+      for (auto& kv : sampled) {
+        for (auto& cs : shifts) cbits[cs.first] = (uint8_t)((kv.first >> cs.second) & 1);
+        counts[format_bits(cbits.data(), nc)] += kv.second;
+      }
+      strategy = "terminal_sampling";
+      parallel_shots = 1;
+      working_bytes = worker_bytes + cb;
+      prefix_ops = prefix_end;
+      return finish();
+    }
+    // Fall back to trajectories when the sampling table exceeds the budget.
+  }
+
+  // Trajectories: a batch of `workers` independent shots per launch sequence.
+  const uint64_t budget = o.budget;
+  const bool cache_prefix = shots > 1 && prefix_end > 0 && lay.bytes <= budget - worker_bytes;
+  const uint64_t prefix_bytes = cache_prefix ? lay.bytes : 0;
+  const uint64_t kMaxBatch = 1024;
+  uint64_t workers = std::min<uint64_t>(shots, o.has_max_parallel_shots ? o.max_parallel_shots : kMaxBatch);
+  workers = std::min<uint64_t>(workers, kMaxBatch);
+  workers = std::min<uint64_t>(workers, (budget - prefix_bytes) / worker_bytes);
+  workers = std::max<uint64_t>(workers, 1);
+
+  Stream st;
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, n);
+  plan.upload(st.s);
+  // Fused runs never cross a non-unitary op, so the plan's leading unitary
+  // launches cover exactly ops[0, prefix_end).
+  size_t plan_prefix = 0;
+  while (plan_prefix < plan.ops.size()) {
+    const DevOp& d = plan.ops[plan_prefix];
+    if (!d.conds.empty() || d.kind == D_MEAS || d.kind == D_RESET) break;
+    ++plan_prefix;
+  }
+  DeviceMem prefix;
+  if (cache_prefix) {
+    Exec pex(o.prec, n, nc, 1, st.s);
+    pex.cbase = plan.base();
+    k::init_zero(o.prec, pex.sp, pex.stride, 1, n, st.s);
+    for (size_t i = 0; i < plan_prefix; ++i) pex.run(plan.ops[i], 1);
+    st.sync();
+    prefix = std::move(pex.owned);
+  }
+  const size_t tail_begin = cache_prefix ? plan_prefix : 0;
+  Exec ex(o.prec, n, nc, (uint32_t)workers, st.s);
+  ex.cbase = plan.base();
+  std::vector<uint8_t> all(shots * ex.ncs);
+  if (profile) shot_times.assign(shots, 0.0);
+  for (uint64_t s0 = 0; s0 < shots; s0 += workers) {
+    const uint32_t w = (uint32_t)std::min<uint64_t>(workers, shots - s0);
+    Event a, b;
+    if (profile) HIPCHK(hipEventRecord(a.e, st.s));
+    if (cache_prefix) k::broadcast(o.prec, ex.sp, ex.stride, w, prefix.get(), n, st.s);
+    else k::init_zero(o.prec, ex.sp, ex.stride, w, n, st.s);
+    HIPCHK(hipMemsetAsync(ex.cbits.get(), 0, ex.ncs * w, st.s));
+    k::rng_init(ex.rng.as<k::RngState>(), w, seed, s0, st.s);
+    for (size_t i = tail_begin; i < plan.ops.size(); ++i) ex.run(plan.ops[i], w);
+    if (profile) HIPCHK(hipEventRecord(b.e, st.s));
+    HIPCHK(hipMemcpyAsync(all.data() + s0 * ex.ncs, ex.cbits.get(), ex.ncs * w, hipMemcpyDeviceToHost, st.s));
+    if (profile) {
+      st.sync();
+      double t = elapsed_ms(a, b) / 1e3;
+      for (uint32_t i = 0; i < w; ++i) shot_times[s0 + i] = t;
+    }
+  }
+  ex.check_err();
+  for (uint64_t s = 0; s < shots; ++s) counts[format_bits(all.data() + s * ex.ncs, nc)]++;
+  strategy = "trajectories";
+  parallel_shots = workers;
+  working_bytes = prefix_bytes + workers * worker_bytes;
+  prefix_ops = cache_prefix ? prefix_end : 0;
+  return finish();
+}
+
+// ---------------------------------------------------------------------------
+// SimulationResult queries (simulator.rs:110-191)
+// ---------------------------------------------------------------------------
+template <typename F>
+static auto with_state(const Result& r, F fn) {
+  Stream st;
+  Exec ex(r.prec, (int)r.n, 0, 1, st.s, r.state.get());
+  return fn(ex);
+}
+
+void result_probabilities(const Result& r, const std::vector<uint64_t>& qubits, double* out) {
+  with_state(r, [&](Exec& ex) {
+    const int m = (int)qubits.size();
+    std::vector<uint8_t> map = marginal_map(ex.n, qubits);
+    DeviceMem bins = reduce_virtual(ex, map, ex.n - m);
+    const uint64_t count = 1ull << m;
+    std::vector<double> tmp(2 * count);
+    HIPCHK(hipMemcpyAsync(tmp.data(), bins.get(), 16 * count, hipMemcpyDeviceToHost, ex.st));
+    HIPCHK(hipStreamSynchronize(ex.st));
+    for (uint64_t i = 0; i < count; ++i) out[i] = tmp[2 * i];
+    return 0;
+  });
+}
+
+Counts result_counts(const Result& r, uint64_t shots, const std::vector<uint64_t>& qubits, uint64_t seed) {
+  Counts out;
+  if (shots == 0) return out;
+  return with_state(r, [&](Exec& ex) {
+    std::vector<uint64_t> idx = sample_indices(ex, qubits, shots, seed);
+    std::unordered_map<uint64_t, uint64_t> sampled;
+    for (uint64_t x : idx) sampled[x]++;
+    const size_t w = qubits.size();
+    std::map<std::string, uint64_t> c;
+    for (auto& kv : sampled) {
+      std::string bits(w, '0');
+      for (size_t i = 0; i < w; ++i)
+        if ((kv.first >> (w - 1 - i)) & 1) bits[i] = '1';
+      c[bits] += kv.second;
+    }
+    return Counts(c.begin(), c.end());
+  });
+}
+
+double result_fidelity(const Result& r, const void* other, uint64_t len) {
+  if (len != (1ull << r.n)) return 0.0;
+  return with_state(r, [&](Exec& ex) {
+    DeviceMem o(16 * len);
+    HIPCHK(hipMemcpyAsync(o.get(), other, 16 * len, hipMemcpyHostToDevice, ex.st));
+    const int n = ex.n;
+    const int seg1 = n < 11 ? n : 11;
+    uint64_t count = 1ull << (n - seg1);
+    DeviceMem a(16 * count), b(16 * count);
+    k::seg_reduce_dot(ex.prec, ex.sp, n, o.get(), seg1, a.get(), ex.st);
+    void* src = a.get();
+    void* dst = b.get();
+    while (count > 1) {
+      int lg = 0;
+      while ((1ull << lg) < count) ++lg;
+      int s = lg < 11 ? lg : 11;
+      k::seg_reduce_f64(src, 0, count, 1, s, dst, 0, ex.st);
+      count >>= s;
+      std::swap(src, dst);
+    }
+    double d[2];
+    HIPCHK(hipMemcpyAsync(d, src, 16, hipMemcpyDeviceToHost, ex.st));
+    HIPCHK(hipStreamSynchronize(ex.st));
+    return d[0] * d[0] + d[1] * d[1];
+  });
+}
+
+void result_statevector(const Result& r, void* out) {
+  const uint64_t len = 1ull << r.n;
+  Stream st;
+  if (r.prec == 16) {
+    HIPCHK(hipMemcpyAsync(out, r.state.get(), 16 * len, hipMemcpyDeviceToHost, st.s));
+  } else {
+    const uint64_t chunk = 1ull << 26;
+    DeviceMem tmp(16 * std::min(len, chunk));
+    for (uint64_t off = 0; off < len; off += chunk) {
+      uint64_t c = std::min(chunk, len - off);
+      k::to_c128(8, (const char*)r.state.get() + 8 * off, c, tmp.get(), st.s);
+      HIPCHK(hipMemcpyAsync((char*)out + 16 * off, tmp.get(), 16 * c, hipMemcpyDeviceToHost, st.s));
+    }
+  }
+  st.sync();
+}
+
+void result_amplitude(const Result& r, uint64_t index, double* out) {
+  Stream st;
+  if (r.prec == 16) {
+    HIPCHK(hipMemcpyAsync(out, (const char*)r.state.get() + 16 * index, 16, hipMemcpyDeviceToHost, st.s));
+    st.sync();
+  } else {
+    float f[2];
+    HIPCHK(hipMemcpyAsync(f, (const char*)r.state.get() + 8 * index, 8, hipMemcpyDeviceToHost, st.s));
+    st.sync();
+    out[0] = f[0];
+    out[1] = f[1];
+  }
+}
+
+void result_compare(const Result& a, const Result& b, double* max_abs, double* l2) {
+  if (a.n != b.n) throw ValueErr("results have different qubit counts");
+  const uint64_t len = 1ull << a.n;
+  Stream st;
+  const uint64_t nb = k::diff_blocks(len);
+  DeviceMem part(16 * nb);
+  k::diff(a.prec, a.state.get(), b.prec, b.state.get(), len, part.get(), st.s);
+  std::vector<double> h(2 * nb);
+  HIPCHK(hipMemcpyAsync(h.data(), part.get(), 16 * nb, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+  double m = 0, s = 0;
+  for (uint64_t i = 0; i < nb; ++i) {
+    m = std::max(m, h[2 * i]);
+    s += h[2 * i + 1];
+  }
+  *max_abs = m;
+  *l2 = std::sqrt(s);
+}
+
+// ---------------------------------------------------------------------------
+// Timed evolution (benchmarks, planner training data)
+// ---------------------------------------------------------------------------
+EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int repeats, int record,
+                         std::unique_ptr<Result>* keep) {
+  Parsed p = read_and_validate(json, len);
+  Layout lay = state_layout(p.n, o.prec);
+  check_budget(lay.bytes + p.nc, o.budget);
+  p.ops = compile(p.c.instructions);
+  const size_t pe = prefix_end_of(p.ops);
+  const int n = (int)p.n;
+  Stream st;
+  Plan plan = build_plan(p.ops, 0, pe, o.prec, o.fusion, o.max_fused, n);
+  plan.upload(st.s);
+  Exec ex(o.prec, n, p.nc, 1, st.s);
+  ex.cbase = plan.base();
+  EvolveStats stats;
+  stats.gates = count_gates(p.c.instructions);
+  stats.launches = plan.ops.size();
+  for (int rep = 0; rep < std::max(1, repeats); ++rep) {
+    k::init_zero(o.prec, ex.sp, ex.stride, 1, n, st.s);
+    const bool rec = record && rep == std::max(1, repeats) - 1;
+    Event t0, t1;
+    std::vector<Event> evs;
+    if (rec) evs.resize(plan.ops.size() + 1);
+    HIPCHK(hipEventRecord(t0.e, st.s));
+    for (size_t i = 0; i < plan.ops.size(); ++i) {
+      if (rec) HIPCHK(hipEventRecord(evs[i].e, st.s));
+      ex.run(plan.ops[i], 1);
+    }
+    if (rec) HIPCHK(hipEventRecord(evs[plan.ops.size()].e, st.s));
+    HIPCHK(hipEventRecord(t1.e, st.s));
+    st.sync();
+    stats.total_ms = elapsed_ms(t0, t1);
+    if (rec) {
+      stats.ops.clear();
+      for (size_t i = 0; i < plan.ops.size(); ++i) {
+        const DevOp& d = plan.ops[i];
+        OpTiming t;
+        t.kind = op_kind_code(d);
+        t.k = d.kq;
+        t.low = d.low;
+        t.gates = d.gates;
+        t.ms = elapsed_ms(evs[i], evs[i + 1]);
+        t.bytes = op_bytes(d, n, o.prec);
+        stats.ops.push_back(t);
+      }
+    }
+  }
+  ex.check_err();
+  if (keep) {
+    auto res = std::make_unique<Result>();
+    res->prec = o.prec;
+    res->n = (uint32_t)n;
+    res->nc = p.nc;
+    res->cbits.assign(p.nc, 0);
+    res->state = std::move(ex.owned);
+    *keep = std::move(res);
+  }
+  return stats;
+}
+
+// ---------------------------------------------------------------------------
+// Plan description (host only)
+// ---------------------------------------------------------------------------
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused) {
+  Parsed p = read_and_validate(json, len);
+  p.ops = compile(p.c.instructions);
+  const size_t pe = prefix_end_of(p.ops);
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), prec, fusion, max_fused, (int)p.n);
+  static const char* names[] = {"one", "x", "swap", "diagonal", "dense", "fused", "measure", "reset", "noop"};
+  std::ostringstream js;
+  js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
+     << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions) << ", \"launches\": [";
+  for (size_t i = 0; i < plan.ops.size(); ++i) {
+    const DevOp& d = plan.ops[i];
+    js << (i ? ", " : "") << "{\"kind\": \"" << names[d.kind] << "\", \"k\": " << d.kq
+       << ", \"gates\": " << d.gates << ", \"conditional\": " << (d.conds.empty() ? "false" : "true")
+       << ", \"mfma\": " << (d.mfma ? "true" : "false") << ", \"entries\": " << d.pat.size()
+       << ", \"category\": " << d.category << ", \"bytes\": " << json_num(op_bytes(d, (int)p.n, prec)) << "}";
+  }
+  js << "]}";
+  return js.str();
+}
+
+// ---------------------------------------------------------------------------
+// Learned planner hook
+// ---------------------------------------------------------------------------
+void set_planner_weights(const float* w, size_t count) {
+  if (!w || count == 0) g_planner.clear();
+  else g_planner.assign(w, w + count);
+}
+
+namespace {
+double planner_score(const FusedBlock&, const FusedBlock&, const std::vector<Op>&, int) { return 1.0; }
+}  // namespace
+
+}  // namespace dq

@@ -1,0 +1,137 @@
+// Device statevector engine: simulator options, plan lowering, execution on
+// the GPU, terminal sampling, trajectories and result queries.
+//
+// Follows src/monolithic/statevector/{shots.rs,simulator.rs} and
+// src/engine.rs of the reference for semantics; the kernels are in
+// kernels.hip.
+#pragma once
+
+#include <cstdint>
+#include <exception>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "circuit.hpp"
+
+namespace dq {
+
+struct MemErr : std::exception {
+  std::string msg;
+  explicit MemErr(std::string m) : msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+struct RuntimeErr : std::exception {
+  std::string msg;
+  explicit RuntimeErr(std::string m) : msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+struct DeviceErr : std::exception {
+  std::string msg;
+  explicit DeviceErr(std::string m) : msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+
+enum Fusion { FUSE_NONE = 0, FUSE_GREEDY = 1, FUSE_LEARNED = 2 };
+
+struct Options {
+  bool has_seed = false;
+  uint64_t seed = 0;
+  bool profile = false;
+  uint64_t budget = 1024ull << 20;  // bytes
+  bool has_max_parallel_shots = false;
+  uint64_t max_parallel_shots = 0;
+  bool sample_terminal = true;
+  int prec = 16;  // bytes per amplitude: 16 = complex128, 8 = complex64
+  int fusion = FUSE_NONE;
+  int max_fused = 4;
+};
+
+struct SimProfile {
+  uint64_t oq_calls = 0, nq_calls = 0, mq_calls = 0;
+  double oq_time = 0, nq_time = 0, mq_time = 0, total_time = 0;
+};
+
+class DeviceMem {
+ public:
+  DeviceMem() = default;
+  explicit DeviceMem(size_t bytes) { alloc(bytes); }
+  DeviceMem(const DeviceMem&) = delete;
+  DeviceMem& operator=(const DeviceMem&) = delete;
+  DeviceMem(DeviceMem&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  DeviceMem& operator=(DeviceMem&& o) noexcept;
+  ~DeviceMem();
+  void alloc(size_t bytes);
+  void reset();
+  void* get() const { return p_; }
+  size_t size() const { return n_; }
+  template <typename T>
+  T* as() const {
+    return (T*)p_;
+  }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+struct Result {
+  int prec = 16;
+  uint32_t n = 0;
+  uint64_t nc = 0;
+  DeviceMem state;
+  std::vector<uint8_t> cbits;
+  bool has_profile = false;
+  SimProfile prof;
+};
+
+using Counts = std::vector<std::pair<std::string, uint64_t>>;
+
+// Per-launch timing record (engine-recorded kernel timings).
+struct OpTiming {
+  int kind;      // 0 one, 1 x, 2 swap, 3 diag, 4 dense, 5 fused, 6 measure, 7 reset
+  int k;         // qubits touched
+  int low;       // lowest qubit touched
+  int gates;     // source gates folded into the launch
+  double ms;     // device time
+  double bytes;  // algorithmic HBM bytes moved
+};
+
+struct EvolveStats {
+  std::vector<OpTiming> ops;
+  double total_ms = 0;      // event time over the whole evolution
+  uint64_t gates = 0;       // unitary gate instructions in the input
+  uint64_t launches = 0;
+};
+
+std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len);
+Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t shots, bool profile);
+
+void result_probabilities(const Result& r, const std::vector<uint64_t>& qubits, double* out);
+Counts result_counts(const Result& r, uint64_t shots, const std::vector<uint64_t>& qubits, uint64_t seed);
+double result_fidelity(const Result& r, const void* other_c128, uint64_t len);
+void result_statevector(const Result& r, void* out_c128);
+void result_amplitude(const Result& r, uint64_t index, double* out);
+
+// Evolve the unitary prefix of a circuit from |0...0> `repeats` times and
+// report per-launch device timings of the last repeat. The final state can
+// be kept (returned in *keep) for accuracy checks.
+EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int repeats, int record,
+                         std::unique_ptr<Result>* keep);
+
+// Max |a-b| and sqrt(sum |a-b|^2) between two results of equal size.
+void result_compare(const Result& a, const Result& b, double* max_abs, double* l2);
+
+// Compile-only plan description (no device use), as JSON.
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused);
+
+int device_count();
+
+// Learned fusion planner weights (2 hidden layers of 64, FP32). Null clears.
+void set_planner_weights(const float* w, size_t count);
+
+}  // namespace dq

@@ -1,0 +1,894 @@
+// CDNA4 (gfx950) statevector kernels.
+//
+// The plain kernels restate the reference CPU loops of src/engine.rs with the
+// same complex arithmetic (num-complex operation order, no FMA contraction:
+// this file is built with -ffp-contract=off), so a complex128 run reproduces
+// the FP64 results of the reference algorithm bit for bit:
+//   one / xgate ...... for_each_pair + apply_one_qubit / apply_x (engine.rs:64-155)
+//   diag ............. apply_diagonal (engine.rs:157-181), identity entries skipped
+//   swap ............. apply_swap (engine.rs:183-241)
+//   dense ............ DenseLayout::apply (engine.rs:253-348)
+//   seg_reduce/meas .. measure_qubit (engine.rs:367-435), marginal_probs (:437-453)
+//   cdf/sample ....... terminal sampling (shots.rs:137-158), sample_counts (engine.rs:457-507)
+// Controlled gates enumerate only the amplitudes whose controls hold, and
+// sparse diagonals only the non-identity entries, so a launch moves exactly
+// the bytes the gate changes.
+//
+// Reductions use one canonical summation order, the pairwise tree over the
+// index range (sum(x[0:2^k]) = sum(lower half) + sum(upper half)), which is
+// what the butterfly over lanes plus the LDS tree computes; the CPU oracle
+// uses the same tree, so FP64 probabilities agree bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace dq {
+namespace k {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int ITEMS = 4;
+
+template <typename T>
+struct V2;
+template <>
+struct V2<float> {
+  using t = float2;
+};
+template <>
+struct V2<double> {
+  using t = double2;
+};
+
+template <typename T>
+struct M2 {
+  T v[8];  // u00r u00i u01r u01i u10r u10i u11r u11i
+};
+
+struct Offsets {
+  uint64_t o[32];
+};
+
+__device__ __forceinline__ uint64_t expand(uint64_t p, const Expand& e) {
+#pragma unroll
+  for (int i = 0; i < kMaxExpand; ++i) {
+    if ((uint32_t)i < e.nbits) {
+      uint64_t low = (1ull << e.pos[i]) - 1;
+      p = (p & low) | ((p & ~low) << 1);
+    }
+  }
+  return p | e.fixed;
+}
+
+__device__ __forceinline__ bool flag_off(const Flag& f, uint32_t b) {
+  return f.ptr != nullptr && f.ptr[(uint64_t)b * f.stride] == 0;
+}
+
+inline uint32_t blocks_for(uint64_t count, uint64_t per_block) {
+  uint64_t b = (count + per_block - 1) / per_block;
+  return (uint32_t)(b == 0 ? 1 : b);
+}
+
+// ---------------------------------------------------------------------------
+// 1-qubit update on amplitude pairs (engine.rs:113-128)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_one(typename V2<T>::t* __restrict__ s, uint64_t stride, uint64_t count,
+                                             Expand e, uint64_t tbit, M2<T> u, Flag f) {
+  using V = typename V2<T>::t;
+  const uint32_t b = blockIdx.y;
+  if (flag_off(f, b)) return;
+  V* st = s + (uint64_t)b * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+  V a[ITEMS], c[ITEMS];
+  uint64_t idx[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t p = base + (uint64_t)j * TPB;
+    idx[j] = p < count ? expand(p, e) : ~0ull;
+    if (p < count) {
+      a[j] = st[idx[j]];
+      c[j] = st[idx[j] | tbit];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    if (idx[j] == ~0ull) continue;
+    V x = a[j], y = c[j], na, nc;
+    na.x = (u.v[0] * x.x - u.v[1] * x.y) + (u.v[2] * y.x - u.v[3] * y.y);
+    na.y = (u.v[0] * x.y + u.v[1] * x.x) + (u.v[2] * y.y + u.v[3] * y.x);
+    nc.x = (u.v[4] * x.x - u.v[5] * x.y) + (u.v[6] * y.x - u.v[7] * y.y);
+    nc.y = (u.v[4] * x.y + u.v[5] * x.x) + (u.v[6] * y.y + u.v[7] * y.x);
+    st[idx[j]] = na;
+    st[idx[j] | tbit] = nc;
+  }
+}
+
+// Controlled permutation: swap the pair members (engine.rs:152-155); also
+// SWAP/CSWAP with tbit = a_bit and obit = b_bit (engine.rs:183-241).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_pairswap(typename V2<T>::t* __restrict__ s, uint64_t stride,
+                                                  uint64_t count, Expand e, uint64_t abit, uint64_t bbit, Flag f) {
+  using V = typename V2<T>::t;
+  const uint32_t b = blockIdx.y;
+  if (flag_off(f, b)) return;
+  V* st = s + (uint64_t)b * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+  V a[ITEMS], c[ITEMS];
+  uint64_t idx[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t p = base + (uint64_t)j * TPB;
+    idx[j] = p < count ? expand(p, e) : ~0ull;
+    if (p < count) {
+      a[j] = st[idx[j] | abit];
+      c[j] = st[idx[j] | bbit];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    if (idx[j] == ~0ull) continue;
+    st[idx[j] | abit] = c[j];
+    st[idx[j] | bbit] = a[j];
+  }
+}
+
+// Sparse diagonal (engine.rs:157-181): only entries != 1 are visited.
+template <typename T>
+struct DiagArgs {
+  uint64_t pat[32];
+  T val[64];
+};
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_diag(typename V2<T>::t* __restrict__ s, uint64_t stride, uint64_t count,
+                                              Expand e, int nent, DiagArgs<T> d, Flag f) {
+  using V = typename V2<T>::t;
+  const uint32_t b = blockIdx.y;
+  if (flag_off(f, b)) return;
+  V* st = s + (uint64_t)b * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t p = base + (uint64_t)j * TPB;
+    if (p >= count) continue;
+    uint64_t i0 = expand(p, e);
+    for (int t = 0; t < nent; ++t) {
+      uint64_t i = i0 | d.pat[t];
+      V a = st[i], r;
+      T dr = d.val[2 * t], di = d.val[2 * t + 1];
+      r.x = a.x * dr - a.y * di;
+      r.y = a.x * di + a.y * dr;
+      st[i] = r;
+    }
+  }
+}
+
+// Dense k<=5 gather / matvec / scatter (engine.rs:290-347).
+template <typename T, int K>
+__global__ __launch_bounds__(TPB) void k_dense(typename V2<T>::t* __restrict__ s, uint64_t stride, uint64_t count,
+                                               Expand e, const T* __restrict__ m, Offsets off, Flag f) {
+  using V = typename V2<T>::t;
+  constexpr int D = 1 << K;
+  const uint32_t b = blockIdx.y;
+  if (flag_off(f, b)) return;
+  V* st = s + (uint64_t)b * stride;
+  const uint64_t p = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= count) return;
+  const uint64_t base = expand(p, e);
+  V v[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) v[j] = st[base | off.o[j]];
+#pragma unroll
+  for (int row = 0; row < D; ++row) {
+    T ar = 0, ai = 0;
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      T mr = m[2 * (row * D + col)], mi = m[2 * (row * D + col) + 1];
+      ar = ar + (mr * v[col].x - mi * v[col].y);
+      ai = ai + (mr * v[col].y + mi * v[col].x);
+    }
+    V r;
+    r.x = ar;
+    r.y = ai;
+    // Outputs are written after all inputs are read: v[] is private.
+    st[base | off.o[row]] = r;
+  }
+}
+
+// Fused complex64 block: FMA arithmetic, matrix in FP32.
+template <int K>
+__global__ __launch_bounds__(TPB) void k_fused(float2* __restrict__ s, uint64_t stride, uint64_t count, Expand e,
+                                               const float* __restrict__ m, Offsets off) {
+  constexpr int D = 1 << K;
+  float2* st = s + (uint64_t)blockIdx.y * stride;
+  const uint64_t p = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (p >= count) return;
+  const uint64_t base = expand(p, e);
+  float2 v[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) v[j] = st[base | off.o[j]];
+#pragma unroll
+  for (int row = 0; row < D; ++row) {
+    float ar = 0.f, ai = 0.f;
+#pragma unroll
+    for (int col = 0; col < D; ++col) {
+      float mr = m[2 * (row * D + col)], mi = m[2 * (row * D + col) + 1];
+      ar = __builtin_fmaf(mr, v[col].x, ar);
+      ar = __builtin_fmaf(-mi, v[col].y, ar);
+      ai = __builtin_fmaf(mr, v[col].y, ai);
+      ai = __builtin_fmaf(mi, v[col].x, ai);
+    }
+    st[base | off.o[row]] = make_float2(ar, ai);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// State set-up
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_init_zero(typename V2<T>::t* __restrict__ s, uint64_t stride,
+                                                   uint64_t len) {
+  using V = typename V2<T>::t;
+  V* st = s + (uint64_t)blockIdx.y * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * TPB;
+    if (i < len) {
+      V z;
+      z.x = i == 0 ? T(1) : T(0);
+      z.y = T(0);
+      st[i] = z;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_broadcast(typename V2<T>::t* __restrict__ d, uint64_t stride,
+                                                   const typename V2<T>::t* __restrict__ src, uint64_t len) {
+  using V = typename V2<T>::t;
+  V* st = d + (uint64_t)blockIdx.y * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * TPB;
+    if (i < len) st[i] = src[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pairwise-tree segmented reductions (FP64, two channels)
+// ---------------------------------------------------------------------------
+struct Map64 {
+  uint8_t q[64];
+};
+
+template <typename T>
+struct ProbSrc {
+  const typename V2<T>::t* s;
+  uint64_t stride;
+  int branch;
+  int identity;
+  int n;
+  Map64 map;
+  __device__ __forceinline__ uint64_t phys(uint64_t v) const {
+    if (identity) return v;
+    uint64_t p = 0;
+    for (int j = 0; j < n; ++j) p |= ((v >> j) & 1ull) << map.q[j];
+    return p;
+  }
+  __device__ __forceinline__ double2 operator()(uint32_t b, uint64_t v) const {
+    uint64_t i = phys(v);
+    typename V2<T>::t a = s[(uint64_t)b * stride + i];
+    double re = (double)a.x, im = (double)a.y;
+    double pr = re * re + im * im;
+    double2 r;
+    if (branch < 0) {
+      r.x = pr;
+      r.y = 0.0;
+    } else if ((i >> branch) & 1ull) {
+      r.x = 0.0;
+      r.y = pr;
+    } else {
+      r.x = pr;
+      r.y = 0.0;
+    }
+    return r;
+  }
+};
+
+template <typename T>
+struct DotSrc {
+  const typename V2<T>::t* s;
+  const double2* other;
+  __device__ __forceinline__ double2 operator()(uint32_t, uint64_t v) const {
+    typename V2<T>::t a = s[v];
+    double2 o = other[v];
+    double ar = (double)a.x, ai = (double)a.y;
+    double2 r;
+    // conj(a) * o = (ar*or - (-ai)*oi, ar*oi + (-ai)*or)
+    r.x = ar * o.x - (-ai) * o.y;
+    r.y = ar * o.y + (-ai) * o.x;
+    return r;
+  }
+};
+
+struct F64Src {
+  const double2* p;
+  uint64_t stride;
+  __device__ __forceinline__ double2 operator()(uint32_t b, uint64_t v) const { return p[(uint64_t)b * stride + v]; }
+};
+
+__device__ __forceinline__ double2 add2(double2 a, double2 b) {
+  double2 r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+
+__device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
+  double2 r;
+  r.x = __shfl_xor(v.x, m, 64);
+  r.y = __shfl_xor(v.y, m, 64);
+  return r;
+}
+
+// Block = 2048 consecutive virtual elements; element j*256 + t is loaded by
+// thread t (coalesced). Lane butterflies give the pairwise sums of aligned
+// runs of up to 64 elements; the 32 run sums per block are combined in LDS.
+template <class Src>
+__global__ __launch_bounds__(TPB) void k_seg_reduce(Src src, uint64_t count, int seg, double2* __restrict__ out,
+                                                    uint64_t out_stride) {
+  __shared__ double2 runs[32];
+  const uint32_t b = blockIdx.y;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63, wave = t >> 6;
+  const uint64_t blk0 = (uint64_t)blockIdx.x * 2048;
+  double2 val[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    uint64_t v = blk0 + (uint64_t)j * 256 + t;
+    if (v < count) {
+      val[j] = src(b, v);
+    } else {
+      val[j].x = 0.0;
+      val[j].y = 0.0;
+    }
+  }
+  const int s1 = seg < 6 ? seg : 6;
+  for (int l = 0; l < s1; ++l) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) val[j] = add2(val[j], shfl_xor2(val[j], 1 << l));
+  }
+  double2* o = out + (uint64_t)b * out_stride;
+  if (seg <= 6) {
+    const uint64_t mask = (1ull << seg) - 1;
+    if ((lane & mask) == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        uint64_t v = blk0 + (uint64_t)j * 256 + t;
+        if (v < count) o[v >> seg] = val[j];
+      }
+    }
+    return;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) runs[4 * j + wave] = val[j];
+  }
+  __syncthreads();
+  const int width = 1 << (seg - 6);  // runs per output segment
+  const int nout = 32 / width;
+  if ((int)t < nout) {
+    double2* r = runs + t * width;
+    for (int w = 1; w < width; w *= 2)
+      for (int k2 = 0; k2 < width; k2 += 2 * w) r[k2] = add2(r[k2], r[k2 + w]);
+    uint64_t v = blk0 + ((uint64_t)t << seg);
+    if (v < count) o[v >> seg] = r[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Measurement (engine.rs:367-435)
+// ---------------------------------------------------------------------------
+__global__ void k_meas_decide(const double2* __restrict__ tot, uint64_t tot_stride, uint32_t batch,
+                              MeasState* __restrict__ ms, RngState* __restrict__ rng, uint8_t* __restrict__ cbits,
+                              uint64_t nc, int64_t cbit, Flag f, int* err) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  MeasState m = ms[b];
+  m.active = 0;
+  m.xflag = 0;
+  if (!flag_off(f, b)) {
+    double2 p = tot[(uint64_t)b * tot_stride];
+    double total = p.x + p.y;
+    if (!(isfinite(total) && total > 0.0)) {
+      atomicOr(err, 1);
+    } else {
+      RngState r = rng[b];
+      double u = uniform53(r.seed, r.stream, r.draw);
+      rng[b].draw = r.draw + 1;
+      int outcome = u < p.y / total ? 1 : 0;
+      double po = outcome ? p.y : p.x;
+      m.p0 = p.x;
+      m.p1 = p.y;
+      m.scale = 1.0 / sqrt(po);
+      m.outcome = outcome;
+      m.active = 1;
+      m.xflag = outcome;
+      if (cbit >= 0) cbits[(uint64_t)b * nc + (uint64_t)cbit] = (uint8_t)outcome;
+    }
+  }
+  ms[b] = m;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_collapse(typename V2<T>::t* __restrict__ s, uint64_t stride, uint64_t len,
+                                                  uint32_t q, const MeasState* __restrict__ ms) {
+  using V = typename V2<T>::t;
+  const uint32_t b = blockIdx.y;
+  const MeasState m = ms[b];
+  if (!m.active) return;
+  V* st = s + (uint64_t)b * stride;
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)j * TPB;
+    if (i >= len) continue;
+    V a = st[i];
+    if ((int)((i >> q) & 1ull) == m.outcome) {
+      a.x = (T)((double)a.x * m.scale);
+      a.y = (T)((double)a.y * m.scale);
+    } else {
+      a.x = T(0);
+      a.y = T(0);
+    }
+    st[i] = a;
+  }
+}
+
+// Conditional evaluation (execution.rs:286-295): the tested bits are read as
+// a little-endian integer and compared with the value; nested conditions AND.
+__global__ void k_cond_eval(const uint8_t* __restrict__ cbits, uint64_t nc, uint32_t batch,
+                            const Cond* __restrict__ conds, int ncond, int* __restrict__ flags) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  int ok = 1;
+  for (int c = 0; c < ncond; ++c) {
+    uint64_t actual = 0;
+    for (uint64_t bit = 0; bit < conds[c].size; ++bit)
+      actual |= (uint64_t)cbits[(uint64_t)b * nc + conds[c].base + bit] << bit;
+    ok = ok && (actual == conds[c].value);
+  }
+  flags[b] = ok;
+}
+
+__global__ void k_rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  rng[b].seed = seed;
+  rng[b].stream = stream0 + b;
+  rng[b].draw = 0;
+}
+
+// ---------------------------------------------------------------------------
+// CDF (grouped sequential prefix) and sampling
+// ---------------------------------------------------------------------------
+constexpr uint64_t kGroup = 1024;
+
+__global__ void k_cdf_local(const double* __restrict__ bins, uint64_t bstride, uint64_t count,
+                            double* __restrict__ C, double* __restrict__ gt) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t lo = g * kGroup;
+  if (lo >= count) return;
+  const uint64_t hi = lo + kGroup < count ? lo + kGroup : count;
+  double acc = 0.0;
+  for (uint64_t j = lo; j < hi; ++j) {
+    acc = acc + bins[j * bstride];
+    C[j] = acc;
+  }
+  gt[g] = acc;
+}
+
+__global__ void k_cdf_offsets(double* __restrict__ gt, uint64_t ngroups, double* __restrict__ total,
+                              int* __restrict__ err) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double off = 0.0, last = 0.0;
+  for (uint64_t g = 0; g < ngroups; ++g) {
+    double tg = gt[g];
+    gt[g] = off;
+    last = off + tg;
+    off = last;
+  }
+  *total = last;
+  if (!(isfinite(last) && last > 0.0)) atomicOr(err, 2);
+}
+
+__global__ __launch_bounds__(TPB) void k_cdf_add(double* __restrict__ C, uint64_t count,
+                                                 const double* __restrict__ gt) {
+  const uint64_t j = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (j >= count) return;
+  C[j] = gt[j / kGroup] + C[j];
+}
+
+__device__ __forceinline__ double cdf_u(const double* C, uint64_t j, uint64_t count, double total) {
+  return j + 1 == count ? 1.0 : C[j] / total;
+}
+
+// First j with U(j) > r (partition_point of p <= r), clamped to count-1.
+__device__ __forceinline__ uint64_t partition(const double* C, uint64_t count, double total, double r) {
+  uint64_t lo = 0, hi = count;
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    if (cdf_u(C, mid, count, total) <= r) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < count ? lo : count - 1;
+}
+
+__global__ __launch_bounds__(TPB) void k_sample_cdf(const double* __restrict__ C, uint64_t count,
+                                                    const double* __restrict__ total, uint64_t seed,
+                                                    uint64_t stream0, uint64_t shots, uint64_t* __restrict__ out) {
+  const uint64_t shot = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (shot >= shots) return;
+  double r = uniform53(seed, stream0 + shot, 0);
+  out[shot] = partition(C, count, *total, r);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_sample_blocked(ProbSrc<T> src, int binseg, const double* __restrict__ C,
+                                                        uint64_t nblocks, const double* __restrict__ total,
+                                                        uint64_t seed, uint64_t stream0, uint64_t shots,
+                                                        uint64_t* __restrict__ out) {
+  const uint64_t shot = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (shot >= shots) return;
+  const double tot = *total;
+  double r = uniform53(seed, stream0 + shot, 0);
+  uint64_t kb = partition(C, nblocks, tot, r);
+  double acc = kb == 0 ? 0.0 : C[kb - 1];
+  uint64_t found = ~0ull, last_pos = kb * 2048;
+  for (uint64_t j = 0; j < 2048; ++j) {
+    uint64_t bin = kb * 2048 + j;
+    // Pairwise sum of the bin's 2^binseg amplitudes (binary-counter stack).
+    double stack[12];
+    int level[12];
+    int sp = 0;
+    uint64_t nb = 1ull << binseg;
+    for (uint64_t e = 0; e < nb; ++e) {
+      double x = src(0, (bin << binseg) | e).x;
+      int lv = 0;
+      while (sp > 0 && level[sp - 1] == lv) {
+        x = stack[sp - 1] + x;
+        --sp;
+        ++lv;
+      }
+      stack[sp] = x;
+      level[sp] = lv;
+      ++sp;
+    }
+    double p = stack[0];
+    if (p > 0.0) last_pos = bin;
+    acc = acc + p;
+    if (acc / tot > r) {
+      found = bin;
+      break;
+    }
+  }
+  out[shot] = found != ~0ull ? found : last_pos;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_to_c128(const typename V2<T>::t* __restrict__ s, uint64_t count,
+                                                 double2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= count) return;
+  typename V2<T>::t a = s[i];
+  double2 r;
+  r.x = (double)a.x;
+  r.y = (double)a.y;
+  out[i] = r;
+}
+
+// Per-block max |a-b| and pairwise sum of |a-b|^2 between two states of
+// possibly different precision (accuracy reports).
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(TPB) void k_diff(const typename V2<Ta>::t* __restrict__ a,
+                                              const typename V2<Tb>::t* __restrict__ b, uint64_t count,
+                                              double2* __restrict__ out) {
+  __shared__ double mx[TPB], sq[TPB];
+  const uint64_t base = (uint64_t)blockIdx.x * (TPB * ITEMS);
+  double m = 0.0, acc[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    uint64_t i = base + (uint64_t)threadIdx.x * ITEMS + j;
+    acc[j] = 0.0;
+    if (i < count) {
+      double dr = (double)a[i].x - (double)b[i].x, di = (double)a[i].y - (double)b[i].y;
+      double d2 = dr * dr + di * di;
+      acc[j] = d2;
+      m = fmax(m, sqrt(d2));
+    }
+  }
+  double t = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  mx[threadIdx.x] = m;
+  sq[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w /= 2) {
+    if ((int)threadIdx.x < w) {
+      mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
+      sq[threadIdx.x] = sq[threadIdx.x] + sq[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = make_double2(mx[0], sq[0]);
+}
+
+template <typename T>
+M2<T> to_m2(const double* u) {
+  M2<T> m;
+  for (int i = 0; i < 8; ++i) m.v[i] = (T)u[i];
+  return m;
+}
+
+template <typename T>
+ProbSrc<T> make_prob(const void* s, uint64_t stride, int n, const ProbSpec& ps) {
+  ProbSrc<T> src;
+  src.s = (const typename V2<T>::t*)s;
+  src.stride = stride;
+  src.branch = ps.branch;
+  src.n = n;
+  src.identity = 1;
+  for (int j = 0; j < 64; ++j) src.map.q[j] = (uint8_t)j;
+  if (ps.map) {
+    for (int j = 0; j < n; ++j) {
+      src.map.q[j] = ps.map[j];
+      if (ps.map[j] != j) src.identity = 0;
+    }
+  }
+  return src;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+#define DQ_DISPATCH(prec, F) \
+  do {                       \
+    if ((prec) == 16) {      \
+      F(double);             \
+    } else {                 \
+      F(float);              \
+    }                        \
+  } while (0)
+
+void one(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t target,
+         const double* u, Flag f, hipStream_t st) {
+  const uint64_t count = (1ull << n) >> e.nbits;
+  dim3 grid(blocks_for(count, TPB * ITEMS), batch);
+#define L(T)                                                                                        \
+  hipLaunchKernelGGL(k_one<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, count, e, \
+                     1ull << target, to_m2<T>(u), f)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void xgate(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t target, Flag f,
+           hipStream_t st) {
+  const uint64_t count = (1ull << n) >> e.nbits;
+  dim3 grid(blocks_for(count, TPB * ITEMS), batch);
+#define L(T)                                                                                                \
+  hipLaunchKernelGGL(k_pairswap<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, count, e, 0ull, \
+                     1ull << target, f)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void swap(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t qa, uint32_t qb,
+          Flag f, hipStream_t st) {
+  const uint64_t count = (1ull << n) >> e.nbits;
+  dim3 grid(blocks_for(count, TPB * ITEMS), batch);
+#define L(T)                                                                                                  \
+  hipLaunchKernelGGL(k_pairswap<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, count, e, 1ull << qa, \
+                     1ull << qb, f)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void diag(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, int nentries,
+          const uint64_t* patterns, const double* values, Flag f, hipStream_t st) {
+  if (nentries == 0) return;
+  const uint64_t count = (1ull << n) >> e.nbits;
+  dim3 grid(blocks_for(count, TPB * ITEMS), batch);
+#define L(T)                                                                                                  \
+  {                                                                                                           \
+    DiagArgs<T> d;                                                                                            \
+    for (int i = 0; i < nentries; ++i) {                                                                      \
+      d.pat[i] = patterns[i];                                                                                 \
+      d.val[2 * i] = (T)values[2 * i];                                                                        \
+      d.val[2 * i + 1] = (T)values[2 * i + 1];                                                                \
+    }                                                                                                         \
+    hipLaunchKernelGGL(k_diag<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, count, e, nentries, d, f); \
+  }
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void dense(int prec, void* s, uint64_t stride, uint32_t batch, int n, int kq, const Expand& e, const void* mat_dev,
+           const uint64_t* offsets, Flag f, hipStream_t st) {
+  const uint64_t count = (1ull << n) >> kq;
+  dim3 grid(blocks_for(count, TPB), batch);
+  Offsets off;
+  for (int j = 0; j < 32; ++j) off.o[j] = j < (1 << kq) ? offsets[j] : 0;
+#define L(T, KK)                                                                                               \
+  hipLaunchKernelGGL((k_dense<T, KK>), grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, count, e,      \
+                     (const T*)mat_dev, off, f)
+#define LK(T)                  \
+  switch (kq) {                \
+    case 0: L(T, 0); break;    \
+    case 1: L(T, 1); break;    \
+    case 2: L(T, 2); break;    \
+    case 3: L(T, 3); break;    \
+    case 4: L(T, 4); break;    \
+    default: L(T, 5); break;   \
+  }
+  DQ_DISPATCH(prec, LK);
+#undef LK
+#undef L
+}
+
+void fused_block(void* s, uint64_t stride, uint32_t batch, int n, int kq, const Expand& e, const float* mat_dev,
+                 const uint64_t* offsets, hipStream_t st) {
+  const uint64_t count = (1ull << n) >> kq;
+  dim3 grid(blocks_for(count, TPB), batch);
+  Offsets off;
+  for (int j = 0; j < 32; ++j) off.o[j] = j < (1 << kq) ? offsets[j] : 0;
+#define L(KK) hipLaunchKernelGGL(k_fused<KK>, grid, dim3(TPB), 0, st, (float2*)s, stride, count, e, mat_dev, off)
+  switch (kq) {
+    case 0: L(0); break;
+    case 1: L(1); break;
+    case 2: L(2); break;
+    case 3: L(3); break;
+    case 4: L(4); break;
+    default: L(5); break;
+  }
+#undef L
+}
+
+bool fused_block_mfma(void*, uint64_t, uint32_t, int, int, const Expand&, const float*, const uint64_t*,
+                      hipStream_t) {
+  return false;
+}
+
+void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st) {
+  const uint64_t len = 1ull << n;
+  dim3 grid(blocks_for(len, TPB * ITEMS), batch);
+#define L(T) hipLaunchKernelGGL(k_init_zero<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, len)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void broadcast(int prec, void* d, uint64_t stride, uint32_t batch, const void* src, int n, hipStream_t st) {
+  const uint64_t len = 1ull << n;
+  dim3 grid(blocks_for(len, TPB * ITEMS), batch);
+#define L(T)                                                                                   \
+  hipLaunchKernelGGL(k_broadcast<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)d, stride, \
+                     (const typename V2<T>::t*)src, len)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void seg_reduce_prob(int prec, const void* s, uint64_t stride, uint32_t batch, int n, const ProbSpec& ps, int seg,
+                     void* out, uint64_t out_stride, hipStream_t st) {
+  const uint64_t count = 1ull << n;
+  dim3 grid(blocks_for(count, 2048), batch);
+#define L(T)                                                                                          \
+  hipLaunchKernelGGL(k_seg_reduce<ProbSrc<T>>, grid, dim3(TPB), 0, st, make_prob<T>(s, stride, n, ps), \
+                     count, seg, (double2*)out, out_stride)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void seg_reduce_dot(int prec, const void* s, int n, const void* other, int seg, void* out, hipStream_t st) {
+  const uint64_t count = 1ull << n;
+  dim3 grid(blocks_for(count, 2048), 1);
+#define L(T)                                                                                                    \
+  {                                                                                                             \
+    DotSrc<T> src;                                                                                              \
+    src.s = (const typename V2<T>::t*)s;                                                                        \
+    src.other = (const double2*)other;                                                                          \
+    hipLaunchKernelGGL(k_seg_reduce<DotSrc<T>>, grid, dim3(TPB), 0, st, src, count, seg, (double2*)out, 0ull); \
+  }
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void seg_reduce_f64(const void* in, uint64_t in_stride, uint64_t count, uint32_t batch, int seg, void* out,
+                    uint64_t out_stride, hipStream_t st) {
+  dim3 grid(blocks_for(count, 2048), batch);
+  F64Src src{(const double2*)in, in_stride};
+  hipLaunchKernelGGL(k_seg_reduce<F64Src>, grid, dim3(TPB), 0, st, src, count, seg, (double2*)out, out_stride);
+}
+
+void meas_decide(const void* totals, uint64_t tot_stride, uint32_t batch, MeasState* ms, RngState* rng,
+                 uint8_t* cbits, uint64_t nc, int64_t cbit, Flag f, int* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_meas_decide, dim3(blocks_for(batch, 64)), dim3(64), 0, st, (const double2*)totals, tot_stride,
+                     batch, ms, rng, cbits, nc, cbit, f, err);
+}
+
+void collapse(int prec, void* s, uint64_t stride, uint32_t batch, int n, uint32_t qubit, const MeasState* ms,
+              hipStream_t st) {
+  const uint64_t len = 1ull << n;
+  dim3 grid(blocks_for(len, TPB * ITEMS), batch);
+#define L(T) \
+  hipLaunchKernelGGL(k_collapse<T>, grid, dim3(TPB), 0, st, (typename V2<T>::t*)s, stride, len, qubit, ms)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+void cond_eval(const uint8_t* cbits, uint64_t nc, uint32_t batch, const Cond* conds, int ncond, int* flags,
+               hipStream_t st) {
+  hipLaunchKernelGGL(k_cond_eval, dim3(blocks_for(batch, 64)), dim3(64), 0, st, cbits, nc, batch, conds, ncond,
+                     flags);
+}
+
+void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hipStream_t st) {
+  hipLaunchKernelGGL(k_rng_init, dim3(blocks_for(batch, 64)), dim3(64), 0, st, rng, batch, seed, stream0);
+}
+
+void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* gt, double* total, int* err,
+               hipStream_t st) {
+  const uint64_t ngroups = (count + kGroup - 1) / kGroup;
+  hipLaunchKernelGGL(k_cdf_local, dim3(blocks_for(ngroups, 64)), dim3(64), 0, st, bins, bstride, count, C, gt);
+  hipLaunchKernelGGL(k_cdf_offsets, dim3(1), dim3(1), 0, st, gt, ngroups, total, err);
+  hipLaunchKernelGGL(k_cdf_add, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, C, count, (const double*)gt);
+}
+
+void sample_cdf(const double* C, uint64_t count, const double* total, uint64_t seed, uint64_t stream0,
+                uint64_t shots, uint64_t* out, hipStream_t st) {
+  if (shots == 0) return;
+  hipLaunchKernelGGL(k_sample_cdf, dim3(blocks_for(shots, TPB)), dim3(TPB), 0, st, C, count, total, seed, stream0,
+                     shots, out);
+}
+
+void sample_blocked(int prec, const void* s, int n, const uint8_t* map, int binseg, const double* C,
+                    uint64_t nblocks, const double* total, uint64_t seed, uint64_t stream0, uint64_t shots,
+                    uint64_t* out, hipStream_t st) {
+  if (shots == 0) return;
+  ProbSpec ps;
+  ps.map = map;
+#define L(T)                                                                                                   \
+  hipLaunchKernelGGL(k_sample_blocked<T>, dim3(blocks_for(shots, TPB)), dim3(TPB), 0, st,                    \
+                     make_prob<T>(s, 0, n, ps), binseg, C, nblocks, total, seed, stream0, shots, out)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+uint64_t diff_blocks(uint64_t count) { return blocks_for(count, TPB * ITEMS); }
+
+void diff(int prec_a, const void* a, int prec_b, const void* b, uint64_t count, void* out, hipStream_t st) {
+  dim3 grid(blocks_for(count, TPB * ITEMS));
+#define L2(TA, TB)                                                                                       \
+  hipLaunchKernelGGL((k_diff<TA, TB>), grid, dim3(TPB), 0, st, (const typename V2<TA>::t*)a,         \
+                     (const typename V2<TB>::t*)b, count, (double2*)out)
+  if (prec_a == 16 && prec_b == 16) L2(double, double);
+  else if (prec_a == 16) L2(double, float);
+  else if (prec_b == 16) L2(float, double);
+  else L2(float, float);
+#undef L2
+}
+
+void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st) {
+#define L(T)                                                                                      \
+  hipLaunchKernelGGL(k_to_c128<T>, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st,             \
+                     (const typename V2<T>::t*)s, count, (double2*)out)
+  DQ_DISPATCH(prec, L);
+#undef L
+}
+
+}  // namespace k
+}  // namespace dq

@@ -1,0 +1,134 @@
+// Host-side launch interface of the statevector kernels (kernels.hip).
+//
+// State layout in HBM: `batch` states back to back, each 2^n interleaved
+// complex amplitudes ({re, im} as float2 for complex64 or double2 for
+// complex128), little-endian qubit order (qubit 0 = index bit 0).
+// Every launcher takes the element type as `prec` (8 or 16 bytes/amplitude).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace dq {
+namespace k {
+
+constexpr int kMaxExpand = 7;
+
+// Bit-insertion description: a compact counter p is widened by inserting a
+// zero at each position in `pos` (ascending), then `fixed` is OR-ed in.
+struct Expand {
+  uint32_t nbits = 0;
+  uint32_t pos[kMaxExpand] = {};
+  uint64_t fixed = 0;
+};
+
+// Per-trajectory measurement record (one per batch slot).
+struct MeasState {
+  double p0, p1, scale;
+  int outcome;  // last outcome
+  int active;   // 1 if the last measure ran (condition held, finite norm)
+  int xflag;    // reset: apply X (active && outcome == 1)
+  int pad;
+};
+
+// Per-trajectory RNG record.
+struct RngState {
+  uint64_t seed, stream, draw;
+};
+
+struct Cond {
+  uint64_t base, size, value;
+};
+
+// Optional per-slot enable flag: the kernel for slot b runs only if
+// flag[b * stride] != 0. A null pointer means unconditional.
+struct Flag {
+  const int* ptr = nullptr;
+  uint32_t stride = 1;
+};
+
+// ---- gates (reference-semantics arithmetic, no FMA contraction) ----
+// u: 4 complex FP64 values (re, im interleaved), row-major.
+void one(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t target,
+         const double* u, Flag f, hipStream_t st);
+void xgate(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t target, Flag f,
+           hipStream_t st);
+void swap(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, uint32_t qa, uint32_t qb,
+          Flag f, hipStream_t st);
+// Sparse diagonal: entries with value != 1. qpos: the gate's qubits sorted
+// ascending (k of them); patterns: for each entry, the index bits to set.
+void diag(int prec, void* s, uint64_t stride, uint32_t batch, int n, const Expand& e, int nentries,
+          const uint64_t* patterns, const double* values, Flag f, hipStream_t st);
+// Dense k<=5 gather/matvec/scatter. mat_dev: device pointer to 4^k complex
+// values (FP64 for prec 16, FP32 for prec 8), row-major with qubits[0] MSB.
+// offsets_dev: device pointer to 2^k index offsets.
+void dense(int prec, void* s, uint64_t stride, uint32_t batch, int n, int k, const Expand& e, const void* mat_dev,
+           const uint64_t* offsets_dev, Flag f, hipStream_t st);
+// Fused complex64 block (FMA arithmetic, matrix FP32 on device).
+void fused_block(void* s, uint64_t stride, uint32_t batch, int n, int k, const Expand& e, const float* mat_dev,
+                 const uint64_t* offsets_dev, hipStream_t st);
+// Fused complex64 block on FP32 MFMA (k = 4 or 5). Returns false when the
+// layout is not supported by the MFMA kernel.
+bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, const Expand& e,
+                      const float* embed_dev, const uint64_t* offsets_dev, hipStream_t st);
+
+// ---- state set-up ----
+void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
+void broadcast(int prec, void* dst, uint64_t stride, uint32_t batch, const void* src, int n, hipStream_t st);
+
+// ---- deterministic FP64 reductions ----
+// Channel pair (x, y) per element. Source kinds:
+//   PROB:  x = |a|^2 if bit `branch` of the physical index is 0 (or branch<0),
+//          y = |a|^2 if that bit is 1.
+//   DOT:   (re, im) of conj(a) * other, other complex128 on device.
+//   F64:   a double2 array.
+// Virtual index v maps to physical index phys(v) = sum_j bit_j(v) << map[j]
+// (identity when map == nullptr).
+struct ProbSpec {
+  int branch = -1;
+  const uint8_t* map = nullptr;  // host pointer, n entries; copied into the launch
+};
+// Reduce `count` virtual elements in segments of 2^seg (seg <= 11) with the
+// pairwise tree; writes count >> seg double2 per batch slot at out + b*out_stride.
+void seg_reduce_prob(int prec, const void* s, uint64_t stride, uint32_t batch, int n, const ProbSpec& ps, int seg,
+                     void* out, uint64_t out_stride, hipStream_t st);
+void seg_reduce_dot(int prec, const void* s, int n, const void* other, int seg, void* out, hipStream_t st);
+void seg_reduce_f64(const void* in, uint64_t in_stride, uint64_t count, uint32_t batch, int seg, void* out,
+                    uint64_t out_stride, hipStream_t st);
+
+// ---- measurement ----
+void meas_decide(const void* totals, uint64_t tot_stride, uint32_t batch, MeasState* ms, RngState* rng,
+                 uint8_t* cbits, uint64_t nc, int64_t cbit, Flag f, int* err, hipStream_t st);
+void collapse(int prec, void* s, uint64_t stride, uint32_t batch, int n, uint32_t qubit, const MeasState* ms,
+              hipStream_t st);
+void cond_eval(const uint8_t* cbits, uint64_t nc, uint32_t batch, const Cond* conds_dev, int ncond, int* flags,
+               hipStream_t st);
+void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hipStream_t st);
+
+// ---- CDF and sampling ----
+// Grouped CDF: C[j] = O[g] + L[j] with L the sequential inclusive prefix in
+// groups of 1024 and O the sequential exclusive prefix of group totals.
+// Writes C (count entries) and total = C[count-1] into *total.
+void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* group_tmp, double* total,
+               int* err, hipStream_t st);
+// Materialised sampling: idx = first j with C[j]/total > r (last -> 1.0).
+void sample_cdf(const double* C, uint64_t count, const double* total, uint64_t seed, uint64_t stream0,
+                uint64_t shots, uint64_t* out, hipStream_t st);
+// Blocked sampling: C over blocks of 2^11 bins, bins recomputed per shot as
+// pairwise sums of 2^binseg amplitudes under `map`.
+void sample_blocked(int prec, const void* s, int n, const uint8_t* map, int binseg, const double* C,
+                    uint64_t nblocks, const double* total, uint64_t seed, uint64_t stream0, uint64_t shots,
+                    uint64_t* out, hipStream_t st);
+
+// ---- accuracy ----
+// Per-block (max |a-b|, sum |a-b|^2) for two states; diff_blocks(count)
+// double2 values are written.
+uint64_t diff_blocks(uint64_t count);
+void diff(int prec_a, const void* a, int prec_b, const void* b, uint64_t count, void* out, hipStream_t st);
+
+// ---- conversion ----
+void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st);
+
+}  // namespace k
+}  // namespace dq

@@ -1,0 +1,165 @@
+/*
+ * dqsim_sv.h - C ABI of the MI355X dense statevector engine (libdqsim_sv.so).
+ *
+ * This is the drop-in boundary for the reference's statevector backend. The
+ * reference exposes that backend only as the PyO3 module `dqsim._core`
+ * (src/lib.rs:16-32); every entry point below names the reference item it
+ * replaces, so a binding (PyO3 `extern "C"` block, ctypes, cffi) can map the
+ * Python API onto it one to one.
+ *
+ * Conventions
+ *  - Circuits are the JSON transport of src/types.rs:7-296, passed as UTF-8
+ *    text with an explicit length (what `model_dump_json()` returns).
+ *  - Amplitudes use little-endian wire order (qubit 0 = index bit 0). Query
+ *    qubit lists put qubits[0] at the most significant output bit.
+ *  - Every call returns DQ_OK or an error code; dq_last_error() then gives the
+ *    message for the calling thread. Error codes map to Python exceptions:
+ *      DQ_EVALUE  -> ValueError    DQ_EMEMORY -> MemoryError
+ *      DQ_ETYPE   -> TypeError     DQ_ERUNTIME / DQ_EDEVICE -> RuntimeError
+ *  - All calls are re-entrant: each call uses its own HIP stream, and a
+ *    result may be queried from several threads at once.
+ *  - Nothing in these signatures is a framework type: plain pointers and
+ *    sizes only.
+ */
+#ifndef DQSIM_SV_H
+#define DQSIM_SV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQ_OK 0
+#define DQ_EVALUE 1
+#define DQ_EMEMORY 2
+#define DQ_ETYPE 3
+#define DQ_ERUNTIME 4
+#define DQ_EDEVICE 5
+
+/* Amplitude precision (bytes per amplitude). */
+#define DQ_PREC_C128 16
+#define DQ_PREC_C64 8
+
+/* Gate fusion planner. */
+#define DQ_FUSE_NONE 0
+#define DQ_FUSE_GREEDY 1
+#define DQ_FUSE_LEARNED 2
+
+typedef struct dq_sv dq_sv;
+typedef struct dq_result dq_result;
+
+/* Constructor options of StatevectorSimulator
+ * (src/monolithic/statevector/simulator.rs:225-254, docs/simulators.md:31-39).
+ * precision / fusion / max_fused_qubits are additive: the defaults
+ * (complex128, no block fusion) give the reference's FP64 results. */
+typedef struct dq_sv_opts {
+  uint64_t seed;
+  int32_t has_seed;
+  int32_t profile;
+  uint64_t max_memory_mb;
+  uint64_t max_parallel_shots;
+  int32_t has_max_parallel_shots;
+  int32_t sample_terminal;
+  int32_t precision;        /* DQ_PREC_C128 (default) or DQ_PREC_C64 */
+  int32_t fusion;           /* DQ_FUSE_NONE (default), _GREEDY, _LEARNED */
+  int32_t max_fused_qubits; /* 1..5, default 4 */
+} dq_sv_opts;
+
+/* SimulationProfile fields (simulator.rs:43-59). */
+typedef struct dq_sim_profile {
+  uint64_t apply_one_qubit_calls;
+  double apply_one_qubit_time;
+  uint64_t apply_n_qubit_calls;
+  double apply_n_qubit_time;
+  uint64_t measure_qubit_calls;
+  double measure_qubit_time;
+  double total_time;
+} dq_sim_profile;
+
+/* One (bitstring, count) entry; return nonzero to abort the iteration. */
+typedef int (*dq_count_cb)(void* user, const char* bits, size_t len, uint64_t count);
+
+/* Per-launch device timing (engine-recorded). */
+typedef struct dq_op_timing {
+  int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA) */
+  int32_t k;     /* qubits touched */
+  int32_t low;   /* lowest qubit touched */
+  int32_t gates; /* compiled ops folded into the launch */
+  double ms;
+  double bytes;  /* algorithmic HBM bytes */
+} dq_op_timing;
+
+/* Fill `o` with the reference defaults (max_memory_mb = 1024,
+ * sample_terminal = 1, no seed, complex128, no fusion). */
+void dq_sv_opts_default(dq_sv_opts* o);
+
+/* StatevectorSimulator::new (simulator.rs:225-254). DQ_EVALUE when
+ * max_memory_mb is 0 or overflows bytes, or max_parallel_shots is 0. */
+int dq_sv_create(const dq_sv_opts* o, dq_sv** out);
+void dq_sv_free(dq_sv* sim);
+
+/* StatevectorSimulator::simulate_shots (simulator.rs:256-317, shots.rs:58-260).
+ * Calls `cb` once per distinct classical bitstring (highest cbit first). */
+int dq_sv_simulate_shots(dq_sv* sim, const char* json, size_t len, uint64_t shots, int32_t profile, dq_count_cb cb,
+                         void* user);
+
+/* StatevectorSimulator::simulate (simulator.rs:319-371). */
+int dq_sv_simulate(dq_sv* sim, const char* json, size_t len, dq_result** out);
+
+/* SimulationResult (simulator.rs:85-191). */
+uint32_t dq_result_num_qubits(const dq_result* r);
+uint64_t dq_result_num_cbits(const dq_result* r);
+/* classical_bits getter: out[i] = bit i, len = num_cbits. */
+int dq_result_classical_bits(const dq_result* r, uint8_t* out, uint64_t len);
+/* profile getter: returns 1 and fills `out` when profiling was on, else 0. */
+int dq_result_profile(const dq_result* r, dq_sim_profile* out);
+/* statevector getter: independent complex128 copy, len = 2^n elements. */
+int dq_result_statevector_c128(const dq_result* r, void* out, uint64_t len);
+/* probabilities(qubits): out has 2^k FP64 entries (all, including zeros). */
+int dq_result_probabilities(const dq_result* r, const uint64_t* qubits, uint64_t k, double* out);
+/* counts(shots, qubits, seed). has_seed = 0 draws a fresh seed. */
+int dq_result_counts(const dq_result* r, uint64_t shots, const uint64_t* qubits, uint64_t k, int32_t has_seed,
+                     uint64_t seed, dq_count_cb cb, void* user);
+/* fidelity(other): |<self|other>|^2, 0.0 on a length mismatch. */
+int dq_result_fidelity(const dq_result* r, const void* other_c128, uint64_t len, double* out);
+/* Bounded single-amplitude query (the reference's MpsResult.amplitude,
+ * docs/simulators.md:122-123): re, im of amplitude `index`. */
+int dq_result_amplitude(const dq_result* r, uint64_t index, double* re_im);
+void dq_result_free(dq_result* r);
+
+/* Accuracy report (no reference counterpart): max |a-b| and the L2 norm of
+ * a-b over two results with the same qubit count (any precisions). */
+int dq_result_compare(const dq_result* a, const dq_result* b, double* max_abs, double* l2);
+
+/* Message of the calling thread's last error ("" if none). */
+const char* dq_last_error(void);
+
+/* ---- no reference counterpart: host-only inspection and measurement ---- */
+
+/* Parse, validate, compile and fuse without touching the GPU; writes a JSON
+ * description of the launch plan. Free it with dq_string_free. */
+int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
+                 char** out);
+void dq_string_free(char* s);
+
+/* Number of visible GPUs (0 on a host without one). */
+int dq_device_count(void);
+
+/* Evolve the circuit's unitary prefix from |0...0> `repeats` times on the
+ * simulator's settings; returns per-launch timings of the last repeat
+ * (up to `cap` records), the launch count and the total device time. With
+ * `keep` non-null the final state is returned as a result. */
+int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats, dq_op_timing* out, uint64_t cap,
+                       uint64_t* nops, uint64_t* gates, double* total_ms, dq_result** keep);
+
+/* Weights of the learned fusion planner (FP32, layout documented in
+ * the planner module); count = 0 clears them. */
+int dq_set_planner_weights(const float* w, uint64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DQSIM_SV_H */
