@@ -1,0 +1,255 @@
+"""Host-side checks of libdqsim_sv.so that need no GPU: the C ABI loads and
+exports every declared symbol, and parsing, validation, budget checks,
+option handling and compile/fusion planning behave like the reference
+(src/validation.rs, src/monolithic/statevector/{validation,shots,simulator}.rs,
+src/codecs.rs, tests/test_statevector.py:215-298)."""
+
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import dqsim
+from dqsim import _core, workloads
+from dqsim.workloads import Circuit
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def inst(kind, **f):
+    return {"kind": kind, **f}
+
+
+def cond(op, value, base=0, size=1):
+    return {"kind": "conditional", "condition": {"creg_base": base, "creg_size": size, "creg_value": value}, "op": op}
+
+
+class Raw:
+    """Circuit with an arbitrary transport document."""
+
+    def __init__(self, data):
+        self.data = data
+
+    def model_dump_json(self):
+        return json.dumps(self.data) if not isinstance(self.data, str) else self.data
+
+
+def reg(name, base, size):
+    return {"name": name, "base": base, "size": size}
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "dqsim_sv.h")).read()
+    names = set(re.findall(r"\b(dq_[a-z0-9_]+)\s*\(", header))
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_core.library_path())
+    for name in sorted(names):
+        assert hasattr(lib, name), name
+
+
+def test_package_exports_reference_names():
+    for name in ("StatevectorSimulator", "SimulationResult", "SimulationProfile", "simulate_monolithic",
+                 "simulate_monolithic_shots", "MpsSimulator", "MpsResult", "PBlockSimulator", "PBlockResult",
+                 "StabilizerSimulator", "StabilizerResult", "simulate_distributed", "simulate_distributed_shots",
+                 "ImportedCircuit", "QASMImportError", "from_qiskit", "load_qasm"):
+        assert hasattr(dqsim, name), name
+    with pytest.raises(NotImplementedError):
+        dqsim.MpsSimulator(seed=1)
+
+
+INVALID = [
+    Circuit(12, [inst("cx", control=0, target=12)]),
+    Circuit(12, [inst("x", qubit=12)]),
+    Circuit(2, [inst("cx", control=1, target=1)]),
+    Circuit(3, [inst("ccx", control1=0, control2=0, target=2)]),
+    Circuit(2, [inst("measure", qubit=0, cbit=2)], cbits=2),
+    Circuit(1, [cond(inst("x", qubit=1), 1)], cbits=1),
+    Circuit(1, [cond(inst("x", qubit=0), 0, base=1)], cbits=1),
+    Circuit(2, [inst("gate", name="remote_cx", params=[], qubits=[0])]),
+    Circuit(2, [inst("gate", name="remote_cu1", params=[], qubits=[0, 1])]),
+]
+
+
+@pytest.mark.parametrize("idx", range(len(INVALID)))
+def test_invalid_qubits_and_operands_raise_value_error(idx):
+    sim = dqsim.StatevectorSimulator(seed=1)
+    for method in (sim.simulate, sim.simulate_shots):
+        with pytest.raises(ValueError):
+            method(INVALID[idx])
+
+
+@pytest.mark.parametrize("op", [inst("gate", name="circuit-123", params=[], qubits=[0]),
+                                inst("gate", name="unknown", params=[], qubits=[0]),
+                                inst("classical", name="unsupported_classical_operation")])
+def test_unsupported_operations_are_rejected_even_when_unreachable(op):
+    sim = dqsim.StatevectorSimulator(seed=1)
+    for instruction in (op, cond(op, 1)):
+        for method in (sim.simulate, sim.simulate_shots):
+            with pytest.raises(ValueError):
+                method(Circuit(1, [instruction], cbits=1))
+
+
+def _doc(instructions, qregs=None, cregs=None):
+    return {"qregs": qregs or {"q": reg("q", 0, 3)}, "cregs": cregs or {"c": reg("c", 0, 3)},
+            "instructions": instructions}
+
+
+@pytest.mark.parametrize("doc", [
+    _doc([inst("cx", control=0, target=3)]),
+    _doc([inst("id", qubit=3)]),
+    _doc([inst("measure", qubit=0, cbit=3)]),
+    _doc([cond(inst("x", qubit=99), 1)]),
+    _doc([inst("x", qubit=1)], qregs={"a": reg("a", 0, 1), "b": reg("b", 2, 1)}),  # hole between registers
+    _doc([], qregs={"a": reg("a", 0, 2), "b": reg("b", 1, 2)}),  # overlap
+    _doc([], qregs={"a": reg("a", 0, 64)}),  # n >= 64
+    _doc([], qregs={"a": reg("a", 2**64 - 1, 1)}),  # overflowing range
+    _doc([], cregs={"c": reg("c", 0, 3), "o": reg("o", 1, 1)}),
+    _doc([inst("u3", qubit=0, theta=0.0, phi=float("nan"), lam=0.0)]),
+    _doc([inst("u2", qubit=0, phi=1.7976931348623157e308, lam=1.7976931348623157e308)]),
+    _doc([inst("gate", name="remote_rzz", qubits=[0, 1], params=[float("-inf")])]),
+    _doc([inst("gate", name="remote_epr", qubits=[0, 1, 2], params=[])]),
+    _doc([inst("gate", name="remote_cz", qubits=[0, 1], params=[0.2])]),
+    _doc([inst("gate", name="remote_barrier", qubits=[1, 1], params=[])]),
+    _doc([cond(inst("x", qubit=0), 0, 0, 0)]),
+    _doc([cond(inst("x", qubit=0), 0, 0, 65)]),
+    _doc([cond(inst("x", qubit=0), 2, 0, 1)]),
+    _doc([cond(inst("x", qubit=0), 0, 2, 2)]),
+    _doc([cond(inst("x", qubit=0), 0, 2**64 - 1, 2)]),
+], ids=lambda d: json.dumps(d["instructions"])[:60])
+def test_validation_rejections(doc):
+    # src/monolithic/statevector/validation.rs:56-199 (NaN is sent as a JSON
+    # number the reference cannot parse either).
+    text = json.dumps(doc).replace("NaN", "1e999").replace("-Infinity", "-1e999")
+    with pytest.raises(ValueError):
+        _core.plan_json(text)
+
+
+@pytest.mark.parametrize("doc", [
+    _doc([inst("gate", name="remote_cu1", qubits=[0, 1], params=[0.4]),
+          inst("gate", name="remote_rzz", qubits=[0, 1], params=[0.4]),
+          inst("gate", name="REMOTE_CX", qubits=[0, 2], params=[]),
+          inst("gate", name="remote_barrier", qubits=[0, 1, 2], params=[])]),
+    _doc([inst("h", qubit=0), inst("cx", control=0, target=2), inst("measure", qubit=2, cbit=1),
+          cond(cond(inst("reset", qubit=0), 1, 1, 1), 2, 0, 3)]),
+    _doc([cond(inst("x", qubit=0), 2**64 - 1, 0, 64)], cregs={"c": reg("c", 0, 64)}),
+])
+def test_validation_acceptances(doc):
+    _core.plan_json(json.dumps(doc))
+
+
+@pytest.mark.parametrize("text", ["{", "[]", '{"qregs": {}}', '{"qregs": {}, "instructions": [{"kind": "nope"}]}',
+                                  '{"qregs": {"q": {"name": "q", "base": 0, "size": 1.0}}, "instructions": []}',
+                                  '{"qregs": {"q": {"name": "q", "base": 0, "size": -1}}, "instructions": []}',
+                                  '{"qregs": {}, "instructions": [{"kind": "x"}]}'])
+def test_malformed_json_is_a_value_error(text):
+    with pytest.raises(ValueError, match="parse error"):
+        _core.plan_json(text)
+
+
+def test_memory_budget_rejects_before_allocation():
+    sim = dqsim.StatevectorSimulator(max_memory_mb=1)
+    for method in (sim.simulate, sim.simulate_shots):
+        with pytest.raises(MemoryError):
+            method(Circuit(20))
+        with pytest.raises(MemoryError):
+            method(Circuit(1, cbits=2_000_000))
+    with pytest.raises(MemoryError):
+        dqsim.StatevectorSimulator(max_memory_mb=16, precision="complex64").simulate(Circuit(22))
+
+
+def test_zero_shots_need_no_device():
+    sim = dqsim.StatevectorSimulator(seed=30)
+    assert sim.simulate_shots(Circuit(1, [inst("h", qubit=0)], cbits=3), shots=0) == {}
+
+
+def test_constructor_and_option_errors():
+    for opts in ({"max_memory_mb": 0}, {"max_parallel_shots": 0}, {"max_fused_qubits": 6},
+                 {"precision": "complex32"}, {"fusion": "magic"}):
+        with pytest.raises(ValueError):
+            dqsim.StatevectorSimulator(**opts)
+    with pytest.raises(ValueError):
+        dqsim.StatevectorSimulator(max_memory_mb=2**63)
+    with pytest.raises(TypeError):
+        dqsim.simulate_monolithic(Circuit(1), nonexistent_option=True)
+    with pytest.raises(TypeError):
+        dqsim.StatevectorSimulator(seed=1.5)
+    with pytest.raises(OverflowError):
+        dqsim.StatevectorSimulator(seed=-1)
+    with pytest.raises(TypeError):
+        dqsim.StatevectorSimulator(profile=1)
+    with pytest.raises(TypeError):
+        dqsim.StatevectorSimulator(1, False, 1024)  # max_memory_mb is keyword-only
+    with pytest.raises(ValueError):
+        dqsim.simulate_monolithic(Circuit(1), mode="tensor")
+    for mode in ("state_vector", "StateVector", " sv ", "state-vector"):
+        dqsim._core._parse_mode(mode)
+
+
+def _product_kinds(plan):
+    out = []
+    for l in plan["launches"]:
+        out.append("conditional" if l["conditional"] else l["kind"])
+    return out
+
+
+def test_compile_matches_oracle_on_random_circuits():
+    rng = np.random.default_rng(7)
+    kinds = ["h", "x", "rz", "cx", "cz", "cp", "swap", "ccx", "rxx", "rzz", "cswap", "rccx", "u3", "id", "measure",
+             "reset", "barrier", "c3sqrtx"]
+    for _ in range(40):
+        n = 5
+        insts = []
+        for _ in range(30):
+            k = kinds[int(rng.integers(len(kinds)))]
+            q = rng.choice(n, size=4, replace=False).tolist()
+            if k in ("h", "x", "id", "reset"):
+                i = inst(k, qubit=q[0])
+            elif k == "rz":
+                i = inst(k, qubit=q[0], phi=0.3)
+            elif k == "u3":
+                i = inst(k, qubit=q[0], theta=0.3, phi=0.2, lam=0.1)
+            elif k in ("cx", "cz"):
+                i = inst(k, control=q[0], target=q[1])
+            elif k == "cp":
+                i = inst(k, control=q[0], target=q[1], lam=0.7)
+            elif k == "swap":
+                i = inst(k, a=q[0], b=q[1])
+            elif k in ("rxx", "rzz"):
+                i = inst(k, a=q[0], b=q[1], theta=0.4)
+            elif k in ("ccx", "rccx"):
+                i = inst(k, control1=q[0], control2=q[1], target=q[2])
+            elif k == "c3sqrtx":
+                i = inst(k, control1=q[0], control2=q[1], control3=q[2], target=q[3])
+            elif k == "cswap":
+                i = inst(k, control=q[0], target1=q[1], target2=q[2])
+            elif k == "measure":
+                i = inst(k, qubit=q[0], cbit=q[1])
+            else:
+                i = inst(k)
+            if rng.random() < 0.15 and k != "barrier":
+                i = cond(i, int(rng.integers(2)), int(rng.integers(n)))
+            insts.append(i)
+        c = Circuit(n, insts, cbits=n)
+        assert _product_kinds(_core.plan_json(c)) == oracle.compile_kinds(insts)
+
+
+def test_fusion_plan_conserves_gates_and_respects_width():
+    circ = workloads.random_su2(n=20, depth=8, seed=3)
+    plain = _core.plan_json(circ)
+    for k in (2, 3, 4, 5):
+        fused = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k)
+        assert sum(l["gates"] for l in fused["launches"]) == plain["compiled_ops"]
+        assert all(l["k"] <= k for l in fused["launches"] if l["gates"] > 1)
+        assert len(fused["launches"]) < len(plain["launches"])
+        assert all(l["mfma"] == (l["kind"] == "fused" and l["k"] >= 4) for l in fused["launches"])
+
+
+def test_workload_gate_counts():
+    assert workloads.gate_count(workloads.brickwork().data["instructions"]) == 590
+    assert workloads.gate_count(workloads.qft(30, measure=False)[0].data["instructions"]) == 510
+    assert workloads.gate_count(workloads.qft(33, measure=False)[0].data["instructions"]) == 610
+    assert workloads.gate_count(workloads.random_su2(32, 40).data["instructions"]) == 1920
