@@ -226,6 +226,82 @@ __global__ __launch_bounds__(TPB) void k_fused(float2* __restrict__ s, uint64_t 
   }
 }
 
+// Fused complex64 block on FP32 MFMA (k = 4 or 5).
+//
+// A block of 2^k amplitudes (one "group", the amplitudes that differ only on
+// the block's qubits) is the vector v; the block matrix U is applied as the
+// real product [Re w; Im w] = E [Re v; Im v] with E = [[Ur, -Ui], [Ui, Ur]]
+// (R = 2^(k+1) rows). A wave takes 32 groups as the 32 columns of B and runs
+// v_mfma_f32_32x32x2f32 over K = R in steps of 2:
+//   A fragment (step s, row tile t): lane l holds E[32t + l%32][2s + l/32]
+//     -- the whole of E stays in registers for the kernel's lifetime;
+//   B fragment: lane l holds component 2s + l/32 of group l%32. Components
+//     < 2^k are real parts of input 2s + l/32, the rest imaginary parts, so
+//     lane l only ever needs the inputs of parity l/32 of its group: it loads
+//     those 2^(k-1) amplitudes once and feeds B from registers.
+//   D: lane l, register r holds row 8(r/4) + 4(l/32) + r%4 of column l%32.
+//     For k = 4 rows < 16 are real parts and rows >= 16 the matching
+//     imaginary parts (registers r and r+8); for k = 5 tile 0 holds real
+//     and tile 1 imaginary parts of the same 16 outputs. Each lane therefore
+//     stores whole complex outputs of its own group.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int K>
+__global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint64_t stride, uint64_t ngroups,
+                                                    Expand e, const float* __restrict__ emb, Offsets off) {
+  constexpr int D = 1 << K;
+  constexpr int R = 2 * D;
+  constexpr int T = R / 32;
+  constexpr int STEPS = R / 2;
+  constexpr int NIN = D / 2;
+  float2* st = s + (uint64_t)blockIdx.y * stride;
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 31, h = lane >> 5;
+  float a[T][STEPS];
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int q = 0; q < STEPS; ++q) a[t][q] = emb[(32 * t + j) * R + 2 * q + h];
+  uint64_t offs[NIN];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) offs[i] = off.o[2 * i + h];
+  const uint64_t waves = (uint64_t)gridDim.x * (TPB / 64);
+  const uint64_t ntiles = (ngroups + 31) / 32;
+  for (uint64_t tile = (uint64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6); tile < ntiles; tile += waves) {
+    const uint64_t g = tile * 32 + j;
+    const bool valid = g < ngroups;
+    const uint64_t base = expand(valid ? g : 0, e);
+    float2 v[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) v[i] = valid ? st[base | offs[i]] : make_float2(0.f, 0.f);
+    floatx16 acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+#pragma unroll
+    for (int q = 0; q < STEPS; ++q) {
+      const float b = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][q], b, acc[t], 0, 0, 0);
+    }
+    if (!valid) continue;
+    if constexpr (K == 4) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int o = 8 * (r / 4) + 4 * h + (r % 4);
+        st[base | off.o[o]] = make_float2(acc[0][r], acc[0][r + 8]);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int o = 8 * (r / 4) + 4 * h + (r % 4);
+        st[base | off.o[o]] = make_float2(acc[0][r], acc[1][r]);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // State set-up
 // ---------------------------------------------------------------------------
@@ -759,9 +835,23 @@ void fused_block(void* s, uint64_t stride, uint32_t batch, int n, int kq, const 
 #undef L
 }
 
-bool fused_block_mfma(void*, uint64_t, uint32_t, int, int, const Expand&, const float*, const uint64_t*,
-                      hipStream_t) {
-  return false;
+bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, const Expand& e, const float* emb,
+                      const uint64_t* offsets, hipStream_t st) {
+  if (kq != 4 && kq != 5) return false;
+  const uint64_t ngroups = (1ull << n) >> kq;
+  const uint64_t ntiles = (ngroups + 31) / 32;
+  const uint64_t waves_wanted = ntiles;
+  uint64_t blocks = (waves_wanted + 3) / 4;
+  const uint64_t cap = 8192;
+  if (blocks > cap) blocks = cap;
+  dim3 grid((uint32_t)(blocks == 0 ? 1 : blocks), batch);
+  Offsets off;
+  for (int j = 0; j < 32; ++j) off.o[j] = j < (1 << kq) ? offsets[j] : 0;
+  if (kq == 4)
+    hipLaunchKernelGGL(k_fused_mfma<4>, grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
+  else
+    hipLaunchKernelGGL(k_fused_mfma<5>, grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
+  return true;
 }
 
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st) {
