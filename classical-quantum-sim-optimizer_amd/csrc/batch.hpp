@@ -1,0 +1,51 @@
+// Device structures and launcher of the persistent batched-circuit kernel.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dq {
+namespace bk {
+
+struct BatchCircuit {
+  uint32_t n, nops;
+  uint64_t op_off;  // first op in the op array
+  uint64_t seed;
+  uint32_t shots, pad;
+  uint64_t out_off;  // first sample slot in the output array
+};
+
+struct BatchOp {
+  uint32_t k;        // 1..4 qubits
+  uint32_t mat_off;  // float offset of the 4^k complex FP32 matrix
+  uint32_t q[4];     // qubits, q[0] = MSB of the local index
+  uint32_t s[4];     // the same qubits sorted ascending
+};
+
+struct LaunchArgs {
+  const BatchCircuit* circ;
+  uint32_t ncirc;
+  const uint32_t* order;
+  const BatchOp* ops;
+  const float* mats;
+  float2* scratch;
+  uint64_t slot_amps;
+  int lmax;
+  int* queue;
+  uint32_t* out;
+  uint64_t* times;
+  float2* dump;
+  const uint64_t* dump_off;
+  uint32_t grid;
+  int block;  // 256, 512 or 1024
+};
+
+size_t lds_bytes(int lmax, int block);
+void set_lds_limit(int block, size_t bytes);
+// Queue reset (a fill kernel, graph-capturable) + the persistent kernel.
+void launch(const LaunchArgs& a, hipStream_t st);
+
+}  // namespace bk
+}  // namespace dq

@@ -6,12 +6,16 @@
 
 #include "../../include/dqsim_sv.h"
 #include "engine.hpp"
+#include "planner.hpp"
 
 struct dq_sv {
   dq::Options o;
 };
 struct dq_result {
   std::unique_ptr<dq::Result> r;
+};
+struct dq_batch {
+  std::unique_ptr<dq::BatchOutcome> b;
 };
 
 namespace {
@@ -217,6 +221,7 @@ int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats
       out[i].gates = s.ops[i].gates;
       out[i].ms = s.ops[i].ms;
       out[i].bytes = s.ops[i].bytes;
+      std::memcpy(out[i].feat, s.ops[i].feat, sizeof out[i].feat);
     }
     if (nops) *nops = s.launches;
     if (gates) *gates = s.gates;
@@ -229,8 +234,91 @@ int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats
   });
 }
 
+int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
+                 const uint64_t* seeds, const dq_batch_opts* o, dq_batch** out) {
+  return guard([&] {
+    std::vector<std::pair<const char*, size_t>> cs;
+    for (uint64_t i = 0; i < ncirc; ++i) cs.push_back({jsons[i], lens[i]});
+    dq::BatchOptions opt;
+    if (o) {
+      opt.block = o->block;
+      opt.max_fused = o->max_fused_qubits;
+      opt.autotune = o->autotune != 0;
+      opt.replays = o->replays;
+      opt.dump_states = o->dump_states != 0;
+    }
+    auto* b = new dq_batch;
+    try {
+      b->b = dq::batch_run(cs, std::vector<uint64_t>(shots, shots + ncirc), std::vector<uint64_t>(seeds, seeds + ncirc),
+                           opt);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int dq_batch_counts(const dq_batch* b, uint64_t i, dq_count_cb cb, void* user) {
+  return guard([&] {
+    if (i >= b->b->nq.size()) throw dq::ValueErr("circuit index out of range");
+    emit(dq::batch_counts(*b->b, i), cb, user);
+  });
+}
+
+int dq_batch_indices(const dq_batch* b, uint64_t i, uint32_t* out, uint64_t len) {
+  return guard([&] {
+    const auto& x = *b->b;
+    if (i >= x.nq.size()) throw dq::ValueErr("circuit index out of range");
+    const uint64_t n = (i + 1 < x.offsets.size() ? x.offsets[i + 1] : x.idx.size()) - x.offsets[i];
+    if (len != n) throw dq::ValueErr("length mismatch");
+    std::memcpy(out, x.idx.data() + x.offsets[i], 4 * n);
+  });
+}
+
+int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* block, int32_t* max_fused, int32_t* lds_qubits,
+                  uint32_t* grid, double* latency_us, uint64_t ncirc) {
+  return guard([&] {
+    const auto& x = *b->b;
+    if (total_ms) *total_ms = x.total_ms;
+    if (block) *block = x.block;
+    if (max_fused) *max_fused = x.fused;
+    if (lds_qubits) *lds_qubits = x.lmax;
+    if (grid) *grid = x.grid;
+    if (latency_us)
+      for (uint64_t i = 0; i < ncirc && i < x.latency_us.size(); ++i) latency_us[i] = x.latency_us[i];
+  });
+}
+
+int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len) {
+  return guard([&] {
+    const auto& x = *b->b;
+    if (x.state_off.empty()) throw dq::ValueErr("states were not kept (dump_states = 0)");
+    if (i >= x.nq.size() || len != (1ull << x.nq[i])) throw dq::ValueErr("index or length mismatch");
+    double* o = (double*)out;
+    for (uint64_t k = 0; k < 2 * len; ++k) o[k] = x.states[2 * x.state_off[i] + k];
+  });
+}
+
+void dq_batch_free(dq_batch* b) { delete b; }
+
+uint64_t dq_mlp_param_count(void) { return (uint64_t)dq::mlp::kMlpParams; }
+
 int dq_set_planner_weights(const float* w, uint64_t count) {
   return guard([&] { dq::set_planner_weights(w, (size_t)count); });
+}
+
+int dq_mlp_train(float* w, const float* X, const float* y, uint64_t n, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float* loss_curve) {
+  return guard([&] { dq::mlp_train(w, X, y, n, epochs, batch, lr, seed, loss_curve); });
+}
+
+int dq_mlp_grad(const float* w, const float* X, const float* y, int32_t n, float* grad, float* loss) {
+  return guard([&] { dq::mlp_grad(w, X, y, n, grad, loss); });
+}
+
+int dq_mlp_predict(const float* w, const float* X, uint64_t n, float* out) {
+  return guard([&] { dq::mlp_predict(w, X, n, out); });
 }
 
 }  // extern "C"

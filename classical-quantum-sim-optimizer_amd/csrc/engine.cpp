@@ -26,8 +26,10 @@
 #include <sstream>
 #include <unordered_map>
 
+#include "batch.hpp"
 #include "fuse.hpp"
 #include "kernels.hpp"
+#include "planner.hpp"
 #include "philox.hpp"
 
 namespace dq {
@@ -163,7 +165,64 @@ struct DevOp {
   int gates = 1;
   int low = 0;
   int nctl = 0;
+  float feat[mlp::kFeatures] = {};
 };
+
+// Planner features of a block of compiled ops (also used for single ops):
+//  0 k/5             1 lowest qubit/34     2 highest qubit/34   3 qubits < 4, /4
+//  4 ops folded/32   5 diagonal-op share   6 Clifford-op share  7 all ops diagonal
+//  8 n/34            9 k >= 4 (MFMA)      10 span/34           11 touched share of amplitudes
+bool clifford_entry(const cplx& e) {
+  const double r = std::fabs(e.re), i = std::fabs(e.im), t = 1e-12, h = 0.70710678118654752440;
+  auto is = [&](double a, double b) { return std::fabs(r - a) < t && std::fabs(i - b) < t; };
+  return is(0, 0) || is(1, 0) || is(0, 1) || is(h, 0) || is(0, h) || is(0.5, 0.5);
+}
+void block_features(const std::vector<uint32_t>& qubits, const std::vector<int>& opidx, const std::vector<Op>& ops,
+                    int n, float* f) {
+  const int k = (int)qubits.size();
+  uint32_t lo = 63, hi = 0;
+  int low4 = 0;
+  for (uint32_t q : qubits) {
+    lo = std::min(lo, q);
+    hi = std::max(hi, q);
+    low4 += q < 4;
+  }
+  int diag = 0, cliff = 0;
+  for (int oi : opidx) {
+    const Op& o = ops[oi];
+    diag += o.kind == OpKind::Diagonal;
+    bool c = o.kind == OpKind::X || o.kind == OpKind::Swap;
+    if (!c && !o.mat.empty()) {
+      c = true;
+      for (const cplx& e : o.mat) c = c && clifford_entry(e);
+    }
+    cliff += c;
+  }
+  const float no = (float)std::max<size_t>(1, opidx.size());
+  double touched = 1.0;
+  if (opidx.size() == 1) {
+    const Op& o = ops[opidx[0]];
+    if (o.kind == OpKind::One || o.kind == OpKind::X) touched = std::ldexp(1.0, -(int)o.controls.size());
+    else if (o.kind == OpKind::Swap) touched = std::ldexp(1.0, -1 - (int)o.controls.size());
+    else if (o.kind == OpKind::Diagonal) {
+      int ne = 0;
+      for (const cplx& e : o.mat) ne += !(e.re == 1.0 && e.im == 0.0);
+      touched = (double)ne / (double)o.mat.size();
+    }
+  }
+  f[0] = k / 5.f;
+  f[1] = lo / 34.f;
+  f[2] = hi / 34.f;
+  f[3] = low4 / 4.f;
+  f[4] = opidx.size() / 32.f;
+  f[5] = diag / no;
+  f[6] = cliff / no;
+  f[7] = diag == (int)opidx.size() ? 1.f : 0.f;
+  f[8] = n / 34.f;
+  f[9] = k >= 4 ? 1.f : 0.f;
+  f[10] = (hi - lo) / 34.f;
+  f[11] = (float)touched;
+}
 
 struct Plan {
   std::vector<DevOp> ops;
@@ -354,6 +413,8 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
   while (i < end) {
     if (!ops[i].is_unitary() || fusion == FUSE_NONE) {
       plan.ops.push_back(lower(ops[i], plan, prec));
+      if (ops[i].is_unitary())
+        block_features(op_qubits(ops[i]), {(int)i}, ops, n, plan.ops.back().feat);
       ++i;
       continue;
     }
@@ -366,8 +427,11 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
       };
     std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
     for (FusedBlock& blk : blocks) {
+      float feat[mlp::kFeatures];
+      block_features(blk.qubits, blk.ops, ops, n, feat);
       if (blk.ops.size() == 1) {
         plan.ops.push_back(lower(ops[blk.ops[0]], plan, prec));
+        std::copy(feat, feat + mlp::kFeatures, plan.ops.back().feat);
         continue;
       }
       build_block_matrix(blk, ops);
@@ -389,6 +453,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
           d.mfma = true;
         }
       }
+      std::copy(feat, feat + mlp::kFeatures, d.feat);
       plan.ops.push_back(std::move(d));
     }
     i = j;
@@ -1078,6 +1143,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
         t.gates = d.gates;
         t.ms = elapsed_ms(evs[i], evs[i + 1]);
         t.bytes = op_bytes(d, n, o.prec);
+        std::copy(d.feat, d.feat + mlp::kFeatures, t.feat);
         stats.ops.push_back(t);
       }
     }
@@ -1119,15 +1185,418 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
 }
 
 // ---------------------------------------------------------------------------
+// Batched small circuits
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchProgram {
+  std::vector<bk::BatchCircuit> circ;
+  std::vector<bk::BatchOp> ops;
+  std::vector<float> mats;
+  std::vector<uint32_t> order;
+  uint64_t total_shots = 0;
+  uint32_t max_n = 0;
+};
+
+struct ParsedBatch {
+  std::vector<Parsed> p;
+  std::vector<size_t> pe;
+};
+
+BatchProgram build_batch(const ParsedBatch& pb, const std::vector<uint64_t>& shots,
+                         const std::vector<uint64_t>& seeds, int max_fused, const std::vector<uint32_t>& subset) {
+  BatchProgram prog;
+  std::vector<double> cost;
+  for (uint32_t idx : subset) {
+    const Parsed& p = pb.p[idx];
+    bk::BatchCircuit c{};
+    c.n = (uint32_t)p.n;
+    c.op_off = prog.ops.size();
+    c.seed = seeds[idx];
+    c.shots = (uint32_t)shots[idx];
+    c.out_off = prog.total_shots;
+    prog.total_shots += shots[idx];
+    std::vector<FusedBlock> blocks = greedy_fuse(p.ops, 0, pb.pe[idx], max_fused);
+    for (FusedBlock& b : blocks) {
+      build_block_matrix(b, p.ops);
+      bk::BatchOp op{};
+      op.k = (uint32_t)b.qubits.size();
+      op.mat_off = (uint32_t)prog.mats.size();
+      for (uint32_t i = 0; i < op.k; ++i) op.q[i] = b.qubits[i];
+      std::vector<uint32_t> srt(b.qubits.begin(), b.qubits.end());
+      std::sort(srt.begin(), srt.end());
+      for (uint32_t i = 0; i < op.k; ++i) op.s[i] = srt[i];
+      for (const cplx& v : b.mat) {
+        prog.mats.push_back((float)v.re);
+        prog.mats.push_back((float)v.im);
+      }
+      prog.ops.push_back(op);
+    }
+    c.nops = (uint32_t)(prog.ops.size() - c.op_off);
+    prog.circ.push_back(c);
+    cost.push_back(std::ldexp((double)(c.nops + 2), (int)c.n));
+    prog.max_n = std::max(prog.max_n, c.n);
+  }
+  prog.order.resize(prog.circ.size());
+  for (uint32_t i = 0; i < prog.order.size(); ++i) prog.order[i] = i;
+  std::stable_sort(prog.order.begin(), prog.order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  return prog;
+}
+
+struct BatchDevice {
+  DeviceMem circ, order, ops, mats, scratch, queue, out, times, dump, dump_off;
+  bk::LaunchArgs args{};
+};
+
+int device_attr(hipDeviceAttribute_t a) {
+  int dev = 0, v = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&v, a, dev));
+  return v;
+}
+
+void upload_batch(BatchDevice& d, const BatchProgram& prog, int block, int lmax, bool dump, hipStream_t st,
+                  const std::vector<uint64_t>* dump_off) {
+  auto up = [&](DeviceMem& m, const void* p, size_t bytes) {
+    m.alloc(bytes);
+    if (bytes) HIPCHK(hipMemcpyAsync(m.get(), p, bytes, hipMemcpyHostToDevice, st));
+  };
+  up(d.circ, prog.circ.data(), prog.circ.size() * sizeof(bk::BatchCircuit));
+  up(d.order, prog.order.data(), prog.order.size() * sizeof(uint32_t));
+  up(d.ops, prog.ops.data(), prog.ops.size() * sizeof(bk::BatchOp));
+  up(d.mats, prog.mats.data(), prog.mats.size() * sizeof(float));
+  const int cus = device_attr(hipDeviceAttributeMultiprocessorCount);
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(prog.circ.size(), (uint64_t)cus * 2));
+  uint64_t slot = 0;
+  for (auto& c : prog.circ)
+    if ((int)c.n > lmax) slot = std::max<uint64_t>(slot, 1ull << c.n);
+  d.scratch.alloc(std::max<uint64_t>(8, 8 * slot * grid));
+  d.queue.alloc(sizeof(int));
+  d.out.alloc(4 * std::max<uint64_t>(1, prog.total_shots));
+  d.times.alloc(16 * std::max<size_t>(1, prog.circ.size()));
+  if (dump) {
+    up(d.dump_off, dump_off->data(), dump_off->size() * sizeof(uint64_t));
+    d.dump.alloc(8 * std::max<uint64_t>(1, dump_off->back()));
+  }
+  bk::LaunchArgs& a = d.args;
+  a.circ = d.circ.as<bk::BatchCircuit>();
+  a.ncirc = (uint32_t)prog.circ.size();
+  a.order = d.order.as<uint32_t>();
+  a.ops = d.ops.as<bk::BatchOp>();
+  a.mats = d.mats.as<float>();
+  a.scratch = d.scratch.as<float2>();
+  a.slot_amps = slot;
+  a.lmax = lmax;
+  a.queue = d.queue.as<int>();
+  a.out = d.out.as<uint32_t>();
+  a.times = d.times.as<uint64_t>();
+  a.dump = dump ? d.dump.as<float2>() : nullptr;
+  a.dump_off = dump ? d.dump_off.as<uint64_t>() : nullptr;
+  a.grid = grid;
+  a.block = block;
+}
+
+int lmax_for(int block) {
+  const int lim = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
+  int l = 0;
+  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= lim) ++l;
+  return l;
+}
+
+// Time one launch of a captured graph (warm-up launch first).
+double time_graph(const BatchDevice& d, hipStream_t st, int reps, hipGraphExec_t* keep) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  bk::launch(d.args, st);
+  HIPCHK(hipStreamEndCapture(st, &g));
+  hipGraphExec_t ge;
+  HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(g));
+  HIPCHK(hipGraphLaunch(ge, st));
+  Event a, b;
+  HIPCHK(hipEventRecord(a.e, st));
+  for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge, st));
+  HIPCHK(hipEventRecord(b.e, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double ms = elapsed_ms(a, b) / std::max(1, reps);
+  if (keep) *keep = ge;
+  else HIPCHK(hipGraphExecDestroy(ge));
+  return ms;
+}
+
+}  // namespace
+
+std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
+                                        const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
+                                        const BatchOptions& opt) {
+  const size_t nc = circuits.size();
+  if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
+  auto out = std::make_unique<BatchOutcome>();
+  ParsedBatch pb;
+  pb.p.resize(nc);
+  pb.pe.resize(nc);
+  out->nq.resize(nc);
+  out->nc.resize(nc);
+  out->cmap.resize(nc);
+  for (size_t i = 0; i < nc; ++i) {
+    Parsed& p = pb.p[i];
+    try {
+      p = read_and_validate(circuits[i].first, circuits[i].second);
+      p.ops = compile(p.c.instructions);
+    } catch (const ValueErr& e) {
+      throw ValueErr("Circuit " + std::to_string(i) + ": " + e.msg);
+    }
+    if (p.n > 30 || p.n < 1) throw ValueErr("Circuit " + std::to_string(i) + ": batched circuits need 1..30 qubits");
+    if (shots[i] > 0xFFFFFFFFull) throw ValueErr("too many shots for one batched circuit");
+    size_t pe = prefix_end_of(p.ops);
+    std::map<uint64_t, uint64_t> mapping;
+    for (size_t j = pe; j < p.ops.size(); ++j) {
+      if (p.ops[j].kind != OpKind::Measure)
+        throw ValueErr("Circuit " + std::to_string(i) + ": batched execution needs terminal measurements only");
+      mapping[p.ops[j].cbit] = p.ops[j].qubit;
+    }
+    pb.pe[i] = pe;
+    out->nq[i] = p.n;
+    out->nc[i] = p.nc;
+    out->cmap[i].assign(mapping.begin(), mapping.end());
+  }
+  Stream st;
+  std::vector<uint32_t> all(nc);
+  for (uint32_t i = 0; i < nc; ++i) all[i] = i;
+
+  int block = opt.block ? opt.block : 256;
+  int fused = opt.max_fused ? opt.max_fused : 3;
+  if (block != 256 && block != 512 && block != 1024) throw ValueErr("block must be 256, 512 or 1024");
+  if (fused < 1 || fused > 4) throw ValueErr("batched fusion width must be 1..4");
+  if (opt.autotune && nc > 0) {
+    // Calibration subset: evenly spaced circuits of the cost order.
+    BatchProgram probe = build_batch(pb, shots, seeds, 2, all);
+    std::vector<uint32_t> sub;
+    const size_t want = std::min<size_t>(nc, 64);
+    for (size_t i = 0; i < want; ++i) sub.push_back(probe.order[(i * nc) / want]);
+    double best = 1e300;
+    const int blocks[3] = {256, 512, 1024};
+    for (int f = 2; f <= 4; ++f) {
+      if (opt.max_fused && f != opt.max_fused) continue;
+      BatchProgram prog = build_batch(pb, shots, seeds, f, sub);
+      for (int b : blocks) {
+        if (opt.block && b != opt.block) continue;
+        const int lm = lmax_for(b);
+        bk::set_lds_limit(b, bk::lds_bytes(lm, b));
+        BatchDevice dev;
+        upload_batch(dev, prog, b, lm, false, st.s, nullptr);
+        const double ms = time_graph(dev, st.s, 1, nullptr);
+        out->tuning.push_back({"block=" + std::to_string(b) + ",k<=" + std::to_string(f), ms});
+        if (ms < best) {
+          best = ms;
+          block = b;
+          fused = f;
+        }
+      }
+    }
+  }
+  const int lmax = lmax_for(block);
+  bk::set_lds_limit(block, bk::lds_bytes(lmax, block));
+  BatchProgram prog = build_batch(pb, shots, seeds, fused, all);
+  std::vector<uint64_t> dump_off;
+  if (opt.dump_states) {
+    uint64_t acc = 0;
+    for (size_t i = 0; i < nc; ++i) {
+      dump_off.push_back(acc);
+      acc += 1ull << out->nq[i];
+    }
+    dump_off.push_back(acc);
+  }
+  BatchDevice dev;
+  upload_batch(dev, prog, block, lmax, opt.dump_states, st.s, &dump_off);
+  out->total_ms = time_graph(dev, st.s, std::max(1, opt.replays), nullptr);
+  out->block = block;
+  out->fused = fused;
+  out->lmax = lmax;
+  out->grid = dev.args.grid;
+  out->idx.resize(prog.total_shots);
+  if (prog.total_shots)
+    HIPCHK(hipMemcpyAsync(out->idx.data(), dev.out.get(), 4 * prog.total_shots, hipMemcpyDeviceToHost, st.s));
+  std::vector<uint64_t> times(2 * nc);
+  if (nc) HIPCHK(hipMemcpyAsync(times.data(), dev.times.get(), 16 * nc, hipMemcpyDeviceToHost, st.s));
+  if (opt.dump_states) {
+    out->states.resize(2 * dump_off.back());
+    HIPCHK(hipMemcpyAsync(out->states.data(), dev.dump.get(), 8 * dump_off.back(), hipMemcpyDeviceToHost, st.s));
+    dump_off.pop_back();
+    out->state_off = dump_off;
+  }
+  st.sync();
+  const double khz = (double)device_attr(hipDeviceAttributeWallClockRate);
+  out->latency_us.resize(nc);
+  for (size_t i = 0; i < nc; ++i) out->latency_us[i] = (double)(times[2 * i + 1] - times[2 * i]) / khz * 1e3;
+  out->offsets.resize(nc);
+  for (size_t i = 0; i < nc; ++i) out->offsets[i] = prog.circ[i].out_off;
+  return out;
+}
+
+Counts batch_counts(const BatchOutcome& b, uint64_t i) {
+  std::map<std::string, uint64_t> c;
+  const uint64_t nshots = (i + 1 < b.offsets.size() ? b.offsets[i + 1] : b.idx.size()) - b.offsets[i];
+  const uint64_t ncb = b.nc[i];
+  std::unordered_map<uint32_t, uint64_t> hist;
+  for (uint64_t s = 0; s < nshots; ++s) hist[b.idx[b.offsets[i] + s]]++;
+  std::vector<uint8_t> bits(ncb, 0);
+  for (auto& kv : hist) {
+    for (auto& m : b.cmap[i]) bits[m.first] = (uint8_t)((kv.first >> m.second) & 1);
+    c[format_bits(bits.data(), ncb)] += kv.second;
+  }
+  return Counts(c.begin(), c.end());
+}
+
+// ---------------------------------------------------------------------------
 // Learned planner hook
 // ---------------------------------------------------------------------------
 void set_planner_weights(const float* w, size_t count) {
-  if (!w || count == 0) g_planner.clear();
-  else g_planner.assign(w, w + count);
+  if (!w || count == 0) {
+    g_planner.clear();
+    return;
+  }
+  if (count != (size_t)mlp::kMlpParams)
+    throw ValueErr("planner weights must have " + std::to_string(mlp::kMlpParams) + " entries");
+  g_planner.assign(w, w + count);
+}
+
+float mlp_eval_host(const float* w, const float* x) {
+  constexpr int F = mlp::kFeatures, H = mlp::kHidden;
+  const float* W1 = w;
+  const float* b1 = W1 + F * H;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* w3 = b2 + H;
+  const float b3 = w3[H];
+  float h1[H], h2[H];
+  for (int j = 0; j < H; ++j) {
+    float a = b1[j];
+    for (int f = 0; f < F; ++f) a = std::fma(x[f], W1[f * H + j], a);
+    h1[j] = a > 0.f ? a : 0.f;
+  }
+  for (int j = 0; j < H; ++j) {
+    float c = b2[j];
+    for (int i = 0; i < H; ++i) c = std::fma(h1[i], W2[i * H + j], c);
+    h2[j] = c > 0.f ? c : 0.f;
+  }
+  float o = b3;
+  for (int i = 0; i < H; ++i) o = std::fma(h2[i], w3[i], o);
+  return o;
 }
 
 namespace {
-double planner_score(const FusedBlock&, const FusedBlock&, const std::vector<Op>&, int) { return 1.0; }
+// Predicted time (relative units) of one launch for the block; the model
+// predicts log(ms per 2^24 amplitudes).
+double predicted_time(const std::vector<uint32_t>& q, const std::vector<int>& opidx, const std::vector<Op>& ops,
+                      int n) {
+  float f[mlp::kFeatures];
+  block_features(q, opidx, ops, n, f);
+  return std::exp((double)mlp_eval_host(g_planner.data(), f));
+}
+
+// Benefit of running blocks a and b as one launch instead of two.
+double planner_score(const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& ops, int n) {
+  std::vector<uint32_t> u(a.qubits);
+  for (uint32_t q : b.qubits)
+    if (std::find(u.begin(), u.end(), q) == u.end()) u.push_back(q);
+  std::vector<int> both(a.ops);
+  both.insert(both.end(), b.ops.begin(), b.ops.end());
+  return predicted_time(a.qubits, a.ops, ops, n) + predicted_time(b.qubits, b.ops, ops, n) -
+         predicted_time(u, both, ops, n);
+}
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// MLP training on the GPU
+// ---------------------------------------------------------------------------
+namespace {
+struct MlpDev {
+  DeviceMem w, g, m, v, x, t, h1, h2, y, dy, d1, d2, loss;
+  mlp::Scratch s{};
+  explicit MlpDev(int nb) {
+    const size_t P = mlp::kMlpParams, F = mlp::kFeatures, H = mlp::kHidden;
+    w.alloc(4 * P);
+    g.alloc(4 * P);
+    m.alloc(4 * P);
+    v.alloc(4 * P);
+    x.alloc(4 * F * nb);
+    t.alloc(4 * nb);
+    h1.alloc(4 * H * nb);
+    h2.alloc(4 * H * nb);
+    y.alloc(4 * nb);
+    dy.alloc(4 * nb);
+    d1.alloc(4 * H * nb);
+    d2.alloc(4 * H * nb);
+    loss.alloc(4);
+    s = {h1.as<float>(), h2.as<float>(), y.as<float>(), dy.as<float>(), d1.as<float>(), d2.as<float>()};
+  }
+};
+}  // namespace
+
+void mlp_grad(const float* w, const float* X, const float* T, int nb, float* g, float* loss) {
+  if (nb <= 0) throw ValueErr("empty batch");
+  Stream st;
+  MlpDev d(nb);
+  HIPCHK(hipMemcpyAsync(d.w.get(), w, 4 * mlp::kMlpParams, hipMemcpyHostToDevice, st.s));
+  HIPCHK(hipMemcpyAsync(d.x.get(), X, 4 * (size_t)mlp::kFeatures * nb, hipMemcpyHostToDevice, st.s));
+  HIPCHK(hipMemcpyAsync(d.t.get(), T, 4 * (size_t)nb, hipMemcpyHostToDevice, st.s));
+  mlp::grad(d.w.as<float>(), d.x.as<float>(), d.t.as<float>(), nb, d.s, d.g.as<float>(), d.loss.as<float>(), st.s);
+  HIPCHK(hipMemcpyAsync(g, d.g.get(), 4 * mlp::kMlpParams, hipMemcpyDeviceToHost, st.s));
+  HIPCHK(hipMemcpyAsync(loss, d.loss.get(), 4, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+}
+
+void mlp_predict(const float* w, const float* X, uint64_t n, float* y) {
+  Stream st;
+  DeviceMem dw(4 * mlp::kMlpParams), dx(4 * mlp::kFeatures * std::max<uint64_t>(n, 1)), dyv(4 * std::max<uint64_t>(n, 1));
+  HIPCHK(hipMemcpyAsync(dw.get(), w, 4 * mlp::kMlpParams, hipMemcpyHostToDevice, st.s));
+  HIPCHK(hipMemcpyAsync(dx.get(), X, 4 * mlp::kFeatures * n, hipMemcpyHostToDevice, st.s));
+  mlp::forward(dw.as<float>(), dx.as<float>(), (int)n, nullptr, nullptr, dyv.as<float>(), st.s);
+  HIPCHK(hipMemcpyAsync(y, dyv.get(), 4 * n, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+}
+
+void mlp_train(float* w, const float* X, const float* T, uint64_t n, int epochs, int batch, float lr, uint64_t seed,
+               float* loss_curve) {
+  if (n == 0 || batch <= 0 || epochs <= 0) throw ValueErr("empty training set");
+  const int nb = (int)std::min<uint64_t>((uint64_t)batch, n);
+  Stream st;
+  MlpDev d(nb);
+  DeviceMem X_d(4 * mlp::kFeatures * n), T_d(4 * n), idx_d(4 * n);
+  HIPCHK(hipMemcpyAsync(d.w.get(), w, 4 * mlp::kMlpParams, hipMemcpyHostToDevice, st.s));
+  HIPCHK(hipMemsetAsync(d.m.get(), 0, 4 * mlp::kMlpParams, st.s));
+  HIPCHK(hipMemsetAsync(d.v.get(), 0, 4 * mlp::kMlpParams, st.s));
+  HIPCHK(hipMemcpyAsync(X_d.get(), X, 4 * mlp::kFeatures * n, hipMemcpyHostToDevice, st.s));
+  HIPCHK(hipMemcpyAsync(T_d.get(), T, 4 * n, hipMemcpyHostToDevice, st.s));
+  std::vector<uint32_t> order(n);
+  for (uint64_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
+  int step = 0;
+  const uint64_t per_epoch = n / (uint64_t)nb;
+  for (int e = 0; e < epochs; ++e) {
+    // Fisher-Yates with the Philox stream (seed, stream = epoch).
+    for (uint64_t i = n - 1; i > 0; --i) {
+      const uint64_t j = (uint64_t)(uniform53(seed, (uint64_t)e, i) * (double)(i + 1));
+      std::swap(order[i], order[std::min(j, i)]);
+    }
+    HIPCHK(hipMemcpyAsync(idx_d.get(), order.data(), 4 * n, hipMemcpyHostToDevice, st.s));
+    for (uint64_t b = 0; b < std::max<uint64_t>(1, per_epoch); ++b) {
+      mlp::gather(X_d.as<float>(), T_d.as<float>(), idx_d.as<uint32_t>() + b * nb, nb, d.x.as<float>(),
+                  d.t.as<float>(), st.s);
+      mlp::grad(d.w.as<float>(), d.x.as<float>(), d.t.as<float>(), nb, d.s, d.g.as<float>(),
+                b + 1 == std::max<uint64_t>(1, per_epoch) ? d.loss.as<float>() : nullptr, st.s);
+      mlp::adam(d.w.as<float>(), d.g.as<float>(), d.m.as<float>(), d.v.as<float>(), lr, ++step, st.s);
+    }
+    if (loss_curve) {
+      HIPCHK(hipMemcpyAsync(loss_curve + e, d.loss.get(), 4, hipMemcpyDeviceToHost, st.s));
+    }
+    // The host shuffle of the next epoch overwrites `order` only after the
+    // copy of this epoch's indices has completed.
+    st.sync();
+  }
+  HIPCHK(hipMemcpyAsync(w, d.w.get(), 4 * mlp::kMlpParams, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+  HIPCHK(hipGetLastError());
+}
 
 }  // namespace dq

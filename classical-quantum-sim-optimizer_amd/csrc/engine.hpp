@@ -99,6 +99,7 @@ struct OpTiming {
   int gates;     // source gates folded into the launch
   double ms;     // device time
   double bytes;  // algorithmic HBM bytes moved
+  float feat[12];  // planner features of the launch (block_features)
 };
 
 struct EvolveStats {
@@ -131,7 +132,42 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
 
 int device_count();
 
+// ---- batched small circuits (persistent kernel, batch.hip) ----
+struct BatchOptions {
+  int block = 0;       // 0 = autotune / default
+  int max_fused = 0;   // 0 = autotune / default
+  bool autotune = true;
+  int replays = 1;     // timed graph launches
+  bool dump_states = false;
+};
+struct BatchOutcome {
+  std::vector<uint64_t> nq, nc;             // per circuit
+  std::vector<uint64_t> offsets;            // sample offset per circuit
+  std::vector<uint32_t> idx;                // full-register sample indices
+  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> cmap;  // (cbit, qubit)
+  std::vector<double> latency_us;           // per circuit, last replay
+  std::vector<uint64_t> state_off;          // dumped states (complex64 -> c128 on query)
+  std::vector<float> states;                // interleaved re, im
+  double total_ms = 0;                      // device time per graph launch
+  int block = 0, fused = 0, lmax = 0;
+  uint32_t grid = 0;
+  std::vector<std::pair<std::string, double>> tuning;  // candidate -> ms
+};
+std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
+                                        const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
+                                        const BatchOptions& opt);
+Counts batch_counts(const BatchOutcome& b, uint64_t i);
+
 // Learned fusion planner weights (2 hidden layers of 64, FP32). Null clears.
 void set_planner_weights(const float* w, size_t count);
+// Train the planner MLP on the GPU (Adam, mean squared error); w is in/out.
+void mlp_train(float* w, const float* X, const float* T, uint64_t n, int epochs, int batch, float lr, uint64_t seed,
+               float* loss_curve);
+// One forward + backward pass on the GPU: gradients and loss of a batch.
+void mlp_grad(const float* w, const float* X, const float* T, int nb, float* g, float* loss);
+// GPU forward pass.
+void mlp_predict(const float* w, const float* X, uint64_t n, float* y);
+// Host FP32 forward pass (the planner's own inference path).
+float mlp_eval_host(const float* w, const float* x);
 
 }  // namespace dq

@@ -76,6 +76,17 @@ class OpTiming(ctypes.Structure):
         ("gates", ctypes.c_int32),
         ("ms", ctypes.c_double),
         ("bytes", ctypes.c_double),
+        ("feat", ctypes.c_float * 12),
+    ]
+
+
+class _BatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int32),
+        ("max_fused_qubits", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("replays", ctypes.c_int32),
+        ("dump_states", ctypes.c_int32),
     ]
 
 
@@ -118,6 +129,17 @@ def _load():
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
         "dq_set_planner_weights": (ctypes.c_int, [P, U64]),
+        "dq_batch_run": (ctypes.c_int, [P, P, U64, P, P, ctypes.POINTER(_BatchOpts), ctypes.POINTER(P)]),
+        "dq_batch_counts": (ctypes.c_int, [P, U64, _COUNT_CB, P]),
+        "dq_batch_indices": (ctypes.c_int, [P, U64, P, U64]),
+        "dq_batch_info": (ctypes.c_int, [P, ctypes.POINTER(D), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                         ctypes.POINTER(I32), ctypes.POINTER(ctypes.c_uint32), P, U64]),
+        "dq_batch_state_c128": (ctypes.c_int, [P, U64, P, U64]),
+        "dq_batch_free": (None, [P]),
+        "dq_mlp_param_count": (U64, []),
+        "dq_mlp_train": (ctypes.c_int, [P, P, P, U64, I32, I32, ctypes.c_float, U64, P]),
+        "dq_mlp_grad": (ctypes.c_int, [P, P, P, I32, P, P]),
+        "dq_mlp_predict": (ctypes.c_int, [P, P, U64, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -389,7 +411,7 @@ class StatevectorSimulator:
             for i in range(min(nops.value, cap)):
                 r = recs[i]
                 rows.append({"kind": r.kind, "k": r.k, "low": r.low, "gates": r.gates, "ms": r.ms,
-                             "bytes": r.bytes})
+                             "bytes": r.bytes, "feat": list(r.feat)})
         return rows, nops.value, gates.value, total.value, (SimulationResult(keep_h) if keep else None)
 
 

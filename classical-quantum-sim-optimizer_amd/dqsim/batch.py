@@ -1,0 +1,81 @@
+"""Batched execution of many small circuits in one persistent GPU launch.
+
+No reference counterpart: the reference would run such a workload as a loop of
+``StatevectorSimulator.simulate_shots`` calls. Every circuit must end in
+terminal measurements; each gets its own shot count and seed and is sampled
+with the Philox stream (seed, stream = shot).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _core
+from ._core import _BatchOpts, _check, _circuit_json, _collect_counts, _lib
+
+
+class BatchResult:
+    """Counts, sample indices, timings and the tuned launch shape of a batch."""
+
+    def __init__(self, handle, ncirc: int, shots: List[int], nqubits: List[int]):
+        self._h = handle
+        self.num_circuits = ncirc
+        self._shots = shots
+        self._nq = nqubits
+        total = ctypes.c_double()
+        block, fused, lmax = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        grid = ctypes.c_uint32()
+        lat = np.zeros(max(1, ncirc), dtype=np.float64)
+        _check(_lib.dq_batch_info(handle, ctypes.byref(total), ctypes.byref(block), ctypes.byref(fused),
+                                  ctypes.byref(lmax), ctypes.byref(grid), lat.ctypes.data, ncirc))
+        self.device_ms = total.value
+        self.block = block.value
+        self.max_fused_qubits = fused.value
+        self.lds_qubits = lmax.value
+        self.grid = grid.value
+        self.latency_us = lat[:ncirc]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.dq_batch_free(h)
+            self._h = None
+
+    def counts(self, i: int) -> Dict[str, int]:
+        out, cb = _collect_counts()
+        _check(_lib.dq_batch_counts(self._h, i, cb, None))
+        return out
+
+    def indices(self, i: int) -> np.ndarray:
+        out = np.empty(max(1, self._shots[i]), dtype=np.uint32)
+        _check(_lib.dq_batch_indices(self._h, i, out.ctypes.data, self._shots[i]))
+        return out[: self._shots[i]]
+
+    def statevector(self, i: int) -> np.ndarray:
+        out = np.empty(1 << self._nq[i], dtype=np.complex128)
+        _check(_lib.dq_batch_state_c128(self._h, i, out.ctypes.data, out.size))
+        return out
+
+
+def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = None, base_seed: int = 0,
+              block: int = 0, max_fused_qubits: int = 0, autotune: bool = True, replays: int = 1,
+              keep_states: bool = False) -> BatchResult:
+    """Run circuits as one batched launch. ``shots`` is an int or a list;
+    ``seeds`` defaults to base_seed + index."""
+    texts = [_circuit_json(c) for c in circuits]
+    n = len(texts)
+    shots_l = [int(shots)] * n if isinstance(shots, (int, np.integer)) else [int(s) for s in shots]
+    seeds_l = [base_seed + i for i in range(n)] if seeds is None else [int(s) for s in seeds]
+    bufs = [ctypes.create_string_buffer(t, len(t)) for t in texts]
+    ptrs = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(b) for b in bufs])
+    lens = (ctypes.c_size_t * max(1, n))(*[len(t) for t in texts])
+    sh = (ctypes.c_uint64 * max(1, n))(*shots_l)
+    sd = (ctypes.c_uint64 * max(1, n))(*seeds_l)
+    o = _BatchOpts(block, max_fused_qubits, 1 if autotune else 0, replays, 1 if keep_states else 0)
+    h = ctypes.c_void_p()
+    _check(_lib.dq_batch_run(ptrs, lens, n, sh, sd, ctypes.byref(o), ctypes.byref(h)))
+    nq = [int(json_n) for json_n in (_core.plan_json(t)["num_qubits"] for t in texts)] if keep_states else [0] * n
+    return BatchResult(h, n, shots_l, nq)

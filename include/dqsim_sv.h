@@ -89,6 +89,7 @@ typedef struct dq_op_timing {
   int32_t gates; /* compiled ops folded into the launch */
   double ms;
   double bytes;  /* algorithmic HBM bytes */
+  float feat[12];/* fusion-planner features of the launch */
 } dq_op_timing;
 
 /* Fill `o` with the reference defaults (max_memory_mb = 1024,
@@ -154,9 +155,47 @@ int dq_device_count(void);
 int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats, dq_op_timing* out, uint64_t cap,
                        uint64_t* nops, uint64_t* gates, double* total_ms, dq_result** keep);
 
-/* Weights of the learned fusion planner (FP32, layout documented in
- * the planner module); count = 0 clears them. */
+/* ---- batched small circuits (no reference counterpart: the reference runs
+ * such workloads as a loop of simulate_shots calls) ---- */
+typedef struct dq_batch dq_batch;
+typedef struct dq_batch_opts {
+  int32_t block;            /* threads per workgroup: 256, 512, 1024; 0 = tuned */
+  int32_t max_fused_qubits; /* 1..4; 0 = tuned */
+  int32_t autotune;         /* time candidate (block, width) pairs on a sample */
+  int32_t replays;          /* timed graph launches (>= 1) */
+  int32_t dump_states;      /* keep final states for dq_batch_state_c128 */
+} dq_batch_opts;
+/* Run `ncirc` circuits (terminal measurements only) with shots[i] shots and
+ * seed seeds[i] each, in one persistent kernel launched through a hipGraph. */
+int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
+                 const uint64_t* seeds, const dq_batch_opts* o, dq_batch** out);
+/* Counts of circuit i (same format as simulate_shots). */
+int dq_batch_counts(const dq_batch* b, uint64_t i, dq_count_cb cb, void* user);
+/* Full-register sample indices of circuit i (len = shots[i]). */
+int dq_batch_indices(const dq_batch* b, uint64_t i, uint32_t* out, uint64_t len);
+/* Device ms per graph launch, chosen block / width / LDS qubit limit /
+ * grid, and per-circuit latency in microseconds (ncirc entries). */
+int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* block, int32_t* max_fused, int32_t* lds_qubits,
+                  uint32_t* grid, double* latency_us, uint64_t ncirc);
+/* Final state of circuit i as complex128 (needs dump_states). */
+int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len);
+void dq_batch_free(dq_batch* b);
+
+/* ---- learned fusion planner (no reference counterpart) ----
+ * A 12 -> 64 -> 64 -> 1 FP32 MLP predicting log(ms per 2^24 amplitudes) of a
+ * fused launch from its features. Packed weights: W1[12][64], b1[64],
+ * W2[64][64], b2[64], w3[64], b3 (dq_mlp_param_count() entries). */
+uint64_t dq_mlp_param_count(void);
+/* Install weights for fusion = DQ_FUSE_LEARNED; count = 0 clears them. */
 int dq_set_planner_weights(const float* w, uint64_t count);
+/* Train on the GPU (Adam, mean squared error, Philox-shuffled mini-batches);
+ * w is read and written; loss_curve gets one value per epoch (optional). */
+int dq_mlp_train(float* w, const float* X, const float* y, uint64_t n, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float* loss_curve);
+/* One GPU forward + backward pass over a batch: gradients and loss. */
+int dq_mlp_grad(const float* w, const float* X, const float* y, int32_t n, float* grad, float* loss);
+/* GPU forward pass. */
+int dq_mlp_predict(const float* w, const float* X, uint64_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -118,6 +118,8 @@ def _load():
     lib.or_fidelity.restype = D
     lib.or_fidelity.argtypes = [P, P, ctypes.c_int]
     lib.or_set_threads.argtypes = [ctypes.c_int]
+    lib.or_batch_sample.restype = ctypes.c_int
+    lib.or_batch_sample.argtypes = [P, ctypes.c_int, U64, U64, ctypes.c_int, P]
     return lib
 
 
@@ -334,3 +336,12 @@ def measure(state, qubit, r):
 def ptree(x: np.ndarray) -> float:
     x = np.ascontiguousarray(x, dtype=np.float64)
     return lib().or_ptree(x.ctypes.data, x.size)
+
+
+def batch_sample(state: np.ndarray, shots: int, seed: int, block: int) -> np.ndarray:
+    """Sample indices exactly as the batched-circuit kernel does."""
+    state = np.ascontiguousarray(state, dtype=np.complex128)
+    n = int(state.size).bit_length() - 1
+    out = np.empty(max(1, shots), dtype=np.uint64)
+    lib().or_batch_sample(state.ctypes.data, n, shots, seed, block, out.ctypes.data)
+    return out[:shots]

@@ -770,6 +770,49 @@ int or_sample_marginal(const double* s, int n, const uint64_t* qubits, int k, ui
   return 0;
 }
 
+int or_batch_sample(const double* s, int n, uint64_t shots, uint64_t seed, int block, uint64_t* out) {
+  const cx* st = (const cx*)s;
+  const uint64_t len = 1ull << n;
+  const uint64_t nch = len < (uint64_t)block ? len : (uint64_t)block, cl = len / nch;
+  double* part = (double*)malloc(sizeof(double) * nch);
+  for (uint64_t t = 0; t < nch; ++t) {
+    double acc = 0.0;
+    for (uint64_t i = t * cl; i < (t + 1) * cl; ++i) acc = acc + prob(st, i);
+    part[t] = acc;
+  }
+  double run = 0.0;
+  for (uint64_t t = 0; t < nch; ++t) {
+    run = run + part[t];
+    part[t] = run;
+  }
+  const double total = part[nch - 1];
+  for (uint64_t sh = 0; sh < shots; ++sh) {
+    const double r = or_uniform53(seed, sh, 0);
+    uint64_t lo = 0, hi = nch;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) / 2;
+      double u = mid + 1 == nch ? 1.0 : part[mid] / total;
+      if (u <= r) lo = mid + 1;
+      else hi = mid;
+    }
+    uint64_t t = lo < nch ? lo : nch - 1;
+    double acc = t ? part[t - 1] : 0.0;
+    uint64_t found = ~0ull, lastpos = t * cl;
+    for (uint64_t i = t * cl; i < (t + 1) * cl; ++i) {
+      double p = prob(st, i);
+      if (p > 0.0) lastpos = i;
+      acc = acc + p;
+      if (acc / total > r) {
+        found = i;
+        break;
+      }
+    }
+    out[sh] = found != ~0ull ? found : lastpos;
+  }
+  free(part);
+  return 0;
+}
+
 double or_fidelity(const double* a, const double* b, int n) {
   const uint64_t len = 1ull << n;
   double* re = (double*)malloc(sizeof(double) * len);

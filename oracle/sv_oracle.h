@@ -110,6 +110,11 @@ int or_sample_marginal(const double* s, int n, const uint64_t* qubits, int k, ui
 /* |<a|b>|^2 with the pairwise tree. */
 double or_fidelity(const double* a, const double* b, int n);
 
+/* Sampling of the batched-circuit kernel: `nch` = min(2^n, block) equal
+ * chunks, sequential FP64 chunk sums and prefix, binary search then a
+ * sequential in-chunk scan (see batch.hip). */
+int or_batch_sample(const double* s, int n, uint64_t shots, uint64_t seed, int block, uint64_t* out);
+
 /* Threads used by the parallel loops (OpenMP); 0 = library default. */
 void or_set_threads(int t);
 

@@ -1,0 +1,44 @@
+"""Batched small circuits (persistent kernel): each circuit is pinned by the
+FP64 oracle (final state <= 1e-5) and its samples by the oracle's restatement
+of the kernel's sampler on the kernel's own final state (exact indices)."""
+
+import numpy as np
+import pytest
+
+from dqsim import batch, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def circuits(count, nmin, nmax, dmin=2, dmax=8, base=77):
+    return [workloads.batch_circuit(i, base_seed=base, nmin=nmin, nmax=nmax, dmin=dmin, dmax=dmax)
+            for i in range(count)]
+
+
+@pytest.mark.parametrize("block,k", [(256, 2), (512, 3), (1024, 4), (256, 4)])
+def test_batch_states_and_samples_match_oracle(block, k):
+    cs = circuits(12, 3, 12) + circuits(3, 15, 16, base=5)
+    res = batch.run_batch(cs, shots=300, base_seed=11, block=block, max_fused_qubits=k, autotune=False,
+                          keep_states=True)
+    assert res.block == block and res.max_fused_qubits == k
+    for i, c in enumerate(cs):
+        n = c.num_qubits
+        unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
+        want, _ = oracle.simulate(workloads.Circuit(n, unitary).data, 1)
+        got = res.statevector(i)
+        assert np.abs(got - want).max() < 1e-5, (i, n)
+        idx = oracle.batch_sample(got, 300, 11 + i, block)
+        assert np.array_equal(res.indices(i).astype(np.uint64), idx), i
+        counts = res.counts(i)
+        assert sum(counts.values()) == 300
+        assert counts == oracle.counts_from_indices(idx, n)
+
+
+def test_batch_autotune_and_latency():
+    cs = circuits(40, 4, 14, dmin=4, dmax=12, base=9)
+    res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True)
+    assert res.block in (256, 512, 1024) and 2 <= res.max_fused_qubits <= 4
+    assert res.device_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
+    for i in range(40):
+        assert sum(res.counts(i).values()) == 64

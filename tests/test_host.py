@@ -253,3 +253,33 @@ def test_workload_gate_counts():
     assert workloads.gate_count(workloads.qft(30, measure=False)[0].data["instructions"]) == 510
     assert workloads.gate_count(workloads.qft(33, measure=False)[0].data["instructions"]) == 610
     assert workloads.gate_count(workloads.random_su2(32, 40).data["instructions"]) == 1920
+
+
+def test_learned_planner_merge_decisions_follow_the_model():
+    from dqsim import planner
+
+    circ = workloads.random_su2(n=16, depth=6, seed=2)
+    greedy = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=4)
+    w = np.zeros(planner.N_PARAMS, dtype=np.float32)
+    try:
+        # Constant prediction: every merge saves one launch -> greedy plan.
+        w[-1] = 1.0
+        planner.install(w)
+        same = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4)
+        assert [l["k"] for l in same["launches"]] == [l["k"] for l in greedy["launches"]]
+        # Time growing steeply with width (feature 0 through one hidden unit):
+        # only merges that do not widen a block pay off, so no launch is wider
+        # than the widest compiled gate (2 qubits).
+        w[:] = 0
+        w[0 * 64 + 0] = 1.0       # W1[k][0]
+        w[12 * 64 + 64 + 0] = 60.0  # W2[0][0]
+        w[12 * 64 + 64 + 64 * 64 + 64 + 0] = 1.0  # w3[0]
+        planner.install(w)
+        narrow = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4)
+        assert max(l["k"] for l in narrow["launches"]) == 2
+        assert len(narrow["launches"]) > len(greedy["launches"])
+        assert sum(l["gates"] for l in narrow["launches"]) == narrow["compiled_ops"]
+    finally:
+        planner.install(None)
+    with pytest.raises(ValueError):
+        planner.install(np.zeros(5, dtype=np.float32))

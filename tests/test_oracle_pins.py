@@ -288,3 +288,22 @@ def test_reference_deterministic_results():
             for c in case["counts"]:
                 idx = oracle.sample_marginal(st, c["qubits"], c["shots"], 2)
                 assert oracle.counts_from_indices(idx, len(c["qubits"])) == c["expected"]
+
+
+# ---------------------------------------------------------------------------
+# Planner MLP oracle (oracle/mlp.py): central finite differences
+# ---------------------------------------------------------------------------
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+
+    rng = np.random.default_rng(4)
+    w = rng.normal(0, 0.3, size=12 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+    X = rng.normal(0, 1, size=(9, 12))
+    y = rng.normal(0, 1, size=9)
+    g, _ = mlp.grad(w, X, y)
+    idx = rng.choice(w.size, size=60, replace=False).tolist() + [w.size - 1, 0, 12 * 64]
+    for i in idx:
+        e = np.zeros_like(w)
+        e[i] = 1e-6
+        fd = (mlp.loss(w + e, X, y) - mlp.loss(w - e, X, y)) / 2e-6
+        assert abs(fd - g[i]) < 1e-6 + 1e-5 * abs(g[i]), i
