@@ -73,7 +73,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=24)
     ap.add_argument("--cpu-depth", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_calibration, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs one state per GPU and this benchmark uses exactly one GPU "
               f"(--gpus {args.gpus} requested); multi-GPU sharding is not built.", file=sys.stderr)

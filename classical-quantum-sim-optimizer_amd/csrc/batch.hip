@@ -26,22 +26,23 @@ namespace {
 
 template <int K, typename P>
 __device__ __forceinline__ void apply_block(P st, uint32_t n, const BatchOp& op, const float* __restrict__ m, int bs) {
+  // Batched circuits have at most 30 qubits: 32-bit indices throughout.
   constexpr int D = 1 << K;
-  uint64_t off[D];
+  uint32_t off[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    uint64_t o = 0;
+    uint32_t o = 0;
 #pragma unroll
     for (int i = 0; i < K; ++i)
-      if ((j >> (K - 1 - i)) & 1) o |= 1ull << op.q[i];
+      if ((j >> (K - 1 - i)) & 1) o |= 1u << op.q[i];
     off[j] = o;
   }
-  const uint64_t groups = (1ull << n) >> K;
-  for (uint64_t g = threadIdx.x; g < groups; g += bs) {
-    uint64_t base = g;
+  const uint32_t groups = (1u << n) >> K;
+  for (uint32_t g = threadIdx.x; g < groups; g += bs) {
+    uint32_t base = g;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      const uint64_t low = (1ull << op.s[i]) - 1;
+      const uint32_t low = (1u << op.s[i]) - 1;
       base = (base & low) | ((base & ~low) << 1);
     }
     float2 v[D];

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <random>
 #include <sstream>
 #include <unordered_map>
@@ -86,16 +87,70 @@ DeviceMem& DeviceMem::operator=(DeviceMem&& o) noexcept {
   }
   return *this;
 }
+// Device allocation cache: freed blocks of at least 1 MiB are kept (at most
+// kCacheBlocks of them) and handed out again for a request of the same size,
+// so repeated simulations of one size do not pay for hipMalloc/hipFree of
+// multi-GiB states. The cache is emptied and the allocation retried when
+// hipMalloc reports out-of-memory.
+namespace {
+constexpr size_t kCacheMin = 1 << 20;
+constexpr size_t kCacheBlocks = 4;
+std::mutex g_cache_mu;
+std::vector<std::pair<size_t, void*>> g_cache;
+
+void* cache_take(size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (size_t i = 0; i < g_cache.size(); ++i)
+    if (g_cache[i].first == bytes) {
+      void* p = g_cache[i].second;
+      g_cache.erase(g_cache.begin() + (long)i);
+      return p;
+    }
+  return nullptr;
+}
+bool cache_put(size_t bytes, void* p) {
+  if (bytes < kCacheMin) return false;
+  // hipFree waits for the device; keep that guarantee for cached blocks so a
+  // block is never handed out while queued work may still touch it.
+  if (hipDeviceSynchronize() != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  if (g_cache.size() >= kCacheBlocks) {
+    (void)hipFree(g_cache.front().second);
+    g_cache.erase(g_cache.begin());
+  }
+  g_cache.push_back({bytes, p});
+  return true;
+}
+void cache_release() {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (auto& b : g_cache) (void)hipFree(b.second);
+  g_cache.clear();
+}
+}  // namespace
+
 DeviceMem::~DeviceMem() { reset(); }
 void DeviceMem::reset() {
-  if (p_) (void)hipFree(p_);
+  if (p_ && !cache_put(n_, p_)) (void)hipFree(p_);
   p_ = nullptr;
   n_ = 0;
 }
 void DeviceMem::alloc(size_t bytes) {
   reset();
   if (bytes == 0) bytes = 16;
+  if (void* c = cache_take(bytes)) {
+    p_ = c;
+    n_ = bytes;
+    return;
+  }
   hipError_t e = hipMalloc(&p_, bytes);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    cache_release();
+    e = hipMalloc(&p_, bytes);
+  }
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
     (void)hipGetLastError();
     p_ = nullptr;
