@@ -75,7 +75,16 @@ def main():
     ap.add_argument("--cpu-depth", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--report", choices=["precision", "batch", "planner"], default=None,
+                    help="run one secondary report instead of the headline benchmark")
     args = ap.parse_args()
+    if args.report:
+        from dqsim import reports
+
+        fn = {"precision": reports.precision_report, "batch": reports.batch_report,
+              "planner": reports.planner_report}[args.report]
+        print(json.dumps({"report": args.report, **fn(log=log)}), flush=True)
+        return
     if args.copy_calibration:
         import torch
 

@@ -191,6 +191,10 @@ int dq_result_compare(const dq_result* a, const dq_result* b, double* max_abs, d
   return guard([&] { dq::result_compare(*a->r, *b->r, max_abs, l2); });
 }
 
+int dq_result_gather(const dq_result* r, const uint64_t* indices, uint64_t count, double* out) {
+  return guard([&] { dq::result_gather(*r->r, indices, count, out); });
+}
+
 void dq_result_free(dq_result* r) { delete r; }
 
 const char* dq_last_error(void) { return g_err.c_str(); }

@@ -1153,6 +1153,19 @@ void result_compare(const Result& a, const Result& b, double* max_abs, double* l
   *l2 = std::sqrt(s);
 }
 
+void result_gather(const Result& r, const uint64_t* idx, uint64_t count, double* out) {
+  const uint64_t len = 1ull << r.n;
+  for (uint64_t i = 0; i < count; ++i)
+    if (idx[i] >= len) throw ValueErr("amplitude index out of range");
+  if (count == 0) return;
+  Stream st;
+  DeviceMem di(8 * count), dout(16 * count);
+  HIPCHK(hipMemcpyAsync(di.get(), idx, 8 * count, hipMemcpyHostToDevice, st.s));
+  k::gather(r.prec, r.state.get(), di.as<uint64_t>(), count, dout.get(), st.s);
+  HIPCHK(hipMemcpyAsync(out, dout.get(), 16 * count, hipMemcpyDeviceToHost, st.s));
+  st.sync();
+}
+
 // ---------------------------------------------------------------------------
 // Timed evolution (benchmarks, planner training data)
 // ---------------------------------------------------------------------------

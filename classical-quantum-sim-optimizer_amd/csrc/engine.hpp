@@ -117,6 +117,7 @@ Counts result_counts(const Result& r, uint64_t shots, const std::vector<uint64_t
 double result_fidelity(const Result& r, const void* other_c128, uint64_t len);
 void result_statevector(const Result& r, void* out_c128);
 void result_amplitude(const Result& r, uint64_t index, double* out);
+void result_gather(const Result& r, const uint64_t* idx, uint64_t count, double* out);
 
 // Evolve the unitary prefix of a circuit from |0...0> `repeats` times and
 // report per-launch device timings of the last repeat. The final state can

@@ -704,6 +704,16 @@ __global__ __launch_bounds__(TPB) void k_diff(const typename V2<Ta>::t* __restri
 }
 
 template <typename T>
+__global__ __launch_bounds__(TPB) void k_gather(const typename V2<T>::t* __restrict__ s,
+                                                const uint64_t* __restrict__ idx, uint64_t count,
+                                                double2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= count) return;
+  const typename V2<T>::t a = s[idx[i]];
+  out[i] = make_double2((double)a.x, (double)a.y);
+}
+
+template <typename T>
 M2<T> to_m2(const double* u) {
   M2<T> m;
   for (int i = 0; i < 8; ++i) m.v[i] = (T)u[i];
@@ -970,6 +980,15 @@ void diff(int prec_a, const void* a, int prec_b, const void* b, uint64_t count, 
   else if (prec_b == 16) L2(float, double);
   else L2(float, float);
 #undef L2
+}
+
+void gather(int prec, const void* s, const uint64_t* idx, uint64_t count, void* out, hipStream_t st) {
+  if (count == 0) return;
+#define L(T)                                                                                       \
+  hipLaunchKernelGGL(k_gather<T>, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st,              \
+                     (const typename V2<T>::t*)s, idx, count, (double2*)out)
+  DQ_DISPATCH(prec, L);
+#undef L
 }
 
 void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st) {

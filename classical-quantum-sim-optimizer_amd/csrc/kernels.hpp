@@ -127,6 +127,9 @@ void sample_blocked(int prec, const void* s, int n, const uint8_t* map, int bins
 uint64_t diff_blocks(uint64_t count);
 void diff(int prec_a, const void* a, int prec_b, const void* b, uint64_t count, void* out, hipStream_t st);
 
+// out[i] = state[idx[i]] as complex128.
+void gather(int prec, const void* s, const uint64_t* idx, uint64_t count, void* out, hipStream_t st);
+
 // ---- conversion ----
 void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st);
 

@@ -120,6 +120,7 @@ def _load():
         "dq_result_fidelity": (ctypes.c_int, [P, P, U64, ctypes.POINTER(D)]),
         "dq_result_amplitude": (ctypes.c_int, [P, U64, P]),
         "dq_result_free": (None, [P]),
+        "dq_result_gather": (ctypes.c_int, [P, P, U64, P]),
         "dq_result_compare": (ctypes.c_int, [P, P, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_last_error": (ctypes.c_char_p, []),
         "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
@@ -327,6 +328,13 @@ class SimulationResult:
         mx, l2 = ctypes.c_double(), ctypes.c_double()
         _check(_lib.dq_result_compare(self._h, other._h, ctypes.byref(mx), ctypes.byref(l2)))
         return mx.value, l2.value
+
+    def amplitudes(self, indices) -> np.ndarray:
+        """Amplitudes at the given basis indices (complex128), gathered on the GPU."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        out = np.empty(idx.size, dtype=np.complex128)
+        _check(_lib.dq_result_gather(self._h, idx.ctypes.data, idx.size, out.ctypes.data))
+        return out
 
     def amplitude(self, index) -> complex:
         """One amplitude by basis index, or by bitstring with the highest qubit first."""

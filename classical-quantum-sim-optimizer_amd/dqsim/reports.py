@@ -1,0 +1,140 @@
+"""Measurement reports for the secondary workloads of the benchmark plan.
+
+  precision_report ... one U3 layer + QFT: complex64 (fused) at n64 qubits and
+                       complex128 (plain) at n128 qubits against the exact
+                       closed form at sampled indices, the full-state L2 error
+                       of complex64 against the complex128 GPU result, and the
+                       total-variation distance of shots drawn with the same
+                       Philox stream from both states.
+  batch_report ....... thousands of small uneven circuits in one persistent
+                       graph-launched kernel: circuits/s, p50/p99 latency.
+  planner_report ..... learned fusion planner trained on engine-recorded launch
+                       timings vs the greedy fuser on held-out circuits.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from typing import Dict, List
+
+import numpy as np
+
+from . import _core, batch, planner, workloads
+
+
+def _tvd(a: Dict[str, int], b: Dict[str, int]) -> float:
+    na, nb = sum(a.values()), sum(b.values())
+    keys = set(a) | set(b)
+    return 0.5 * sum(abs(a.get(k, 0) / na - b.get(k, 0) / nb) for k in keys)
+
+
+def precision_report(n64: int = 30, n128: int = 33, shots: int = 4096, samples: int = 1 << 16, seed: int = 7,
+                     log=print) -> dict:
+    out = {}
+    rng = np.random.default_rng(seed)
+    circ, angles = workloads.qft(n64, seed=seed, measure=False)
+    gates = workloads.gate_count(circ.data["instructions"])
+    mem = (1 << n64) * 16 * 3 // (1 << 20) + 4096
+    idx = np.unique(np.concatenate([np.arange(64), rng.integers(0, 1 << n64, size=samples)])).astype(np.uint64)
+    exact = workloads.qft_closed_form(angles, idx)
+    f = _core.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=4,
+                                   max_memory_mb=mem)
+    _, fl, _, fms, fres = f.evolve_timed(circ, record=False, keep=True)
+    p = _core.StatevectorSimulator(seed=1, precision="complex128", max_memory_mb=mem)
+    _, pl, _, pms, pres = p.evolve_timed(circ, record=False, keep=True)
+    f_amp, p_amp = fres.amplitudes(idx), pres.amplitudes(idx)
+    full_max, full_l2 = fres.compare(pres)
+    c64 = fres.counts(shots=shots, seed=11)
+    c128 = pres.counts(shots=shots, seed=11)
+    out["complex64"] = {"qubits": n64, "gates": gates, "launches": fl, "device_ms": fms,
+                        "max_err_vs_closed_form": float(np.abs(f_amp - exact).max()),
+                        "max_err_vs_fp64_gpu": full_max, "l2_err_vs_fp64_gpu": full_l2,
+                        "tvd_vs_fp64_same_stream": _tvd(c64, c128), "shots": shots,
+                        "samples": int(idx.size)}
+    out["complex128_same_n"] = {"qubits": n64, "launches": pl, "device_ms": pms,
+                                "max_err_vs_closed_form": float(np.abs(p_amp - exact).max())}
+    log(f"[precision] c64 n={n64}: {out['complex64']}")
+    del fres, pres
+    if n128:
+        circ, angles = workloads.qft(n128, seed=seed, measure=False)
+        idx = np.unique(np.concatenate([np.arange(64), rng.integers(0, 1 << n128, size=samples)])).astype(np.uint64)
+        exact = workloads.qft_closed_form(angles, idx)
+        mem = (1 << n128) * 16 // (1 << 20) + 4096
+        p = _core.StatevectorSimulator(seed=1, precision="complex128", max_memory_mb=mem)
+        _, pl, _, pms, pres = p.evolve_timed(circ, record=False, keep=True)
+        amp = pres.amplitudes(idx)
+        out["complex128"] = {"qubits": n128, "gates": workloads.gate_count(circ.data["instructions"]),
+                             "launches": pl, "device_ms": pms,
+                             "max_err_vs_closed_form": float(np.abs(amp - exact).max()),
+                             "samples": int(idx.size)}
+        log(f"[precision] c128 n={n128}: {out['complex128']}")
+    return out
+
+
+def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, replays: int = 2, log=print) -> dict:
+    t0 = time.perf_counter()
+    circs = [workloads.batch_circuit(i, base_seed=base_seed) for i in range(count)]
+    gen_s = time.perf_counter() - t0
+    work = sum(workloads.gate_count(c.data["instructions"]) * float(1 << c.num_qubits) for c in circs)
+    t0 = time.perf_counter()
+    res = batch.run_batch(circs, shots=shots, base_seed=base_seed, autotune=True, replays=replays)
+    wall = time.perf_counter() - t0
+    lat = np.sort(res.latency_us)
+    out = {"circuits": count, "shots_each": shots, "device_ms": res.device_ms,
+           "circuits_per_s": count / (res.device_ms / 1e3), "gate_amp_per_s": work / (res.device_ms / 1e3),
+           "latency_us_p50": float(np.percentile(lat, 50)), "latency_us_p99": float(np.percentile(lat, 99)),
+           "block": res.block, "max_fused_qubits": res.max_fused_qubits, "lds_qubits": res.lds_qubits,
+           "grid": res.grid, "host_wall_s_incl_parse_fuse_tune": wall, "generate_s": gen_s}
+    log(f"[batch] {out}")
+    return out
+
+
+def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(22, 26), eval_n=(28, 30),
+                   seed: int = 5, epochs: int = 300, log=print) -> dict:
+    t0 = time.perf_counter()
+    X, y = planner.dataset(planner.training_circuits(train_circuits, nmin=train_n[0], nmax=train_n[1], dmin=20,
+                                                     dmax=80, seed=seed), seed=seed)
+    collect_s = time.perf_counter() - t0
+    perm = np.random.default_rng(seed).permutation(len(y))
+    cut = int(0.9 * len(y))
+    tr, te = perm[:cut], perm[cut:]
+    t0 = time.perf_counter()
+    w, curve = planner.train(X[tr], y[tr], epochs=epochs, batch=256, lr=2e-3, seed=seed)
+    train_s = time.perf_counter() - t0
+    pred = planner.predict(w, X[te])
+    test_mse = float(np.mean((pred - y[te]) ** 2))
+    base_mse = float(np.mean((y[te] - y[tr].mean()) ** 2))
+    rows = []
+    rng = np.random.default_rng(seed + 1)
+    for i in range(eval_circuits):
+        n = int(rng.integers(eval_n[0], eval_n[1] + 1))
+        d = int(rng.integers(20, 81))
+        c = workloads.random_su2(n=n, depth=d, seed=10_000 + i)
+        mem = (1 << n) * 16 * 3 // (1 << 20) + 4096
+        ref = _core.StatevectorSimulator(seed=1, max_memory_mb=mem)
+        _, _, _, _, rres = ref.evolve_timed(c, record=False, keep=True)
+        row = {"qubits": n, "depth": d}
+        for name, fusion, k in (("greedy_k4", "greedy", 4), ("greedy_k5", "greedy", 5), ("learned_k5", "learned", 5)):
+            if fusion == "learned":
+                planner.install(w)
+            try:
+                sim = _core.StatevectorSimulator(seed=1, precision="complex64", fusion=fusion, max_fused_qubits=k,
+                                                 max_memory_mb=mem)
+                sim.evolve_timed(c, record=False)  # warm-up
+                _, launches, _, ms, res = sim.evolve_timed(c, record=False, keep=True)
+                err, _ = res.compare(rres)
+                del res
+            finally:
+                planner.install(None)
+            row[name] = {"ms": ms, "launches": launches, "max_err": err}
+        rows.append(row)
+        log(f"[planner] {row}")
+    tot = {k: sum(r[k]["ms"] for r in rows) for k in ("greedy_k4", "greedy_k5", "learned_k5")}
+    out = {"train_samples": int(len(tr)), "test_samples": int(len(te)), "collect_s": collect_s, "train_s": train_s,
+           "final_train_loss": float(curve[-1]), "test_mse": test_mse, "test_mse_predict_mean": base_mse,
+           "eval": rows, "total_ms": tot,
+           "learned_vs_best_greedy": min(tot["greedy_k4"], tot["greedy_k5"]) / tot["learned_k5"],
+           "max_err": max(r[k]["max_err"] for r in rows for k in ("greedy_k4", "greedy_k5", "learned_k5"))}
+    log(f"[planner] summary {out['total_ms']} gain {out['learned_vs_best_greedy']:.3f}")
+    return out

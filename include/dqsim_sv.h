@@ -128,6 +128,9 @@ int dq_result_fidelity(const dq_result* r, const void* other_c128, uint64_t len,
 /* Bounded single-amplitude query (the reference's MpsResult.amplitude,
  * docs/simulators.md:122-123): re, im of amplitude `index`. */
 int dq_result_amplitude(const dq_result* r, uint64_t index, double* re_im);
+/* Bulk form of dq_result_amplitude: out[2i], out[2i+1] = amplitude
+ * indices[i] (no reference counterpart). */
+int dq_result_gather(const dq_result* r, const uint64_t* indices, uint64_t count, double* out);
 void dq_result_free(dq_result* r);
 
 /* Accuracy report (no reference counterpart): max |a-b| and the L2 norm of
