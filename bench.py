@@ -71,8 +71,8 @@ def main():
     ap.add_argument("--fusion", default="greedy")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-qubits", type=int, default=24)
-    ap.add_argument("--cpu-depth", type=int, default=4)
+    ap.add_argument("--cpu-qubits", type=int, default=26)
+    ap.add_argument("--cpu-depth", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--report", choices=["precision", "batch", "planner"], default=None,
