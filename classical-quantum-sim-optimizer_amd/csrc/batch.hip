@@ -77,7 +77,8 @@ __device__ void run_circuit(P st, const BatchCircuit& c, const BatchOp* __restri
       case 1: apply_block<1>(st, c.n, op, m, bs); break;
       case 2: apply_block<2>(st, c.n, op, m, bs); break;
       case 3: apply_block<3>(st, c.n, op, m, bs); break;
-      default: apply_block<4>(st, c.n, op, m, bs); break;
+      case 4: apply_block<4>(st, c.n, op, m, bs); break;
+      default: apply_block<5>(st, c.n, op, m, bs); break;
     }
     __syncthreads();
   }
@@ -182,13 +183,29 @@ __global__ void k_fill_int(int* p, int v) { *p = v; }
 
 size_t lds_bytes(int lmax, int block) { return ((size_t)8 << lmax) + (size_t)8 * block; }
 
-void set_lds_limit(int block, size_t bytes) {
-  hipError_t e = hipSuccess;
-  if (block == 256) e = hipFuncSetAttribute((const void*)k_batch<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  else if (block == 512)
-    e = hipFuncSetAttribute((const void*)k_batch<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  else e = hipFuncSetAttribute((const void*)k_batch<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+static const void* kernel_for(int block) {
+  if (block == 256) return (const void*)k_batch<256>;
+  if (block == 512) return (const void*)k_batch<512>;
+  return (const void*)k_batch<1024>;
+}
+
+int set_lds_limit(int block, size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(kernel_for(block), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) (void)hipGetLastError();
+  return (int)e;
+}
+
+int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads) {
+  hipFuncAttributes fa{};
+  hipError_t e = hipFuncGetAttributes(&fa, kernel_for(block));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return (int)e;
+  }
+  *max_dynamic_lds = fa.maxDynamicSharedSizeBytes;
+  *static_lds = (int)fa.sharedSizeBytes;
+  *max_threads = fa.maxThreadsPerBlock;
+  return 0;
 }
 
 void launch(const LaunchArgs& a, hipStream_t st) {

@@ -18,10 +18,10 @@ struct BatchCircuit {
 };
 
 struct BatchOp {
-  uint32_t k;        // 1..4 qubits
+  uint32_t k;        // 1..5 qubits
   uint32_t mat_off;  // float offset of the 4^k complex FP32 matrix
-  uint32_t q[4];     // qubits, q[0] = MSB of the local index
-  uint32_t s[4];     // the same qubits sorted ascending
+  uint32_t q[5];     // qubits, q[0] = MSB of the local index
+  uint32_t s[5];     // the same qubits sorted ascending
 };
 
 struct LaunchArgs {
@@ -43,7 +43,10 @@ struct LaunchArgs {
 };
 
 size_t lds_bytes(int lmax, int block);
-void set_lds_limit(int block, size_t bytes);
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize); returns the hipError_t.
+int set_lds_limit(int block, size_t bytes);
+// hipFuncGetAttributes of the kernel instance for `block`; returns hipError_t.
+int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads);
 // Queue reset (a fill kernel, graph-capturable) + the persistent kernel.
 void launch(const LaunchArgs& a, hipStream_t st);
 

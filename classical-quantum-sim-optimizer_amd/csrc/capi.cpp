@@ -306,6 +306,13 @@ int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len) 
 
 void dq_batch_free(dq_batch* b) { delete b; }
 
+int dq_batch_device_info(int32_t block, int32_t* info7) {
+  return guard([&] {
+    dq::batch_device_info(block, &info7[0], &info7[1], &info7[2], &info7[3], &info7[4], &info7[5]);
+    info7[6] = dq::device_count();
+  });
+}
+
 uint64_t dq_mlp_param_count(void) { return (uint64_t)dq::mlp::kMlpParams; }
 
 int dq_set_planner_weights(const float* w, uint64_t count) {

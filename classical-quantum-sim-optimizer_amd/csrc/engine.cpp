@@ -221,6 +221,7 @@ struct DevOp {
   int low = 0;
   int nctl = 0;
   float feat[mlp::kFeatures] = {};
+  std::vector<uint32_t> touched;  // qubits the launch touches (plan description)
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -468,8 +469,10 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
   while (i < end) {
     if (!ops[i].is_unitary() || fusion == FUSE_NONE) {
       plan.ops.push_back(lower(ops[i], plan, prec));
-      if (ops[i].is_unitary())
-        block_features(op_qubits(ops[i]), {(int)i}, ops, n, plan.ops.back().feat);
+      if (ops[i].is_unitary()) {
+        plan.ops.back().touched = op_qubits(ops[i]);
+        block_features(plan.ops.back().touched, {(int)i}, ops, n, plan.ops.back().feat);
+      }
       ++i;
       continue;
     }
@@ -487,6 +490,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
       if (blk.ops.size() == 1) {
         plan.ops.push_back(lower(ops[blk.ops[0]], plan, prec));
         std::copy(feat, feat + mlp::kFeatures, plan.ops.back().feat);
+        plan.ops.back().touched = blk.qubits;
         continue;
       }
       build_block_matrix(blk, ops);
@@ -509,6 +513,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         }
       }
       std::copy(feat, feat + mlp::kFeatures, d.feat);
+      d.touched = blk.qubits;
       plan.ops.push_back(std::move(d));
     }
     i = j;
@@ -1246,7 +1251,10 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
     js << (i ? ", " : "") << "{\"kind\": \"" << names[d.kind] << "\", \"k\": " << d.kq
        << ", \"gates\": " << d.gates << ", \"conditional\": " << (d.conds.empty() ? "false" : "true")
        << ", \"mfma\": " << (d.mfma ? "true" : "false") << ", \"entries\": " << d.pat.size()
-       << ", \"category\": " << d.category << ", \"bytes\": " << json_num(op_bytes(d, (int)p.n, prec)) << "}";
+       << ", \"category\": " << d.category << ", \"bytes\": " << json_num(op_bytes(d, (int)p.n, prec))
+       << ", \"qubits\": [";
+    for (size_t t = 0; t < d.touched.size(); ++t) js << (t ? ", " : "") << d.touched[t];
+    js << "]}";
   }
   js << "]}";
   return js.str();
@@ -1290,6 +1298,7 @@ BatchProgram build_batch(const ParsedBatch& pb, const std::vector<uint64_t>& sho
       bk::BatchOp op{};
       op.k = (uint32_t)b.qubits.size();
       op.mat_off = (uint32_t)prog.mats.size();
+      if (op.k > 5) throw ValueErr("batched blocks are limited to 5 qubits");
       for (uint32_t i = 0; i < op.k; ++i) op.q[i] = b.qubits[i];
       std::vector<uint32_t> srt(b.qubits.begin(), b.qubits.end());
       std::sort(srt.begin(), srt.end());
@@ -1364,6 +1373,20 @@ void upload_batch(BatchDevice& d, const BatchProgram& prog, int block, int lmax,
   a.block = block;
 }
 
+}  // namespace
+
+void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_dynamic, int* static_lds,
+                       int* max_threads, int* lmax) {
+  *max_lds_per_block = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
+  int l = 0;
+  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= *max_lds_per_block) ++l;
+  *lmax = l;
+  *set_rc = bk::set_lds_limit(block, bk::lds_bytes(l, block));
+  *max_dynamic = *static_lds = *max_threads = -1;
+  bk::func_info(block, max_dynamic, static_lds, max_threads);
+}
+
+namespace {
 int lmax_for(int block) {
   const int lim = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
   int l = 0;
@@ -1399,6 +1422,10 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
                                         const BatchOptions& opt) {
   const size_t nc = circuits.size();
   if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
+  // The persistent launcher stalled (no progress for 180 s) on its first GPU
+  // test and the cause is not yet identified, so it stays disabled.
+  throw RuntimeErr("the batched persistent launcher is disabled in this build (it stalled on its first GPU run; "
+                   "the cause is not identified yet)");
   auto out = std::make_unique<BatchOutcome>();
   ParsedBatch pb;
   pb.p.resize(nc);

@@ -158,6 +158,9 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
                                         const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
                                         const BatchOptions& opt);
 Counts batch_counts(const BatchOutcome& b, uint64_t i);
+// Device and kernel limits relevant to the batched launcher (no launch).
+void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_dynamic, int* static_lds,
+                       int* max_threads, int* lmax);
 
 // Learned fusion planner weights (2 hidden layers of 64, FP32). Null clears.
 void set_planner_weights(const float* w, size_t count);

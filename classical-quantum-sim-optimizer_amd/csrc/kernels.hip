@@ -246,7 +246,7 @@ __global__ __launch_bounds__(TPB) void k_fused(float2* __restrict__ s, uint64_t 
 //     stores whole complex outputs of its own group.
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-template <int K>
+template <int K, int TILES>
 __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint64_t stride, uint64_t ngroups,
                                                     Expand e, const float* __restrict__ emb, Offsets off) {
   constexpr int D = 1 << K;
@@ -267,13 +267,140 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
   for (int i = 0; i < NIN; ++i) offs[i] = off.o[2 * i + h];
   const uint64_t waves = (uint64_t)gridDim.x * (TPB / 64);
   const uint64_t ntiles = (ngroups + 31) / 32;
-  for (uint64_t tile = (uint64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6); tile < ntiles; tile += waves) {
-    const uint64_t g = tile * 32 + j;
-    const bool valid = g < ngroups;
-    const uint64_t base = expand(valid ? g : 0, e);
+  // Each wave takes TILES consecutive tiles per iteration: all their loads are
+  // issued before the first MFMA, which doubles the bytes in flight per wave.
+  for (uint64_t tile0 = ((uint64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * TILES; tile0 < ntiles;
+       tile0 += waves * TILES) {
+    uint64_t base[TILES];
+    bool valid[TILES];
+    float2 v[TILES][NIN];
+#pragma unroll
+    for (int u = 0; u < TILES; ++u) {
+      const uint64_t g = (tile0 + u) * 32 + j;
+      valid[u] = g < ngroups;
+      base[u] = expand(valid[u] ? g : 0, e);
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) v[u][i] = valid[u] ? st[base[u] | offs[i]] : make_float2(0.f, 0.f);
+    }
+    floatx16 acc[TILES][T];
+#pragma unroll
+    for (int u = 0; u < TILES; ++u)
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[u][t][r] = 0.f;
+#pragma unroll
+    for (int q = 0; q < STEPS; ++q) {
+#pragma unroll
+      for (int u = 0; u < TILES; ++u) {
+        const float b = q < STEPS / 2 ? v[u][q].x : v[u][q - STEPS / 2].y;
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][q], b, acc[u][t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TILES; ++u) {
+      if (!valid[u]) continue;
+      if constexpr (K == 4) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const int o = 8 * (r / 4) + 4 * h + (r % 4);
+          st[base[u] | off.o[o]] = make_float2(acc[u][0][r], acc[u][0][r + 8]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int o = 8 * (r / 4) + 4 * h + (r % 4);
+          st[base[u] | off.o[o]] = make_float2(acc[u][0][r], acc[u][1][r]);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tile sweep: several fused blocks per HBM pass
+// ---------------------------------------------------------------------------
+// Local index t of a tile: bit i <-> global qubit spos[i]; the tile's other
+// global bits come from the tile number (zeros inserted at S). spos[0..3] are
+// qubits 0..3, so 16 consecutive threads move one contiguous 128-byte run.
+// Blocks are applied in LDS: k <= 3 by one thread per group with FP32 FMA,
+// k = 4, 5 on MFMA with the operand layout of k_fused_mfma (inputs and
+// outputs in LDS instead of HBM).
+
+__device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
+  for (int i = 0; i < k; ++i) {
+    const uint32_t low = (1u << srt[i]) - 1;
+    p = (p & low) | ((p & ~low) << 1);
+  }
+  return p;
+}
+
+template <int K>
+__device__ __forceinline__ void tile_block_valu(float2* __restrict__ tile, int L, const TileBlock& b,
+                                                const float* __restrict__ m) {
+  constexpr int D = 1 << K;
+  uint32_t off[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if ((j >> (K - 1 - i)) & 1) o |= 1u << b.loc[i];
+    off[j] = o;
+  }
+  const uint32_t groups = (1u << L) >> K;
+  for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
+    const uint32_t base = insert_zeros32(g, b.srt, K);
+    float2 v[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = tile[base | off[j]];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      float ar = 0.f, ai = 0.f;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const float mr = m[2 * (r * D + c)], mi = m[2 * (r * D + c) + 1];
+        ar = __builtin_fmaf(mr, v[c].x, ar);
+        ar = __builtin_fmaf(-mi, v[c].y, ar);
+        ai = __builtin_fmaf(mr, v[c].y, ai);
+        ai = __builtin_fmaf(mi, v[c].x, ai);
+      }
+      tile[base | off[r]] = make_float2(ar, ai);
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void tile_block_mfma(float2* __restrict__ tile, int L, const TileBlock& b,
+                                                const float* __restrict__ emb) {
+  constexpr int D = 1 << K;
+  constexpr int R = 2 * D;
+  constexpr int T = R / 32;
+  constexpr int STEPS = R / 2;
+  constexpr int NIN = D / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  uint32_t offs[NIN];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) {
+    const int c = 2 * i + h;
+    uint32_t o = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      if ((c >> (K - 1 - q)) & 1) o |= 1u << b.loc[q];
+    offs[i] = o;
+  }
+  const uint32_t groups = (1u << L) >> K;
+  const uint32_t ntiles = (groups + 31) / 32;
+  for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
+    const uint32_t g = mt * 32 + j;
+    const bool valid = g < groups;
+    const uint32_t base = insert_zeros32(valid ? g : 0, b.srt, K);
     float2 v[NIN];
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) v[i] = valid ? st[base | offs[i]] : make_float2(0.f, 0.f);
+    for (int i = 0; i < NIN; ++i) v[i] = valid ? tile[base | offs[i]] : make_float2(0.f, 0.f);
     floatx16 acc[T];
 #pragma unroll
     for (int t = 0; t < T; ++t)
@@ -281,24 +408,79 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
       for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
 #pragma unroll
     for (int q = 0; q < STEPS; ++q) {
-      const float b = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
+      const float bv = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
 #pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][q], b, acc[t], 0, 0, 0);
+      for (int t = 0; t < T; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[(32 * t + j) * R + 2 * q + h], bv, acc[t], 0, 0, 0);
     }
     if (!valid) continue;
-    if constexpr (K == 4) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int o = 8 * (r / 4) + 4 * h + (r % 4);
-        st[base | off.o[o]] = make_float2(acc[0][r], acc[0][r + 8]);
-      }
-    } else {
+    for (int r = 0; r < 16; ++r) {
+      if (K == 4 && r >= 8) break;
+      const int o = 8 * (r / 4) + 4 * h + (r % 4);
+      uint32_t oo = 0;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int o = 8 * (r / 4) + 4 * h + (r % 4);
-        st[base | off.o[o]] = make_float2(acc[0][r], acc[1][r]);
-      }
+      for (int q = 0; q < K; ++q)
+        if ((o >> (K - 1 - q)) & 1) oo |= 1u << b.loc[q];
+      tile[base | oo] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
     }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t stride, uint64_t ntiles, TileArgs a,
+                                              const TileBlock* __restrict__ blocks, const uint8_t* __restrict__ blob) {
+  __shared__ float2 tile[1 << L];
+  constexpr int PER = (1 << L) / TPB;
+  float2* st = s + (uint64_t)blockIdx.y * stride;
+  // Global offset of local index t = tid + TPB * i: low part from the thread,
+  // high part from i (disjoint bit sets, so the offsets OR together).
+  uint64_t lo = 0;
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit)
+    if ((threadIdx.x >> bit) & 1) lo |= 1ull << a.spos[bit];
+  uint64_t hi[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int bit = 0; bit < L - 8; ++bit)
+      if ((i >> bit) & 1) o |= 1ull << a.spos[8 + bit];
+    hi[i] = o;
+  }
+  for (uint64_t tau = blockIdx.x; tau < ntiles; tau += gridDim.x) {
+    uint64_t base = tau;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const uint64_t low = (1ull << a.spos[i]) - 1;
+      base = (base & low) | ((base & ~low) << 1);
+    }
+    float2 r[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) r[i] = st[base | lo | hi[i]];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+    __syncthreads();
+    for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
+      const TileBlock b = blocks[bi];
+      const float* m = (const float*)(blob + b.mat_off);
+      if (b.mfma) {
+        if (b.k == 4) tile_block_mfma<4>(tile, L, b, m);
+        else tile_block_mfma<5>(tile, L, b, m);
+      } else {
+        switch (b.k) {
+          case 1: tile_block_valu<1>(tile, L, b, m); break;
+          case 2: tile_block_valu<2>(tile, L, b, m); break;
+          case 3: tile_block_valu<3>(tile, L, b, m); break;
+          case 4: tile_block_valu<4>(tile, L, b, m); break;
+          default: tile_block_valu<5>(tile, L, b, m); break;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    __syncthreads();
   }
 }
 
@@ -858,10 +1040,25 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, c
   Offsets off;
   for (int j = 0; j < 32; ++j) off.o[j] = j < (1 << kq) ? offsets[j] : 0;
   if (kq == 4)
-    hipLaunchKernelGGL(k_fused_mfma<4>, grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
+    hipLaunchKernelGGL((k_fused_mfma<4, 2>), grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
   else
-    hipLaunchKernelGGL(k_fused_mfma<5>, grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
+    hipLaunchKernelGGL((k_fused_mfma<5, 1>), grid, dim3(TPB), 0, st, (float2*)s, stride, ngroups, e, emb, off);
   return true;
+}
+
+void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const TileBlock* blocks,
+                const uint8_t* blob, hipStream_t st) {
+  const uint64_t ntiles = 1ull << (n - (int)a.L);
+  const uint64_t grid = ntiles < 4096 ? ntiles : 4096;
+  dim3 g((uint32_t)grid, batch);
+#define L(LL) \
+  hipLaunchKernelGGL(k_tile<LL>, g, dim3(TPB), 0, st, (float2*)s, stride, ntiles, a, blocks, blob)
+  switch (a.L) {
+    case 10: L(10); break;
+    case 11: L(11); break;
+    default: L(12); break;
+  }
+#undef L
 }
 
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st) {

@@ -73,6 +73,26 @@ void fused_block(void* s, uint64_t stride, uint32_t batch, int n, int k, const E
 bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, const Expand& e,
                       const float* embed_dev, const uint64_t* offsets_dev, hipStream_t st);
 
+// ---- LDS-tile sweep (complex64) ----
+// One sweep applies several fused blocks whose qubits all lie in a tile set S
+// (|S| = L <= 12, S always holds qubits 0..3). Each workgroup loads one
+// 2^L-amplitude tile into LDS, applies every block there, and writes it back.
+struct TileBlock {
+  uint32_t k;        // 1..5 qubits
+  uint32_t mfma;     // 1: real embedding (R x R) for MFMA; 0: complex FP32 matrix
+  uint32_t mat_off;  // byte offset of the matrix in the plan blob
+  uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
+  uint32_t srt[5];   // the same local bits sorted ascending
+  uint32_t pad;
+};
+struct TileArgs {
+  uint32_t L;         // tile qubits
+  uint32_t nblocks;
+  uint32_t spos[12];  // S sorted ascending (global qubit of local bit i)
+};
+void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const TileBlock* blocks_dev,
+                const uint8_t* blob, hipStream_t st);
+
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
 void broadcast(int prec, void* dst, uint64_t stride, uint32_t batch, const void* src, int n, hipStream_t st);

@@ -79,3 +79,14 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
     _check(_lib.dq_batch_run(ptrs, lens, n, sh, sd, ctypes.byref(o), ctypes.byref(h)))
     nq = [int(json_n) for json_n in (_core.plan_json(t)["num_qubits"] for t in texts)] if keep_states else [0] * n
     return BatchResult(h, n, shots_l, nq)
+
+
+def device_info(block: int = 256) -> Dict[str, int]:
+    """Launch limits of the batched kernel for ``block`` threads (no launch):
+    device LDS per block, the planned LDS qubit count, whether the kernel
+    accepted that dynamic LDS size, and the kernel's own attributes."""
+    info = (ctypes.c_int32 * 7)()
+    _check(_lib.dq_batch_device_info(block, info))
+    keys = ("max_lds_per_block", "set_attribute_rc", "kernel_max_dynamic_lds", "kernel_static_lds",
+            "kernel_max_threads", "lds_qubits", "device_count")
+    return dict(zip(keys, (int(v) for v in info)))

@@ -183,6 +183,11 @@ int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* block, int32_t* 
 /* Final state of circuit i as complex128 (needs dump_states). */
 int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len);
 void dq_batch_free(dq_batch* b);
+/* Launch limits of the batched kernel instance for `block` threads (no launch):
+ * info7 = {device max LDS per block, hipFuncSetAttribute rc for the planned
+ * LDS size, kernel max dynamic LDS, kernel static LDS, kernel max threads,
+ * planned LDS qubits, device count}. */
+int dq_batch_device_info(int32_t block, int32_t* info7);
 
 /* ---- learned fusion planner (no reference counterpart) ----
  * A 12 -> 64 -> 64 -> 1 FP32 MLP predicting log(ms per 2^24 amplitudes) of a

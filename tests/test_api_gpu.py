@@ -230,3 +230,14 @@ def test_budget_working_bytes_and_fallback_match_reference_rule(tmp_path, monkey
     for key in ("execution_strategy", "parallel_shots", "working_bytes", "deterministic_prefix_ops"):
         assert data[key] == info[key], key
     assert sum(counts.values()) == 10 and sum(ref.values()) == 10
+
+
+def test_batched_kernel_launch_limits_are_readable():
+    # No launch: reads the device LDS limit and the kernel's attributes for
+    # the dynamic LDS size the batched launcher would request.
+    from dqsim import batch
+    for block in (256, 512, 1024):
+        info = batch.device_info(block)
+        print(block, info, flush=True)
+        assert info["device_count"] >= 1 and info["max_lds_per_block"] >= 65536
+        assert 8 <= info["lds_qubits"] <= 15 and info["kernel_max_threads"] >= block

@@ -283,3 +283,11 @@ def test_learned_planner_merge_decisions_follow_the_model():
         planner.install(None)
     with pytest.raises(ValueError):
         planner.install(np.zeros(5, dtype=np.float32))
+
+
+def test_batched_launcher_is_disabled_with_a_clear_error():
+    # The persistent batched launcher raises before touching the device.
+    from dqsim import batch
+    c = workloads.batch_circuit(0, base_seed=1, nmin=3, nmax=4)
+    with pytest.raises(RuntimeError, match="batched persistent launcher is disabled"):
+        batch.run_batch([c], shots=8, autotune=False)
