@@ -21,9 +21,10 @@ sys.path.insert(0, os.path.join(ROOT, "classical-quantum-sim-optimizer_amd"))
 sys.path.insert(0, ROOT)
 
 KIND_NAMES = {0: "one", 1: "x", 2: "swap", 3: "diag", 4: "dense", 5: "fused_valu", 6: "measure", 7: "reset",
-              8: "fused_mfma"}
+              8: "fused_mfma", 9: "tile"}
 KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag": "k_diag", "dense": "k_dense",
-                "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse"}
+                "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse",
+                "tile": "k_tile"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -69,6 +70,7 @@ def main():
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--max-fused", type=int, default=4)
     ap.add_argument("--fusion", default="greedy")
+    ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=26)
@@ -108,8 +110,8 @@ def main():
     gates = workloads.gate_count(circ.data["instructions"])
     mem_mb = (1 << n) * 16 * 2 // (1 << 20) + 4096
     fused = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion=args.fusion,
-                                       max_fused_qubits=args.max_fused, max_memory_mb=mem_mb)
-    log(f"[bench] n={n} depth={depth} gates={gates} fusion={args.fusion} k<={args.max_fused}")
+                                       max_fused_qubits=args.max_fused, max_memory_mb=mem_mb, tile_qubits=args.tile)
+    log(f"[bench] n={n} depth={depth} gates={gates} fusion={args.fusion} k<={args.max_fused} tile={args.tile}")
     for w in range(args.warmup):
         t = time.perf_counter()
         fused.evolve_timed(circ, repeats=1, record=False)
@@ -176,7 +178,7 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, complex64, {args.fusion} fusion "
-                               f"k<={args.max_fused}", "qubits": n, "depth": depth, "gates": gates,
+                               f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},

@@ -73,6 +73,7 @@ void dq_sv_opts_default(dq_sv_opts* o) {
   o->precision = DQ_PREC_C128;
   o->fusion = DQ_FUSE_NONE;
   o->max_fused_qubits = 4;
+  o->tile_qubits = dq::kTileDefault;
 }
 
 int dq_sv_create(const dq_sv_opts* o, dq_sv** out) {
@@ -87,6 +88,8 @@ int dq_sv_create(const dq_sv_opts* o, dq_sv** out) {
       throw dq::ValueErr("precision must be complex128 or complex64");
     if (o->fusion < DQ_FUSE_NONE || o->fusion > DQ_FUSE_LEARNED) throw dq::ValueErr("unknown fusion mode");
     if (o->max_fused_qubits < 1 || o->max_fused_qubits > 5) throw dq::ValueErr("max_fused_qubits must be in 1..5");
+    if (o->tile_qubits != 0 && (o->tile_qubits < dq::kTileMin || o->tile_qubits > dq::kTileMax))
+      throw dq::ValueErr("tile_qubits must be 0 or in 10..12");
     auto* s = new dq_sv;
     s->o.has_seed = o->has_seed != 0;
     s->o.seed = o->seed;
@@ -98,6 +101,7 @@ int dq_sv_create(const dq_sv_opts* o, dq_sv** out) {
     s->o.prec = o->precision;
     s->o.fusion = o->fusion;
     s->o.max_fused = o->max_fused_qubits;
+    s->o.tile = o->tile_qubits;
     *out = s;
   });
 }
@@ -199,9 +203,12 @@ void dq_result_free(dq_result* r) { delete r; }
 
 const char* dq_last_error(void) { return g_err.c_str(); }
 
-int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused, char** out) {
+int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
+                 int32_t tile_qubits, char** out) {
   return guard([&] {
-    std::string s = dq::plan_description(json, len, precision, fusion, max_fused);
+    if (tile_qubits != 0 && (tile_qubits < dq::kTileMin || tile_qubits > dq::kTileMax))
+      throw dq::ValueErr("tile_qubits must be 0 or in 10..12");
+    std::string s = dq::plan_description(json, len, precision, fusion, max_fused, tile_qubits);
     char* p = (char*)std::malloc(s.size() + 1);
     if (!p) throw std::bad_alloc();
     std::memcpy(p, s.c_str(), s.size() + 1);

@@ -199,7 +199,7 @@ uint64_t random_seed() {
 // ---------------------------------------------------------------------------
 // Device plan
 // ---------------------------------------------------------------------------
-enum DKind { D_ONE, D_X, D_SWAP, D_DIAG, D_DENSE, D_FUSED, D_MEAS, D_RESET, D_NOOP };
+enum DKind { D_ONE, D_X, D_SWAP, D_DIAG, D_DENSE, D_FUSED, D_MEAS, D_RESET, D_NOOP, D_TILE };
 
 struct DevOp {
   DKind kind = D_NOOP;
@@ -222,6 +222,12 @@ struct DevOp {
   int nctl = 0;
   float feat[mlp::kFeatures] = {};
   std::vector<uint32_t> touched;  // qubits the launch touches (plan description)
+  std::vector<int> src;           // compiled ops the launch applies, in order
+  // D_TILE: tile qubits and the blocks applied in LDS (TileBlock array in the blob).
+  k::TileArgs targs{};
+  size_t tile_off = 0;
+  std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
+  std::vector<int> tile_mfma;
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -463,7 +469,159 @@ std::vector<float> g_planner;  // learned planner weights (empty = greedy)
 
 double planner_score(const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& ops, int n);
 
-Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n) {
+// One fused block as its own launch.
+void emit_block(Plan& plan, FusedBlock& blk, const std::vector<Op>& ops, int prec, int n) {
+  float feat[mlp::kFeatures];
+  block_features(blk.qubits, blk.ops, ops, n, feat);
+  if (blk.ops.size() == 1) {
+    plan.ops.push_back(lower(ops[blk.ops[0]], plan, prec));
+    std::copy(feat, feat + mlp::kFeatures, plan.ops.back().feat);
+    plan.ops.back().touched = blk.qubits;
+    plan.ops.back().src = blk.ops;
+    return;
+  }
+  if (blk.mat.empty()) build_block_matrix(blk, ops);
+  DevOp d;
+  d.category = 1;
+  d.gates = (int)blk.ops.size();
+  if (blk.diagonal) {
+    std::vector<cplx> dg;
+    const size_t dim = (size_t)1 << blk.qubits.size();
+    for (size_t r = 0; r < dim; ++r) dg.push_back(blk.mat[r * dim + r]);
+    set_diag(d, blk.qubits, dg);
+  } else {
+    d.kind = prec == 8 ? D_FUSED : D_DENSE;
+    set_dense_offsets(d, blk.qubits);
+    push_matrix(plan, d, blk.mat, prec);
+    if (prec == 8 && d.kq >= 4) {
+      std::vector<float> emb = real_embedding(blk.mat, d.kq);
+      d.emb_off = plan.push(emb.data(), emb.size() * sizeof(float));
+      d.mfma = true;
+    }
+  }
+  std::copy(feat, feat + mlp::kFeatures, d.feat);
+  d.touched = blk.qubits;
+  d.src = blk.ops;
+  plan.ops.push_back(std::move(d));
+}
+
+uint64_t qubit_mask(const std::vector<uint32_t>& q) {
+  uint64_t m = 0;
+  for (uint32_t x : q) m |= 1ull << x;
+  return m;
+}
+
+// Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
+// read and write of the state for all of them.
+void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int L, const std::vector<Op>& ops) {
+  DevOp d;
+  d.kind = D_TILE;
+  d.category = 1;
+  d.gates = 0;
+  // Pad the tile to exactly L qubits with the lowest free qubits.
+  for (uint32_t q = 0; __builtin_popcountll(smask) < L; ++q) smask |= 1ull << q;
+  std::vector<int> local(64, -1);
+  int nl = 0;
+  for (uint32_t q = 0; q < 64; ++q)
+    if ((smask >> q) & 1) {
+      d.targs.spos[nl] = q;
+      local[q] = nl++;
+      d.touched.push_back(q);
+    }
+  d.targs.L = (uint32_t)L;
+  d.targs.nblocks = (uint32_t)blocks.size();
+  std::vector<k::TileBlock> tb;
+  for (FusedBlock* b : blocks) {
+    if (b->mat.empty()) build_block_matrix(*b, ops);
+    k::TileBlock t{};
+    t.k = (uint32_t)b->qubits.size();
+    t.mfma = t.k >= 4;
+    if (t.mfma) {
+      std::vector<float> emb = real_embedding(b->mat, (int)t.k);
+      t.mat_off = (uint32_t)plan.push(emb.data(), emb.size() * sizeof(float));
+    } else {
+      std::vector<float> v;
+      for (auto& c : b->mat) {
+        v.push_back((float)c.re);
+        v.push_back((float)c.im);
+      }
+      t.mat_off = (uint32_t)plan.push(v.data(), v.size() * sizeof(float));
+    }
+    for (uint32_t i = 0; i < t.k; ++i) {
+      t.loc[i] = (uint32_t)local[b->qubits[i]];
+      t.srt[i] = t.loc[i];
+    }
+    std::sort(t.srt, t.srt + t.k);
+    tb.push_back(t);
+    d.gates += (int)b->ops.size();
+    d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
+    d.tile_blocks.push_back(b->qubits);
+    d.tile_mfma.push_back((int)t.mfma);
+  }
+  d.tile_off = plan.push(tb.data(), tb.size() * sizeof(k::TileBlock));
+  d.kq = L;
+  d.low = 0;
+  plan.ops.push_back(std::move(d));
+}
+
+// Groups fused blocks into tile sweeps. Starting from the first remaining
+// block, later blocks join the sweep while the union of tile qubits (always
+// including qubits 0..3, so a tile row is a 128-byte run) stays within L and
+// they commute with every block passed over (no shared qubit); a passed-over
+// block keeps its place for the next sweep. A sweep of one block is emitted
+// as an ordinary launch.
+void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<Op>& ops, int prec, int n, int L) {
+  constexpr size_t kWindow = 512;
+  std::vector<int> rem(blocks.size());
+  for (size_t b = 0; b < blocks.size(); ++b) rem[b] = (int)b;
+  const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  auto tileable = [&](const FusedBlock& b) { return !b.qubits.empty() && b.qubits.size() <= 5; };
+  size_t head = 0;
+  std::vector<char> used(blocks.size(), 0);
+  while (head < rem.size()) {
+    FusedBlock& first = blocks[rem[head]];
+    if (used[rem[head]]) {
+      ++head;
+      continue;
+    }
+    if (!tileable(first)) {
+      used[rem[head]] = 1;
+      emit_block(plan, first, ops, prec, n);
+      ++head;
+      continue;
+    }
+    uint64_t smask = 0xFull | qubit_mask(first.qubits);
+    uint64_t blocked = 0;
+    std::vector<int> take{rem[head]};
+    size_t seen = 0;
+    for (size_t r = head + 1; r < rem.size() && seen < kWindow; ++r) {
+      const int bi = rem[r];
+      if (used[bi]) continue;
+      ++seen;
+      const FusedBlock& b = blocks[bi];
+      const uint64_t m = qubit_mask(b.qubits);
+      if (tileable(b) && !(m & blocked) && __builtin_popcountll(smask | m) <= L) {
+        smask |= m;
+        take.push_back(bi);
+      } else {
+        blocked |= m;
+        if ((blocked & all) == all) break;
+      }
+    }
+    for (int bi : take) used[bi] = 1;
+    if (take.size() == 1) {
+      emit_block(plan, first, ops, prec, n);
+    } else {
+      std::vector<FusedBlock*> bp;
+      for (int bi : take) bp.push_back(&blocks[bi]);
+      emit_tile(plan, bp, smask, L, ops);
+    }
+    ++head;
+  }
+}
+
+Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n,
+                int tile) {
   Plan plan;
   size_t i = begin;
   while (i < end) {
@@ -473,6 +631,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         plan.ops.back().touched = op_qubits(ops[i]);
         block_features(plan.ops.back().touched, {(int)i}, ops, n, plan.ops.back().feat);
       }
+      plan.ops.back().src = {(int)i};
       ++i;
       continue;
     }
@@ -484,37 +643,10 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         return planner_score(a, b, o, n);
       };
     std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
-    for (FusedBlock& blk : blocks) {
-      float feat[mlp::kFeatures];
-      block_features(blk.qubits, blk.ops, ops, n, feat);
-      if (blk.ops.size() == 1) {
-        plan.ops.push_back(lower(ops[blk.ops[0]], plan, prec));
-        std::copy(feat, feat + mlp::kFeatures, plan.ops.back().feat);
-        plan.ops.back().touched = blk.qubits;
-        continue;
-      }
-      build_block_matrix(blk, ops);
-      DevOp d;
-      d.category = 1;
-      d.gates = (int)blk.ops.size();
-      if (blk.diagonal) {
-        std::vector<cplx> dg;
-        const size_t dim = (size_t)1 << blk.qubits.size();
-        for (size_t r = 0; r < dim; ++r) dg.push_back(blk.mat[r * dim + r]);
-        set_diag(d, blk.qubits, dg);
-      } else {
-        d.kind = prec == 8 ? D_FUSED : D_DENSE;
-        set_dense_offsets(d, blk.qubits);
-        push_matrix(plan, d, blk.mat, prec);
-        if (prec == 8 && d.kq >= 4) {
-          std::vector<float> emb = real_embedding(blk.mat, d.kq);
-          d.emb_off = plan.push(emb.data(), emb.size() * sizeof(float));
-          d.mfma = true;
-        }
-      }
-      std::copy(feat, feat + mlp::kFeatures, d.feat);
-      d.touched = blk.qubits;
-      plan.ops.push_back(std::move(d));
+    if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
+      emit_sweeps(plan, blocks, ops, prec, n, tile);
+    } else {
+      for (FusedBlock& blk : blocks) emit_block(plan, blk, ops, prec, n);
     }
     i = j;
   }
@@ -629,6 +761,9 @@ struct Exec {
         k::xgate(prec, s, stride, nb, n, e, d.qubit, xf, st);
         break;
       }
+      case D_TILE:
+        k::tile_sweep(s, stride, nb, n, d.targs, (const k::TileBlock*)(cbase + d.tile_off), cbase, st);
+        break;
       case D_NOOP: break;
     }
   }
@@ -642,7 +777,8 @@ double op_bytes(const DevOp& d, int n, int prec) {
     case D_SWAP: return 2.0 * prec * len / std::ldexp(1.0, 1 + d.nctl);
     case D_DIAG: return 2.0 * prec * len / std::ldexp(1.0, d.kq) * (double)d.pat.size();
     case D_DENSE:
-    case D_FUSED: return 2.0 * prec * len;
+    case D_FUSED:
+    case D_TILE: return 2.0 * prec * len;
     case D_MEAS:
     case D_RESET: return 3.0 * prec * len;
     default: return 0.0;
@@ -657,6 +793,7 @@ int op_kind_code(const DevOp& d) {
     case D_DIAG: return 3;
     case D_DENSE: return 4;
     case D_FUSED: return d.mfma ? 8 : 5;
+    case D_TILE: return 9;
     case D_MEAS: return 6;
     case D_RESET: return 7;
     default: return -1;
@@ -833,7 +970,7 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   p.ops = compile(p.c.instructions);
   const uint64_t seed = o.has_seed ? o.seed : random_seed();
   Stream st;
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n);
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.tile);
   plan.upload(st.s);
   Exec ex(o.prec, (int)p.n, p.nc, 1, st.s);
   ex.cbase = plan.base();
@@ -953,7 +1090,7 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
     const uint64_t cb = cdf_bytes(m);
     if (worker_bytes <= UINT64_MAX - cb && worker_bytes + cb <= o.budget) {
       Stream st;
-      Plan plan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n);
+      Plan plan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n, o.tile);
       plan.upload(st.s);
       Exec ex(o.prec, n, nc, 1, st.s);
       ex.cbase = plan.base();
@@ -988,7 +1125,7 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
   workers = std::max<uint64_t>(workers, 1);
 
   Stream st;
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, n);
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, n, o.tile);
   plan.upload(st.s);
   // Fused runs never cross a non-unitary op, so the plan's leading unitary
   // launches cover exactly ops[0, prefix_end).
@@ -1183,7 +1320,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
   const size_t pe = prefix_end_of(p.ops);
   const int n = (int)p.n;
   Stream st;
-  Plan plan = build_plan(p.ops, 0, pe, o.prec, o.fusion, o.max_fused, n);
+  Plan plan = build_plan(p.ops, 0, pe, o.prec, o.fusion, o.max_fused, n, o.tile);
   plan.upload(st.s);
   Exec ex(o.prec, n, p.nc, 1, st.s);
   ex.cbase = plan.base();
@@ -1237,12 +1374,12 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
 // ---------------------------------------------------------------------------
 // Plan description (host only)
 // ---------------------------------------------------------------------------
-std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused) {
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile) {
   Parsed p = read_and_validate(json, len);
   p.ops = compile(p.c.instructions);
   const size_t pe = prefix_end_of(p.ops);
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), prec, fusion, max_fused, (int)p.n);
-  static const char* names[] = {"one", "x", "swap", "diagonal", "dense", "fused", "measure", "reset", "noop"};
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), prec, fusion, max_fused, (int)p.n, tile);
+  static const char* names[] = {"one", "x", "swap", "diagonal", "dense", "fused", "measure", "reset", "noop", "tile"};
   std::ostringstream js;
   js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
      << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions) << ", \"launches\": [";
@@ -1254,7 +1391,19 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
        << ", \"category\": " << d.category << ", \"bytes\": " << json_num(op_bytes(d, (int)p.n, prec))
        << ", \"qubits\": [";
     for (size_t t = 0; t < d.touched.size(); ++t) js << (t ? ", " : "") << d.touched[t];
-    js << "]}";
+    js << "], \"ops\": [";
+    for (size_t t = 0; t < d.src.size(); ++t) js << (t ? ", " : "") << d.src[t];
+    js << "]";
+    if (d.kind == D_TILE) {
+      js << ", \"blocks\": [";
+      for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
+        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";
+        for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
+        js << "]}";
+      }
+      js << "]";
+    }
+    js << "}";
   }
   js << "]}";
   return js.str();

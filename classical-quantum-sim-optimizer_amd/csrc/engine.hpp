@@ -35,6 +35,9 @@ struct DeviceErr : std::exception {
 
 enum Fusion { FUSE_NONE = 0, FUSE_GREEDY = 1, FUSE_LEARNED = 2 };
 
+// LDS tile sweeps (complex64 with fusion): tile sizes the kernel is built for.
+constexpr int kTileMin = 10, kTileMax = 12, kTileDefault = 12;
+
 struct Options {
   bool has_seed = false;
   uint64_t seed = 0;
@@ -46,6 +49,7 @@ struct Options {
   int prec = 16;  // bytes per amplitude: 16 = complex128, 8 = complex64
   int fusion = FUSE_NONE;
   int max_fused = 4;
+  int tile = kTileDefault;  // LDS tile qubits for complex64 fused sweeps; 0 = off
 };
 
 struct SimProfile {
@@ -129,7 +133,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
 void result_compare(const Result& a, const Result& b, double* max_abs, double* l2);
 
 // Compile-only plan description (no device use), as JSON.
-std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused);
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile);
 
 int device_count();
 

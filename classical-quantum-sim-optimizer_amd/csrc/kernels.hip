@@ -464,17 +464,13 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
       const TileBlock b = blocks[bi];
       const float* m = (const float*)(blob + b.mat_off);
-      if (b.mfma) {
-        if (b.k == 4) tile_block_mfma<4>(tile, L, b, m);
-        else tile_block_mfma<5>(tile, L, b, m);
-      } else {
-        switch (b.k) {
-          case 1: tile_block_valu<1>(tile, L, b, m); break;
-          case 2: tile_block_valu<2>(tile, L, b, m); break;
-          case 3: tile_block_valu<3>(tile, L, b, m); break;
-          case 4: tile_block_valu<4>(tile, L, b, m); break;
-          default: tile_block_valu<5>(tile, L, b, m); break;
-        }
+      // k >= 4 blocks carry the real embedding (MFMA), k <= 3 the complex matrix.
+      switch (b.k) {
+        case 1: tile_block_valu<1>(tile, L, b, m); break;
+        case 2: tile_block_valu<2>(tile, L, b, m); break;
+        case 3: tile_block_valu<3>(tile, L, b, m); break;
+        case 4: tile_block_mfma<4>(tile, L, b, m); break;
+        default: tile_block_mfma<5>(tile, L, b, m); break;
       }
       __syncthreads();
     }

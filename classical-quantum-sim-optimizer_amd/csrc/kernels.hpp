@@ -79,7 +79,7 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, co
 // 2^L-amplitude tile into LDS, applies every block there, and writes it back.
 struct TileBlock {
   uint32_t k;        // 1..5 qubits
-  uint32_t mfma;     // 1: real embedding (R x R) for MFMA; 0: complex FP32 matrix
+  uint32_t mfma;     // k >= 4: real embedding (R x R) for MFMA; k <= 3: complex FP32 matrix
   uint32_t mat_off;  // byte offset of the matrix in the plan blob
   uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
   uint32_t srt[5];   // the same local bits sorted ascending

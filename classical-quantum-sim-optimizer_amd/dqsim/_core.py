@@ -53,6 +53,7 @@ class _Opts(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("fusion", ctypes.c_int32),
         ("max_fused_qubits", ctypes.c_int32),
+        ("tile_qubits", ctypes.c_int32),
     ]
 
 
@@ -123,7 +124,7 @@ def _load():
         "dq_result_gather": (ctypes.c_int, [P, P, U64, P]),
         "dq_result_compare": (ctypes.c_int, [P, P, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_last_error": (ctypes.c_char_p, []),
-        "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
+        "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
         "dq_string_free": (None, [P]),
         "dq_device_count": (ctypes.c_int, []),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
@@ -356,7 +357,8 @@ class StatevectorSimulator:
     """Dense statevector simulation on the GPU."""
 
     def __init__(self, seed=None, profile=False, *, max_memory_mb=1024, max_parallel_shots=None,
-                 sample_terminal=True, precision="complex128", fusion="none", max_fused_qubits=4):
+                 sample_terminal=True, precision="complex128", fusion="none", max_fused_qubits=4,
+                 tile_qubits=12):
         o = _Opts()
         _lib.dq_sv_opts_default(ctypes.byref(o))
         seed = _opt_u64(seed, "seed")
@@ -375,6 +377,7 @@ class StatevectorSimulator:
         o.precision = _PRECISIONS[precision]
         o.fusion = _FUSIONS[fusion]
         o.max_fused_qubits = min(_as_u64(max_fused_qubits, "max_fused_qubits"), 99)
+        o.tile_qubits = min(_as_u64(tile_qubits, "tile_qubits"), 99)
         h = ctypes.c_void_p()
         _check(_lib.dq_sv_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -504,14 +507,14 @@ def simulate_distributed_shots(distributed, mode="p_block", shots=1000, seed=Non
 # ---------------------------------------------------------------------------
 # Host-only helpers
 # ---------------------------------------------------------------------------
-def plan_json(circuit_or_json, precision="complex128", fusion="none", max_fused_qubits=4) -> dict:
+def plan_json(circuit_or_json, precision="complex128", fusion="none", max_fused_qubits=4, tile_qubits=12) -> dict:
     """Launch plan of a circuit (parse, validate, compile, fuse; no GPU)."""
     data = circuit_or_json if isinstance(circuit_or_json, (bytes, str)) else _circuit_json(circuit_or_json)
     if isinstance(data, str):
         data = data.encode("utf-8")
     out = ctypes.c_void_p()
     _check(_lib.dq_plan_json(data, len(data), _PRECISIONS[precision], _FUSIONS[fusion], int(max_fused_qubits),
-                             ctypes.byref(out)))
+                             int(tile_qubits), ctypes.byref(out)))
     try:
         return json.loads(ctypes.string_at(out).decode("utf-8"))
     finally:

@@ -83,7 +83,7 @@ def install(w: Optional[np.ndarray]) -> None:
 
 def launch_records(circuit, max_fused_qubits: int, precision: str = "complex64", fusion: str = "greedy"):
     sim = _core.StatevectorSimulator(seed=1, precision=precision, fusion=fusion, max_fused_qubits=max_fused_qubits,
-                                     max_memory_mb=1 << 20)
+                                     max_memory_mb=1 << 20, tile_qubits=0)
     rows, launches, gates, total_ms, _ = sim.evolve_timed(circuit, repeats=1, record=True)
     return rows, total_ms
 

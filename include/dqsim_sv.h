@@ -65,6 +65,9 @@ typedef struct dq_sv_opts {
   int32_t precision;        /* DQ_PREC_C128 (default) or DQ_PREC_C64 */
   int32_t fusion;           /* DQ_FUSE_NONE (default), _GREEDY, _LEARNED */
   int32_t max_fused_qubits; /* 1..5, default 4 */
+  int32_t tile_qubits;      /* complex64 + fusion: fused blocks are grouped into
+                               sweeps over 2^tile_qubits-amplitude LDS tiles
+                               (10..12, default 12); 0 = one launch per block */
 } dq_sv_opts;
 
 /* SimulationProfile fields (simulator.rs:43-59). */
@@ -83,7 +86,8 @@ typedef int (*dq_count_cb)(void* user, const char* bits, size_t len, uint64_t co
 
 /* Per-launch device timing (engine-recorded). */
 typedef struct dq_op_timing {
-  int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA) */
+  int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA),
+                    9 tile sweep (k = tile qubits, gates = ops applied) */
   int32_t k;     /* qubits touched */
   int32_t low;   /* lowest qubit touched */
   int32_t gates; /* compiled ops folded into the launch */
@@ -145,7 +149,7 @@ const char* dq_last_error(void);
 /* Parse, validate, compile and fuse without touching the GPU; writes a JSON
  * description of the launch plan. Free it with dq_string_free. */
 int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
-                 char** out);
+                 int32_t tile_qubits, char** out);
 void dq_string_free(char* s);
 
 /* Number of visible GPUs (0 on a host without one). */

@@ -1,0 +1,97 @@
+"""LDS tile sweeps (complex64 + fusion): fused blocks grouped into sweeps over
+2^L-amplitude tiles held in LDS, against the FP64 oracle (max amplitude error
+<= 1e-5) and against the one-launch-per-block fused plan."""
+
+import numpy as np
+import pytest
+
+import dqsim
+from dqsim import workloads
+from dqsim.workloads import Circuit
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_circuit(n, depth, seed):
+    """U3 layers plus seeded multi-qubit gates of every lowering kind (dense,
+    controlled, diagonal, swap, 3-5 qubit controls)."""
+    rng = np.random.default_rng(seed)
+    ins = []
+    for _ in range(depth):
+        for q in range(n):
+            t, p, l = rng.uniform(-np.pi, np.pi, 3)
+            ins.append({"kind": "u3", "qubit": q, "theta": float(t), "phi": float(p), "lam": float(l)})
+        for _ in range(n // 2):
+            kind = rng.choice(["cx", "cz", "swap", "rzz", "cp", "ccx", "cswap", "c3x", "c4x", "ch"])
+            q = [int(x) for x in rng.choice(n, 5, replace=False)]
+            if kind in ("cx", "cz", "ch"):
+                ins.append({"kind": kind, "control": q[0], "target": q[1]})
+            elif kind == "cp":
+                ins.append({"kind": kind, "control": q[0], "target": q[1], "lam": float(rng.uniform(0, 3))})
+            elif kind == "swap":
+                ins.append({"kind": kind, "a": q[0], "b": q[1]})
+            elif kind == "rzz":
+                ins.append({"kind": kind, "a": q[0], "b": q[1], "theta": float(rng.uniform(0, 3))})
+            elif kind == "ccx":
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "target": q[2]})
+            elif kind == "cswap":
+                ins.append({"kind": kind, "control": q[0], "target1": q[1], "target2": q[2]})
+            elif kind == "c3x":
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "control3": q[2], "target": q[3]})
+            else:
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "control3": q[2], "control4": q[3],
+                            "target": q[4]})
+    return Circuit(n, ins)
+
+
+CASES = [
+    ("su2", 16, 10, 2), ("su2", 16, 10, 3), ("su2", 16, 10, 4), ("su2", 16, 10, 5),
+    ("su2", 17, 11, 4), ("su2", 18, 12, 4), ("su2", 20, 12, 5), ("su2", 20, 12, 3),
+    ("qft", 16, 12, 4), ("mixed", 17, 12, 3), ("mixed", 18, 11, 5), ("mixed", 16, 10, 4),
+]
+
+
+def build(kind, n):
+    if kind == "su2":
+        return workloads.random_su2(n=n, depth=12, seed=n)
+    if kind == "qft":
+        return workloads.qft(n, seed=3, measure=False)[0]
+    return mixed_circuit(n, 6, seed=n)
+
+
+@pytest.mark.parametrize("kind,n,L,k", CASES, ids=lambda v: str(v))
+def test_tile_sweeps_match_oracle(kind, n, L, k):
+    print(f"[tiles] {kind} n={n} L={L} k={k}", flush=True)
+    circ = build(kind, n)
+    plan = dqsim._core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=L)
+    assert any(x["kind"] == "tile" for x in plan["launches"])
+    want = oracle.simulate(circ.data, 1)[0]
+    sim = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                     tile_qubits=L)
+    got = sim.simulate(circ).statevector
+    assert np.abs(got - want).max() < 1e-5
+    flat = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                      tile_qubits=0)
+    assert np.abs(flat.simulate(circ).statevector - got).max() < 1e-5
+
+
+def test_tile_sweeps_with_trajectory_batches():
+    # Mid-circuit measurements split the circuit into fused runs; every slot
+    # of a trajectory batch is swept (grid.y = slot).
+    n = 16
+    ins = workloads.random_su2(n=n, depth=6, seed=2).data["instructions"]
+    ins = ins + [{"kind": "measure", "qubit": 3, "cbit": 0}] + \
+        workloads.random_su2(n=n, depth=4, seed=3).data["instructions"] + \
+        [{"kind": "measure", "qubit": q, "cbit": q} for q in range(n)]
+    circ = Circuit(n, ins, cbits=n)
+    res = {}
+    for tile in (0, 12):
+        sim = dqsim.StatevectorSimulator(seed=9, precision="complex64", fusion="greedy", max_fused_qubits=4,
+                                         tile_qubits=tile, sample_terminal=False, max_parallel_shots=64)
+        res[tile] = sim.simulate_shots(circ, shots=64)
+        assert sum(res[tile].values()) == 64
+    # Same Philox draws on states that agree to ~1e-6: outcomes may differ only
+    # where a draw falls within that distance of a CDF boundary.
+    common = sum(min(res[0].get(b, 0), res[12].get(b, 0)) for b in set(res[0]) | set(res[12]))
+    assert common >= 60

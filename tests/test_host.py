@@ -241,11 +241,16 @@ def test_fusion_plan_conserves_gates_and_respects_width():
     circ = workloads.random_su2(n=20, depth=8, seed=3)
     plain = _core.plan_json(circ)
     for k in (2, 3, 4, 5):
-        fused = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k)
+        fused = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=0)
         assert sum(l["gates"] for l in fused["launches"]) == plain["compiled_ops"]
         assert all(l["k"] <= k for l in fused["launches"] if l["gates"] > 1)
         assert len(fused["launches"]) < len(plain["launches"])
         assert all(l["mfma"] == (l["kind"] == "fused" and l["k"] >= 4) for l in fused["launches"])
+        tiled = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=12)
+        assert sum(l["gates"] for l in tiled["launches"]) == plain["compiled_ops"]
+        assert len(tiled["launches"]) < len(fused["launches"])
+        assert all(len(b["qubits"]) <= k for l in tiled["launches"] if l["kind"] == "tile" for b in l["blocks"]
+                   if len(b["qubits"]) > 2)
 
 
 def test_workload_gate_counts():
@@ -259,13 +264,13 @@ def test_learned_planner_merge_decisions_follow_the_model():
     from dqsim import planner
 
     circ = workloads.random_su2(n=16, depth=6, seed=2)
-    greedy = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=4)
+    greedy = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=4, tile_qubits=0)
     w = np.zeros(planner.N_PARAMS, dtype=np.float32)
     try:
         # Constant prediction: every merge saves one launch -> greedy plan.
         w[-1] = 1.0
         planner.install(w)
-        same = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4)
+        same = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4, tile_qubits=0)
         assert [l["k"] for l in same["launches"]] == [l["k"] for l in greedy["launches"]]
         # Time growing steeply with width (feature 0 through one hidden unit):
         # only merges that do not widen a block pay off, so no launch is wider
@@ -275,7 +280,7 @@ def test_learned_planner_merge_decisions_follow_the_model():
         w[12 * 64 + 64 + 0] = 60.0  # W2[0][0]
         w[12 * 64 + 64 + 64 * 64 + 64 + 0] = 1.0  # w3[0]
         planner.install(w)
-        narrow = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4)
+        narrow = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4, tile_qubits=0)
         assert max(l["k"] for l in narrow["launches"]) == 2
         assert len(narrow["launches"]) > len(greedy["launches"])
         assert sum(l["gates"] for l in narrow["launches"]) == narrow["compiled_ops"]
@@ -291,3 +296,83 @@ def test_batched_launcher_is_disabled_with_a_clear_error():
     c = workloads.batch_circuit(0, base_seed=1, nmin=3, nmax=4)
     with pytest.raises(RuntimeError, match="batched persistent launcher is disabled"):
         batch.run_batch([c], shots=8, autotune=False)
+
+
+def _mixed_tile_circuit(n, depth, seed):
+    rng = np.random.default_rng(seed)
+    ins = []
+    for _ in range(depth):
+        for q in range(n):
+            ins.append({"kind": "u3", "qubit": q, "theta": float(rng.uniform(0, 3)), "phi": 0.3, "lam": -0.2})
+        for _ in range(n // 2):
+            q = [int(x) for x in rng.choice(n, 5, replace=False)]
+            kind = ["cx", "cz", "swap", "ccx", "c4x", "rzz"][int(rng.integers(6))]
+            if kind in ("cx", "cz"):
+                ins.append({"kind": kind, "control": q[0], "target": q[1]})
+            elif kind == "swap":
+                ins.append({"kind": kind, "a": q[0], "b": q[1]})
+            elif kind == "rzz":
+                ins.append({"kind": kind, "a": q[0], "b": q[1], "theta": 0.4})
+            elif kind == "ccx":
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "target": q[2]})
+            else:
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "control3": q[2],
+                            "control4": q[3], "target": q[4]})
+    return Circuit(n, ins)
+
+
+@pytest.mark.parametrize("L,k", [(10, 2), (10, 5), (11, 4), (12, 3), (12, 4), (12, 5)])
+@pytest.mark.parametrize("kind", ["su2", "mixed", "qft"])
+def test_tile_sweep_plan_is_a_valid_reordering(kind, L, k):
+    # Every compiled op is applied exactly once, ops that share a qubit keep
+    # their order, and each sweep's blocks fit its L tile qubits (which always
+    # include qubits 0..3).
+    n = 20
+    if kind == "su2":
+        c = workloads.random_su2(n=n, depth=12, seed=L * 10 + k)
+    elif kind == "qft":
+        c = workloads.qft(n, seed=k, measure=False)[0]
+    else:
+        c = _mixed_tile_circuit(n, 6, seed=L + k)
+    plain = _core.plan_json(c, precision="complex64", fusion="none")
+    opq = {x["ops"][0]: set(x["qubits"]) for x in plain["launches"]}
+    plan = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=L)
+    assert plan["compiled_ops"] == plain["compiled_ops"]
+    order = [o for x in plan["launches"] for o in x["ops"]]
+    assert sorted(order) == list(range(plan["compiled_ops"]))
+    pos = {o: i for i, o in enumerate(order)}
+    last = {}
+    for o in range(plan["compiled_ops"]):  # original order
+        for q in opq.get(o, ()):
+            if q in last:
+                assert pos[last[q]] < pos[o], (last[q], o, q)
+            last[q] = o
+    tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+    assert tiles
+    for t in tiles:
+        s = set(t["qubits"])
+        assert len(s) == L and {0, 1, 2, 3} <= s and t["k"] == L
+        assert len(t["blocks"]) >= 2
+        for b in t["blocks"]:
+            assert set(b["qubits"]) <= s and 1 <= len(b["qubits"]) <= 5
+            assert b["mfma"] == (len(b["qubits"]) >= 4)
+    # Block ops inside a tile: the union of the tile's blocks covers its ops.
+    for t in tiles:
+        for o in t["ops"]:
+            assert opq[o] <= set(t["qubits"])
+
+
+def test_tile_sweeps_off_and_limits():
+    c = workloads.random_su2(n=18, depth=6, seed=1)
+    off = _core.plan_json(c, precision="complex64", fusion="greedy", tile_qubits=0)
+    assert not any(x["kind"] == "tile" for x in off["launches"])
+    # complex128 and small states never use tiles.
+    assert not any(x["kind"] == "tile" for x in _core.plan_json(c, fusion="greedy")["launches"])
+    small = workloads.random_su2(n=15, depth=6, seed=1)
+    assert not any(x["kind"] == "tile"
+                   for x in _core.plan_json(small, precision="complex64", fusion="greedy")["launches"])
+    for bad in (9, 13, 64):
+        with pytest.raises(ValueError):
+            _core.plan_json(c, precision="complex64", fusion="greedy", tile_qubits=bad)
+        with pytest.raises(ValueError):
+            dqsim.StatevectorSimulator(precision="complex64", tile_qubits=bad)
