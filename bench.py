@@ -44,6 +44,43 @@ def per_kind(rows):
     return agg
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summaries (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter's reading for a plain copy of known size); None if that
+    kernel/workload pair has not been profiled."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path, encoding="utf-8"):
+            if not line.startswith("traffic-per-launch:"):
+                continue
+            kv = dict(f.split("=", 1) for f in line.split(":", 1)[1].split())
+            if kv.get("kernel") == kernel and kv.get("workload") == workload:
+                return float(kv["bytes"])
+    return None
+
+
+def device_copy_gbs(nbytes=4 << 30, reps=5):
+    """Read+write rate of a plain device-to-device copy (the practical
+    streaming ceiling beside the 8 TB/s peak)."""
+    import torch
+
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms / 1e3) / 1e9
+
+
 def cpu_baseline(n, depth, threads):
     """FP64 CPU restatement of the reference algorithm (the oracle) on the same
     circuit family at a size the host finishes in ~10-30 s."""
@@ -105,6 +142,8 @@ def main():
     import dqsim
     from dqsim import workloads
 
+    copy_gbs = device_copy_gbs()
+    log(f"[bench] device copy (4 GiB, torch copy_): {copy_gbs:.0f} GB/s read+write")
     n, depth = args.qubits, args.depth
     circ = workloads.random_su2(n=n, depth=depth, seed=1234)
     gates = workloads.gate_count(circ.data["instructions"])
@@ -136,7 +175,8 @@ def main():
     extra = {"launches_per_step": launches, "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                             for kk, vv in v.items()} for k, v in agg.items()},
              "device_ms_per_step": sum(dev_ms) / len(dev_ms),
-             "hbm_gbs_whole_step": total_bytes / (sum(dev_ms) / len(dev_ms) / 1e3) / 1e9}
+             "hbm_gbs_whole_step": total_bytes / (sum(dev_ms) / len(dev_ms) / 1e3) / 1e9,
+             "device_copy_gbs": copy_gbs}
 
     if not args.no_plain:
         plain = dqsim.StatevectorSimulator(seed=1, precision="complex128", fusion="none", max_memory_mb=mem_mb)
@@ -158,6 +198,7 @@ def main():
         extra["max_amp_err_vs_fp64"] = max_err
         extra["l2_err_vs_fp64"] = l2_err
 
+    workload_key = f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -181,7 +222,9 @@ def main():
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(KERNEL_NAMES.get(dname, dname), workload_key),
+                     "algorithmic_bytes_per_launch": d["bytes"] / d["launches"]},
         "cpu_baseline": cpu,
         "detail": extra,
     }
