@@ -511,8 +511,42 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
   return m;
 }
 
+// Tile-sweep matrix form of a block: kind 0 (k <= 2) complex FP32 matrix,
+// kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA, kind 2
+// (k = 4, 5) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
+int tile_kind(int k) { return k <= 2 ? 0 : k == 3 ? 1 : 2; }
+size_t tile_matrix_bytes(int k) {
+  const size_t d = (size_t)1 << k;
+  return tile_kind(k) == 0 ? 8 * d * d : 4 * (2 * d) * (2 * d);
+}
+std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
+  const int kind = tile_kind(k);
+  if (kind == 2) return real_embedding(m, k);
+  std::vector<float> v;
+  if (kind == 0) {
+    for (auto& c : m) {
+      v.push_back((float)c.re);
+      v.push_back((float)c.im);
+    }
+    return v;
+  }
+  // Row 2c+p: output c, part p (0 real, 1 imaginary). Column 4s+u: input
+  // 2u + s/2, part s%2. Entry = d(out part)/d(in part) of out = U in.
+  v.assign(256, 0.f);
+  for (int row = 0; row < 16; ++row)
+    for (int col = 0; col < 16; ++col) {
+      const int c = row >> 1, p = row & 1, s = col >> 2, u = col & 3;
+      const int cin = 2 * u + (s >> 1), q = s & 1;
+      const cplx e = m[(size_t)c * 8 + cin];
+      const double val = p == 0 ? (q == 0 ? e.re : -e.im) : (q == 0 ? e.im : e.re);
+      v[(size_t)row * 16 + col] = (float)val;
+    }
+  return v;
+}
+
 // Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
-// read and write of the state for all of them.
+// read and write of the state for all of them. The blob (descriptors, then
+// matrices) is what every workgroup copies into LDS.
 void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int L, const std::vector<Op>& ops) {
   DevOp d;
   d.kind = D_TILE;
@@ -531,22 +565,17 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   d.targs.L = (uint32_t)L;
   d.targs.nblocks = (uint32_t)blocks.size();
   std::vector<k::TileBlock> tb;
+  std::vector<float> mats;
+  const size_t head = blocks.size() * sizeof(k::TileBlock);
   for (FusedBlock* b : blocks) {
     if (b->mat.empty()) build_block_matrix(*b, ops);
     k::TileBlock t{};
     t.k = (uint32_t)b->qubits.size();
-    t.mfma = t.k >= 4;
-    if (t.mfma) {
-      std::vector<float> emb = real_embedding(b->mat, (int)t.k);
-      t.mat_off = (uint32_t)plan.push(emb.data(), emb.size() * sizeof(float));
-    } else {
-      std::vector<float> v;
-      for (auto& c : b->mat) {
-        v.push_back((float)c.re);
-        v.push_back((float)c.im);
-      }
-      t.mat_off = (uint32_t)plan.push(v.data(), v.size() * sizeof(float));
-    }
+    t.kind = (uint32_t)tile_kind((int)t.k);
+    t.mat_off = (uint32_t)(head + mats.size() * sizeof(float));
+    std::vector<float> m = tile_matrix(b->mat, (int)t.k);
+    mats.insert(mats.end(), m.begin(), m.end());
+    while (mats.size() % 4) mats.push_back(0.f);  // 16-byte alignment
     for (uint32_t i = 0; i < t.k; ++i) {
       t.loc[i] = (uint32_t)local[b->qubits[i]];
       t.srt[i] = t.loc[i];
@@ -556,9 +585,13 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
     d.gates += (int)b->ops.size();
     d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
     d.tile_blocks.push_back(b->qubits);
-    d.tile_mfma.push_back((int)t.mfma);
+    d.tile_mfma.push_back((int)(t.kind != 0));
   }
-  d.tile_off = plan.push(tb.data(), tb.size() * sizeof(k::TileBlock));
+  std::vector<uint8_t> blob(head + mats.size() * sizeof(float));
+  std::memcpy(blob.data(), tb.data(), head);
+  std::memcpy(blob.data() + head, mats.data(), mats.size() * sizeof(float));
+  d.targs.blob16 = (uint32_t)(blob.size() / 16);
+  d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
   d.low = 0;
   plan.ops.push_back(std::move(d));
@@ -572,6 +605,8 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
 // as an ordinary launch.
 void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<Op>& ops, int prec, int n, int L) {
   constexpr size_t kWindow = 512;
+  // LDS per workgroup = 8 * 2^L + blob; the blob cap keeps it within 64 KiB.
+  constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = 64;
   std::vector<int> rem(blocks.size());
   for (size_t b = 0; b < blocks.size(); ++b) rem[b] = (int)b;
   const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
@@ -592,6 +627,7 @@ void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<
     }
     uint64_t smask = 0xFull | qubit_mask(first.qubits);
     uint64_t blocked = 0;
+    size_t blob = sizeof(k::TileBlock) + tile_matrix_bytes((int)first.qubits.size());
     std::vector<int> take{rem[head]};
     size_t seen = 0;
     for (size_t r = head + 1; r < rem.size() && seen < kWindow; ++r) {
@@ -600,8 +636,11 @@ void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<
       ++seen;
       const FusedBlock& b = blocks[bi];
       const uint64_t m = qubit_mask(b.qubits);
-      if (tileable(b) && !(m & blocked) && __builtin_popcountll(smask | m) <= L) {
+      const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)b.qubits.size());
+      if (tileable(b) && !(m & blocked) && __builtin_popcountll(smask | m) <= L && blob + bb <= kTileBlobMax &&
+          take.size() < kTileBlocksMax) {
         smask |= m;
+        blob += bb;
         take.push_back(bi);
       } else {
         blocked |= m;
@@ -762,7 +801,7 @@ struct Exec {
         break;
       }
       case D_TILE:
-        k::tile_sweep(s, stride, nb, n, d.targs, (const k::TileBlock*)(cbase + d.tile_off), cbase, st);
+        k::tile_sweep(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
         break;
       case D_NOOP: break;
     }

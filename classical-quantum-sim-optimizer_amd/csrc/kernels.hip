@@ -20,7 +20,11 @@
 // uses the same tree, so FP64 probabilities agree bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <stdexcept>
+#include <string>
 
 #include "kernels.hpp"
 #include "philox.hpp"
@@ -325,9 +329,16 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // Local index t of a tile: bit i <-> global qubit spos[i]; the tile's other
 // global bits come from the tile number (zeros inserted at S). spos[0..3] are
 // qubits 0..3, so 16 consecutive threads move one contiguous 128-byte run.
-// Blocks are applied in LDS: k <= 3 by one thread per group with FP32 FMA,
-// k = 4, 5 on MFMA with the operand layout of k_fused_mfma (inputs and
-// outputs in LDS instead of HBM).
+// A workgroup first copies the sweep's blob (block descriptors, then their
+// matrices) into LDS, then walks its tiles persistently: while the blocks of
+// one tile are applied in LDS, the loads of its next tile are in flight (the
+// block arithmetic reads only LDS, so it never waits on those loads).
+// Block kinds (TileBlock.kind):
+//   0  k <= 2: complex FP32 matrix, FP32 FMA, one thread per group;
+//   1  k = 3: permuted 16x16 real embedding on v_mfma_f32_16x16x4f32, each
+//      lane holding whole complex inputs and outputs;
+//   2  k = 4, 5: 2^(k+1) real embedding on v_mfma_f32_32x32x2f32 with the
+//      operand layout of k_fused_mfma.
 
 __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
   for (int i = 0; i < k; ++i) {
@@ -338,18 +349,26 @@ __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* s
 }
 
 template <int K>
-__device__ __forceinline__ void tile_block_valu(float2* __restrict__ tile, int L, const TileBlock& b,
-                                                const float* __restrict__ m) {
+__device__ __forceinline__ uint32_t local_offset(const TileBlock& b, int c) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    if ((c >> (K - 1 - q)) & 1) o |= 1u << b.loc[q];
+  return o;
+}
+
+template <int K>
+__device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileBlock& b, const float* m) {
   constexpr int D = 1 << K;
+  float mr[D * D], mi[D * D];
+#pragma unroll
+  for (int i = 0; i < D * D; ++i) {
+    mr[i] = m[2 * i];
+    mi[i] = m[2 * i + 1];
+  }
   uint32_t off[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int i = 0; i < K; ++i)
-      if ((j >> (K - 1 - i)) & 1) o |= 1u << b.loc[i];
-    off[j] = o;
-  }
+  for (int j = 0; j < D; ++j) off[j] = local_offset<K>(b, j);
   const uint32_t groups = (1u << L) >> K;
   for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
     const uint32_t base = insert_zeros32(g, b.srt, K);
@@ -361,20 +380,48 @@ __device__ __forceinline__ void tile_block_valu(float2* __restrict__ tile, int L
       float ar = 0.f, ai = 0.f;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        const float mr = m[2 * (r * D + c)], mi = m[2 * (r * D + c) + 1];
-        ar = __builtin_fmaf(mr, v[c].x, ar);
-        ar = __builtin_fmaf(-mi, v[c].y, ar);
-        ai = __builtin_fmaf(mr, v[c].y, ai);
-        ai = __builtin_fmaf(mi, v[c].x, ai);
+        ar = __builtin_fmaf(mr[r * D + c], v[c].x, ar);
+        ar = __builtin_fmaf(-mi[r * D + c], v[c].y, ar);
+        ai = __builtin_fmaf(mr[r * D + c], v[c].y, ai);
+        ai = __builtin_fmaf(mi[r * D + c], v[c].x, ai);
       }
       tile[base | off[r]] = make_float2(ar, ai);
     }
   }
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// k = 3 on v_mfma_f32_16x16x4f32 (A lane l: A[l%16][l/16]; B lane l:
+// B[l/16][l%16]; D lane l, register r: D[4(l/16)+r][l%16]). The embedding
+// e (16x16, row-major) is permuted so that
+//   row 2c+p    = real (p=0) / imaginary (p=1) part of output c,
+//   column 4s+u = input 2u + s/2, real (s even) / imaginary (s odd) part,
+// so lane l (u = l/16) feeds the four K steps from inputs 2u and 2u+1 of
+// group l%16 and receives outputs 2u and 2u+1 of that group whole.
+__device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const TileBlock& b, const float* e) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, u = lane >> 4;
+  float a[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) a[s] = e[col * 16 + 4 * s + u];
+  const uint32_t o0 = local_offset<3>(b, 2 * u), o1 = local_offset<3>(b, 2 * u + 1);
+  const uint32_t ntiles = ((1u << L) >> 3) / 16;
+  for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
+    const uint32_t base = insert_zeros32(mt * 16 + col, b.srt, 3);
+    const float2 x0 = tile[base | o0], x1 = tile[base | o1];
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x0.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, acc, 0, 0, 0);
+    tile[base | o0] = make_float2(acc[0], acc[1]);
+    tile[base | o1] = make_float2(acc[2], acc[3]);
+  }
+}
+
 template <int K>
-__device__ __forceinline__ void tile_block_mfma(float2* __restrict__ tile, int L, const TileBlock& b,
-                                                const float* __restrict__ emb) {
+__device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const TileBlock& b, const float* emb) {
   constexpr int D = 1 << K;
   constexpr int R = 2 * D;
   constexpr int T = R / 32;
@@ -384,23 +431,14 @@ __device__ __forceinline__ void tile_block_mfma(float2* __restrict__ tile, int L
   const int j = lane & 31, h = lane >> 5;
   uint32_t offs[NIN];
 #pragma unroll
-  for (int i = 0; i < NIN; ++i) {
-    const int c = 2 * i + h;
-    uint32_t o = 0;
-#pragma unroll
-    for (int q = 0; q < K; ++q)
-      if ((c >> (K - 1 - q)) & 1) o |= 1u << b.loc[q];
-    offs[i] = o;
-  }
+  for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h);
   const uint32_t groups = (1u << L) >> K;
-  const uint32_t ntiles = (groups + 31) / 32;
+  const uint32_t ntiles = groups / 32;
   for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
-    const uint32_t g = mt * 32 + j;
-    const bool valid = g < groups;
-    const uint32_t base = insert_zeros32(valid ? g : 0, b.srt, K);
+    const uint32_t base = insert_zeros32(mt * 32 + j, b.srt, K);
     float2 v[NIN];
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) v[i] = valid ? tile[base | offs[i]] : make_float2(0.f, 0.f);
+    for (int i = 0; i < NIN; ++i) v[i] = tile[base | offs[i]];
     floatx16 acc[T];
 #pragma unroll
     for (int t = 0; t < T; ++t)
@@ -413,25 +451,24 @@ __device__ __forceinline__ void tile_block_mfma(float2* __restrict__ tile, int L
       for (int t = 0; t < T; ++t)
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[(32 * t + j) * R + 2 * q + h], bv, acc[t], 0, 0, 0);
     }
-    if (!valid) continue;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      if (K == 4 && r >= 8) break;
+    for (int r = 0; r < (K == 4 ? 8 : 16); ++r) {
       const int o = 8 * (r / 4) + 4 * h + (r % 4);
-      uint32_t oo = 0;
-#pragma unroll
-      for (int q = 0; q < K; ++q)
-        if ((o >> (K - 1 - q)) & 1) oo |= 1u << b.loc[q];
-      tile[base | oo] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
+      tile[base | local_offset<K>(b, o)] =
+          K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
     }
   }
 }
 
 template <int L>
 __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t stride, uint64_t ntiles, TileArgs a,
-                                              const TileBlock* __restrict__ blocks, const uint8_t* __restrict__ blob) {
-  __shared__ float2 tile[1 << L];
+                                              const float4* __restrict__ blob) {
+  extern __shared__ float4 lds4[];
   constexpr int PER = (1 << L) / TPB;
+  float2* tile = (float2*)lds4;
+  const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L) / 2);
+  for (uint32_t i = threadIdx.x; i < a.blob16; i += TPB) lds4[(1 << L) / 2 + i] = blob[i];
+  const TileBlock* blocks = (const TileBlock*)meta;
   float2* st = s + (uint64_t)blockIdx.y * stride;
   // Global offset of local index t = tid + TPB * i: low part from the thread,
   // high part from i (disjoint bit sets, so the offsets OR together).
@@ -448,29 +485,44 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       if ((i >> bit) & 1) o |= 1ull << a.spos[8 + bit];
     hi[i] = o;
   }
-  for (uint64_t tau = blockIdx.x; tau < ntiles; tau += gridDim.x) {
+  auto tile_base = [&](uint64_t tau) {
     uint64_t base = tau;
 #pragma unroll
     for (int i = 0; i < L; ++i) {
       const uint64_t low = (1ull << a.spos[i]) - 1;
       base = (base & low) | ((base & ~low) << 1);
     }
-    float2 r[PER];
+    return base;
+  };
+  float2 r[PER];
+  uint64_t tau = blockIdx.x;
+  if (tau < ntiles) {
+    const uint64_t b0 = tile_base(tau);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) r[i] = st[base | lo | hi[i]];
+    for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi[i]];
+  }
+  for (; tau < ntiles; tau += gridDim.x) {
+    const uint64_t base = tile_base(tau);
 #pragma unroll
     for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
-    __syncthreads();
+    __syncthreads();  // the tile (and, on the first pass, the blob) is in LDS
+    const uint64_t nxt = tau + gridDim.x;
+    if (nxt < ntiles) {
+      const uint64_t nb = tile_base(nxt);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) r[i] = st[nb | lo | hi[i]];
+    }
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
-      const TileBlock b = blocks[bi];
-      const float* m = (const float*)(blob + b.mat_off);
-      // k >= 4 blocks carry the real embedding (MFMA), k <= 3 the complex matrix.
-      switch (b.k) {
-        case 1: tile_block_valu<1>(tile, L, b, m); break;
-        case 2: tile_block_valu<2>(tile, L, b, m); break;
-        case 3: tile_block_valu<3>(tile, L, b, m); break;
-        case 4: tile_block_mfma<4>(tile, L, b, m); break;
-        default: tile_block_mfma<5>(tile, L, b, m); break;
+      const TileBlock& b = blocks[bi];
+      const float* m = (const float*)(meta + b.mat_off);
+      if (b.kind == 1) {
+        tile_block_mfma16(tile, L, b, m);
+      } else if (b.kind == 2) {
+        if (b.k == 4) tile_block_mfma32<4>(tile, L, b, m);
+        else tile_block_mfma32<5>(tile, L, b, m);
+      } else {
+        if (b.k == 1) tile_block_valu<1>(tile, L, b, m);
+        else tile_block_valu<2>(tile, L, b, m);
       }
       __syncthreads();
     }
@@ -1042,13 +1094,39 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, c
   return true;
 }
 
-void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const TileBlock* blocks,
-                const uint8_t* blob, hipStream_t st) {
+#define HIPCHK_K(x)                                                                             \
+  do {                                                                                          \
+    const hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t tile_lds_bytes(const TileArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
+
+// Persistent grid: as many workgroups as fit on the device at once (for this
+// tile size and LDS footprint), capped by the tile count.
+static uint64_t tile_grid(int L, size_t lds, uint64_t ntiles) {
+  static thread_local std::map<std::pair<int, int>, int> cache;  // (device, L<<20 | lds) -> workgroups
+  int dev = 0;
+  HIPCHK_K(hipGetDevice(&dev));
+  const auto key = std::make_pair(dev, (int)((L << 20) | (int)lds));
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    int per_cu = 0, cus = 0;
+    const void* f = L == 10 ? (const void*)k_tile<10> : L == 11 ? (const void*)k_tile<11> : (const void*)k_tile<12>;
+    HIPCHK_K(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, TPB, lds));
+    HIPCHK_K(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    it = cache.emplace(key, std::max(1, per_cu) * std::max(1, cus)).first;
+  }
+  return std::min<uint64_t>(ntiles, (uint64_t)it->second);
+}
+
+void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob,
+                hipStream_t st) {
   const uint64_t ntiles = 1ull << (n - (int)a.L);
-  const uint64_t grid = ntiles < 4096 ? ntiles : 4096;
-  dim3 g((uint32_t)grid, batch);
+  const size_t lds = tile_lds_bytes(a);
+  dim3 g((uint32_t)tile_grid((int)a.L, lds, ntiles), batch);
 #define L(LL) \
-  hipLaunchKernelGGL(k_tile<LL>, g, dim3(TPB), 0, st, (float2*)s, stride, ntiles, a, blocks, blob)
+  hipLaunchKernelGGL(k_tile<LL>, g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
   switch (a.L) {
     case 10: L(10); break;
     case 11: L(11); break;

@@ -77,21 +77,28 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, co
 // One sweep applies several fused blocks whose qubits all lie in a tile set S
 // (|S| = L <= 12, S always holds qubits 0..3). Each workgroup loads one
 // 2^L-amplitude tile into LDS, applies every block there, and writes it back.
+// Blob of one sweep (copied to LDS by every workgroup): nblocks TileBlock
+// descriptors, then the block matrices (16-byte aligned).
 struct TileBlock {
   uint32_t k;        // 1..5 qubits
-  uint32_t mfma;     // k >= 4: real embedding (R x R) for MFMA; k <= 3: complex FP32 matrix
-  uint32_t mat_off;  // byte offset of the matrix in the plan blob
+  uint32_t kind;     // 0: complex FP32 matrix (k <= 2); 1: permuted 16x16 embedding (k = 3);
+                     // 2: 2^(k+1) real embedding (k = 4, 5)
+  uint32_t mat_off;  // byte offset of the matrix in the blob
   uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
   uint32_t srt[5];   // the same local bits sorted ascending
-  uint32_t pad;
+  uint32_t pad[3];
 };
+static_assert(sizeof(TileBlock) == 64, "TileBlock is 64 bytes");
 struct TileArgs {
   uint32_t L;         // tile qubits
   uint32_t nblocks;
+  uint32_t blob16;    // blob size in 16-byte units
   uint32_t spos[12];  // S sorted ascending (global qubit of local bit i)
 };
-void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const TileBlock* blocks_dev,
-                const uint8_t* blob, hipStream_t st);
+// Dynamic LDS of one workgroup: the tile plus the blob.
+size_t tile_lds_bytes(const TileArgs& a);
+void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob_dev,
+                hipStream_t st);
 
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
