@@ -99,10 +99,27 @@ def library_path() -> str:
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), _LIB_NAME)
 
 
+def _share_hip_runtime():
+    """Keep one HIP runtime per process. PyTorch's ROCm libraries request the
+    runtime as ``libamdhip64.so`` (their bundled copy); this library requests
+    it by its soname ``libamdhip64.so.7``. Two copies in one process cannot
+    both open the GPU: the second to initialise sees no device. Loading the
+    runtime under the unversioned name first makes every later request bind
+    to one copy: if torch is already loaded this returns torch's copy (and
+    the library's soname request binds to it); otherwise the system runtime
+    is loaded under that name and a later ``import torch`` binds to it."""
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):  # the HSA runtime under HIP likewise
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            pass  # not on the search path under that name: the soname request decides
+
+
 def _load():
     path = library_path()
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built; build it with __graft_entry__.build() or `make` in the package")
+    _share_hip_runtime()
     lib = ctypes.CDLL(path)
     P, U64, I32, SZ, D = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_size_t, ctypes.c_double
     sigs = {

@@ -241,3 +241,48 @@ def test_batched_kernel_launch_limits_are_readable():
         print(block, info, flush=True)
         assert info["device_count"] >= 1 and info["max_lds_per_block"] >= 65536
         assert 8 <= info["lds_qubits"] <= 15 and info["kernel_max_threads"] >= block
+
+
+_TORCH_THEN_DQSIM = """
+import sys
+sys.path[:0] = {paths!r}
+import torch
+x = torch.ones(1 << 20, device="cuda") * 2
+torch.cuda.synchronize()
+import dqsim
+from dqsim.workloads import Circuit
+c = Circuit(3, [{{"kind": "h", "qubit": 0}}, {{"kind": "cx", "control": 0, "target": 2}}])
+sv = dqsim.StatevectorSimulator(seed=1).simulate(c).statevector
+assert abs(abs(sv[0]) ** 2 - 0.5) < 1e-12 and abs(abs(sv[5]) ** 2 - 0.5) < 1e-12
+assert float(x.sum()) == 2 * (1 << 20)
+print("ok")
+"""
+
+_DQSIM_THEN_TORCH = """
+import sys
+sys.path[:0] = {paths!r}
+import dqsim
+from dqsim.workloads import Circuit
+c = Circuit(3, [{{"kind": "h", "qubit": 0}}, {{"kind": "cx", "control": 0, "target": 2}}])
+sv = dqsim.StatevectorSimulator(seed=1).simulate(c).statevector
+import torch
+x = torch.ones(1 << 20, device="cuda") * 2
+torch.cuda.synchronize()
+assert float(x.sum()) == 2 * (1 << 20)
+sv2 = dqsim.StatevectorSimulator(seed=1).simulate(c).statevector
+assert (sv == sv2).all()
+print("ok")
+"""
+
+
+@pytest.mark.parametrize("script", [_TORCH_THEN_DQSIM, _DQSIM_THEN_TORCH], ids=["torch-first", "dqsim-first"])
+def test_one_hip_runtime_shared_with_torch(script):
+    # PyTorch and the engine in one process, in either import order.
+    import subprocess
+    import sys
+
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "classical-quantum-sim-optimizer_amd")
+    out = subprocess.run([sys.executable, "-c", script.format(paths=[pkg])], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stderr[-2000:]
