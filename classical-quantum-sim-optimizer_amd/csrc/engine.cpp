@@ -521,7 +521,17 @@ size_t tile_matrix_bytes(int k) {
 }
 std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
   const int kind = tile_kind(k);
-  if (kind == 2) return real_embedding(m, k);
+  if (kind == 2) {
+    // Fragment-major: [t][q][lane] = E[32 t + lane % 32][2 q + lane / 32].
+    const int R = 2 << k, T = R / 32, STEPS = R / 2;
+    const std::vector<float> e = real_embedding(m, k);
+    std::vector<float> f((size_t)R * R);
+    for (int t = 0; t < T; ++t)
+      for (int q = 0; q < STEPS; ++q)
+        for (int l = 0; l < 64; ++l)
+          f[((size_t)t * STEPS + q) * 64 + l] = e[(size_t)(32 * t + l % 32) * R + 2 * q + l / 32];
+    return f;
+  }
   std::vector<float> v;
   if (kind == 0) {
     for (auto& c : m) {
@@ -530,17 +540,21 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
     }
     return v;
   }
-  // Row 2c+p: output c, part p (0 real, 1 imaginary). Column 4s+u: input
-  // 2u + s/2, part s%2. Entry = d(out part)/d(in part) of out = U in.
-  v.assign(256, 0.f);
+  // Embedding row 2c+p: output c, part p (0 real, 1 imaginary). Column
+  // 4s+u: input 2u + s/2, part s%2. Entry = d(out part)/d(in part) of
+  // out = U in. Stored fragment-major: [s][lane] = E[lane % 16][4 s + lane / 16].
+  std::vector<float> e(256, 0.f);
   for (int row = 0; row < 16; ++row)
     for (int col = 0; col < 16; ++col) {
       const int c = row >> 1, p = row & 1, s = col >> 2, u = col & 3;
       const int cin = 2 * u + (s >> 1), q = s & 1;
-      const cplx e = m[(size_t)c * 8 + cin];
-      const double val = p == 0 ? (q == 0 ? e.re : -e.im) : (q == 0 ? e.im : e.re);
-      v[(size_t)row * 16 + col] = (float)val;
+      const cplx x = m[(size_t)c * 8 + cin];
+      const double val = p == 0 ? (q == 0 ? x.re : -x.im) : (q == 0 ? x.im : x.re);
+      e[(size_t)row * 16 + col] = (float)val;
     }
+  v.assign(256, 0.f);
+  for (int s = 0; s < 4; ++s)
+    for (int l = 0; l < 64; ++l) v[(size_t)s * 64 + l] = e[(size_t)(l % 16) * 16 + 4 * s + l / 16];
   return v;
 }
 

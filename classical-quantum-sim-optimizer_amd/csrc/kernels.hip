@@ -339,6 +339,8 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 //      lane holding whole complex inputs and outputs;
 //   2  k = 4, 5: 2^(k+1) real embedding on v_mfma_f32_32x32x2f32 with the
 //      operand layout of k_fused_mfma.
+// MFMA matrices are stored fragment-major (the A value of K step q for lane l
+// at [q][l]), so every operand read is one conflict-free 256-byte LDS row.
 
 __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
   for (int i = 0; i < k; ++i) {
@@ -399,12 +401,13 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 //   column 4s+u = input 2u + s/2, real (s even) / imaginary (s odd) part,
 // so lane l (u = l/16) feeds the four K steps from inputs 2u and 2u+1 of
 // group l%16 and receives outputs 2u and 2u+1 of that group whole.
+// `e` holds the A fragments: e[64 s + l] = embedding[l % 16][4 s + l / 16].
 __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const TileBlock& b, const float* e) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, u = lane >> 4;
   float a[4];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) a[s] = e[col * 16 + 4 * s + u];
+  for (int s = 0; s < 4; ++s) a[s] = e[64 * s + lane];
   const uint32_t o0 = local_offset<3>(b, 2 * u), o1 = local_offset<3>(b, 2 * u + 1);
   const uint32_t ntiles = ((1u << L) >> 3) / 16;
   for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
@@ -420,6 +423,7 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   }
 }
 
+// `emb` holds the A fragments: emb[(STEPS t + q) 64 + l] = E[32 t + l % 32][2 q + l / 32].
 template <int K>
 __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const TileBlock& b, const float* emb) {
   constexpr int D = 1 << K;
@@ -429,9 +433,12 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
   constexpr int NIN = D / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 31, h = lane >> 5;
-  uint32_t offs[NIN];
+  constexpr int NOUT = K == 4 ? 8 : 16;
+  uint32_t offs[NIN], outs[NOUT];
 #pragma unroll
   for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h);
+#pragma unroll
+  for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4));
   const uint32_t groups = (1u << L) >> K;
   const uint32_t ntiles = groups / 32;
   for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
@@ -449,14 +456,11 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
       const float bv = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
 #pragma unroll
       for (int t = 0; t < T; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[(32 * t + j) * R + 2 * q + h], bv, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[(STEPS * t + q) * 64 + lane], bv, acc[t], 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < (K == 4 ? 8 : 16); ++r) {
-      const int o = 8 * (r / 4) + 4 * h + (r % 4);
-      tile[base | local_offset<K>(b, o)] =
-          K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
-    }
+    for (int r = 0; r < NOUT; ++r)
+      tile[base | outs[r]] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
   }
 }
 
