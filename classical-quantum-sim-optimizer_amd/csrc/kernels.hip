@@ -341,6 +341,19 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 //      operand layout of k_fused_mfma.
 // MFMA matrices are stored fragment-major (the A value of K step q for lane l
 // at [q][l]), so every operand read is one conflict-free 256-byte LDS row.
+// Tile element t lives at LDS slot swz(t) = t ^ M(t >> 5), a linear XOR
+// swizzle (M maps each of bits 5..11 to a fixed 5-bit pattern) that spreads
+// the strided group accesses of blocks on low local bits over the banks
+// (modelled conflict degree 1.3 instead of 3.4 over random block positions).
+// Being linear, swz(base | off) = swz(base) ^ swz(off) for disjoint bits.
+
+__host__ __device__ constexpr uint32_t swz(uint32_t t) {
+  constexpr uint32_t m[7] = {27, 14, 28, 25, 22, 8, 24};
+  uint32_t x = 0;
+  for (int i = 0; i < 7; ++i)
+    if ((t >> (5 + i)) & 1) x ^= m[i];
+  return t ^ x;
+}
 
 __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
   for (int i = 0; i < k; ++i) {
@@ -370,13 +383,13 @@ __device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileB
   }
   uint32_t off[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) off[j] = local_offset<K>(b, j);
+  for (int j = 0; j < D; ++j) off[j] = swz(local_offset<K>(b, j));
   const uint32_t groups = (1u << L) >> K;
   for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
-    const uint32_t base = insert_zeros32(g, b.srt, K);
+    const uint32_t base = swz(insert_zeros32(g, b.srt, K));
     float2 v[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) v[j] = tile[base | off[j]];
+    for (int j = 0; j < D; ++j) v[j] = tile[base ^ off[j]];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       float ar = 0.f, ai = 0.f;
@@ -387,7 +400,7 @@ __device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileB
         ai = __builtin_fmaf(mr[r * D + c], v[c].y, ai);
         ai = __builtin_fmaf(mi[r * D + c], v[c].x, ai);
       }
-      tile[base | off[r]] = make_float2(ar, ai);
+      tile[base ^ off[r]] = make_float2(ar, ai);
     }
   }
 }
@@ -408,18 +421,18 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   float a[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) a[s] = e[64 * s + lane];
-  const uint32_t o0 = local_offset<3>(b, 2 * u), o1 = local_offset<3>(b, 2 * u + 1);
+  const uint32_t o0 = swz(local_offset<3>(b, 2 * u)), o1 = swz(local_offset<3>(b, 2 * u + 1));
   const uint32_t ntiles = ((1u << L) >> 3) / 16;
   for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = insert_zeros32(mt * 16 + col, b.srt, 3);
-    const float2 x0 = tile[base | o0], x1 = tile[base | o1];
+    const uint32_t base = swz(insert_zeros32(mt * 16 + col, b.srt, 3));
+    const float2 x0 = tile[base ^ o0], x1 = tile[base ^ o1];
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x0.y, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1.x, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, acc, 0, 0, 0);
-    tile[base | o0] = make_float2(acc[0], acc[1]);
-    tile[base | o1] = make_float2(acc[2], acc[3]);
+    tile[base ^ o0] = make_float2(acc[0], acc[1]);
+    tile[base ^ o1] = make_float2(acc[2], acc[3]);
   }
 }
 
@@ -436,16 +449,16 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
   constexpr int NOUT = K == 4 ? 8 : 16;
   uint32_t offs[NIN], outs[NOUT];
 #pragma unroll
-  for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h);
+  for (int i = 0; i < NIN; ++i) offs[i] = swz(local_offset<K>(b, 2 * i + h));
 #pragma unroll
-  for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4));
+  for (int r = 0; r < NOUT; ++r) outs[r] = swz(local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)));
   const uint32_t groups = (1u << L) >> K;
   const uint32_t ntiles = groups / 32;
   for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = insert_zeros32(mt * 32 + j, b.srt, K);
+    const uint32_t base = swz(insert_zeros32(mt * 32 + j, b.srt, K));
     float2 v[NIN];
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) v[i] = tile[base | offs[i]];
+    for (int i = 0; i < NIN; ++i) v[i] = tile[base ^ offs[i]];
     floatx16 acc[T];
 #pragma unroll
     for (int t = 0; t < T; ++t)
@@ -460,7 +473,7 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
     }
 #pragma unroll
     for (int r = 0; r < NOUT; ++r)
-      tile[base | outs[r]] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
+      tile[base ^ outs[r]] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
   }
 }
 
@@ -505,10 +518,11 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi[i]];
   }
+  const uint32_t sl = swz(threadIdx.x);
   for (; tau < ntiles; tau += gridDim.x) {
     const uint64_t base = tile_base(tau);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+    for (int i = 0; i < PER; ++i) tile[sl ^ swz(TPB * i)] = r[i];
     __syncthreads();  // the tile (and, on the first pass, the blob) is in LDS
     const uint64_t nxt = tau + gridDim.x;
     if (nxt < ntiles) {
@@ -531,7 +545,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[sl ^ swz(TPB * i)];
     __syncthreads();
   }
 }
@@ -1106,29 +1120,28 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, c
 
 size_t tile_lds_bytes(const TileArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
 
-// Persistent grid: as many workgroups as fit on the device at once (for this
-// tile size and LDS footprint), capped by the tile count.
-static uint64_t tile_grid(int L, size_t lds, uint64_t ntiles) {
-  static thread_local std::map<std::pair<int, int>, int> cache;  // (device, L<<20 | lds) -> workgroups
+// Grid: 64 workgroups per CU (each walks ntiles / grid tiles with its
+// prefetch), capped by the tile count. A grid of exactly the resident
+// workgroup count measured 1.4-1.8x slower outside a profiler: it leaves the
+// kernel's end to the slowest of a few long-lived workgroups.
+static uint64_t tile_grid(uint64_t ntiles) {
+  static thread_local std::map<int, int> cus_of;  // device -> CU count
   int dev = 0;
   HIPCHK_K(hipGetDevice(&dev));
-  const auto key = std::make_pair(dev, (int)((L << 20) | (int)lds));
-  auto it = cache.find(key);
-  if (it == cache.end()) {
-    int per_cu = 0, cus = 0;
-    const void* f = L == 10 ? (const void*)k_tile<10> : L == 11 ? (const void*)k_tile<11> : (const void*)k_tile<12>;
-    HIPCHK_K(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, TPB, lds));
+  auto it = cus_of.find(dev);
+  if (it == cus_of.end()) {
+    int cus = 0;
     HIPCHK_K(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    it = cache.emplace(key, std::max(1, per_cu) * std::max(1, cus)).first;
+    it = cus_of.emplace(dev, std::max(1, cus)).first;
   }
-  return std::min<uint64_t>(ntiles, (uint64_t)it->second);
+  return std::min<uint64_t>(ntiles, (uint64_t)it->second * 64);
 }
 
 void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob,
                 hipStream_t st) {
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = tile_lds_bytes(a);
-  dim3 g((uint32_t)tile_grid((int)a.L, lds, ntiles), batch);
+  dim3 g((uint32_t)tile_grid(ntiles), batch);
 #define L(LL) \
   hipLaunchKernelGGL(k_tile<LL>, g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
   switch (a.L) {
