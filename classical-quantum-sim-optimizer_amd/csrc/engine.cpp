@@ -513,7 +513,7 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
 
 // Tile-sweep matrix form of a block: kind 0 (k <= 2) complex FP32 matrix,
 // kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA, kind 2
-// (k = 4, 5) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
+// (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
 int tile_kind(int k) { return k <= 2 ? 0 : k == 3 ? 1 : 2; }
 size_t tile_matrix_bytes(int k) {
   const size_t d = (size_t)1 << k;
@@ -624,7 +624,7 @@ void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<
   std::vector<int> rem(blocks.size());
   for (size_t b = 0; b < blocks.size(); ++b) rem[b] = (int)b;
   const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
-  auto tileable = [&](const FusedBlock& b) { return !b.qubits.empty() && b.qubits.size() <= 5; };
+  auto tileable = [&](const FusedBlock& b) { return !b.qubits.empty() && b.qubits.size() <= 4; };
   size_t head = 0;
   std::vector<char> used(blocks.size(), 0);
   while (head < rem.size()) {

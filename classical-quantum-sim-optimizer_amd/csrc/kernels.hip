@@ -337,8 +337,10 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 //   0  k <= 2: complex FP32 matrix, FP32 FMA, one thread per group;
 //   1  k = 3: permuted 16x16 real embedding on v_mfma_f32_16x16x4f32, each
 //      lane holding whole complex inputs and outputs;
-//   2  k = 4, 5: 2^(k+1) real embedding on v_mfma_f32_32x32x2f32 with the
+//   2  k = 4: 32x32 real embedding on v_mfma_f32_32x32x2f32 with the
 //      operand layout of k_fused_mfma.
+// 5-qubit blocks stay out of sweeps (ordinary launches): their operands would
+// raise the kernel's register use past three workgroups per CU.
 // MFMA matrices are stored fragment-major (the A value of K step q for lane l
 // at [q][l]), so every operand read is one conflict-free 256-byte LDS row.
 // Tile element t lives at LDS slot swz(t) = t ^ M(t >> 5), a linear XOR
@@ -536,8 +538,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       if (b.kind == 1) {
         tile_block_mfma16(tile, L, b, m);
       } else if (b.kind == 2) {
-        if (b.k == 4) tile_block_mfma32<4>(tile, L, b, m);
-        else tile_block_mfma32<5>(tile, L, b, m);
+        tile_block_mfma32<4>(tile, L, b, m);
       } else {
         if (b.k == 1) tile_block_valu<1>(tile, L, b, m);
         else tile_block_valu<2>(tile, L, b, m);
