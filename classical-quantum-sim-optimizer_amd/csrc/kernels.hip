@@ -374,6 +374,18 @@ __device__ __forceinline__ uint32_t local_offset(const TileBlock& b, int c) {
   return o;
 }
 
+// Group g of a block sits at local index IZ(g) (zeros inserted at the block's
+// local bits). With g = u + v, u and v on disjoint bits, IZ(g) = IZ(u) | IZ(v)
+// and swz(IZ(g)) = swz(IZ(u)) ^ swz(IZ(v)): the lane-dependent part is fixed per
+// block and the loop-dependent part is wave-uniform, so the address update
+// per group runs on the scalar unit (the block's sorted bits are read into
+// scalar registers first).
+template <int K>
+__device__ __forceinline__ void uniform_bits(const TileBlock& b, uint32_t* srt) {
+#pragma unroll
+  for (int i = 0; i < K; ++i) srt[i] = __builtin_amdgcn_readfirstlane(b.srt[i]);
+}
+
 template <int K>
 __device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileBlock& b, const float* m) {
   constexpr int D = 1 << K;
@@ -383,12 +395,15 @@ __device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileB
     mr[i] = m[2 * i];
     mi[i] = m[2 * i + 1];
   }
+  uint32_t srt[K];
+  uniform_bits<K>(b, srt);
   uint32_t off[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) off[j] = swz(local_offset<K>(b, j));
+  const uint32_t lane_part = swz(insert_zeros32(threadIdx.x, srt, K));
   const uint32_t groups = (1u << L) >> K;
-  for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
-    const uint32_t base = swz(insert_zeros32(g, b.srt, K));
+  for (uint32_t g0 = 0; g0 < groups; g0 += TPB) {
+    const uint32_t base = swz(insert_zeros32(g0, srt, K)) ^ lane_part;
     float2 v[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) v[j] = tile[base ^ off[j]];
@@ -423,10 +438,13 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   float a[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) a[s] = e[64 * s + lane];
-  const uint32_t o0 = swz(local_offset<3>(b, 2 * u)), o1 = swz(local_offset<3>(b, 2 * u + 1));
+  uint32_t srt[3];
+  uniform_bits<3>(b, srt);
+  const uint32_t lane_part = swz(insert_zeros32(col, srt, 3));
+  const uint32_t o0 = swz(local_offset<3>(b, 2 * u)) ^ lane_part, o1 = swz(local_offset<3>(b, 2 * u + 1)) ^ lane_part;
   const uint32_t ntiles = ((1u << L) >> 3) / 16;
-  for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = swz(insert_zeros32(mt * 16 + col, b.srt, 3));
+  for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
+    const uint32_t base = swz(insert_zeros32(mt * 16, srt, 3));
     const float2 x0 = tile[base ^ o0], x1 = tile[base ^ o1];
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
@@ -438,45 +456,38 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   }
 }
 
-// `emb` holds the A fragments: emb[(STEPS t + q) 64 + l] = E[32 t + l % 32][2 q + l / 32].
-template <int K>
+// k = 4 on v_mfma_f32_32x32x2f32 with the operand layout of k_fused_mfma.
+// `emb` holds the A fragments: emb[64 q + l] = E[l % 32][2 q + l / 32].
 __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const TileBlock& b, const float* emb) {
-  constexpr int D = 1 << K;
-  constexpr int R = 2 * D;
-  constexpr int T = R / 32;
-  constexpr int STEPS = R / 2;
-  constexpr int NIN = D / 2;
+  constexpr int K = 4, D = 16, STEPS = 16, NIN = 8, NOUT = 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 31, h = lane >> 5;
-  constexpr int NOUT = K == 4 ? 8 : 16;
+  uint32_t srt[K];
+  uniform_bits<K>(b, srt);
+  const uint32_t lane_part = swz(insert_zeros32(j, srt, K));
   uint32_t offs[NIN], outs[NOUT];
 #pragma unroll
-  for (int i = 0; i < NIN; ++i) offs[i] = swz(local_offset<K>(b, 2 * i + h));
+  for (int i = 0; i < NIN; ++i) offs[i] = swz(local_offset<K>(b, 2 * i + h)) ^ lane_part;
 #pragma unroll
-  for (int r = 0; r < NOUT; ++r) outs[r] = swz(local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)));
-  const uint32_t groups = (1u << L) >> K;
-  const uint32_t ntiles = groups / 32;
-  for (uint32_t mt = wave; mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = swz(insert_zeros32(mt * 32 + j, b.srt, K));
+  for (int r = 0; r < NOUT; ++r) outs[r] = swz(local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4))) ^ lane_part;
+  const uint32_t ntiles = ((1u << L) >> K) / 32;
+  for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
+    const uint32_t base = swz(insert_zeros32(mt * 32, srt, K));
     float2 v[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; ++i) v[i] = tile[base ^ offs[i]];
-    floatx16 acc[T];
+    floatx16 acc;
 #pragma unroll
-    for (int t = 0; t < T; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 #pragma unroll
     for (int q = 0; q < STEPS; ++q) {
       const float bv = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
-#pragma unroll
-      for (int t = 0; t < T; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[(STEPS * t + q) * 64 + lane], bv, acc[t], 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[q * 64 + lane], bv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < NOUT; ++r)
-      tile[base ^ outs[r]] = K == 4 ? make_float2(acc[0][r], acc[0][r + 8]) : make_float2(acc[0][r], acc[T - 1][r]);
+    for (int r = 0; r < NOUT; ++r) tile[base ^ outs[r]] = make_float2(acc[r], acc[r + 8]);
   }
+  (void)D;
 }
 
 template <int L>
@@ -538,7 +549,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       if (b.kind == 1) {
         tile_block_mfma16(tile, L, b, m);
       } else if (b.kind == 2) {
-        tile_block_mfma32<4>(tile, L, b, m);
+        tile_block_mfma32(tile, L, b, m);
       } else {
         if (b.k == 1) tile_block_valu<1>(tile, L, b, m);
         else tile_block_valu<2>(tile, L, b, m);
