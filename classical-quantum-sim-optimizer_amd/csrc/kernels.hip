@@ -343,19 +343,6 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // raise the kernel's register use past three workgroups per CU.
 // MFMA matrices are stored fragment-major (the A value of K step q for lane l
 // at [q][l]), so every operand read is one conflict-free 256-byte LDS row.
-// Tile element t lives at LDS slot swz(t) = t ^ M(t >> 5), a linear XOR
-// swizzle (M maps each of bits 5..11 to a fixed 5-bit pattern) that spreads
-// the strided group accesses of blocks on low local bits over the banks
-// (modelled conflict degree 1.3 instead of 3.4 over random block positions).
-// Being linear, swz(base | off) = swz(base) ^ swz(off) for disjoint bits.
-
-__host__ __device__ constexpr uint32_t swz(uint32_t t) {
-  constexpr uint32_t m[7] = {27, 14, 28, 25, 22, 8, 24};
-  uint32_t x = 0;
-  for (int i = 0; i < 7; ++i)
-    if ((t >> (5 + i)) & 1) x ^= m[i];
-  return t ^ x;
-}
 
 __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
   for (int i = 0; i < k; ++i) {
@@ -375,11 +362,12 @@ __device__ __forceinline__ uint32_t local_offset(const TileBlock& b, int c) {
 }
 
 // Group g of a block sits at local index IZ(g) (zeros inserted at the block's
-// local bits). With g = u + v, u and v on disjoint bits, IZ(g) = IZ(u) | IZ(v)
-// and swz(IZ(g)) = swz(IZ(u)) ^ swz(IZ(v)): the lane-dependent part is fixed per
-// block and the loop-dependent part is wave-uniform, so the address update
-// per group runs on the scalar unit (the block's sorted bits are read into
-// scalar registers first).
+// local bits). With g = u + v, u and v on disjoint bits, IZ(g) = IZ(u) | IZ(v):
+// the lane-dependent part is fixed per block and the loop-dependent part is
+// wave-uniform, so the address update per group runs on the scalar unit (the
+// block's sorted bits are read into scalar registers first). An XOR swizzle of
+// the LDS slots halved the bank conflicts but its address arithmetic cost more
+// than it saved (1.66 s vs 1.51 s per 32-qubit step), so tiles are unswizzled.
 template <int K>
 __device__ __forceinline__ void uniform_bits(const TileBlock& b, uint32_t* srt) {
 #pragma unroll
@@ -399,11 +387,11 @@ __device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileB
   uniform_bits<K>(b, srt);
   uint32_t off[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) off[j] = swz(local_offset<K>(b, j));
-  const uint32_t lane_part = swz(insert_zeros32(threadIdx.x, srt, K));
+  for (int j = 0; j < D; ++j) off[j] = local_offset<K>(b, j);
+  const uint32_t lane_part = insert_zeros32(threadIdx.x, srt, K);
   const uint32_t groups = (1u << L) >> K;
   for (uint32_t g0 = 0; g0 < groups; g0 += TPB) {
-    const uint32_t base = swz(insert_zeros32(g0, srt, K)) ^ lane_part;
+    const uint32_t base = insert_zeros32(g0, srt, K) ^ lane_part;
     float2 v[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) v[j] = tile[base ^ off[j]];
@@ -440,11 +428,11 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   for (int s = 0; s < 4; ++s) a[s] = e[64 * s + lane];
   uint32_t srt[3];
   uniform_bits<3>(b, srt);
-  const uint32_t lane_part = swz(insert_zeros32(col, srt, 3));
-  const uint32_t o0 = swz(local_offset<3>(b, 2 * u)) ^ lane_part, o1 = swz(local_offset<3>(b, 2 * u + 1)) ^ lane_part;
+  const uint32_t lane_part = insert_zeros32(col, srt, 3);
+  const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
   const uint32_t ntiles = ((1u << L) >> 3) / 16;
   for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = swz(insert_zeros32(mt * 16, srt, 3));
+    const uint32_t base = insert_zeros32(mt * 16, srt, 3);
     const float2 x0 = tile[base ^ o0], x1 = tile[base ^ o1];
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
@@ -464,15 +452,15 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
   const int j = lane & 31, h = lane >> 5;
   uint32_t srt[K];
   uniform_bits<K>(b, srt);
-  const uint32_t lane_part = swz(insert_zeros32(j, srt, K));
+  const uint32_t lane_part = insert_zeros32(j, srt, K);
   uint32_t offs[NIN], outs[NOUT];
 #pragma unroll
-  for (int i = 0; i < NIN; ++i) offs[i] = swz(local_offset<K>(b, 2 * i + h)) ^ lane_part;
+  for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h) ^ lane_part;
 #pragma unroll
-  for (int r = 0; r < NOUT; ++r) outs[r] = swz(local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4))) ^ lane_part;
+  for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)) ^ lane_part;
   const uint32_t ntiles = ((1u << L) >> K) / 32;
   for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = swz(insert_zeros32(mt * 32, srt, K));
+    const uint32_t base = insert_zeros32(mt * 32, srt, K);
     float2 v[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; ++i) v[i] = tile[base ^ offs[i]];
@@ -531,11 +519,10 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi[i]];
   }
-  const uint32_t sl = swz(threadIdx.x);
   for (; tau < ntiles; tau += gridDim.x) {
     const uint64_t base = tile_base(tau);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[sl ^ swz(TPB * i)] = r[i];
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
     __syncthreads();  // the tile (and, on the first pass, the blob) is in LDS
     const uint64_t nxt = tau + gridDim.x;
     if (nxt < ntiles) {
@@ -557,7 +544,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[sl ^ swz(TPB * i)];
+    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
     __syncthreads();
   }
 }
