@@ -611,65 +611,88 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   plan.ops.push_back(std::move(d));
 }
 
-// Groups fused blocks into tile sweeps. Starting from the first remaining
-// block, later blocks join the sweep while the union of tile qubits (always
-// including qubits 0..3, so a tile row is a 128-byte run) stays within L and
-// they commute with every block passed over (no shared qubit); a passed-over
-// block keeps its place for the next sweep. A sweep of one block is emitted
-// as an ordinary launch.
-void emit_sweeps(Plan& plan, std::vector<FusedBlock>& blocks, const std::vector<Op>& ops, int prec, int n, int L) {
+// Tile sweeps of a unitary run ops[begin, end):
+//  1. fuse to blocks of <= 2 qubits (wider compiled ops stay single blocks);
+//  2. group the blocks into sweeps: a sweep starts from qubits {0,1,2,3} (a
+//     tile row is then a 128-byte run) and repeatedly takes, among the blocks
+//     whose earlier untaken blocks share no qubit with them (so the taken order
+//     is a valid reordering), the one adding the fewest new qubits, while the
+//     union stays within L qubits;
+//  3. fuse each sweep's ops again into blocks of <= kt qubits (fewer passes
+//     over the tile in LDS) and emit one tile launch per sweep, split where the
+//     blob would exceed its LDS budget. A sweep of one block is an ordinary
+//     launch; blocks wider than 4 qubits always are.
+void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int prec, int n, int L, int kt,
+                 const MergeScorer& scorer) {
   constexpr size_t kWindow = 512;
   // LDS per workgroup = 8 * 2^L + blob; the blob cap keeps it within 64 KiB.
   constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = 64;
-  std::vector<int> rem(blocks.size());
-  for (size_t b = 0; b < blocks.size(); ++b) rem[b] = (int)b;
-  const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
-  auto tileable = [&](const FusedBlock& b) { return !b.qubits.empty() && b.qubits.size() <= 4; };
+  std::vector<FusedBlock> pre = greedy_fuse(ops, begin, end, 2);
+  std::vector<uint64_t> mask(pre.size());
+  for (size_t b = 0; b < pre.size(); ++b) mask[b] = qubit_mask(pre[b].qubits);
+  auto tileable = [&](size_t b) { return !pre[b].qubits.empty() && pre[b].qubits.size() <= 4; };
+  std::vector<int> rem(pre.size());
+  for (size_t b = 0; b < pre.size(); ++b) rem[b] = (int)b;
+  std::vector<char> taken(pre.size(), 0);
   size_t head = 0;
-  std::vector<char> used(blocks.size(), 0);
   while (head < rem.size()) {
-    FusedBlock& first = blocks[rem[head]];
-    if (used[rem[head]]) {
+    // rem[head..] holds the untaken blocks in program order.
+    const int h = rem[head];
+    if (!tileable((size_t)h)) {
+      emit_block(plan, pre[h], ops, prec, n);
+      taken[h] = 1;
       ++head;
       continue;
     }
-    if (!tileable(first)) {
-      used[rem[head]] = 1;
-      emit_block(plan, first, ops, prec, n);
-      ++head;
-      continue;
-    }
-    uint64_t smask = 0xFull | qubit_mask(first.qubits);
-    uint64_t blocked = 0;
-    size_t blob = sizeof(k::TileBlock) + tile_matrix_bytes((int)first.qubits.size());
-    std::vector<int> take{rem[head]};
-    size_t seen = 0;
-    for (size_t r = head + 1; r < rem.size() && seen < kWindow; ++r) {
-      const int bi = rem[r];
-      if (used[bi]) continue;
-      ++seen;
-      const FusedBlock& b = blocks[bi];
-      const uint64_t m = qubit_mask(b.qubits);
-      const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)b.qubits.size());
-      if (tileable(b) && !(m & blocked) && __builtin_popcountll(smask | m) <= L && blob + bb <= kTileBlobMax &&
-          take.size() < kTileBlocksMax) {
-        smask |= m;
-        blob += bb;
-        take.push_back(bi);
-      } else {
-        blocked |= m;
-        if ((blocked & all) == all) break;
+    uint64_t smask = 0xFull;
+    std::vector<int> sweep;
+    for (;;) {
+      uint64_t blocked = 0;
+      int best = -1, best_grow = 99;
+      size_t seen = 0;
+      for (size_t r = head; r < rem.size() && seen < kWindow; ++r) {
+        const int b = rem[r];
+        if (taken[b]) continue;
+        ++seen;
+        if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(smask | mask[b]) <= L) {
+          const int grow = __builtin_popcountll(mask[b] & ~smask);
+          if (grow < best_grow) {
+            best = b;
+            best_grow = grow;
+            if (grow == 0) break;
+          }
+        }
+        blocked |= mask[b];
       }
+      if (best < 0 || sweep.size() >= 4 * kTileBlocksMax) break;
+      taken[best] = 1;
+      smask |= mask[best];
+      sweep.push_back(best);
     }
-    for (int bi : take) used[bi] = 1;
-    if (take.size() == 1) {
-      emit_block(plan, first, ops, prec, n);
-    } else {
-      std::vector<FusedBlock*> bp;
-      for (int bi : take) bp.push_back(&blocks[bi]);
-      emit_tile(plan, bp, smask, L, ops);
+    // The head block is always available and fits (<= 4 new qubits), so the
+    // sweep is never empty.
+    std::vector<int> order;
+    for (int b : sweep) order.insert(order.end(), pre[b].ops.begin(), pre[b].ops.end());
+    std::vector<FusedBlock> blocks = greedy_fuse_list(ops, order, std::max(1, std::min(kt, 4)), scorer);
+    size_t i = 0;
+    while (i < blocks.size()) {
+      size_t blob = 0, j = i;
+      while (j < blocks.size() && j - i < kTileBlocksMax) {
+        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
+        if (blob + bb > kTileBlobMax) break;
+        blob += bb;
+        ++j;
+      }
+      if (j == i + 1) {
+        emit_block(plan, blocks[i], ops, prec, n);
+      } else {
+        std::vector<FusedBlock*> bp;
+        for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
+        emit_tile(plan, bp, smask, L, ops);
+      }
+      i = j;
     }
-    ++head;
+    while (head < rem.size() && taken[rem[head]]) ++head;
   }
 }
 
@@ -695,10 +718,10 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
       scorer = [n](const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& o) {
         return planner_score(a, b, o, n);
       };
-    std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
     if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
-      emit_sweeps(plan, blocks, ops, prec, n, tile);
+      emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
     } else {
+      std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
       for (FusedBlock& blk : blocks) emit_block(plan, blk, ops, prec, n);
     }
     i = j;

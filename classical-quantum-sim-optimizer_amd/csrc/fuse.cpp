@@ -105,10 +105,17 @@ void apply_small(const Op& op, std::vector<cplx>& v, const std::vector<int>& loc
 
 std::vector<FusedBlock> greedy_fuse(const std::vector<Op>& ops, size_t begin, size_t end, int max_k,
                                     const MergeScorer& scorer) {
+  std::vector<int> order;
+  for (size_t i = begin; i < end; ++i) order.push_back((int)i);
+  return greedy_fuse_list(ops, order, max_k, scorer);
+}
+
+std::vector<FusedBlock> greedy_fuse_list(const std::vector<Op>& ops, const std::vector<int>& order, int max_k,
+                                         const MergeScorer& scorer) {
   std::vector<FusedBlock> blocks;
   std::vector<std::set<uint32_t>> qsets;
   std::vector<int> last(64, -1);
-  for (size_t i = begin; i < end; ++i) {
+  for (int i : order) {
     std::vector<uint32_t> q = op_qubits(ops[i]);
     int t = -1;
     for (uint32_t x : q) t = std::max(t, last[x]);

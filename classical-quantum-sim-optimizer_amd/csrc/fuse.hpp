@@ -36,6 +36,10 @@ using MergeScorer = std::function<double(const FusedBlock& a, const FusedBlock& 
 // Greedy fuser over ops[begin, end) (all unitary).
 std::vector<FusedBlock> greedy_fuse(const std::vector<Op>& ops, size_t begin, size_t end, int max_k,
                                     const MergeScorer& scorer = nullptr);
+// The same over the ops listed in `order` (a valid execution order of
+// unitary ops; blocks keep the listed ops' relative order per qubit).
+std::vector<FusedBlock> greedy_fuse_list(const std::vector<Op>& ops, const std::vector<int>& order, int max_k,
+                                         const MergeScorer& scorer = nullptr);
 
 // Fill block.mat and block.diagonal from its ops.
 void build_block_matrix(FusedBlock& block, const std::vector<Op>& ops);
