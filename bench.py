@@ -3,9 +3,10 @@ engine on one MI355X.
 
 Workload (one step): evolve a 32-qubit, depth-40 random circuit (Haar SU(2) on
 every qubit plus CZ/CX on a random perfect matching per layer, 1,920 gates,
-seed 1234) from |0...0> in complex64 with greedy <=k-qubit block fusion. The
-same circuit is also run once on the plain complex128 path (one launch per
-compiled gate, reference arithmetic) for the speed-up and the accuracy check.
+seed 1234) from |0...0> in complex64 with greedy block fusion (blocks of <= 3
+qubits by default) grouped into LDS tile sweeps (12-qubit tiles). The same
+circuit is also run once on the plain complex128 path (one launch per compiled
+gate, reference arithmetic) for the speed-up and the accuracy check.
 
 Prints one JSON line. `value` = G * 2^n / seconds per step.
 """
@@ -105,7 +106,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--max-fused", type=int, default=4)
+    ap.add_argument("--max-fused", type=int, default=3)
     ap.add_argument("--fusion", default="greedy")
     ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")

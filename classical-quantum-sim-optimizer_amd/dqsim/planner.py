@@ -88,13 +88,15 @@ def launch_records(circuit, max_fused_qubits: int, precision: str = "complex64",
     return rows, total_ms
 
 
-def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int = 0):
+def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int = 0, log=None):
     """(X, y): features and log(ms per 2^24 amplitudes) of every launch of the
     circuits, each evolved with a width drawn from `widths`."""
     rng = np.random.default_rng(seed)
     X: List[List[float]] = []
     y: List[float] = []
-    for c in circuits:
+    for i, c in enumerate(circuits):
+        if log is not None and i % 25 == 0:
+            log(f"[planner] dataset: circuit {i}, {len(y)} launches so far")
         n = c.num_qubits
         k = int(widths[int(rng.integers(len(widths)))])
         rows, _ = launch_records(c, k)

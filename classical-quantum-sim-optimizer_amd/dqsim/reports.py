@@ -9,7 +9,8 @@
   batch_report ....... thousands of small uneven circuits in one persistent
                        graph-launched kernel: circuits/s, p50/p99 latency.
   planner_report ..... learned fusion planner trained on engine-recorded launch
-                       timings vs the greedy fuser on held-out circuits.
+                       timings vs the greedy fuser on held-out circuits (one
+                       launch per fused block, the plans the model times).
 """
 
 from __future__ import annotations
@@ -94,7 +95,7 @@ def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(2
                    seed: int = 5, epochs: int = 300, log=print) -> dict:
     t0 = time.perf_counter()
     X, y = planner.dataset(planner.training_circuits(train_circuits, nmin=train_n[0], nmax=train_n[1], dmin=20,
-                                                     dmax=80, seed=seed), seed=seed)
+                                                     dmax=80, seed=seed), seed=seed, log=log)
     collect_s = time.perf_counter() - t0
     perm = np.random.default_rng(seed).permutation(len(y))
     cut = int(0.9 * len(y))
@@ -119,8 +120,9 @@ def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(2
             if fusion == "learned":
                 planner.install(w)
             try:
+                # One launch per block: the model predicts standalone launch times.
                 sim = _core.StatevectorSimulator(seed=1, precision="complex64", fusion=fusion, max_fused_qubits=k,
-                                                 max_memory_mb=mem)
+                                                 max_memory_mb=mem, tile_qubits=0)
                 sim.evolve_timed(c, record=False)  # warm-up
                 _, launches, _, ms, res = sim.evolve_timed(c, record=False, keep=True)
                 err, _ = res.compare(rres)
