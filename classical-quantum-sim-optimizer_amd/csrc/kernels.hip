@@ -512,20 +512,26 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     }
     return base;
   };
+  // Pipeline per tile: issue the loads of the next tile, apply the blocks to
+  // this tile in LDS, store it, then move the loaded registers into LDS. The
+  // next tile's loads are consumed in the iteration that issued them, after
+  // this tile's stores, so the wait for them lets the stores stay in flight
+  // (vector-memory completions count in issue order, stores included).
   float2 r[PER];
   uint64_t tau = blockIdx.x;
   if (tau < ntiles) {
     const uint64_t b0 = tile_base(tau);
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi[i]];
-  }
-  for (; tau < ntiles; tau += gridDim.x) {
-    const uint64_t base = tile_base(tau);
 #pragma unroll
     for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
-    __syncthreads();  // the tile (and, on the first pass, the blob) is in LDS
+  }
+  __syncthreads();  // the first tile and the blob are in LDS
+  for (; tau < ntiles; tau += gridDim.x) {
+    const uint64_t base = tile_base(tau);
     const uint64_t nxt = tau + gridDim.x;
-    if (nxt < ntiles) {
+    const bool more = nxt < ntiles;
+    if (more) {
       const uint64_t nb = tile_base(nxt);
 #pragma unroll
       for (int i = 0; i < PER; ++i) r[i] = st[nb | lo | hi[i]];
@@ -545,6 +551,11 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    if (more) {
+      __syncthreads();  // every wave has read this tile out of LDS
+#pragma unroll
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+    }
     __syncthreads();
   }
 }
