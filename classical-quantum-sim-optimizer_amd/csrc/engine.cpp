@@ -511,9 +511,10 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
   return m;
 }
 
-// Tile-sweep matrix form of a block: kind 0 (k <= 2) complex FP32 matrix,
-// kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA, kind 2
-// (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
+// Tile-sweep matrix form of a block: kind 1 (k = 3; narrower blocks are
+// padded to 3 qubits) permuted 16x16 real embedding for the 16x16x4 MFMA,
+// kind 2 (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
+// (kind 0, a complex FP32 matrix, serves no tile block.)
 int tile_kind(int k) { return k <= 2 ? 0 : k == 3 ? 1 : 2; }
 size_t tile_matrix_bytes(int k) {
   const size_t d = (size_t)1 << k;
@@ -569,7 +570,7 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   // Pad the tile to exactly L qubits with the lowest free qubits.
   for (uint32_t q = 0; __builtin_popcountll(smask) < L; ++q) smask |= 1ull << q;
   std::vector<int> local(64, -1);
-  int nl = 0;
+  int nl = 0;  // tile qubits so far
   for (uint32_t q = 0; q < 64; ++q)
     if ((smask >> q) & 1) {
       d.targs.spos[nl] = q;
@@ -581,7 +582,21 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   std::vector<k::TileBlock> tb;
   std::vector<float> mats;
   const size_t head = blocks.size() * sizeof(k::TileBlock);
-  for (FusedBlock* b : blocks) {
+  for (FusedBlock* orig : blocks) {
+    // Blocks of 1 or 2 qubits become 3-qubit blocks (U kron I on the highest
+    // free tile qubits): the 16x16x4 MFMA applies them at the cost of a
+    // 3-qubit block, off the vector ALU.
+    FusedBlock padded;
+    FusedBlock* b = orig;
+    if (orig->qubits.size() < 3) {
+      padded.ops = orig->ops;
+      std::vector<uint32_t> q = orig->qubits;
+      for (int i = nl - 1; i >= 0 && q.size() < 3; --i)
+        if (std::find(q.begin(), q.end(), (uint32_t)d.targs.spos[i]) == q.end()) q.push_back(d.targs.spos[i]);
+      std::sort(q.rbegin(), q.rend());
+      padded.qubits = q;
+      b = &padded;
+    }
     if (b->mat.empty()) build_block_matrix(*b, ops);
     k::TileBlock t{};
     t.k = (uint32_t)b->qubits.size();
@@ -598,7 +613,7 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
     tb.push_back(t);
     d.gates += (int)b->ops.size();
     d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
-    d.tile_blocks.push_back(b->qubits);
+    d.tile_blocks.push_back(orig->qubits);
     d.tile_mfma.push_back((int)(t.kind != 0));
   }
   std::vector<uint8_t> blob(head + mats.size() * sizeof(float));
@@ -678,7 +693,7 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
     while (i < blocks.size()) {
       size_t blob = 0, j = i;
       while (j < blocks.size() && j - i < kTileBlocksMax) {
-        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
+        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes(std::max(3, (int)blocks[j].qubits.size()));
         if (blob + bb > kTileBlobMax) break;
         blob += bb;
         ++j;

@@ -334,9 +334,9 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // one tile are applied in LDS, the loads of its next tile are in flight (the
 // block arithmetic reads only LDS, so it never waits on those loads).
 // Block kinds (TileBlock.kind):
-//   0  k <= 2: complex FP32 matrix, FP32 FMA, one thread per group;
-//   1  k = 3: permuted 16x16 real embedding on v_mfma_f32_16x16x4f32, each
-//      lane holding whole complex inputs and outputs;
+//   1  k = 3 (narrower blocks arrive padded to 3 qubits): permuted 16x16 real
+//      embedding on v_mfma_f32_16x16x4f32, each lane holding whole complex
+//      inputs and outputs;
 //   2  k = 4: 32x32 real embedding on v_mfma_f32_32x32x2f32 with the
 //      operand layout of k_fused_mfma.
 // 5-qubit blocks stay out of sweeps (ordinary launches): their operands would
@@ -374,42 +374,6 @@ __device__ __forceinline__ void uniform_bits(const TileBlock& b, uint32_t* srt) 
   for (int i = 0; i < K; ++i) srt[i] = __builtin_amdgcn_readfirstlane(b.srt[i]);
 }
 
-template <int K>
-__device__ __forceinline__ void tile_block_valu(float2* tile, int L, const TileBlock& b, const float* m) {
-  constexpr int D = 1 << K;
-  float mr[D * D], mi[D * D];
-#pragma unroll
-  for (int i = 0; i < D * D; ++i) {
-    mr[i] = m[2 * i];
-    mi[i] = m[2 * i + 1];
-  }
-  uint32_t srt[K];
-  uniform_bits<K>(b, srt);
-  uint32_t off[D];
-#pragma unroll
-  for (int j = 0; j < D; ++j) off[j] = local_offset<K>(b, j);
-  const uint32_t lane_part = insert_zeros32(threadIdx.x, srt, K);
-  const uint32_t groups = (1u << L) >> K;
-  for (uint32_t g0 = 0; g0 < groups; g0 += TPB) {
-    const uint32_t base = insert_zeros32(g0, srt, K) ^ lane_part;
-    float2 v[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) v[j] = tile[base ^ off[j]];
-#pragma unroll
-    for (int r = 0; r < D; ++r) {
-      float ar = 0.f, ai = 0.f;
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        ar = __builtin_fmaf(mr[r * D + c], v[c].x, ar);
-        ar = __builtin_fmaf(-mi[r * D + c], v[c].y, ar);
-        ai = __builtin_fmaf(mr[r * D + c], v[c].y, ai);
-        ai = __builtin_fmaf(mi[r * D + c], v[c].x, ai);
-      }
-      tile[base ^ off[r]] = make_float2(ar, ai);
-    }
-  }
-}
-
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // k = 3 on v_mfma_f32_16x16x4f32 (A lane l: A[l%16][l/16]; B lane l:
@@ -431,8 +395,13 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   const uint32_t lane_part = insert_zeros32(col, srt, 3);
   const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
   const uint32_t ntiles = ((1u << L) >> 3) / 16;
-  for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = insert_zeros32(mt * 16, srt, 3);
+  // Masked-carry stepping: with the block's bits set, adding IZ(step) carries
+  // over them, so base = IZ(mt * 16) advances by 3 scalar operations.
+  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t step = insert_zeros32(16 * (TPB / 64), srt, 3);
+  uint32_t base = insert_zeros32(w * 16, srt, 3);
+  for (uint32_t mt = w; mt < ntiles; mt += TPB / 64, base = ((base | mask) + step) & ~mask) {
     const float2 x0 = tile[base ^ o0], x1 = tile[base ^ o1];
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
@@ -459,8 +428,11 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
 #pragma unroll
   for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)) ^ lane_part;
   const uint32_t ntiles = ((1u << L) >> K) / 32;
-  for (uint32_t mt = __builtin_amdgcn_readfirstlane(wave); mt < ntiles; mt += TPB / 64) {
-    const uint32_t base = insert_zeros32(mt * 32, srt, K);
+  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]) | (1u << srt[3]);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t step = insert_zeros32(32 * (TPB / 64), srt, K);
+  uint32_t base = insert_zeros32(w * 32, srt, K);
+  for (uint32_t mt = w; mt < ntiles; mt += TPB / 64, base = ((base | mask) + step) & ~mask) {
     float2 v[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; ++i) v[i] = tile[base ^ offs[i]];
@@ -539,14 +511,8 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
       const TileBlock& b = blocks[bi];
       const float* m = (const float*)(meta + b.mat_off);
-      if (b.kind == 1) {
-        tile_block_mfma16(tile, L, b, m);
-      } else if (b.kind == 2) {
-        tile_block_mfma32(tile, L, b, m);
-      } else {
-        if (b.k == 1) tile_block_valu<1>(tile, L, b, m);
-        else tile_block_valu<2>(tile, L, b, m);
-      }
+      if (b.kind == 1) tile_block_mfma16(tile, L, b, m);
+      else tile_block_mfma32(tile, L, b, m);
       __syncthreads();
     }
 #pragma unroll
