@@ -611,6 +611,7 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
     }
     std::sort(t.srt, t.srt + t.k);
     tb.push_back(t);
+    d.targs.kmax = std::max(d.targs.kmax, t.k);
     d.gates += (int)b->ops.size();
     d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
     d.tile_blocks.push_back(orig->qubits);

@@ -384,7 +384,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // so lane l (u = l/16) feeds the four K steps from inputs 2u and 2u+1 of
 // group l%16 and receives outputs 2u and 2u+1 of that group whole.
 // `e` holds the A fragments: e[64 s + l] = embedding[l % 16][4 s + l / 16].
-__device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const TileBlock& b, const float* e) {
+template <int L>
+__device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock& b, const float* e) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, u = lane >> 4;
   float a[4];
@@ -394,29 +395,53 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, int L, const Til
   uniform_bits<3>(b, srt);
   const uint32_t lane_part = insert_zeros32(col, srt, 3);
   const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
-  const uint32_t ntiles = ((1u << L) >> 3) / 16;
+  // 16-group tiles per wave, U of them per iteration with independent MFMA
+  // chains interleaved.
+  constexpr int NT = (1 << L) / 8 / 16;  // 16-group tiles of the block
+  constexpr int PERW = NT / (TPB / 64);   // L >= 10: at least 2 per wave
+  constexpr int U = PERW >= 4 ? 4 : PERW;
+  static_assert(NT % (TPB / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
   // Masked-carry stepping: with the block's bits set, adding IZ(step) carries
   // over them, so base = IZ(mt * 16) advances by 3 scalar operations.
   const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t step = insert_zeros32(16 * (TPB / 64), srt, 3);
   uint32_t base = insert_zeros32(w * 16, srt, 3);
-  for (uint32_t mt = w; mt < ntiles; mt += TPB / 64, base = ((base | mask) + step) & ~mask) {
-    const float2 x0 = tile[base ^ o0], x1 = tile[base ^ o1];
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0.x, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x0.y, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1.x, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1.y, acc, 0, 0, 0);
-    tile[base ^ o0] = make_float2(acc[0], acc[1]);
-    tile[base ^ o1] = make_float2(acc[2], acc[3]);
+#pragma unroll
+  for (int it = 0; it < PERW; it += U) {
+    uint32_t bs[U];
+    float2 x0[U], x1[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      bs[j] = base;
+      base = ((base | mask) + step) & ~mask;
+      x0[j] = tile[bs[j] ^ o0];
+      x1[j] = tile[bs[j] ^ o1];
+    }
+    floatx4 acc[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], x0[j].x, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], x0[j].y, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1[j].x, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[j].y, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
+      tile[bs[j] ^ o1] = make_float2(acc[j][2], acc[j][3]);
+    }
   }
 }
 
 // k = 4 on v_mfma_f32_32x32x2f32 with the operand layout of k_fused_mfma.
 // `emb` holds the A fragments: emb[64 q + l] = E[l % 32][2 q + l / 32].
-__device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const TileBlock& b, const float* emb) {
-  constexpr int K = 4, D = 16, STEPS = 16, NIN = 8, NOUT = 8;
+template <int L>
+__device__ __forceinline__ void tile_block_mfma32(float2* tile, const TileBlock& b, const float* emb) {
+  constexpr int K = 4, STEPS = 16, NIN = 8, NOUT = 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 31, h = lane >> 5;
   uint32_t srt[K];
@@ -427,30 +452,52 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, int L, const Til
   for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h) ^ lane_part;
 #pragma unroll
   for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)) ^ lane_part;
-  const uint32_t ntiles = ((1u << L) >> K) / 32;
+  // 32-group tiles per wave (at L = 10 only waves 0 and 1 have one), U per
+  // iteration with independent MFMA chains sharing each A operand read.
+  constexpr int NT = (1 << L) / 16 / 32;
+  constexpr int PERW = (NT + TPB / 64 - 1) / (TPB / 64);
+  constexpr int U = PERW >= 2 ? 2 : 1;
+  static_assert(PERW % U == 0, "32-group tiles per wave split evenly");
   const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]) | (1u << srt[3]);
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t step = insert_zeros32(32 * (TPB / 64), srt, K);
   uint32_t base = insert_zeros32(w * 32, srt, K);
-  for (uint32_t mt = w; mt < ntiles; mt += TPB / 64, base = ((base | mask) + step) & ~mask) {
-    float2 v[NIN];
+  if (NT < TPB / 64 && (int)w >= NT) return;  // wave-uniform: no 32-group tile for this wave
 #pragma unroll
-    for (int i = 0; i < NIN; ++i) v[i] = tile[base ^ offs[i]];
-    floatx16 acc;
+  for (int it = 0; it < PERW; it += U) {
+    uint32_t bs[U];
+    float2 v[U][NIN];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    for (int u = 0; u < U; ++u) {
+      bs[u] = base;
+      base = ((base | mask) + step) & ~mask;
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) v[u][i] = tile[bs[u] ^ offs[i]];
+    }
+    floatx16 acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[u][r] = 0.f;
 #pragma unroll
     for (int q = 0; q < STEPS; ++q) {
-      const float bv = q < STEPS / 2 ? v[q].x : v[q - STEPS / 2].y;
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(emb[q * 64 + lane], bv, acc, 0, 0, 0);
+      const float av = emb[q * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float bv = q < STEPS / 2 ? v[u][q].x : v[u][q - STEPS / 2].y;
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[u], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int r = 0; r < NOUT; ++r) tile[base ^ outs[r]] = make_float2(acc[r], acc[r + 8]);
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < NOUT; ++r) tile[bs[u] ^ outs[r]] = make_float2(acc[u][r], acc[u][r + 8]);
   }
-  (void)D;
 }
 
-template <int L>
+// K4: the sweep holds 4-qubit blocks (the 32x32x2 path is compiled in only
+// then, keeping the register count of 3-qubit sweeps at four workgroups/CU).
+template <int L, bool K4>
 __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t stride, uint64_t ntiles, TileArgs a,
                                               const float4* __restrict__ blob) {
   extern __shared__ float4 lds4[];
@@ -511,8 +558,8 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
       const TileBlock& b = blocks[bi];
       const float* m = (const float*)(meta + b.mat_off);
-      if (b.kind == 1) tile_block_mfma16(tile, L, b, m);
-      else tile_block_mfma32(tile, L, b, m);
+      if (!K4 || b.kind == 1) tile_block_mfma16<L>(tile, b, m);
+      else tile_block_mfma32<L>(tile, b, m);
       __syncthreads();
     }
 #pragma unroll
@@ -1118,12 +1165,13 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = tile_lds_bytes(a);
   dim3 g((uint32_t)tile_grid(ntiles), batch);
-#define L(LL) \
-  hipLaunchKernelGGL(k_tile<LL>, g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
+#define L(LL, K4) \
+  hipLaunchKernelGGL((k_tile<LL, K4>), g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
+  const bool k4 = a.kmax >= 4;
   switch (a.L) {
-    case 10: L(10); break;
-    case 11: L(11); break;
-    default: L(12); break;
+    case 10: if (k4) L(10, true); else L(10, false); break;
+    case 11: if (k4) L(11, true); else L(11, false); break;
+    default: if (k4) L(12, true); else L(12, false); break;
   }
 #undef L
 }

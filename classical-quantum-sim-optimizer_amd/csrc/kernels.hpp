@@ -93,6 +93,7 @@ struct TileArgs {
   uint32_t L;         // tile qubits
   uint32_t nblocks;
   uint32_t blob16;    // blob size in 16-byte units
+  uint32_t kmax;      // widest block of the sweep (3 or 4)
   uint32_t spos[12];  // S sorted ascending (global qubit of local bit i)
 };
 // Dynamic LDS of one workgroup: the tile plus the blob.
