@@ -99,6 +99,32 @@ def cpu_baseline(n, depth, threads):
             "sample": f"random_su2 n={n} depth={depth} ({g} gates), FP64 oracle, {dt:.2f} s"}
 
 
+def brickwork_cpu_baseline(shots=1000, seed=42, repeats=5):
+    """The oracle (FP64 C restatement of the reference algorithm) on the full
+    20-qubit brickwork with 1000 shots, 4 threads and all threads: median of 5
+    runs after a warm-up."""
+    from dqsim import workloads
+    from oracle import oracle
+
+    circ = workloads.brickwork(n=20, layers=20, seed=seed)
+    gates = workloads.gate_count(circ.data["instructions"])
+    out = {}
+    for threads in sorted({4, min(16, os.cpu_count() or 1)}):
+        oracle.set_threads(threads)
+        oracle.run_shots(circ.data, shots, seed)
+        walls = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            oracle.run_shots(circ.data, shots, seed)
+            walls.append(time.perf_counter() - t0)
+        walls.sort()
+        med = walls[len(walls) // 2]
+        out[f"threads_{threads}"] = {"wall_ms_median": med * 1e3, "gate_amp_per_s": gates * float(1 << 20) / med}
+        log(f"[brickwork] cpu oracle, {threads} threads: {out[f'threads_{threads}']}")
+    out["kind"] = "port"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,15 +141,19 @@ def main():
     ap.add_argument("--cpu-depth", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--report", choices=["precision", "batch", "planner"], default=None,
+    ap.add_argument("--report", choices=["precision", "batch", "planner", "brickwork", "batch-loop"], default=None,
                     help="run one secondary report instead of the headline benchmark")
     args = ap.parse_args()
     if args.report:
         from dqsim import reports
 
         fn = {"precision": reports.precision_report, "batch": reports.batch_report,
-              "planner": reports.planner_report}[args.report]
-        print(json.dumps({"report": args.report, **fn(log=log)}), flush=True)
+              "planner": reports.planner_report, "brickwork": reports.brickwork_report,
+              "batch-loop": reports.batch_loop_report}[args.report]
+        res = fn(log=log)
+        if args.report == "brickwork":
+            res["cpu_baseline"] = brickwork_cpu_baseline()
+        print(json.dumps({"report": args.report, **res}), flush=True)
         return
     if args.copy_calibration:
         import torch

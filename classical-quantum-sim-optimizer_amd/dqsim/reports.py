@@ -8,6 +8,12 @@
                        Philox stream from both states.
   batch_report ....... thousands of small uneven circuits in one persistent
                        graph-launched kernel: circuits/s, p50/p99 latency.
+  brickwork_report ... 20-qubit brickwork (20 layers of U3 + CX, seed 42), 1000
+                       shots: complex128 plain (reference semantics) and
+                       complex64 fused, median of 5 runs after a warm-up.
+  batch_loop_report .. thousands of small uneven circuits through the
+                       per-circuit API (the batched launcher is disabled):
+                       circuits/s, p50/p99 latency, gate-amp/s.
   planner_report ..... learned fusion planner trained on engine-recorded launch
                        timings vs the greedy fuser on held-out circuits (one
                        launch per fused block, the plans the model times).
@@ -139,4 +145,54 @@ def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(2
            "learned_vs_best_greedy": min(tot["greedy_k4"], tot["greedy_k5"]) / tot["learned_k5"],
            "max_err": max(r[k]["max_err"] for r in rows for k in ("greedy_k4", "greedy_k5", "learned_k5"))}
     log(f"[planner] summary {out['total_ms']} gain {out['learned_vs_best_greedy']:.3f}")
+    return out
+
+
+def brickwork_report(n: int = 20, layers: int = 20, shots: int = 1000, seed: int = 42, repeats: int = 5,
+                     log=print) -> dict:
+    circ = workloads.brickwork(n=n, layers=layers, seed=seed)
+    gates = workloads.gate_count(circ.data["instructions"])
+    out = {"qubits": n, "gates": gates, "shots": shots}
+    for name, kw in (("complex128_plain", {}),
+                     ("complex64_fused", {"precision": "complex64", "fusion": "greedy", "max_fused_qubits": 3})):
+        sim = _core.StatevectorSimulator(seed=seed, **kw)
+        sim.simulate_shots(circ, shots=shots)  # warm-up
+        walls = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            counts = sim.simulate_shots(circ, shots=shots)
+            walls.append(time.perf_counter() - t0)
+        rows, launches, _, dev_ms, _ = sim.evolve_timed(circ, record=True)
+        nbytes = sum(r["bytes"] for r in rows)
+        med = float(np.median(walls))
+        out[name] = {"wall_ms_median": med * 1e3, "gate_amp_per_s_wall": gates * float(1 << n) / med,
+                     "evolve_device_ms": dev_ms, "launches": launches,
+                     "evolve_hbm_gbs": nbytes / (dev_ms / 1e3) / 1e9, "distinct_outcomes": len(counts)}
+        log(f"[brickwork] {name}: {out[name]}")
+    return out
+
+
+def batch_loop_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, log=print) -> dict:
+    circs = [workloads.batch_circuit(i, base_seed=base_seed) for i in range(count)]
+    work = sum(workloads.gate_count(c.data["instructions"]) * float(1 << c.num_qubits) for c in circs)
+    lat = np.zeros(count)
+    _core.StatevectorSimulator(seed=0, precision="complex64", fusion="greedy",
+                               max_fused_qubits=3).simulate_shots(circs[0], shots=shots)  # warm-up
+    t0 = time.perf_counter()
+    for i, c in enumerate(circs):
+        t = time.perf_counter()
+        sim = _core.StatevectorSimulator(seed=base_seed + i, precision="complex64", fusion="greedy",
+                                         max_fused_qubits=3)
+        counts = sim.simulate_shots(c, shots=shots)
+        lat[i] = time.perf_counter() - t
+        if sum(counts.values()) != shots:
+            raise RuntimeError(f"circuit {i}: {sum(counts.values())} shots returned")
+        if i % 512 == 0:
+            log(f"[batch-loop] circuit {i}/{count}")
+    wall = time.perf_counter() - t0
+    out = {"circuits": count, "shots_each": shots, "wall_s": wall, "circuits_per_s": count / wall,
+           "gate_amp_per_s": work / wall, "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+           "latency_ms_p99": float(np.percentile(lat, 99) * 1e3), "gate_amp_total": work,
+           "path": "one simulate_shots call per circuit (complex64, greedy <=3-qubit fusion, tile sweeps)"}
+    log(f"[batch-loop] {out}")
     return out
