@@ -210,7 +210,8 @@ def main():
              "device_copy_gbs": copy_gbs}
 
     if not args.no_plain:
-        plain = dqsim.StatevectorSimulator(seed=1, precision="complex128", fusion="none", max_memory_mb=mem_mb)
+        plain = dqsim.StatevectorSimulator(seed=1, precision="complex128", fusion="none", max_memory_mb=mem_mb,
+                                           tile_qubits=0)
         t = time.perf_counter()
         prow, plaunch, _, pms, pres = plain.evolve_timed(circ, repeats=1, record=True, keep=True)
         pwall = time.perf_counter() - t
@@ -228,6 +229,19 @@ def main():
         extra["speedup_vs_plain_complex128"] = pms / (sum(dev_ms) / len(dev_ms))
         extra["max_amp_err_vs_fp64"] = max_err
         extra["l2_err_vs_fp64"] = l2_err
+        # The default complex128 path: the same unfused ops in exact LDS sweeps
+        # (bit-identical to one launch per op).
+        exact = dqsim.StatevectorSimulator(seed=1, precision="complex128", fusion="none", max_memory_mb=mem_mb)
+        t = time.perf_counter()
+        erow, elaunch, _, ems, eres = exact.evolve_timed(circ, repeats=1, record=True, keep=True)
+        ewall = time.perf_counter() - t
+        log(f"[bench] exact complex128 sweeps: {ewall:.2f} s wall, device {ems:.1f} ms, {elaunch} launches")
+        _, _, _, _, pres = plain.evolve_timed(circ, repeats=1, record=False, keep=True)
+        emax, _ = eres.compare(pres)
+        del eres, pres
+        extra["exact_complex128_sweeps"] = {
+            "value": gates * float(1 << n) / (ems / 1e3), "ms": ems, "wall_s": ewall, "launches": elaunch,
+            "speedup_vs_plain_complex128": pms / ems, "max_abs_diff_vs_plain": emax}
 
     workload_key = f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
     cpu = None

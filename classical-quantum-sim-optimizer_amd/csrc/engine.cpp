@@ -228,6 +228,7 @@ struct DevOp {
   size_t tile_off = 0;
   std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
   std::vector<int> tile_mfma;
+  bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -712,11 +713,183 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
   }
 }
 
+// Exact complex128 sweeps: maximal runs of consecutive compiled ops (program
+// order, nothing reordered) whose qubits fit in L tile qubits together with
+// qubits 0..3; each op keeps its per-op kernel's arithmetic, so results are
+// bit-identical to one launch per op. Dense ops wider than 3 qubits and
+// diagonals with more than 32 entries run as their own launches.
+bool exact_tileable(const Op& op) {
+  switch (op.kind) {
+    case OpKind::One:
+    case OpKind::X:
+    case OpKind::Swap:
+    case OpKind::Noop: return true;
+    case OpKind::Diagonal: {
+      int ne = 0;
+      for (const cplx& e : op.mat) ne += !(e.re == 1.0 && e.im == 0.0);
+      return ne <= 32;
+    }
+    case OpKind::Dense: return op.qubits.size() <= 3;
+    default: return false;
+  }
+}
+
+size_t exact_op_bytes(const Op& op) {
+  size_t data = 0;
+  if (op.kind == OpKind::One) data = 64;
+  else if (op.kind == OpKind::Diagonal) data = 128 + 16 * op.mat.size();
+  else if (op.kind == OpKind::Dense) data = 16 * op.mat.size();
+  return sizeof(k::TileOp) + ((data + 15) & ~(size_t)15);
+}
+
+void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, uint64_t smask, int L) {
+  DevOp d;
+  d.kind = D_TILE;
+  d.exact = true;
+  d.category = 1;
+  d.gates = 0;
+  for (uint32_t q = 0; __builtin_popcountll(smask) < L; ++q) smask |= 1ull << q;
+  std::vector<int> local(64, -1);
+  int nl = 0;
+  for (uint32_t q = 0; q < 64; ++q)
+    if ((smask >> q) & 1) {
+      d.targs.spos[nl] = q;
+      local[q] = nl++;
+      d.touched.push_back(q);
+    }
+  auto lmask = [&](uint64_t g) {
+    uint32_t m = 0;
+    for (uint32_t q = 0; q < 64; ++q)
+      if ((g >> q) & 1) m |= 1u << local[q];
+    return m;
+  };
+  std::vector<k::TileOp> tops;
+  std::vector<uint8_t> data;
+  std::vector<std::vector<uint8_t>> datas;
+  for (size_t o = i; o < j; ++o) {
+    if (ops[o].kind == OpKind::Noop) {
+      d.src.push_back((int)o);
+      continue;
+    }
+    DevOp x = lower(ops[o], plan, 16);
+    k::TileOp t{};
+    t.nbits = x.e.nbits;
+    for (uint32_t b = 0; b < x.e.nbits; ++b) t.pos[b] = (uint32_t)local[x.e.pos[b]];
+    t.fixed = lmask(x.e.fixed);
+    std::vector<uint8_t> dd;
+    auto put = [&](const void* p, size_t bytes) {
+      const uint8_t* c = (const uint8_t*)p;
+      dd.insert(dd.end(), c, c + bytes);
+    };
+    switch (x.kind) {
+      case D_ONE:
+        t.kind = 0;
+        t.bbit = 1u << local[x.target];
+        put(x.u, sizeof(x.u));
+        break;
+      case D_X:
+        t.kind = 1;
+        t.abit = 0;
+        t.bbit = 1u << local[x.target];
+        break;
+      case D_SWAP:
+        t.kind = 1;
+        t.abit = 1u << local[x.a];
+        t.bbit = 1u << local[x.b];
+        break;
+      case D_DIAG: {
+        t.kind = 2;
+        t.k = (uint32_t)x.pat.size();
+        uint32_t pats[32] = {};
+        for (size_t e = 0; e < x.pat.size(); ++e) pats[e] = lmask(x.pat[e]);
+        put(pats, sizeof(pats));
+        put(x.dval.data(), x.dval.size() * sizeof(double));
+        break;
+      }
+      case D_DENSE: {
+        t.kind = 3;
+        t.k = (uint32_t)x.kq;
+        for (int b = 0; b < x.kq; ++b) t.q[b] = (uint32_t)local[ops[o].qubits[b]];
+        put(plan.blob.data() + x.mat_off, sizeof(double) * 2 * ((size_t)1 << (2 * x.kq)));
+        break;
+      }
+      default: throw RuntimeErr("internal: op kind not allowed in an exact sweep");
+    }
+    while (dd.size() % 16) dd.push_back(0);
+    tops.push_back(t);
+    datas.push_back(std::move(dd));
+    d.gates += 1;
+    d.src.push_back((int)o);
+    d.tile_blocks.push_back(op_qubits(ops[o]));
+    d.tile_mfma.push_back(0);
+  }
+  const size_t head = tops.size() * sizeof(k::TileOp);
+  size_t off = head;
+  for (size_t t = 0; t < tops.size(); ++t) {
+    tops[t].data_off = (uint32_t)off;
+    off += datas[t].size();
+  }
+  std::vector<uint8_t> blob(off);
+  std::memcpy(blob.data(), tops.data(), head);
+  off = head;
+  for (auto& dd : datas) {
+    std::memcpy(blob.data() + off, dd.data(), dd.size());
+    off += dd.size();
+  }
+  d.targs.L = (uint32_t)L;
+  d.targs.nblocks = (uint32_t)tops.size();
+  d.targs.blob16 = (uint32_t)(blob.size() / 16);
+  d.tile_off = plan.push(blob.data(), blob.size());
+  d.kq = L;
+  d.low = 0;
+  plan.ops.push_back(std::move(d));
+}
+
+void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int n, int L) {
+  // LDS per workgroup = 16 * 2^L + blob; the blob cap keeps it within 64 KiB.
+  constexpr size_t kBlobMax = 32768, kOpsMax = 128;
+  auto plain = [&](size_t o) {
+    plan.ops.push_back(lower(ops[o], plan, 16));
+    plan.ops.back().touched = op_qubits(ops[o]);
+    block_features(plan.ops.back().touched, {(int)o}, ops, n, plan.ops.back().feat);
+    plan.ops.back().src = {(int)o};
+  };
+  size_t i = begin;
+  while (i < end) {
+    if (!exact_tileable(ops[i])) {
+      plain(i++);
+      continue;
+    }
+    uint64_t smask = 0xFull | qubit_mask(op_qubits(ops[i]));
+    size_t blob = exact_op_bytes(ops[i]);
+    size_t j = i + 1;
+    while (j < end && j - i < kOpsMax && exact_tileable(ops[j])) {
+      const uint64_t m = qubit_mask(op_qubits(ops[j]));
+      const size_t bb = exact_op_bytes(ops[j]);
+      if (__builtin_popcountll(smask | m) > L || blob + bb > kBlobMax) break;
+      smask |= m;
+      blob += bb;
+      ++j;
+    }
+    if (j - i == 1) plain(i);
+    else emit_tile128(plan, ops, i, j, smask, L);
+    i = j;
+  }
+}
+
 Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n,
                 int tile) {
   Plan plan;
   size_t i = begin;
+  const int lc = std::min(tile, 11);  // complex128 tiles: 16 * 2^L bytes of LDS
   while (i < end) {
+    if (prec == 16 && fusion == FUSE_NONE && tile >= kTileMin && n >= lc + 4 && ops[i].is_unitary()) {
+      size_t j = i;
+      while (j < end && ops[j].is_unitary()) ++j;
+      emit_exact_sweeps(plan, ops, i, j, n, lc);
+      i = j;
+      continue;
+    }
     if (!ops[i].is_unitary() || fusion == FUSE_NONE) {
       plan.ops.push_back(lower(ops[i], plan, prec));
       if (ops[i].is_unitary()) {
@@ -854,7 +1027,8 @@ struct Exec {
         break;
       }
       case D_TILE:
-        k::tile_sweep(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
+        if (d.exact) k::tile_sweep_c128(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
+        else k::tile_sweep(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
         break;
       case D_NOOP: break;
     }
@@ -1062,7 +1236,9 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   p.ops = compile(p.c.instructions);
   const uint64_t seed = o.has_seed ? o.seed : random_seed();
   Stream st;
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.tile);
+  // A profiled run times each compiled op by its reference category, so it
+  // runs one launch per op.
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.profile ? 0 : o.tile);
   plan.upload(st.s);
   Exec ex(o.prec, (int)p.n, p.nc, 1, st.s);
   ex.cbase = plan.base();

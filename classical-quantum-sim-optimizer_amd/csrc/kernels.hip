@@ -574,6 +574,191 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 }
 
 // ---------------------------------------------------------------------------
+// Exact complex128 sweep: consecutive compiled ops applied to 2^L-amplitude
+// double2 tiles in LDS, each with the arithmetic and index set of its
+// per-op kernel (k_one, k_pairswap, k_diag, k_dense), in program order, so
+// every amplitude sees the same sequence of operations as one launch per op.
+// ---------------------------------------------------------------------------
+// Group index p -> local index: zeros inserted at the op's sorted positions
+// (held in registers; loops fully unrolled so nothing is indexed at run time).
+struct TileGroup {
+  uint32_t low[8];  // (1 << pos[i]) - 1, or all ones past nbits (insertion is then a no-op)
+  uint32_t fixed;
+  __device__ __forceinline__ explicit TileGroup(const TileOp& op) {
+    const uint32_t nb = op.nbits;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) low[i] = (uint32_t)i < nb ? (1u << op.pos[i]) - 1 : 0xFFFFFFFFu;
+    fixed = op.fixed;
+  }
+  __device__ __forceinline__ uint32_t operator()(uint32_t p) const {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p = (p & low[i]) | ((p & ~low[i]) << 1);
+    return p | fixed;
+  }
+};
+
+template <int L>
+__device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, const unsigned char* meta) {
+  const uint32_t count = (1u << L) >> op.nbits;
+  const double* d = (const double*)(meta + op.data_off);
+  const TileGroup tile_group(op);
+  if (op.kind == 0) {
+    double u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u[i] = d[i];
+    for (uint32_t p = threadIdx.x; p < count; p += TPB) {
+      const uint32_t i = tile_group(p);
+      const double2 x = tile[i], y = tile[i | op.bbit];
+      double2 na, nc;
+      na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
+      na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
+      nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
+      nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
+      tile[i] = na;
+      tile[i | op.bbit] = nc;
+    }
+  } else if (op.kind == 1) {
+    for (uint32_t p = threadIdx.x; p < count; p += TPB) {
+      const uint32_t i = tile_group(p);
+      const double2 a = tile[i | op.abit], c = tile[i | op.bbit];
+      tile[i | op.abit] = c;
+      tile[i | op.bbit] = a;
+    }
+  } else if (op.kind == 2) {
+    const uint32_t* pat = (const uint32_t*)d;
+    const double* val = d + 16;  // patterns occupy the first 128 bytes
+    for (uint32_t p = threadIdx.x; p < count; p += TPB) {
+      const uint32_t i0 = tile_group(p);
+      for (uint32_t t = 0; t < op.k; ++t) {
+        const uint32_t i = i0 | pat[t];
+        const double2 a = tile[i];
+        const double dr = val[2 * t], di = val[2 * t + 1];
+        double2 r;
+        r.x = a.x * dr - a.y * di;
+        r.y = a.x * di + a.y * dr;
+        tile[i] = r;
+      }
+    }
+  } else {
+    // Dense, k <= 3 in sweeps.
+    const uint32_t kk = op.k, D = 1u << kk;
+    uint32_t qb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) qb[q] = (uint32_t)q < kk ? 1u << op.q[q] : 0u;
+    uint32_t off[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if ((uint32_t)q < kk && ((j >> (kk - 1 - q)) & 1)) o |= qb[q];
+      off[j] = o;
+    }
+    for (uint32_t p = threadIdx.x; p < count; p += TPB) {
+      const uint32_t base = tile_group(p);
+      double2 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if ((uint32_t)j < D) v[j] = tile[base | off[j]];
+#pragma unroll
+      for (int row = 0; row < 8; ++row) {
+        if ((uint32_t)row >= D) break;
+        double ar = 0, ai = 0;
+#pragma unroll
+        for (int col = 0; col < 8; ++col) {
+          if ((uint32_t)col >= D) break;
+          const double mr = d[2 * (row * D + col)], mi = d[2 * (row * D + col) + 1];
+          ar = ar + (mr * v[col].x - mi * v[col].y);
+          ai = ai + (mr * v[col].y + mi * v[col].x);
+        }
+        double2 r;
+        r.x = ar;
+        r.y = ai;
+        tile[base | off[row]] = r;
+      }
+    }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(TPB) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,
+                                                 TileArgs a, const float4* __restrict__ blob) {
+  extern __shared__ float4 lds4[];
+  constexpr int PER = (1 << L) / TPB;
+  double2* tile = (double2*)lds4;
+  const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L));
+  for (uint32_t i = threadIdx.x; i < a.blob16; i += TPB) lds4[(1 << L) + i] = blob[i];
+  const TileOp* ops = (const TileOp*)meta;
+  double2* st = s + (uint64_t)blockIdx.y * stride;
+  uint64_t lo = 0;
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit)
+    if ((threadIdx.x >> bit) & 1) lo |= 1ull << a.spos[bit];
+  uint64_t hi[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int bit = 0; bit < L - 8; ++bit)
+      if ((i >> bit) & 1) o |= 1ull << a.spos[8 + bit];
+    hi[i] = o;
+  }
+  auto tile_base = [&](uint64_t tau) {
+    uint64_t base = tau;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const uint64_t low = (1ull << a.spos[i]) - 1;
+      base = (base & low) | ((base & ~low) << 1);
+    }
+    return base;
+  };
+  // Same pipeline as k_tile: next tile's loads in flight while this tile's
+  // ops run; the loaded registers move to LDS after this tile's stores.
+  // Real and imaginary parts in separate arrays: an array of double2 was
+  // kept in scratch memory instead of registers.
+  double rx[PER], ry[PER];
+  uint64_t tau = blockIdx.x;
+  if (tau < ntiles) {
+    const uint64_t b0 = tile_base(tau);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const double2 v = st[b0 | lo | hi[i]];
+      rx[i] = v.x;
+      ry[i] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = make_double2(rx[i], ry[i]);
+  }
+  __syncthreads();
+  for (; tau < ntiles; tau += gridDim.x) {
+    const uint64_t base = tile_base(tau);
+    const uint64_t nxt = tau + gridDim.x;
+    const bool more = nxt < ntiles;
+    if (more) {
+      const uint64_t nb = tile_base(nxt);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const double2 v = st[nb | lo | hi[i]];
+        rx[i] = v.x;
+        ry[i] = v.y;
+      }
+    }
+    for (uint32_t oi = 0; oi < a.nblocks; ++oi) {
+      tile128_op<L>(tile, ops[oi], meta);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    if (more) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = make_double2(rx[i], ry[i]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // State set-up
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1142,6 +1327,7 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, c
   } while (0)
 
 size_t tile_lds_bytes(const TileArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
+static size_t tile128_lds_bytes(const TileArgs& a) { return ((size_t)16 << a.L) + (size_t)16 * a.blob16; }
 
 // Grid: 64 workgroups per CU (each walks ntiles / grid tiles with its
 // prefetch), capped by the tile count. A grid of exactly the resident
@@ -1174,6 +1360,17 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
     default: if (k4) L(12, true); else L(12, false); break;
   }
 #undef L
+}
+
+void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob,
+                     hipStream_t st) {
+  const uint64_t ntiles = 1ull << (n - (int)a.L);
+  const size_t lds = tile128_lds_bytes(a);
+  dim3 g((uint32_t)tile_grid(ntiles), batch);
+  if (a.L == 10)
+    hipLaunchKernelGGL(k_tile128<10>, g, dim3(TPB), lds, st, (double2*)s, stride, ntiles, a, (const float4*)blob);
+  else
+    hipLaunchKernelGGL(k_tile128<11>, g, dim3(TPB), lds, st, (double2*)s, stride, ntiles, a, (const float4*)blob);
 }
 
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st) {

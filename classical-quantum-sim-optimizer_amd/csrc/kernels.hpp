@@ -96,6 +96,24 @@ struct TileArgs {
   uint32_t kmax;      // widest block of the sweep (3 or 4)
   uint32_t spos[12];  // S sorted ascending (global qubit of local bit i)
 };
+// Exact complex128 sweep (no fusion): the blob holds nops TileOp descriptors,
+// then each op's FP64 data (16-byte aligned). Ops are applied in program
+// order with the arithmetic of the per-op kernels.
+struct TileOp {
+  uint32_t kind;      // 0: 1-qubit pair update, 1: pair swap (X / SWAP), 2: sparse diagonal, 3: dense
+  uint32_t nbits;     // zero bits inserted into the group counter (sorted local positions in pos)
+  uint32_t fixed;     // local bits ORed into every group index (control values)
+  uint32_t abit, bbit;  // pair members: i | abit and i | bbit
+  uint32_t k;         // dense width or diagonal entry count
+  uint32_t data_off;  // byte offset of the op's data in the blob
+  uint32_t pad0;
+  uint32_t pos[8];    // sorted local positions of the op's qubits (targets and controls)
+  uint32_t q[8];      // dense: local bit of each block qubit, q[0] = MSB of the block index
+};
+static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
+void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob_dev,
+                     hipStream_t st);
+
 // Dynamic LDS of one workgroup: the tile plus the blob.
 size_t tile_lds_bytes(const TileArgs& a);
 void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob_dev,

@@ -1,0 +1,77 @@
+"""Exact complex128 tile sweeps (the default for complex128 without fusion
+at 15+ qubits): consecutive compiled ops applied in LDS with the per-op
+kernels' arithmetic, in program order. Results must equal the oracle bit for
+bit, like one launch per op."""
+
+import numpy as np
+import pytest
+
+import dqsim
+from dqsim import workloads
+from dqsim.workloads import Circuit
+from oracle import oracle
+from test_gpu_parity import entangled_prefix, inst, standard_gate_cases
+from test_gpu_tiles import mixed_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def exact_state(insts, n, tile=12, seed=60):
+    sim = dqsim.StatevectorSimulator(seed=seed, tile_qubits=tile)
+    return sim.simulate(Circuit(n, insts)).statevector
+
+
+def has_exact_tiles(insts, n):
+    plan = dqsim._core.plan_json(Circuit(n, insts))
+    return any(x["kind"] == "tile" for x in plan["launches"])
+
+
+def test_every_gate_kind_bit_exact_in_sweeps():
+    # All standard gates at several placements, many per sweep, after an
+    # entangling prefix, at 16 qubits (12 ... 15 appear as tile qubits).
+    n = 16
+    rng = np.random.default_rng(11)
+    insts = entangled_prefix(n)
+    for kind, names, params in standard_gate_cases():
+        for _ in range(3):
+            qubits = rng.choice(n, size=len(names), replace=False).tolist()
+            insts.append(inst(kind, **dict(zip(names, qubits)), **params))
+    assert has_exact_tiles(insts, n)
+    want = oracle.simulate(Circuit(n, insts).data, 60)[0]
+    assert np.array_equal(exact_state(insts, n), want)
+    assert np.array_equal(exact_state(insts, n, tile=10), want)
+    assert np.array_equal(exact_state(insts, n, tile=0), want)
+
+
+@pytest.mark.parametrize("n,seed", [(16, 1), (17, 2), (18, 3)])
+def test_mixed_circuits_bit_exact(n, seed):
+    insts = mixed_circuit(n, 5, seed).data["instructions"]
+    assert has_exact_tiles(insts, n)
+    want = oracle.simulate(Circuit(n, insts).data, 60)[0]
+    assert np.array_equal(exact_state(insts, n), want)
+
+
+def test_qft_and_brickwork_bit_exact():
+    for circ in (workloads.qft(17, seed=4, measure=False)[0], workloads.brickwork(n=18, layers=8, measure=False)):
+        n = circ.num_qubits
+        insts = circ.data["instructions"]
+        assert has_exact_tiles(insts, n)
+        want = oracle.simulate(circ.data, 60)[0]
+        assert np.array_equal(exact_state(insts, n), want)
+
+
+@pytest.mark.parametrize("terminal", [True, False])
+def test_shots_with_mid_circuit_measurement_match_oracle(terminal):
+    # Unitary runs between measurements are swept; trajectory batches sweep
+    # every slot. Exact states give the oracle's counts exactly.
+    n = 16
+    ins = workloads.random_su2(n=n, depth=3, seed=5).data["instructions"]
+    ins = ins + [inst("measure", qubit=2, cbit=0)] + workloads.random_su2(n=n, depth=2, seed=6).data["instructions"]
+    ins = ins + [inst("measure", qubit=q, cbit=q) for q in range(n)]
+    circ = Circuit(n, ins, cbits=n)
+    assert has_exact_tiles(ins, n)
+    sim = dqsim.StatevectorSimulator(seed=21, sample_terminal=terminal, max_parallel_shots=16,
+                                     max_memory_mb=4096)
+    got = sim.simulate_shots(circ, shots=40)
+    ref, _ = oracle.run_shots(circ.data, 40, 21, budget_mb=4096, sample_terminal=terminal, max_parallel_shots=16)
+    assert got == ref

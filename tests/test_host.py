@@ -239,7 +239,7 @@ def test_compile_matches_oracle_on_random_circuits():
 
 def test_fusion_plan_conserves_gates_and_respects_width():
     circ = workloads.random_su2(n=20, depth=8, seed=3)
-    plain = _core.plan_json(circ)
+    plain = _core.plan_json(circ, tile_qubits=0)
     for k in (2, 3, 4, 5):
         fused = _core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=0)
         assert sum(l["gates"] for l in fused["launches"]) == plain["compiled_ops"]
@@ -376,3 +376,27 @@ def test_tile_sweeps_off_and_limits():
             _core.plan_json(c, precision="complex64", fusion="greedy", tile_qubits=bad)
         with pytest.raises(ValueError):
             dqsim.StatevectorSimulator(precision="complex64", tile_qubits=bad)
+
+
+@pytest.mark.parametrize("kind", ["su2", "mixed", "qft"])
+def test_exact_sweeps_keep_program_order(kind):
+    # complex128 without fusion: sweeps are runs of consecutive compiled ops
+    # (nothing reordered), each within L tile qubits that include 0..3.
+    n = 18
+    if kind == "su2":
+        c = workloads.random_su2(n=n, depth=6, seed=4)
+    elif kind == "qft":
+        c = workloads.qft(n, seed=2, measure=False)[0]
+    else:
+        c = _mixed_tile_circuit(n, 5, seed=9)
+    for L in (10, 11, 12):
+        plan = _core.plan_json(c, tile_qubits=L)
+        order = [o for x in plan["launches"] for o in x["ops"]]
+        assert order == list(range(plan["compiled_ops"]))
+        tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+        assert tiles and len(plan["launches"]) < plan["compiled_ops"]
+        for t in tiles:
+            s = set(t["qubits"])
+            assert len(s) == min(L, 11) and {0, 1, 2, 3} <= s
+            assert all(set(b["qubits"]) <= s for b in t["blocks"])
+    assert not any(x["kind"] == "tile" for x in _core.plan_json(c, tile_qubits=0)["launches"])
