@@ -21,8 +21,8 @@ def exact_state(insts, n, tile=12, seed=60):
     return sim.simulate(Circuit(n, insts)).statevector
 
 
-def has_exact_tiles(insts, n):
-    plan = dqsim._core.plan_json(Circuit(n, insts))
+def has_exact_tiles(insts, n, cbits=0):
+    plan = dqsim._core.plan_json(Circuit(n, insts, cbits=cbits))
     return any(x["kind"] == "tile" for x in plan["launches"])
 
 
@@ -69,7 +69,7 @@ def test_shots_with_mid_circuit_measurement_match_oracle(terminal):
     ins = ins + [inst("measure", qubit=2, cbit=0)] + workloads.random_su2(n=n, depth=2, seed=6).data["instructions"]
     ins = ins + [inst("measure", qubit=q, cbit=q) for q in range(n)]
     circ = Circuit(n, ins, cbits=n)
-    assert has_exact_tiles(ins, n)
+    assert has_exact_tiles(ins, n, cbits=n)
     sim = dqsim.StatevectorSimulator(seed=21, sample_terminal=terminal, max_parallel_shots=16,
                                      max_memory_mb=4096)
     got = sim.simulate_shots(circ, shots=40)
