@@ -22,10 +22,10 @@ sys.path.insert(0, os.path.join(ROOT, "classical-quantum-sim-optimizer_amd"))
 sys.path.insert(0, ROOT)
 
 KIND_NAMES = {0: "one", 1: "x", 2: "swap", 3: "diag", 4: "dense", 5: "fused_valu", 6: "measure", 7: "reset",
-              8: "fused_mfma", 9: "tile"}
+              8: "fused_mfma", 9: "tile", 10: "tile128"}
 KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag": "k_diag", "dense": "k_dense",
                 "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse",
-                "tile": "k_tile"}
+                "tile": "k_tile", "tile128": "k_tile128"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--max-fused", type=int, default=3)
     ap.add_argument("--fusion", default="greedy")
+    ap.add_argument("--precision", default="complex64", choices=["complex64", "complex128"],
+                    help="precision of the timed evolution (complex128 with --fusion none times the exact sweeps)")
     ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -179,9 +181,10 @@ def main():
     circ = workloads.random_su2(n=n, depth=depth, seed=1234)
     gates = workloads.gate_count(circ.data["instructions"])
     mem_mb = (1 << n) * 16 * 2 // (1 << 20) + 4096
-    fused = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion=args.fusion,
+    fused = dqsim.StatevectorSimulator(seed=1, precision=args.precision, fusion=args.fusion,
                                        max_fused_qubits=args.max_fused, max_memory_mb=mem_mb, tile_qubits=args.tile)
-    log(f"[bench] n={n} depth={depth} gates={gates} fusion={args.fusion} k<={args.max_fused} tile={args.tile}")
+    log(f"[bench] n={n} depth={depth} gates={gates} {args.precision} fusion={args.fusion} k<={args.max_fused} "
+        f"tile={args.tile}")
     for w in range(args.warmup):
         t = time.perf_counter()
         fused.evolve_timed(circ, repeats=1, record=False)
@@ -243,7 +246,9 @@ def main():
             "value": gates * float(1 << n) / (ems / 1e3), "ms": ems, "wall_s": ewall, "launches": elaunch,
             "speedup_vs_plain_complex128": pms / ems, "max_abs_diff_vs_plain": emax}
 
-    workload_key = f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
+    workload_key = (f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
+                    if args.precision == "complex64" and args.fusion == "greedy"
+                    else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_k{args.max_fused}_t{args.tile}")
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -261,9 +266,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": "c64" if args.precision == "complex64" else "c128",
         "data": "synthetic",
-        "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, complex64, {args.fusion} fusion "
+        "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,

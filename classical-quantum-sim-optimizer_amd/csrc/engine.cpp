@@ -1059,7 +1059,7 @@ int op_kind_code(const DevOp& d) {
     case D_DIAG: return 3;
     case D_DENSE: return 4;
     case D_FUSED: return d.mfma ? 8 : 5;
-    case D_TILE: return 9;
+    case D_TILE: return d.exact ? 10 : 9;
     case D_MEAS: return 6;
     case D_RESET: return 7;
     default: return -1;

@@ -87,7 +87,8 @@ typedef int (*dq_count_cb)(void* user, const char* bits, size_t len, uint64_t co
 /* Per-launch device timing (engine-recorded). */
 typedef struct dq_op_timing {
   int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA),
-                    9 tile sweep (k = tile qubits, gates = ops applied) */
+                    9 tile sweep (k = tile qubits, gates = ops applied),
+                    10 exact complex128 tile sweep */
   int32_t k;     /* qubits touched */
   int32_t low;   /* lowest qubit touched */
   int32_t gates; /* compiled ops folded into the launch */
