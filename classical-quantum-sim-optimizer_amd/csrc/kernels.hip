@@ -582,17 +582,18 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 // Group index p -> local index: zeros inserted at the op's sorted positions
 // (held in registers; loops fully unrolled so nothing is indexed at run time).
 struct TileGroup {
-  uint32_t low[8];  // (1 << pos[i]) - 1, or all ones past nbits (insertion is then a no-op)
-  uint32_t fixed;
+  uint32_t low[8];  // (1 << pos[i]) - 1 for i < nb
+  uint32_t nb, fixed;
   __device__ __forceinline__ explicit TileGroup(const TileOp& op) {
-    const uint32_t nb = op.nbits;
+    nb = __builtin_amdgcn_readfirstlane(op.nbits);
 #pragma unroll
     for (int i = 0; i < 8; ++i) low[i] = (uint32_t)i < nb ? (1u << op.pos[i]) - 1 : 0xFFFFFFFFu;
     fixed = op.fixed;
   }
   __device__ __forceinline__ uint32_t operator()(uint32_t p) const {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) p = (p & low[i]) | ((p & ~low[i]) << 1);
+    for (int i = 0; i < 8; ++i)
+      if ((uint32_t)i < nb) p = (p & low[i]) | ((p & ~low[i]) << 1);  // wave-uniform branch
     return p | fixed;
   }
 };
@@ -681,7 +682,7 @@ __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, cons
 }
 
 template <int L>
-__global__ __launch_bounds__(TPB) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,
+__global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,
                                                  TileArgs a, const float4* __restrict__ blob) {
   extern __shared__ float4 lds4[];
   constexpr int PER = (1 << L) / TPB;
