@@ -235,6 +235,8 @@ struct DevOp {
 //  0 k/5             1 lowest qubit/34     2 highest qubit/34   3 qubits < 4, /4
 //  4 ops folded/32   5 diagonal-op share   6 Clifford-op share  7 all ops diagonal
 //  8 n/34            9 k >= 4 (MFMA)      10 span/34           11 touched share of amplitudes
+// 12-15 stay zero here; a tile sweep sets 12 = 1, 13 = L/12, 14 = blocks on
+// the 16x16x4 path / 16, 15 = 4-qubit blocks / 16, plus 4 (ops/32) and 8 (n/34).
 bool clifford_entry(const cplx& e) {
   const double r = std::fabs(e.re), i = std::fabs(e.im), t = 1e-12, h = 0.70710678118654752440;
   auto is = [&](double a, double b) { return std::fabs(r - a) < t && std::fabs(i - b) < t; };
@@ -469,6 +471,12 @@ std::vector<float> real_embedding(const std::vector<cplx>& m, int k) {
 std::vector<float> g_planner;  // learned planner weights (empty = greedy)
 
 double planner_score(const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& ops, int n);
+// Predicted device time of a launch (ms): the model is trained on
+// log(ms per 2^24 amplitudes).
+double predicted_ms(const DevOp& d, int n) {
+  if (d.kind == D_NOOP) return 0.0;
+  return std::exp((double)mlp_eval_host(g_planner.data(), d.feat)) * std::ldexp(1.0, n - 24);
+}
 
 // One fused block as its own launch.
 void emit_block(Plan& plan, FusedBlock& blk, const std::vector<Op>& ops, int prec, int n) {
@@ -563,7 +571,8 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
 // Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
 // read and write of the state for all of them. The blob (descriptors, then
 // matrices) is what every workgroup copies into LDS.
-void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int L, const std::vector<Op>& ops) {
+void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int L, const std::vector<Op>& ops,
+               int n) {
   DevOp d;
   d.kind = D_TILE;
   d.category = 1;
@@ -625,6 +634,14 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
   d.low = 0;
+  int n3 = 0, n4 = 0;
+  for (const k::TileBlock& t : tb) (t.kind == 2 ? n4 : n3) += 1;
+  d.feat[4] = d.gates / 32.f;
+  d.feat[8] = n / 34.f;
+  d.feat[12] = 1.f;
+  d.feat[13] = L / 12.f;
+  d.feat[14] = n3 / 16.f;
+  d.feat[15] = n4 / 16.f;
   plan.ops.push_back(std::move(d));
 }
 
@@ -705,7 +722,7 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
       } else {
         std::vector<FusedBlock*> bp;
         for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
-        emit_tile(plan, bp, smask, L, ops);
+        emit_tile(plan, bp, smask, L, ops, n);
       }
       i = j;
     }
@@ -842,6 +859,10 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
   d.low = 0;
+  d.feat[4] = d.gates / 32.f;
+  d.feat[12] = 1.f;
+  d.feat[13] = L / 12.f;
+  d.feat[14] = tops.size() / 16.f;
   plan.ops.push_back(std::move(d));
 }
 
@@ -908,7 +929,27 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         return planner_score(a, b, o, n);
       };
     if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
-      emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
+      if (fusion == FUSE_LEARNED && !g_planner.empty()) {
+        // Choose (block width, tile size) by the model's predicted total time.
+        int best_k = std::min(max_fused, 3), best_l = tile;
+        double best = 1e300;
+        for (int kt = 2; kt <= std::max(2, std::min(max_fused, 4)); ++kt)
+          for (int l = kTileMin; l <= tile; ++l) {
+            if (n < l + 4) continue;
+            Plan cand;
+            emit_sweeps(cand, ops, i, j, prec, n, l, std::min(kt, max_fused), nullptr);
+            double t = 0;
+            for (const DevOp& d : cand.ops) t += predicted_ms(d, n);
+            if (t < best) {
+              best = t;
+              best_k = std::min(kt, max_fused);
+              best_l = l;
+            }
+          }
+        emit_sweeps(plan, ops, i, j, prec, n, best_l, best_k, nullptr);
+      } else {
+        emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
+      }
     } else {
       std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
       for (FusedBlock& blk : blocks) emit_block(plan, blk, ops, prec, n);

@@ -167,6 +167,8 @@ void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_
                        int* max_threads, int* lmax);
 
 // Learned fusion planner weights (2 hidden layers of 64, FP32). Null clears.
+// Host forward pass of the planner MLP (packed weights, one feature vector).
+float mlp_eval_host(const float* w, const float* x);
 void set_planner_weights(const float* w, size_t count);
 // Train the planner MLP on the GPU (Adam, mean squared error); w is in/out.
 void mlp_train(float* w, const float* X, const float* T, uint64_t n, int epochs, int batch, float lr, uint64_t seed,

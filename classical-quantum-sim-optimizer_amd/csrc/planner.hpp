@@ -8,7 +8,7 @@
 namespace dq {
 namespace mlp {
 
-constexpr int kFeatures = 12;
+constexpr int kFeatures = 16;
 constexpr int kHidden = 64;
 constexpr int kMlpParams = kFeatures * kHidden + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
 

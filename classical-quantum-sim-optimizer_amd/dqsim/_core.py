@@ -77,7 +77,7 @@ class OpTiming(ctypes.Structure):
         ("gates", ctypes.c_int32),
         ("ms", ctypes.c_double),
         ("bytes", ctypes.c_double),
-        ("feat", ctypes.c_float * 12),
+        ("feat", ctypes.c_float * 16),
     ]
 
 

@@ -1,15 +1,19 @@
 """Learned fusion planner.
 
-A 12 -> 64 -> 64 -> 1 FP32 MLP predicts the device time of one fused launch
-(log of milliseconds per 2^24 amplitudes) from the launch's features:
+A 16 -> 64 -> 64 -> 1 FP32 MLP predicts the device time of one launch (log of
+milliseconds per 2^24 amplitudes) from the launch's features:
 
    0 k/5              1 lowest qubit/34      2 highest qubit/34    3 qubits < 4, /4
    4 ops folded/32    5 diagonal-op share    6 Clifford-op share   7 all ops diagonal
    8 n/34             9 k >= 4 (MFMA path)  10 span/34            11 share of amplitudes touched
+  12 tile sweep      13 tile qubits/12      14 3-qubit-path blocks/16  15 4-qubit blocks/16
 
-With ``fusion="learned"`` the greedy fuser asks the model before every merge
-and merges blocks a and b only when t(a) + t(b) > t(a+b); without weights it
-falls back to plain greedy merging (every merge that fits the width).
+(0-11 describe a one-block launch and are zero for a tile sweep; 12-15 are
+zero for a one-block launch.) With ``fusion="learned"`` and weights
+installed, complex64 plans with LDS tile sweeps are chosen among candidate
+(block width, tile size) pairs by the model's predicted total time; plans
+without sweeps ask the model before every merge and merge blocks a and b only
+when t(a) + t(b) > t(a+b). Without weights both fall back to plain greedy.
 
 Training data are the engine's own per-launch HIP-event timings
 (``StatevectorSimulator.evolve_timed``) on seeded random circuits, and the
@@ -28,11 +32,12 @@ import numpy as np
 from . import _core, workloads
 from ._core import _check, _lib
 
-N_FEATURES = 12
+N_FEATURES = 16
 HIDDEN = 64
 N_PARAMS = int(_lib.dq_mlp_param_count())
 FEATURE_NAMES = ["k", "lowest_qubit", "highest_qubit", "low_qubits", "ops", "diagonal_share", "clifford_share",
-                 "all_diagonal", "n", "mfma", "span", "touched_share"]
+                 "all_diagonal", "n", "mfma", "span", "touched_share", "tile", "tile_qubits", "tile_blocks3",
+                 "tile_blocks4"]
 
 
 def init_weights(seed: int = 0) -> np.ndarray:
@@ -81,16 +86,19 @@ def install(w: Optional[np.ndarray]) -> None:
     _core.set_planner_weights(w)
 
 
-def launch_records(circuit, max_fused_qubits: int, precision: str = "complex64", fusion: str = "greedy"):
+def launch_records(circuit, max_fused_qubits: int, precision: str = "complex64", fusion: str = "greedy",
+                   tile_qubits: int = 0):
     sim = _core.StatevectorSimulator(seed=1, precision=precision, fusion=fusion, max_fused_qubits=max_fused_qubits,
-                                     max_memory_mb=1 << 20, tile_qubits=0)
+                                     max_memory_mb=1 << 20, tile_qubits=tile_qubits)
     rows, launches, gates, total_ms, _ = sim.evolve_timed(circuit, repeats=1, record=True)
     return rows, total_ms
 
 
-def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int = 0, log=None):
+def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int = 0, log=None,
+            tiles: Sequence[int] = (0, 0, 0, 10, 11, 12)):
     """(X, y): features and log(ms per 2^24 amplitudes) of every launch of the
-    circuits, each evolved with a width drawn from `widths`."""
+    circuits, each evolved with a block width drawn from `widths` and a tile
+    size drawn from `tiles` (0: one launch per block; sweeps use widths <= 4)."""
     rng = np.random.default_rng(seed)
     X: List[List[float]] = []
     y: List[float] = []
@@ -99,7 +107,10 @@ def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int 
             log(f"[planner] dataset: circuit {i}, {len(y)} launches so far")
         n = c.num_qubits
         k = int(widths[int(rng.integers(len(widths)))])
-        rows, _ = launch_records(c, k)
+        tile = int(tiles[int(rng.integers(len(tiles)))])
+        if tile and c.num_qubits < tile + 4:
+            tile = 0
+        rows, _ = launch_records(c, min(k, 4) if tile else k, tile_qubits=tile)
         for r in rows:
             if r["ms"] <= 0:
                 continue
@@ -108,10 +119,26 @@ def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int 
     return np.asarray(X, dtype=np.float32), np.asarray(y, dtype=np.float32)
 
 
-def training_circuits(count: int, nmin: int = 24, nmax: int = 28, dmin: int = 20, dmax: int = 80, seed: int = 0):
-    """Seeded random SU(2) + random-matching circuits of uneven size."""
+FAMILIES = ("su2", "qft", "brickwork", "mixed")
+
+
+def family_circuit(family: str, n: int, depth: int, seed: int):
+    """One seeded circuit of a workload family (random SU(2) + matchings, QFT
+    after a U3 layer, nearest-neighbour brickwork, mixed multi-qubit gates)."""
+    if family == "su2":
+        return workloads.random_su2(n=n, depth=depth, seed=seed)
+    if family == "qft":
+        return workloads.qft(n, seed=seed, measure=False)[0]
+    if family == "brickwork":
+        return workloads.brickwork(n=n, layers=depth, seed=seed, measure=False)
+    return workloads.mixed(n, max(1, depth // 4), seed)
+
+
+def training_circuits(count: int, nmin: int = 24, nmax: int = 28, dmin: int = 20, dmax: int = 80, seed: int = 0,
+                      families: Sequence[str] = FAMILIES):
+    """Seeded circuits of uneven size, rotating through `families`."""
     rng = np.random.default_rng(seed)
     for i in range(count):
         n = int(rng.integers(nmin, nmax + 1))
         d = int(rng.integers(dmin, dmax + 1))
-        yield workloads.random_su2(n=n, depth=d, seed=seed * 1_000_003 + i)
+        yield family_circuit(families[i % len(families)], n, d, seed * 1_000_003 + i)

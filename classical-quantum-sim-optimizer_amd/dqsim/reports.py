@@ -97,8 +97,14 @@ def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, re
     return out
 
 
-def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(22, 26), eval_n=(28, 30),
-                   seed: int = 5, epochs: int = 300, log=print) -> dict:
+def planner_report(train_circuits: int = 300, train_n=(22, 26), eval_n=(28, 30), seed: int = 5, epochs: int = 300,
+                   log=print) -> dict:
+    """Train the planner MLP on engine-recorded launch timings (per-block and
+    tile plans of four circuit families), then on held-out circuits compare
+    (a) tile plans: greedy default (blocks <= 3 qubits, 12-qubit tiles) vs the
+    learned choice of (block width <= 4, tile size); (b) one launch per block:
+    greedy k <= 4 / k <= 5 vs learned merges at k <= 5. Accuracy of every plan
+    against the exact complex128 state."""
     t0 = time.perf_counter()
     X, y = planner.dataset(planner.training_circuits(train_circuits, nmin=train_n[0], nmax=train_n[1], dmin=20,
                                                      dmax=80, seed=seed), seed=seed, log=log)
@@ -112,39 +118,53 @@ def planner_report(train_circuits: int = 300, eval_circuits: int = 6, train_n=(2
     pred = planner.predict(w, X[te])
     test_mse = float(np.mean((pred - y[te]) ** 2))
     base_mse = float(np.mean((y[te] - y[tr].mean()) ** 2))
-    rows = []
-    rng = np.random.default_rng(seed + 1)
-    for i in range(eval_circuits):
-        n = int(rng.integers(eval_n[0], eval_n[1] + 1))
-        d = int(rng.integers(20, 81))
-        c = workloads.random_su2(n=n, depth=d, seed=10_000 + i)
+
+    def timed(c, n, **kw):
         mem = (1 << n) * 16 * 3 // (1 << 20) + 4096
-        ref = _core.StatevectorSimulator(seed=1, max_memory_mb=mem)
-        _, _, _, _, rres = ref.evolve_timed(c, record=False, keep=True)
-        row = {"qubits": n, "depth": d}
-        for name, fusion, k in (("greedy_k4", "greedy", 4), ("greedy_k5", "greedy", 5), ("learned_k5", "learned", 5)):
-            if fusion == "learned":
-                planner.install(w)
-            try:
-                # One launch per block: the model predicts standalone launch times.
-                sim = _core.StatevectorSimulator(seed=1, precision="complex64", fusion=fusion, max_fused_qubits=k,
-                                                 max_memory_mb=mem, tile_qubits=0)
-                sim.evolve_timed(c, record=False)  # warm-up
-                _, launches, _, ms, res = sim.evolve_timed(c, record=False, keep=True)
-                err, _ = res.compare(rres)
-                del res
-            finally:
-                planner.install(None)
-            row[name] = {"ms": ms, "launches": launches, "max_err": err}
-        rows.append(row)
-        log(f"[planner] {row}")
-    tot = {k: sum(r[k]["ms"] for r in rows) for k in ("greedy_k4", "greedy_k5", "learned_k5")}
+        sim = _core.StatevectorSimulator(seed=1, precision="complex64", max_memory_mb=mem, **kw)
+        sim.evolve_timed(c, record=False)  # warm-up
+        _, launches, _, ms, res = sim.evolve_timed(c, record=False, keep=True)
+        return ms, launches, res
+
+    rows = []
+    for fam in planner.FAMILIES:
+        for n in eval_n:
+            c = planner.family_circuit(fam, n, 40, 10_000 + n)
+            mem = (1 << n) * 16 * 3 // (1 << 20) + 4096
+            ref = _core.StatevectorSimulator(seed=1, max_memory_mb=mem)
+            _, _, _, _, rres = ref.evolve_timed(c, record=False, keep=True)
+            row = {"family": fam, "qubits": n}
+            configs = [("greedy_tiles_k3", {"fusion": "greedy", "max_fused_qubits": 3}),
+                       ("learned_tiles_k4", {"fusion": "learned", "max_fused_qubits": 4})]
+            if fam == "su2":
+                configs += [("greedy_blocks_k4", {"fusion": "greedy", "max_fused_qubits": 4, "tile_qubits": 0}),
+                            ("greedy_blocks_k5", {"fusion": "greedy", "max_fused_qubits": 5, "tile_qubits": 0}),
+                            ("learned_blocks_k5", {"fusion": "learned", "max_fused_qubits": 5, "tile_qubits": 0})]
+            for name, kw in configs:
+                if kw["fusion"] == "learned":
+                    planner.install(w)
+                try:
+                    ms, launches, res = timed(c, n, **kw)
+                    err, _ = res.compare(rres)
+                    del res
+                finally:
+                    planner.install(None)
+                row[name] = {"ms": ms, "launches": launches, "max_err": err}
+            del rres
+            rows.append(row)
+            log(f"[planner] {row}")
+    tot = {k: sum(r[k]["ms"] for r in rows) for k in ("greedy_tiles_k3", "learned_tiles_k4")}
+    blk = {k: sum(r[k]["ms"] for r in rows if k in r) for k in ("greedy_blocks_k4", "greedy_blocks_k5",
+                                                                "learned_blocks_k5")}
     out = {"train_samples": int(len(tr)), "test_samples": int(len(te)), "collect_s": collect_s, "train_s": train_s,
            "final_train_loss": float(curve[-1]), "test_mse": test_mse, "test_mse_predict_mean": base_mse,
-           "eval": rows, "total_ms": tot,
-           "learned_vs_best_greedy": min(tot["greedy_k4"], tot["greedy_k5"]) / tot["learned_k5"],
-           "max_err": max(r[k]["max_err"] for r in rows for k in ("greedy_k4", "greedy_k5", "learned_k5"))}
-    log(f"[planner] summary {out['total_ms']} gain {out['learned_vs_best_greedy']:.3f}")
+           "eval": rows, "total_ms_tiles": tot, "total_ms_blocks": blk,
+           "learned_vs_greedy_tiles": tot["greedy_tiles_k3"] / tot["learned_tiles_k4"],
+           "learned_vs_best_greedy_blocks": min(blk["greedy_blocks_k4"], blk["greedy_blocks_k5"]) /
+           blk["learned_blocks_k5"],
+           "max_err": max(v["max_err"] for r in rows for v in r.values() if isinstance(v, dict))}
+    log(f"[planner] tiles {tot} gain {out['learned_vs_greedy_tiles']:.3f}; blocks {blk} gain "
+        f"{out['learned_vs_best_greedy_blocks']:.3f}")
     return out
 
 

@@ -180,3 +180,36 @@ def batch_circuit(index: int, base_seed: int = 2024, nmin: int = 10, nmax: int =
             insts.append({"kind": "cx", "control": int(perm[i]), "target": int(perm[i + 1])})
     insts += measure_all(n)
     return Circuit(n, insts, n)
+
+
+def mixed(n: int, depth: int, seed: int = 0) -> Circuit:
+    """U3 layers plus seeded multi-qubit gates of every lowering kind (dense,
+    controlled, diagonal, swap, 3-5 qubit controls): n // 2 gates per layer,
+    kind and qubits drawn uniformly."""
+    rng = np.random.default_rng(seed)
+    ins: List[dict] = []
+    for _ in range(depth):
+        for q in range(n):
+            t, p, l = rng.uniform(-np.pi, np.pi, 3)
+            ins.append({"kind": "u3", "qubit": q, "theta": float(t), "phi": float(p), "lam": float(l)})
+        for _ in range(n // 2):
+            kind = str(rng.choice(["cx", "cz", "swap", "rzz", "cp", "ccx", "cswap", "c3x", "c4x", "ch"]))
+            q = [int(x) for x in rng.choice(n, 5, replace=False)]
+            if kind in ("cx", "cz", "ch"):
+                ins.append({"kind": kind, "control": q[0], "target": q[1]})
+            elif kind == "cp":
+                ins.append({"kind": kind, "control": q[0], "target": q[1], "lam": float(rng.uniform(0, 3))})
+            elif kind == "swap":
+                ins.append({"kind": kind, "a": q[0], "b": q[1]})
+            elif kind == "rzz":
+                ins.append({"kind": kind, "a": q[0], "b": q[1], "theta": float(rng.uniform(0, 3))})
+            elif kind == "ccx":
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "target": q[2]})
+            elif kind == "cswap":
+                ins.append({"kind": kind, "control": q[0], "target1": q[1], "target2": q[2]})
+            elif kind == "c3x":
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "control3": q[2], "target": q[3]})
+            else:
+                ins.append({"kind": kind, "control1": q[0], "control2": q[1], "control3": q[2], "control4": q[3],
+                            "target": q[4]})
+    return Circuit(n, ins)

@@ -94,7 +94,7 @@ typedef struct dq_op_timing {
   int32_t gates; /* compiled ops folded into the launch */
   double ms;
   double bytes;  /* algorithmic HBM bytes */
-  float feat[12];/* fusion-planner features of the launch */
+  float feat[16];/* fusion-planner features of the launch */
 } dq_op_timing;
 
 /* Fill `o` with the reference defaults (max_memory_mb = 1024,

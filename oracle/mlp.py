@@ -1,14 +1,14 @@
 """FP64 restatement of the fusion-planner MLP -- TEST INFRASTRUCTURE ONLY.
 
 Forward pass, mean-squared-error loss and its gradient for the packed
-parameter layout of csrc/planner.hip (W1[12][64], b1, W2[64][64], b2, w3, b3),
+parameter layout of csrc/planner.hip (W1[16][64], b1, W2[64][64], b2, w3, b3),
 written directly from the chain rule. tests/test_oracle_pins.py pins it by
 central finite differences; the GPU kernels are checked against it.
 """
 
 import numpy as np
 
-F, H = 12, 64
+F, H = 16, 64
 
 
 def unpack(w):

@@ -15,8 +15,9 @@ pytestmark = pytest.mark.gpu
 def test_mlp_gradient_matches_fp64_oracle():
     rng = np.random.default_rng(3)
     w = planner.init_weights(5)
-    w[12 * 64:12 * 64 + 64] = rng.normal(0, 0.1, 64)  # non-zero biases
-    X = rng.normal(0, 1, size=(37, 12)).astype(np.float32)
+    F = planner.N_FEATURES
+    w[F * 64:F * 64 + 64] = rng.normal(0, 0.1, 64)  # non-zero biases
+    X = rng.normal(0, 1, size=(37, planner.N_FEATURES)).astype(np.float32)
     y = rng.normal(0, 1, size=37).astype(np.float32)
     g, loss = planner.grad(w, X, y)
     gref, lref = mlp.grad(w.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
@@ -30,7 +31,7 @@ def test_mlp_gradient_matches_fp64_oracle():
 
 def test_training_reduces_loss_and_fits_a_known_function():
     rng = np.random.default_rng(8)
-    X = rng.uniform(0, 1, size=(2048, 12)).astype(np.float32)
+    X = rng.uniform(0, 1, size=(2048, planner.N_FEATURES)).astype(np.float32)
     y = (2.0 * X[:, 0] - X[:, 3] + 0.5 * np.sin(3 * X[:, 8])).astype(np.float32)
     w0 = planner.init_weights(1)
     _, l0 = planner.grad(w0, X, y)
@@ -45,7 +46,7 @@ def test_training_reduces_loss_and_fits_a_known_function():
 def test_engine_records_training_data_and_learned_plan_is_accurate():
     circs = list(planner.training_circuits(4, nmin=14, nmax=16, dmin=4, dmax=10, seed=1))
     X, y = planner.dataset(circs, seed=1)
-    assert X.shape[1] == 12 and len(X) == len(y) > 20 and np.isfinite(y).all()
+    assert X.shape[1] == planner.N_FEATURES and len(X) == len(y) > 20 and np.isfinite(y).all()
     w, _ = planner.train(X, y, epochs=60, batch=64, lr=3e-3, seed=0)
     planner.install(w)
     try:

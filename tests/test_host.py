@@ -277,8 +277,9 @@ def test_learned_planner_merge_decisions_follow_the_model():
         # than the widest compiled gate (2 qubits).
         w[:] = 0
         w[0 * 64 + 0] = 1.0       # W1[k][0]
-        w[12 * 64 + 64 + 0] = 60.0  # W2[0][0]
-        w[12 * 64 + 64 + 64 * 64 + 64 + 0] = 1.0  # w3[0]
+        F = planner.N_FEATURES
+        w[F * 64 + 64 + 0] = 60.0  # W2[0][0]
+        w[F * 64 + 64 + 64 * 64 + 64 + 0] = 1.0  # w3[0]
         planner.install(w)
         narrow = _core.plan_json(circ, precision="complex64", fusion="learned", max_fused_qubits=4, tile_qubits=0)
         assert max(l["k"] for l in narrow["launches"]) == 2
@@ -400,3 +401,32 @@ def test_exact_sweeps_keep_program_order(kind):
             assert len(s) == min(L, 11) and {0, 1, 2, 3} <= s
             assert all(set(b["qubits"]) <= s for b in t["blocks"])
     assert not any(x["kind"] == "tile" for x in _core.plan_json(c, tile_qubits=0)["launches"])
+
+
+def test_learned_plan_selection_follows_the_model():
+    # complex64 tile plans under fusion="learned" are chosen among (block
+    # width, tile size) candidates by the model's predicted total time.
+    from dqsim import planner
+
+    c = workloads.random_su2(n=20, depth=10, seed=3)
+    F = planner.N_FEATURES
+    w = np.zeros(planner.N_PARAMS, dtype=np.float32)
+    try:
+        # Every launch predicts the same time: the fewest launches win (12-qubit tiles).
+        planner.install(w)
+        plan = _core.plan_json(c, precision="complex64", fusion="learned", max_fused_qubits=4)
+        tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+        assert tiles and all(len(t["qubits"]) == 12 for t in tiles)
+        order = sorted(o for x in plan["launches"] for o in x["ops"])
+        assert order == list(range(plan["compiled_ops"]))
+        # A model that charges heavily for 4-qubit blocks in a sweep (feature 15).
+        w[15 * 64 + 0] = 1.0                       # W1[15][0]
+        w[F * 64 + 64 + 0] = 1.0                   # W2[0][0]
+        w[F * 64 + 64 + 64 * 64 + 64 + 0] = 50.0   # w3[0]
+        planner.install(w)
+        plan = _core.plan_json(c, precision="complex64", fusion="learned", max_fused_qubits=4)
+        assert all(len(b["qubits"]) <= 3 for x in plan["launches"] if x["kind"] == "tile" for b in x["blocks"])
+        greedy4 = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=4)
+        assert any(len(b["qubits"]) == 4 for x in greedy4["launches"] if x["kind"] == "tile" for b in x["blocks"])
+    finally:
+        planner.install(None)

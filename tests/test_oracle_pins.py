@@ -297,11 +297,11 @@ def test_mlp_oracle_gradient_matches_finite_differences():
     from oracle import mlp
 
     rng = np.random.default_rng(4)
-    w = rng.normal(0, 0.3, size=12 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
-    X = rng.normal(0, 1, size=(9, 12))
+    w = rng.normal(0, 0.3, size=mlp.F * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+    X = rng.normal(0, 1, size=(9, mlp.F))
     y = rng.normal(0, 1, size=9)
     g, _ = mlp.grad(w, X, y)
-    idx = rng.choice(w.size, size=60, replace=False).tolist() + [w.size - 1, 0, 12 * 64]
+    idx = rng.choice(w.size, size=60, replace=False).tolist() + [w.size - 1, 0, mlp.F * 64]
     for i in idx:
         e = np.zeros_like(w)
         e[i] = 1e-6
