@@ -232,6 +232,7 @@ int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats
       out[i].gates = s.ops[i].gates;
       out[i].ms = s.ops[i].ms;
       out[i].bytes = s.ops[i].bytes;
+      static_assert(sizeof out[i].feat == sizeof s.ops[i].feat, "planner feature count matches the C ABI");
       std::memcpy(out[i].feat, s.ops[i].feat, sizeof out[i].feat);
     }
     if (nops) *nops = s.launches;

@@ -287,6 +287,7 @@ void block_features(const std::vector<uint32_t>& qubits, const std::vector<int>&
   f[9] = k >= 4 ? 1.f : 0.f;
   f[10] = (hi - lo) / 34.f;
   f[11] = (float)touched;
+  for (int i = 12; i < mlp::kFeatures; ++i) f[i] = 0.f;  // tile-sweep features
 }
 
 struct Plan {

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "circuit.hpp"
+#include "planner.hpp"
 
 namespace dq {
 
@@ -97,13 +98,13 @@ using Counts = std::vector<std::pair<std::string, uint64_t>>;
 
 // Per-launch timing record (engine-recorded kernel timings).
 struct OpTiming {
-  int kind;      // 0 one, 1 x, 2 swap, 3 diag, 4 dense, 5 fused, 6 measure, 7 reset
+  int kind;      // dq_op_timing.kind codes (include/dqsim_sv.h)
   int k;         // qubits touched
   int low;       // lowest qubit touched
   int gates;     // source gates folded into the launch
   double ms;     // device time
   double bytes;  // algorithmic HBM bytes moved
-  float feat[12];  // planner features of the launch (block_features)
+  float feat[mlp::kFeatures];  // planner features of the launch
 };
 
 struct EvolveStats {
