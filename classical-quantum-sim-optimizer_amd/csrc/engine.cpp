@@ -931,22 +931,31 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
       };
     if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
       if (fusion == FUSE_LEARNED && !g_planner.empty()) {
-        // Choose (block width, tile size) by the model's predicted total time.
-        int best_k = std::min(max_fused, 3), best_l = tile;
-        double best = 1e300;
+        // Choose (block width, tile size) by the model's predicted total
+        // time; leave the default (blocks <= 3 qubits, the largest tile) only
+        // for a predicted gain of at least 3 %.
+        const int def_k = std::min(max_fused, 3), def_l = tile;
+        int best_k = def_k, best_l = def_l;
+        double best = 1e300, def_t = 1e300;
         for (int kt = 2; kt <= std::max(2, std::min(max_fused, 4)); ++kt)
           for (int l = kTileMin; l <= tile; ++l) {
             if (n < l + 4) continue;
             Plan cand;
-            emit_sweeps(cand, ops, i, j, prec, n, l, std::min(kt, max_fused), nullptr);
+            const int k = std::min(kt, max_fused);
+            emit_sweeps(cand, ops, i, j, prec, n, l, k, nullptr);
             double t = 0;
             for (const DevOp& d : cand.ops) t += predicted_ms(d, n);
+            if (k == def_k && l == def_l) def_t = t;
             if (t < best) {
               best = t;
-              best_k = std::min(kt, max_fused);
+              best_k = k;
               best_l = l;
             }
           }
+        if (best > 0.97 * def_t) {
+          best_k = def_k;
+          best_l = def_l;
+        }
         emit_sweeps(plan, ops, i, j, prec, n, best_l, best_k, nullptr);
       } else {
         emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
@@ -2056,7 +2065,9 @@ double planner_score(const FusedBlock& a, const FusedBlock& b, const std::vector
     if (std::find(u.begin(), u.end(), q) == u.end()) u.push_back(q);
   std::vector<int> both(a.ops);
   both.insert(both.end(), b.ops.begin(), b.ops.end());
-  return predicted_time(a.qubits, a.ops, ops, n) + predicted_time(b.qubits, b.ops, ops, n) -
+  // A merge saves a launch; it is skipped only when the model predicts the
+  // merged block at least 5 % slower than its two parts.
+  return 1.05 * (predicted_time(a.qubits, a.ops, ops, n) + predicted_time(b.qubits, b.ops, ops, n)) -
          predicted_time(u, both, ops, n);
 }
 }  // namespace
