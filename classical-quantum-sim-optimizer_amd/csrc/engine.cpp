@@ -648,11 +648,13 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
 
 // Tile sweeps of a unitary run ops[begin, end):
 //  1. fuse to blocks of <= 2 qubits (wider compiled ops stay single blocks);
-//  2. group the blocks into sweeps: a sweep starts from qubits {0,1,2,3} (a
-//     tile row is then a 128-byte run) and repeatedly takes, among the blocks
-//     whose earlier untaken blocks share no qubit with them (so the taken order
-//     is a valid reordering), the one adding the fewest new qubits, while the
-//     union stays within L qubits;
+//  2. group the blocks into sweeps: a sweep's qubits always include {0,1,2,3}
+//     (a tile row is then a 128-byte run); it repeatedly takes, among the
+//     blocks whose earlier untaken blocks share no qubit with them (so the
+//     taken order is a valid reordering), the one adding the fewest new
+//     qubits, while the union stays within L qubits. Each sweep tries this
+//     from {0..3} alone and seeded with each of the first 8 available blocks,
+//     and keeps the trial that takes the most blocks;
 //  3. fuse each sweep's ops again into blocks of <= kt qubits (fewer passes
 //     over the tile in LDS) and emit one tile launch per sweep, split where the
 //     blob would exceed its LDS budget. A sweep of one block is an ordinary
@@ -669,25 +671,26 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
   std::vector<int> rem(pre.size());
   for (size_t b = 0; b < pre.size(); ++b) rem[b] = (int)b;
   std::vector<char> taken(pre.size(), 0);
-  size_t head = 0;
-  while (head < rem.size()) {
-    // rem[head..] holds the untaken blocks in program order.
-    const int h = rem[head];
-    if (!tileable((size_t)h)) {
-      emit_block(plan, pre[h], ops, prec, n);
-      taken[h] = 1;
-      ++head;
-      continue;
+  std::vector<int> mark(pre.size(), -1);  // trial number that took the block
+  int trial_no = 0;
+  // One min-growth trial from `seed` (-1: qubits 0..3 only); returns the
+  // blocks it would take, in a valid execution order.
+  auto trial = [&](size_t head, int seed, uint64_t& smask) {
+    const int gen = ++trial_no;
+    std::vector<int> take;
+    smask = 0xFull;
+    if (seed >= 0) {
+      mark[seed] = gen;
+      take.push_back(seed);
+      smask |= mask[seed];
     }
-    uint64_t smask = 0xFull;
-    std::vector<int> sweep;
     for (;;) {
       uint64_t blocked = 0;
       int best = -1, best_grow = 99;
       size_t seen = 0;
       for (size_t r = head; r < rem.size() && seen < kWindow; ++r) {
         const int b = rem[r];
-        if (taken[b]) continue;
+        if (taken[b] || mark[b] == gen) continue;
         ++seen;
         if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(smask | mask[b]) <= L) {
           const int grow = __builtin_popcountll(mask[b] & ~smask);
@@ -699,11 +702,47 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
         }
         blocked |= mask[b];
       }
-      if (best < 0 || sweep.size() >= 4 * kTileBlocksMax) break;
-      taken[best] = 1;
+      if (best < 0 || take.size() >= 4 * kTileBlocksMax) break;
+      mark[best] = gen;
+      take.push_back(best);
       smask |= mask[best];
-      sweep.push_back(best);
     }
+    return take;
+  };
+  size_t head = 0;
+  while (head < rem.size()) {
+    // rem[head..] holds the untaken blocks in program order.
+    const int h = rem[head];
+    if (!tileable((size_t)h)) {
+      emit_block(plan, pre[h], ops, prec, n);
+      taken[h] = 1;
+      ++head;
+      continue;
+    }
+    // Multi-start: grow from qubits 0..3 alone and from each of the first
+    // kStarts available blocks; keep the trial that takes the most blocks.
+    constexpr int kStarts = 8;
+    uint64_t smask = 0;
+    std::vector<int> sweep = trial(head, -1, smask);
+    {
+      uint64_t blocked = 0;
+      int starts = 0;
+      for (size_t r = head; r < rem.size() && starts < kStarts; ++r) {
+        const int b = rem[r];
+        if (taken[b]) continue;
+        if (!(mask[b] & blocked) && tileable((size_t)b)) {
+          ++starts;
+          uint64_t m2 = 0;
+          std::vector<int> t = trial(head, b, m2);
+          if (t.size() > sweep.size()) {
+            sweep = std::move(t);
+            smask = m2;
+          }
+        }
+        blocked |= mask[b];
+      }
+    }
+    for (int b : sweep) taken[b] = 1;
     // The head block is always available and fits (<= 4 new qubits), so the
     // sweep is never empty.
     std::vector<int> order;
