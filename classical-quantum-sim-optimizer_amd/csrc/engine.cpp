@@ -792,7 +792,7 @@ bool exact_tileable(const Op& op) {
 }
 
 size_t exact_op_bytes(const Op& op) {
-  size_t data = 0;
+  size_t data = sizeof(k::TileChain);  // at most one chain per op
   if (op.kind == OpKind::One) data = 64;
   else if (op.kind == OpKind::Diagonal) data = 128 + 16 * op.mat.size();
   else if (op.kind == OpKind::Dense) data = 16 * op.mat.size();
@@ -831,7 +831,10 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
     DevOp x = lower(ops[o], plan, 16);
     k::TileOp t{};
     t.nbits = x.e.nbits;
-    for (uint32_t b = 0; b < x.e.nbits; ++b) t.pos[b] = (uint32_t)local[x.e.pos[b]];
+    for (uint32_t b = 0; b < x.e.nbits; ++b) {
+      t.pos[b] = (uint32_t)local[x.e.pos[b]];
+      t.qmask |= 1u << t.pos[b];
+    }
     t.fixed = lmask(x.e.fixed);
     std::vector<uint8_t> dd;
     auto put = [&](const void* p, size_t bytes) {
@@ -880,21 +883,47 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
     d.tile_blocks.push_back(op_qubits(ops[o]));
     d.tile_mfma.push_back(0);
   }
-  const size_t head = tops.size() * sizeof(k::TileOp);
+  // Register chains: consecutive pair updates, X permutations and diagonals
+  // whose qubits together number at most 3; other ops run alone through LDS.
+  std::vector<k::TileChain> chains;
+  uint32_t umask = 0;
+  auto chainable = [](const k::TileOp& t) {
+    return (t.kind == 0 || (t.kind == 1 && t.abit == 0) || t.kind == 2) && __builtin_popcount(t.qmask) <= 3;
+  };
+  for (size_t t = 0; t < tops.size(); ++t) {
+    const bool ok = chainable(tops[t]);
+    if (ok && !chains.empty() && chains.back().nq > 0 && __builtin_popcount(umask | tops[t].qmask) <= 3) {
+      chains.back().count += 1;
+      umask |= tops[t].qmask;
+    } else {
+      chains.push_back({ok ? 1u : 0u, (uint32_t)t, 1u, 0u});
+      umask = ok ? tops[t].qmask : 0;
+    }
+    if (chains.back().nq > 0) {
+      uint32_t nq = 0, cb = 0;
+      for (uint32_t b = 0; b < 32; ++b)
+        if ((umask >> b) & 1) cb |= b << (8 * nq++);
+      chains.back().nq = nq;
+      chains.back().cbits = cb;
+    }
+  }
+  const size_t chead = chains.size() * sizeof(k::TileChain);
+  const size_t head = chead + tops.size() * sizeof(k::TileOp);
   size_t off = head;
   for (size_t t = 0; t < tops.size(); ++t) {
     tops[t].data_off = (uint32_t)off;
     off += datas[t].size();
   }
   std::vector<uint8_t> blob(off);
-  std::memcpy(blob.data(), tops.data(), head);
+  std::memcpy(blob.data(), chains.data(), chead);
+  std::memcpy(blob.data() + chead, tops.data(), tops.size() * sizeof(k::TileOp));
   off = head;
   for (auto& dd : datas) {
     std::memcpy(blob.data() + off, dd.data(), dd.size());
     off += dd.size();
   }
   d.targs.L = (uint32_t)L;
-  d.targs.nblocks = (uint32_t)tops.size();
+  d.targs.nblocks = (uint32_t)chains.size();
   d.targs.blob16 = (uint32_t)(blob.size() / 16);
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;

@@ -681,6 +681,128 @@ __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, cons
   }
 }
 
+// Register chains: consecutive 1-qubit, X and diagonal ops on <= 3 qubits in
+// total, applied by one thread to the 2^NQ amplitudes of its group held in
+// registers, op after op with each op's own expression (register j holds
+// local index base | combo(j), bit i of j <-> chain qubit c[i]). Every
+// amplitude goes through the same arithmetic as with one LDS pass per op;
+// pair and pattern indices are resolved at compile time per target bit.
+__device__ __forceinline__ uint32_t chain_index(uint32_t m, const uint32_t* c, int nq) {
+  uint32_t j = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < nq && ((m >> c[i]) & 1)) j |= 1u << i;
+  return j;
+}
+
+template <int NQ, int TJ>
+__device__ __forceinline__ void chain_one(double2* v, uint32_t cm, uint32_t cv, const double* u) {
+  constexpr int D = 1 << NQ;
+#pragma unroll
+  for (int j0 = 0; j0 < D; ++j0) {
+    if (j0 & TJ) continue;
+    if (((uint32_t)j0 & cm) != cv) continue;  // wave-uniform
+    const double2 x = v[j0], y = v[j0 | TJ];
+    double2 na, nc;
+    na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
+    na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
+    nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
+    nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
+    v[j0] = na;
+    v[j0 | TJ] = nc;
+  }
+}
+
+template <int NQ, int TJ>
+__device__ __forceinline__ void chain_x(double2* v, uint32_t cm, uint32_t cv) {
+  constexpr int D = 1 << NQ;
+#pragma unroll
+  for (int j0 = 0; j0 < D; ++j0) {
+    if (j0 & TJ) continue;
+    if (((uint32_t)j0 & cm) != cv) continue;
+    const double2 a = v[j0];
+    v[j0] = v[j0 | TJ];
+    v[j0 | TJ] = a;
+  }
+}
+
+template <int L, int NQ>
+__device__ __forceinline__ void tile128_chain(double2* tile, const TileChain& ch, const TileOp* ops,
+                                              const unsigned char* meta) {
+  constexpr int D = 1 << NQ;
+  uint32_t c[3], low[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    c[i] = (ch.cbits >> (8 * i)) & 0xFF;
+    low[i] = (1u << c[i]) - 1;
+  }
+  uint32_t off[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+      if ((j >> i) & 1) o |= 1u << c[i];
+    off[j] = o;
+  }
+  const uint32_t groups = (1u << L) >> NQ;
+  for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
+    uint32_t base = g;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) base = (base & low[i]) | ((base & ~low[i]) << 1);
+    double2 v[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = tile[base | off[j]];
+    for (uint32_t o = 0; o < ch.count; ++o) {
+      const TileOp& op = ops[ch.first + o];
+      const double* d = (const double*)(meta + op.data_off);
+      if (op.kind == 2) {
+        const uint32_t* pat = (const uint32_t*)d;
+        const double* val = d + 16;
+        const uint32_t qm = chain_index(op.qmask, c, NQ);
+        uint32_t key[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) key[t] = (uint32_t)t < op.k ? chain_index(pat[t], c, NQ) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const uint32_t kj = (uint32_t)j & qm;
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (key[t] == kj) {
+              const double2 a = v[j];
+              const double dr = val[2 * t], di = val[2 * t + 1];
+              double2 r;
+              r.x = a.x * dr - a.y * di;
+              r.y = a.x * di + a.y * dr;
+              v[j] = r;
+            }
+        }
+        continue;
+      }
+      const uint32_t tj = chain_index(op.bbit, c, NQ);
+      const uint32_t cm = chain_index(op.qmask & ~op.bbit, c, NQ), cv = chain_index(op.fixed, c, NQ);
+      if (op.kind == 0) {
+        double u[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) u[i] = d[i];
+        if (tj == 1) chain_one<NQ, 1>(v, cm, cv, u);
+        if constexpr (NQ >= 2)
+          if (tj == 2) chain_one<NQ, 2>(v, cm, cv, u);
+        if constexpr (NQ >= 3)
+          if (tj == 4) chain_one<NQ, 4>(v, cm, cv, u);
+      } else {
+        if (tj == 1) chain_x<NQ, 1>(v, cm, cv);
+        if constexpr (NQ >= 2)
+          if (tj == 2) chain_x<NQ, 2>(v, cm, cv);
+        if constexpr (NQ >= 3)
+          if (tj == 4) chain_x<NQ, 4>(v, cm, cv);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) tile[base | off[j]] = v[j];
+  }
+}
+
 template <int L>
 __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,
                                                  TileArgs a, const float4* __restrict__ blob) {
@@ -689,7 +811,8 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
   double2* tile = (double2*)lds4;
   const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L));
   for (uint32_t i = threadIdx.x; i < a.blob16; i += TPB) lds4[(1 << L) + i] = blob[i];
-  const TileOp* ops = (const TileOp*)meta;
+  const TileChain* chains = (const TileChain*)meta;
+  const TileOp* ops = (const TileOp*)(meta + sizeof(TileChain) * a.nblocks);
   double2* st = s + (uint64_t)blockIdx.y * stride;
   uint64_t lo = 0;
 #pragma unroll
@@ -744,8 +867,14 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
         ry[i] = v.y;
       }
     }
-    for (uint32_t oi = 0; oi < a.nblocks; ++oi) {
-      tile128_op<L>(tile, ops[oi], meta);
+    for (uint32_t ci = 0; ci < a.nblocks; ++ci) {
+      const TileChain& ch = chains[ci];
+      switch (ch.nq) {
+        case 0: tile128_op<L>(tile, ops[ch.first], meta); break;
+        case 1: tile128_chain<L, 1>(tile, ch, ops, meta); break;
+        case 2: tile128_chain<L, 2>(tile, ch, ops, meta); break;
+        default: tile128_chain<L, 3>(tile, ch, ops, meta); break;
+      }
       __syncthreads();
     }
 #pragma unroll

@@ -106,11 +106,20 @@ struct TileOp {
   uint32_t abit, bbit;  // pair members: i | abit and i | bbit
   uint32_t k;         // dense width or diagonal entry count
   uint32_t data_off;  // byte offset of the op's data in the blob
-  uint32_t pad0;
+  uint32_t qmask;     // local bits of all the op's qubits (targets and controls)
   uint32_t pos[8];    // sorted local positions of the op's qubits (targets and controls)
   uint32_t q[8];      // dense: local bit of each block qubit, q[0] = MSB of the block index
 };
 static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
+// A chain of consecutive ops applied by one thread to the 2^nq amplitudes of
+// its group held in registers (nq = 1..3 chain qubits), or one op applied
+// through LDS (nq = 0). Blob: nchains TileChain, then the TileOps, then data.
+struct TileChain {
+  uint32_t nq;     // chain qubits (0: single op through LDS)
+  uint32_t first;  // index of the chain's first op
+  uint32_t count;  // ops in the chain
+  uint32_t cbits;  // local bits of the chain qubits, ascending, 8 bits each
+};
 void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob_dev,
                      hipStream_t st);
 
