@@ -884,15 +884,16 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
     d.tile_mfma.push_back(0);
   }
   // Register chains: consecutive pair updates, X permutations and diagonals
-  // whose qubits together number at most 3; other ops run alone through LDS.
+  // whose qubits together number at most 2 (a 3-qubit chain's registers made
+  // the kernel spill to scratch memory); other ops run alone through LDS.
   std::vector<k::TileChain> chains;
   uint32_t umask = 0;
   auto chainable = [](const k::TileOp& t) {
-    return (t.kind == 0 || (t.kind == 1 && t.abit == 0) || t.kind == 2) && __builtin_popcount(t.qmask) <= 3;
+    return (t.kind == 0 || (t.kind == 1 && t.abit == 0) || t.kind == 2) && __builtin_popcount(t.qmask) <= 2;
   };
   for (size_t t = 0; t < tops.size(); ++t) {
     const bool ok = chainable(tops[t]);
-    if (ok && !chains.empty() && chains.back().nq > 0 && __builtin_popcount(umask | tops[t].qmask) <= 3) {
+    if (ok && !chains.empty() && chains.back().nq > 0 && __builtin_popcount(umask | tops[t].qmask) <= 2) {
       chains.back().count += 1;
       umask |= tops[t].qmask;
     } else {

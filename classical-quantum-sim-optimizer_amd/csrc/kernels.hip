@@ -681,8 +681,9 @@ __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, cons
   }
 }
 
-// Register chains: consecutive 1-qubit, X and diagonal ops on <= 3 qubits in
-// total, applied by one thread to the 2^NQ amplitudes of its group held in
+// Register chains: consecutive 1-qubit, X and diagonal ops on <= 2 qubits in
+// total (the host also supports 3 in this code; it issues at most 2),
+// applied by one thread to the 2^NQ amplitudes of its group held in
 // registers, op after op with each op's own expression (register j holds
 // local index base | combo(j), bit i of j <-> chain qubit c[i]). Every
 // amplitude goes through the same arithmetic as with one LDS pass per op;
@@ -872,8 +873,7 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
       switch (ch.nq) {
         case 0: tile128_op<L>(tile, ops[ch.first], meta); break;
         case 1: tile128_chain<L, 1>(tile, ch, ops, meta); break;
-        case 2: tile128_chain<L, 2>(tile, ch, ops, meta); break;
-        default: tile128_chain<L, 3>(tile, ch, ops, meta); break;
+        default: tile128_chain<L, 2>(tile, ch, ops, meta); break;
       }
       __syncthreads();
     }

@@ -112,7 +112,7 @@ struct TileOp {
 };
 static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
 // A chain of consecutive ops applied by one thread to the 2^nq amplitudes of
-// its group held in registers (nq = 1..3 chain qubits), or one op applied
+// its group held in registers (nq = 1..2 chain qubits), or one op applied
 // through LDS (nq = 0). Blob: nchains TileChain, then the TileOps, then data.
 struct TileChain {
   uint32_t nq;     // chain qubits (0: single op through LDS)
