@@ -229,6 +229,7 @@ struct DevOp {
   std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
   std::vector<int> tile_mfma;
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
+  int chains = 0;      // exact sweeps: register chains / single-op passes
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -925,6 +926,7 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
   }
   d.targs.L = (uint32_t)L;
   d.targs.nblocks = (uint32_t)chains.size();
+  d.chains = (int)chains.size();
   d.targs.blob16 = (uint32_t)(blob.size() / 16);
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
@@ -1783,6 +1785,7 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
     for (size_t t = 0; t < d.src.size(); ++t) js << (t ? ", " : "") << d.src[t];
     js << "]";
     if (d.kind == D_TILE) {
+      js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"chains\": " << d.chains;
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";

@@ -430,3 +430,15 @@ def test_learned_plan_selection_follows_the_model():
         assert any(len(b["qubits"]) == 4 for x in greedy4["launches"] if x["kind"] == "tile" for b in x["blocks"])
     finally:
         planner.install(None)
+
+
+def test_exact_sweeps_group_ops_into_register_chains():
+    # Each exact sweep applies its ops in passes: register chains of
+    # consecutive 1-qubit / X / diagonal ops on <= 2 qubits, other ops alone.
+    c = workloads.random_su2(n=18, depth=6, seed=4)
+    plan = _core.plan_json(c)
+    tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+    assert tiles and all(t["exact"] for t in tiles)
+    assert sum(t["chains"] for t in tiles) < sum(len(t["ops"]) for t in tiles)
+    for t in tiles:
+        assert 1 <= t["chains"] <= len(t["ops"])
