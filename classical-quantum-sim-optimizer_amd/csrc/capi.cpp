@@ -216,6 +216,20 @@ int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion
   });
 }
 
+int dq_batch_program_text(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
+                          const uint64_t* seeds, int32_t max_fused, int32_t lds_qubits, char** out) {
+  return guard([&] {
+    std::vector<std::pair<const char*, size_t>> cs;
+    for (uint64_t i = 0; i < ncirc; ++i) cs.push_back({jsons[i], lens[i]});
+    std::string s = dq::batch_program_text(cs, std::vector<uint64_t>(shots, shots + ncirc),
+                                           std::vector<uint64_t>(seeds, seeds + ncirc), max_fused, lds_qubits);
+    char* p = (char*)std::malloc(s.size() + 1);
+    if (!p) throw std::bad_alloc();
+    std::memcpy(p, s.c_str(), s.size() + 1);
+    *out = p;
+  });
+}
+
 void dq_string_free(char* s) { std::free(s); }
 
 int dq_device_count(void) { return dq::device_count(); }

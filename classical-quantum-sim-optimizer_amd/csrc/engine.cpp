@@ -1957,22 +1957,18 @@ double time_graph(const BatchDevice& d, hipStream_t st, int reps, hipGraphExec_t
 
 }  // namespace
 
-std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
-                                        const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
-                                        const BatchOptions& opt) {
+namespace {
+ParsedBatch parse_batch(const std::vector<std::pair<const char*, size_t>>& circuits,
+                        const std::vector<uint64_t>& shots, BatchOutcome* out) {
   const size_t nc = circuits.size();
-  if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
-  // The persistent launcher stalled (no progress for 180 s) on its first GPU
-  // test and the cause is not yet identified, so it stays disabled.
-  throw RuntimeErr("the batched persistent launcher is disabled in this build (it stalled on its first GPU run; "
-                   "the cause is not identified yet)");
-  auto out = std::make_unique<BatchOutcome>();
   ParsedBatch pb;
   pb.p.resize(nc);
   pb.pe.resize(nc);
-  out->nq.resize(nc);
-  out->nc.resize(nc);
-  out->cmap.resize(nc);
+  if (out) {
+    out->nq.resize(nc);
+    out->nc.resize(nc);
+    out->cmap.resize(nc);
+  }
   for (size_t i = 0; i < nc; ++i) {
     Parsed& p = pb.p[i];
     try {
@@ -1991,10 +1987,58 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
       mapping[p.ops[j].cbit] = p.ops[j].qubit;
     }
     pb.pe[i] = pe;
-    out->nq[i] = p.n;
-    out->nc[i] = p.nc;
-    out->cmap[i].assign(mapping.begin(), mapping.end());
+    if (out) {
+      out->nq[i] = p.n;
+      out->nc[i] = p.nc;
+      out->cmap[i].assign(mapping.begin(), mapping.end());
+    }
   }
+  return pb;
+}
+}  // namespace
+
+std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>& circuits,
+                               const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds, int max_fused,
+                               int lmax) {
+  const size_t nc = circuits.size();
+  if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
+  ParsedBatch pb = parse_batch(circuits, shots, nullptr);
+  std::vector<uint32_t> all(nc);
+  for (uint32_t i = 0; i < nc; ++i) all[i] = i;
+  BatchProgram prog = build_batch(pb, shots, seeds, max_fused, all);
+  uint64_t slot = 0;
+  for (auto& c : prog.circ)
+    if ((int)c.n > lmax) slot = std::max<uint64_t>(slot, 1ull << c.n);
+  std::ostringstream t;
+  t.precision(9);
+  t << prog.circ.size() << " " << prog.ops.size() << " " << prog.mats.size() << " " << prog.total_shots << " "
+    << slot << "\n";
+  for (auto& c : prog.circ)
+    t << c.n << " " << c.nops << " " << c.op_off << " " << c.seed << " " << c.shots << " " << c.out_off << "\n";
+  for (uint32_t o : prog.order) t << o << " ";
+  t << "\n";
+  for (auto& op : prog.ops) {
+    t << op.k << " " << op.mat_off;
+    for (int i = 0; i < 5; ++i) t << " " << op.q[i];
+    for (int i = 0; i < 5; ++i) t << " " << op.s[i];
+    t << "\n";
+  }
+  for (float f : prog.mats) t << f << " ";
+  t << "\n";
+  return t.str();
+}
+
+std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
+                                        const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
+                                        const BatchOptions& opt) {
+  const size_t nc = circuits.size();
+  if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
+  // The persistent launcher stalled (no progress for 180 s) on its first GPU
+  // test and the cause is not yet identified, so it stays disabled.
+  throw RuntimeErr("the batched persistent launcher is disabled in this build (it stalled on its first GPU run; "
+                   "the cause is not identified yet)");
+  auto out = std::make_unique<BatchOutcome>();
+  ParsedBatch pb = parse_batch(circuits, shots, out.get());
   Stream st;
   std::vector<uint32_t> all(nc);
   for (uint32_t i = 0; i < nc; ++i) all[i] = i;

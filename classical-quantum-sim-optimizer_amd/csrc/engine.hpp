@@ -163,6 +163,12 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
                                         const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
                                         const BatchOptions& opt);
 Counts batch_counts(const BatchOutcome& b, uint64_t i);
+// Host-only text dump of the program the batched launcher would upload
+// (circuits, launch order, fused ops, FP32 matrices; circuits above `lmax`
+// qubits use an HBM slot of the printed size).
+std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>& circuits,
+                               const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds, int max_fused,
+                               int lmax);
 // Device and kernel limits relevant to the batched launcher (no launch).
 void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_dynamic, int* static_lds,
                        int* max_threads, int* lmax);

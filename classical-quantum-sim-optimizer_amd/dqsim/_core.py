@@ -156,6 +156,7 @@ def _load():
         "dq_batch_state_c128": (ctypes.c_int, [P, U64, P, U64]),
         "dq_batch_free": (None, [P]),
         "dq_batch_device_info": (ctypes.c_int, [I32, P]),
+        "dq_batch_program_text": (ctypes.c_int, [P, P, U64, P, P, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
         "dq_mlp_param_count": (U64, []),
         "dq_mlp_train": (ctypes.c_int, [P, P, P, U64, I32, I32, ctypes.c_float, U64, P]),
         "dq_mlp_grad": (ctypes.c_int, [P, P, P, I32, P, P]),

@@ -81,6 +81,26 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
     return BatchResult(h, n, shots_l, nq)
 
 
+def program_text(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = None, base_seed: int = 0,
+                 max_fused_qubits: int = 3, lds_qubits: int = 14) -> str:
+    """Host-only dump of the program ``run_batch`` would upload (no GPU)."""
+    texts = [_circuit_json(c) for c in circuits]
+    n = len(texts)
+    shots_l = [int(shots)] * n if isinstance(shots, (int, np.integer)) else [int(s) for s in shots]
+    seeds_l = [base_seed + i for i in range(n)] if seeds is None else [int(s) for s in seeds]
+    bufs = [ctypes.create_string_buffer(t, len(t)) for t in texts]
+    ptrs = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(b) for b in bufs])
+    lens = (ctypes.c_size_t * max(1, n))(*[len(t) for t in texts])
+    sh = (ctypes.c_uint64 * max(1, n))(*shots_l)
+    sd = (ctypes.c_uint64 * max(1, n))(*seeds_l)
+    out = ctypes.c_void_p()
+    _check(_lib.dq_batch_program_text(ptrs, lens, n, sh, sd, max_fused_qubits, lds_qubits, ctypes.byref(out)))
+    try:
+        return ctypes.string_at(out.value).decode()
+    finally:
+        _lib.dq_string_free(out)
+
+
 def device_info(block: int = 256) -> Dict[str, int]:
     """Launch limits of the batched kernel for ``block`` threads (no launch):
     device LDS per block, the planned LDS qubit count, whether the kernel

@@ -193,6 +193,13 @@ void dq_batch_free(dq_batch* b);
  * LDS size, kernel max dynamic LDS, kernel static LDS, kernel max threads,
  * planned LDS qubits, device count}. */
 int dq_batch_device_info(int32_t block, int32_t* info7);
+/* Host-only text dump of the program dq_batch_run would upload for these
+ * circuits at fusion width `max_fused` and `lds_qubits` LDS-resident qubits
+ * (header line: circuits, ops, matrix floats, total shots, HBM slot
+ * amplitudes; then one line per circuit, the launch order, one line per op,
+ * the FP32 matrices). Free it with dq_string_free. */
+int dq_batch_program_text(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
+                          const uint64_t* seeds, int32_t max_fused, int32_t lds_qubits, char** out);
 
 /* ---- learned fusion planner (no reference counterpart) ----
  * A 12 -> 64 -> 64 -> 1 FP32 MLP predicting log(ms per 2^24 amplitudes) of a

@@ -291,6 +291,45 @@ def test_learned_planner_merge_decisions_follow_the_model():
         planner.install(np.zeros(5, dtype=np.float32))
 
 
+def _parse_batch_program(text):
+    lines = text.split("\n")
+    nc, nops, nmats, total_shots, slot = map(int, lines[0].split())
+    circ = [list(map(int, lines[1 + i].split())) for i in range(nc)]
+    order = list(map(int, lines[1 + nc].split()))
+    ops = [list(map(int, lines[2 + nc + j].split())) for j in range(nops)]
+    mats = np.array(lines[2 + nc + nops].split(), dtype=np.float64)
+    return nc, nops, nmats, total_shots, slot, circ, order, ops, mats
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_batched_program_stays_inside_its_buffers(k):
+    # Every index the batched kernel derives from the uploaded program lies
+    # inside the buffer it addresses: op ranges, matrix ranges, qubits, output
+    # ranges, the HBM slot of circuits above the LDS limit, and the launch
+    # order is a permutation (largest estimated cost first).
+    from dqsim import batch
+    cs = ([workloads.batch_circuit(i, base_seed=77, nmin=3, nmax=12) for i in range(12)]
+          + [workloads.batch_circuit(i, base_seed=5, nmin=15, nmax=16) for i in range(3)])
+    text = batch.program_text(cs, shots=300, base_seed=11, max_fused_qubits=k, lds_qubits=14)
+    nc, nops, nmats, total_shots, slot, circ, order, ops, mats = _parse_batch_program(text)
+    assert nc == len(cs) and len(ops) == nops and mats.size == nmats and total_shots == 300 * nc
+    assert sorted(order) == list(range(nc))
+    cost = [(c[1] + 2) * 2.0 ** c[0] for c in circ]
+    assert all(cost[order[i]] >= cost[order[i + 1]] for i in range(nc - 1))
+    assert slot == max(1 << c[0] for c in circ if c[0] > 14)
+    out_end = 0
+    for i, (n, nop, op_off, seed, shots, out_off) in enumerate(circ):
+        assert n == cs[i].num_qubits and seed == 11 + i and shots == 300 and out_off == out_end
+        out_end += shots
+        assert op_off + nop <= nops
+        for kk, mat_off, *qs in ops[op_off:op_off + nop]:
+            q, srt = qs[:5], qs[5:]
+            assert 1 <= kk <= k and mat_off + 2 * 4 ** kk <= nmats
+            assert len(set(q[:kk])) == kk and all(x < n for x in q[:kk]) and srt[:kk] == sorted(q[:kk])
+    assert out_end == total_shots
+    assert np.isfinite(mats).all()
+
+
 def test_batched_launcher_is_disabled_with_a_clear_error():
     # The persistent batched launcher raises before touching the device.
     from dqsim import batch
