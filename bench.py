@@ -27,6 +27,7 @@ KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag":
                 "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse",
                 "tile": "k_tile", "tile128": "k_tile128"}
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -60,6 +61,40 @@ def profiled_traffic(kernel, workload):
             if kv.get("kernel") == kernel and kv.get("workload") == workload:
                 return float(kv["bytes"])
     return None
+
+
+def profiled_mfma_busy(kernel, workload):
+    """Fraction of SIMD cycles with the matrix core busy for `kernel` on
+    `workload` (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x SIMDs) from
+    the committed rocprofv3 PMC summaries; None if not profiled."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path, encoding="utf-8"):
+            if not line.startswith("mfma-busy:"):
+                continue
+            kv = dict(f.split("=", 1) for f in line.split(":", 1)[1].split())
+            if kv.get("kernel") == kernel and kv.get("workload") == workload:
+                return float(kv["frac"])
+    return None
+
+
+def mfma_flops_per_step(circ, precision, fusion, max_fused, tile):
+    """FP32 matrix-core flops of one step from the launch plan: a k-qubit
+    block pass is 8 * 2^k * 2^n (one complex MAC = 8 real flops); blocks of
+    fewer than 3 qubits run padded to 3 in tile sweeps."""
+    from dqsim import _core
+
+    plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile)
+    n = plan["num_qubits"]
+    flops = 0.0
+    for op in plan["launches"]:
+        if op["kind"] == "tile" and not op.get("exact"):
+            for b in op["blocks"]:
+                flops += 8.0 * (1 << max(3, len(b["qubits"]))) * float(1 << n)
+        elif op["kind"] == "fused_mfma":
+            flops += 8.0 * (1 << len(op["qubits"])) * float(1 << n)
+    return flops
 
 
 def device_copy_gbs(nbytes=4 << 30, reps=5):
@@ -177,6 +212,8 @@ def main():
 
     copy_gbs = device_copy_gbs()
     log(f"[bench] device copy (4 GiB, torch copy_): {copy_gbs:.0f} GB/s read+write")
+    mfma_peak_tflops, _ = dqsim._core.mfma_fp32_peak(3)
+    log(f"[bench] FP32 matrix-core microbenchmark: {mfma_peak_tflops:.1f} TFLOP/s")
     n, depth = args.qubits, args.depth
     circ = workloads.random_su2(n=n, depth=depth, seed=1234)
     gates = workloads.gate_count(circ.data["instructions"])
@@ -206,6 +243,15 @@ def main():
     total_bytes = sum(a["bytes"] for a in agg.values()) / args.steps
     value = gates * float(1 << n) / (ms_per_step / 1e3)
 
+    mflops = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused, args.tile)
+    mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "fused_mfma")) / args.steps
+    mfma = None
+    if mflops > 0 and mfma_ms > 0:
+        ach = mflops / (mfma_ms / 1e3) / 1e12
+        mfma = {"achieved_tflops": ach, "datasheet_peak_tflops": MFMA_FP32_PEAK_TFLOPS,
+                "measured_peak_tflops": mfma_peak_tflops, "frac_of_datasheet": ach / MFMA_FP32_PEAK_TFLOPS,
+                "frac_of_measured_peak": ach / mfma_peak_tflops, "flops_per_step": mflops,
+                "busy_frac_profiled": None}
     extra = {"launches_per_step": launches, "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                             for kk, vv in v.items()} for k, v in agg.items()},
              "device_ms_per_step": sum(dev_ms) / len(dev_ms),
@@ -249,6 +295,9 @@ def main():
     workload_key = (f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
                     if args.precision == "complex64" and args.fusion == "greedy"
                     else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_k{args.max_fused}_t{args.tile}")
+    if mfma is not None:
+        mfma["busy_frac_profiled"] = profiled_mfma_busy(KERNEL_NAMES.get(dname, dname), workload_key)
+        extra["mfma"] = mfma
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -234,6 +234,10 @@ void dq_string_free(char* s) { std::free(s); }
 
 int dq_device_count(void) { return dq::device_count(); }
 
+int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch) {
+  return guard([&] { *tflops = dq::mfma_fp32_peak_tflops(reps, ms_per_launch); });
+}
+
 int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats, dq_op_timing* out, uint64_t cap,
                        uint64_t* nops, uint64_t* gates, double* total_ms, dq_result** keep) {
   return guard([&] {

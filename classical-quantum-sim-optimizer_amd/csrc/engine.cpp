@@ -169,6 +169,27 @@ int device_count() {
   return n;
 }
 
+double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
+  int dev = 0, cus = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t grid = (uint32_t)cus * 8;  // 8 waves per SIMD
+  const int iters = 20000;
+  DeviceMem out;
+  out.alloc((size_t)grid * 256 * sizeof(float));
+  Stream st;
+  k::mfma_peak(out.as<float>(), grid, iters, st.s);  // warm-up
+  Event a, b;
+  HIPCHK(hipEventRecord(a.e, st.s));
+  for (int r = 0; r < std::max(1, reps); ++r) k::mfma_peak(out.as<float>(), grid, iters, st.s);
+  HIPCHK(hipEventRecord(b.e, st.s));
+  st.sync();
+  HIPCHK(hipGetLastError());
+  const double ms = elapsed_ms(a, b) / std::max(1, reps);
+  if (ms_per_launch) *ms_per_launch = ms;
+  return (double)k::mfma_peak_flops_per_launch(grid, iters) / (ms * 1e-3) / 1e12;
+}
+
 // ---------------------------------------------------------------------------
 // Budget (shots.rs:22-48)
 // ---------------------------------------------------------------------------

@@ -166,6 +166,9 @@ Counts batch_counts(const BatchOutcome& b, uint64_t i);
 // Host-only text dump of the program the batched launcher would upload
 // (circuits, launch order, fused ops, FP32 matrices; circuits above `lmax`
 // qubits use an HBM slot of the printed size).
+// Measured FP32 matrix-core throughput (TFLOP/s) of a register-only
+// v_mfma_f32_16x16x4f32 loop over every CU; also the time per launch.
+double mfma_fp32_peak_tflops(int reps, double* ms_per_launch);
 std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>& circuits,
                                const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds, int max_fused,
                                int lmax);

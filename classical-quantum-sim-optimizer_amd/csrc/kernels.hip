@@ -1456,6 +1456,37 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int kq, c
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// ---------------------------------------------------------------------------
+// FP32 matrix-core peak microbenchmark: every wave runs `iters` rounds of 8
+// independent v_mfma_f32_16x16x4f32 chains on register operands (no memory
+// traffic in the loop); each lane stores one sum so the chains stay live.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ __launch_bounds__(TPB) void k_mfma_peak(float* __restrict__ out, int iters) {
+  floatx4 acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const float a = 1e-3f * (float)(threadIdx.x & 63), b = 1.0f + 1e-6f * (float)blockIdx.x;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
+  }
+  float sum = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) sum += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  out[(uint64_t)blockIdx.x * TPB + threadIdx.x] = sum;
+}
+}  // namespace
+
+uint64_t mfma_peak_flops_per_launch(uint32_t grid, int iters) {
+  // 2 * 16 * 16 * 4 flops per instruction, 8 per round, one wave per 64 threads.
+  return (uint64_t)grid * (TPB / 64) * (uint64_t)iters * 8ull * 2048ull;
+}
+
+void mfma_peak(float* out, uint32_t grid, int iters, hipStream_t st) {
+  hipLaunchKernelGGL(k_mfma_peak, dim3(grid), dim3(TPB), 0, st, out, iters);
+}
+
 size_t tile_lds_bytes(const TileArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
 static size_t tile128_lds_bytes(const TileArgs& a) { return ((size_t)16 << a.L) + (size_t)16 * a.blob16; }
 

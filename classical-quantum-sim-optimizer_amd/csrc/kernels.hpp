@@ -130,6 +130,9 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
 
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
+// FP32 matrix-core peak microbenchmark (`out` holds grid * 256 floats).
+void mfma_peak(float* out, uint32_t grid, int iters, hipStream_t st);
+uint64_t mfma_peak_flops_per_launch(uint32_t grid, int iters);
 void broadcast(int prec, void* dst, uint64_t stride, uint32_t batch, const void* src, int n, hipStream_t st);
 
 // ---- deterministic FP64 reductions ----

@@ -144,6 +144,7 @@ def _load():
         "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
         "dq_string_free": (None, [P]),
         "dq_device_count": (ctypes.c_int, []),
+        "dq_mfma_fp32_peak": (ctypes.c_int, [I32, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
@@ -541,6 +542,14 @@ def plan_json(circuit_or_json, precision="complex128", fusion="none", max_fused_
 
 def device_count() -> int:
     return int(_lib.dq_device_count())
+
+
+def mfma_fp32_peak(reps: int = 5):
+    """Measured FP32 matrix-core throughput of the current GPU:
+    (TFLOP/s, ms per launch) of a register-only MFMA loop."""
+    tf, ms = ctypes.c_double(), ctypes.c_double()
+    _check(_lib.dq_mfma_fp32_peak(reps, ctypes.byref(tf), ctypes.byref(ms)))
+    return tf.value, ms.value
 
 
 def set_planner_weights(weights) -> None:

@@ -156,6 +156,11 @@ void dq_string_free(char* s);
 /* Number of visible GPUs (0 on a host without one). */
 int dq_device_count(void);
 
+/* Measured FP32 matrix-core throughput: `reps` launches of a register-only
+ * v_mfma_f32_16x16x4f32 loop (8 independent chains per wave, 8 waves per
+ * SIMD) on the current device; TFLOP/s and milliseconds per launch. */
+int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch);
+
 /* Evolve the circuit's unitary prefix from |0...0> `repeats` times on the
  * simulator's settings; returns per-launch timings of the last repeat
  * (up to `cap` records), the launch count and the total device time. With

@@ -243,6 +243,15 @@ def test_batched_kernel_launch_limits_are_readable():
         assert 8 <= info["lds_qubits"] <= 15 and info["kernel_max_threads"] >= block
 
 
+def test_fp32_matrix_core_microbenchmark_runs():
+    # The register-only MFMA loop measures the FP32 matrix-core ceiling the
+    # bench reports beside the 157.3 TFLOP/s datasheet figure.
+    from dqsim import _core
+    tflops, ms = _core.mfma_fp32_peak(1)
+    print("mfma fp32", tflops, ms, flush=True)
+    assert ms > 0 and 20.0 < tflops < 200.0
+
+
 _TORCH_THEN_DQSIM = """
 import sys
 sys.path[:0] = {paths!r}
