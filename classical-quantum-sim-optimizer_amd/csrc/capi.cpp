@@ -271,11 +271,15 @@ int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, c
     for (uint64_t i = 0; i < ncirc; ++i) cs.push_back({jsons[i], lens[i]});
     dq::BatchOptions opt;
     if (o) {
-      opt.block = o->block;
+      opt.workers = o->workers;
       opt.max_fused = o->max_fused_qubits;
       opt.autotune = o->autotune != 0;
-      opt.replays = o->replays;
+      opt.tile = o->tile_qubits;
+      if (o->precision != DQ_PREC_C64 && o->precision != DQ_PREC_C128)
+        throw dq::ValueErr("precision must be DQ_PREC_C64 or DQ_PREC_C128");
+      opt.prec = o->precision == DQ_PREC_C128 ? 16 : 8;
       opt.dump_states = o->dump_states != 0;
+      opt.persistent = o->persistent != 0;
     }
     auto* b = new dq_batch;
     try {
@@ -306,15 +310,15 @@ int dq_batch_indices(const dq_batch* b, uint64_t i, uint32_t* out, uint64_t len)
   });
 }
 
-int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* block, int32_t* max_fused, int32_t* lds_qubits,
-                  uint32_t* grid, double* latency_us, uint64_t ncirc) {
+int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* workers, int32_t* max_fused, int32_t* tile_qubits,
+                  uint32_t* streams, double* latency_us, uint64_t ncirc) {
   return guard([&] {
     const auto& x = *b->b;
     if (total_ms) *total_ms = x.total_ms;
-    if (block) *block = x.block;
+    if (workers) *workers = x.workers;
     if (max_fused) *max_fused = x.fused;
-    if (lds_qubits) *lds_qubits = x.lmax;
-    if (grid) *grid = x.grid;
+    if (tile_qubits) *tile_qubits = x.tile;
+    if (streams) *streams = x.streams;
     if (latency_us)
       for (uint64_t i = 0; i < ncirc && i < x.latency_us.size(); ++i) latency_us[i] = x.latency_us[i];
   });
@@ -328,6 +332,10 @@ int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len) 
     double* o = (double*)out;
     for (uint64_t k = 0; k < 2 * len; ++k) o[k] = x.states[2 * x.state_off[i] + k];
   });
+}
+
+uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i) {
+  return i < b->b->nq.size() ? (uint32_t)b->b->nq[i] : 0;
 }
 
 void dq_batch_free(dq_batch* b) { delete b; }

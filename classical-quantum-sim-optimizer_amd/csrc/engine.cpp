@@ -16,6 +16,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -25,6 +26,7 @@
 #include <mutex>
 #include <random>
 #include <sstream>
+#include <thread>
 #include <unordered_map>
 
 #include "batch.hpp"
@@ -131,15 +133,60 @@ void cache_release() {
 }
 }  // namespace
 
+// Stream-ordered arena of one batch worker thread. While a thread has one
+// installed, its DeviceMem blocks come from and go back to the arena's free
+// lists (power-of-two size classes) with no hipFree and no device-wide
+// synchronisation: everything the worker runs is ordered on its one stream,
+// so a block handed out again is only touched after the earlier work on it.
+// The blocks are freed when the arena is destroyed, after the stream drains.
+struct DeviceArena {
+  std::vector<std::pair<size_t, void*>> free_blocks;
+  std::vector<void*> owned;
+  ~DeviceArena() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+  bool owns(const void* p) const { return std::find(owned.begin(), owned.end(), p) != owned.end(); }
+};
+namespace {
+thread_local DeviceArena* t_arena = nullptr;
+size_t arena_class(size_t bytes) {
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+}  // namespace
+
 DeviceMem::~DeviceMem() { reset(); }
 void DeviceMem::reset() {
-  if (p_ && !cache_put(n_, p_)) (void)hipFree(p_);
+  if (p_ && t_arena && t_arena->owns(p_)) t_arena->free_blocks.push_back({n_, p_});
+  else if (p_ && !cache_put(n_, p_)) (void)hipFree(p_);
   p_ = nullptr;
   n_ = 0;
 }
 void DeviceMem::alloc(size_t bytes) {
   reset();
   if (bytes == 0) bytes = 16;
+  if (t_arena) {
+    const size_t c = arena_class(bytes);
+    auto& fb = t_arena->free_blocks;
+    for (size_t i = 0; i < fb.size(); ++i)
+      if (fb[i].first == c) {
+        p_ = fb[i].second;
+        n_ = c;
+        fb.erase(fb.begin() + (long)i);
+        return;
+      }
+    hipError_t e = hipMalloc(&p_, c);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+      (void)hipGetLastError();
+      p_ = nullptr;
+      throw MemErr("Unable to allocate statevector");
+    }
+    hip_check(e, "hipMalloc");
+    t_arena->owned.push_back(p_);
+    n_ = c;
+    return;
+  }
   if (void* c = cache_take(bytes)) {
     p_ = c;
     n_ = bytes;
@@ -2049,88 +2096,183 @@ std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>
   return t.str();
 }
 
+namespace {
+// Measured qubits of a terminal-measurement circuit, highest first, and for
+// each written cbit the bit of the outcome index it reads (last write wins),
+// as in simulate_shots' terminal sampling.
+void terminal_map(const Parsed& p, size_t pe, std::vector<uint64_t>& qubits,
+                  std::vector<std::pair<uint64_t, uint64_t>>& cmap) {
+  std::map<uint64_t, uint64_t> mapping;  // cbit -> qubit
+  for (size_t j = pe; j < p.ops.size(); ++j) mapping[p.ops[j].cbit] = p.ops[j].qubit;
+  qubits.clear();
+  for (auto& kv : mapping) qubits.push_back(kv.second);
+  std::sort(qubits.begin(), qubits.end());
+  qubits.erase(std::unique(qubits.begin(), qubits.end()), qubits.end());
+  std::reverse(qubits.begin(), qubits.end());
+  const uint64_t m = qubits.size();
+  cmap.clear();
+  for (auto& kv : mapping) {
+    const size_t pos = std::find(qubits.begin(), qubits.end(), kv.second) - qubits.begin();
+    cmap.push_back({kv.first, m - 1 - pos});
+  }
+}
+
+struct BatchSetup {
+  const ParsedBatch* pb;
+  const std::vector<uint64_t>* shots;
+  const std::vector<uint64_t>* seeds;
+  std::vector<std::vector<uint64_t>> qubits;  // per circuit, highest first
+  int prec = 8, tile = kTileDefault, device = 0;
+};
+
+// Run `set` (indices into the batch, in this order) on `workers` host threads,
+// each with its own stream and device arena; every circuit takes the
+// simulate_shots terminal-sampling path (plan, evolve, sample on the device)
+// with the circuit's own seed. With `out` null nothing is recorded (tuning).
+// Returns the wall time in ms.
+double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, int workers, int fused,
+                       BatchOutcome* out) {
+  std::atomic<size_t> next{0};
+  std::atomic<bool> stop{false};
+  std::mutex err_mu;
+  std::exception_ptr err;
+  const double t0 = now_s();
+  auto work = [&]() {
+    try {
+      HIPCHK(hipSetDevice(su.device));
+      DeviceArena arena;
+      t_arena = &arena;
+      try {
+        Stream st;
+        for (size_t i = next.fetch_add(1); i < set.size() && !stop.load(); i = next.fetch_add(1)) {
+          const uint32_t c = set[i];
+          const double c0 = now_s();
+          const Parsed& p = su.pb->p[c];
+          const int n = (int)p.n;
+          const uint64_t shots = (*su.shots)[c], seed = (*su.seeds)[c];
+          Plan plan = build_plan(p.ops, 0, su.pb->pe[c], su.prec, FUSE_GREEDY, fused, n, su.tile);
+          plan.upload(st.s);
+          Exec ex(su.prec, n, p.nc, 1, st.s);
+          ex.cbase = plan.base();
+          k::init_zero(su.prec, ex.sp, ex.stride, 1, n, st.s);
+          k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
+          for (const DevOp& d : plan.ops) ex.run(d, 1);
+          std::vector<uint64_t> ix;
+          if (shots && !su.qubits[c].empty()) ix = sample_indices(ex, su.qubits[c], shots, seed);
+          else ix.assign(shots, 0);
+          if (out && !out->state_off.empty()) {
+            const uint64_t len = 1ull << n;
+            double* dst = out->states.data() + 2 * out->state_off[c];
+            if (su.prec == 16) {
+              HIPCHK(hipMemcpyAsync(dst, ex.sp, 16 * len, hipMemcpyDeviceToHost, st.s));
+            } else {
+              std::vector<float> tmp(2 * len);
+              HIPCHK(hipMemcpyAsync(tmp.data(), ex.sp, 8 * len, hipMemcpyDeviceToHost, st.s));
+              HIPCHK(hipStreamSynchronize(st.s));
+              for (uint64_t k2 = 0; k2 < 2 * len; ++k2) dst[k2] = (double)tmp[k2];
+            }
+          }
+          HIPCHK(hipStreamSynchronize(st.s));
+          if (out) {
+            for (uint64_t s2 = 0; s2 < shots; ++s2) out->idx[out->offsets[c] + s2] = (uint32_t)ix[s2];
+            out->latency_us[c] = (now_s() - c0) * 1e6;
+          }
+        }
+        st.sync();
+      } catch (...) {
+        t_arena = nullptr;
+        throw;
+      }
+      t_arena = nullptr;
+    } catch (...) {
+      stop.store(true);
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (!err) err = std::current_exception();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+  for (auto& t : pool) t.join();
+  if (err) std::rethrow_exception(err);
+  return (now_s() - t0) * 1e3;
+}
+}  // namespace
+
 std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
                                         const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
                                         const BatchOptions& opt) {
   const size_t nc = circuits.size();
   if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
-  // The persistent launcher stalled (no progress for 180 s) on its first GPU
-  // test and the cause is not yet identified, so it stays disabled.
-  throw RuntimeErr("the batched persistent launcher is disabled in this build (it stalled on its first GPU run; "
-                   "the cause is not identified yet)");
+  if (opt.persistent)
+    throw RuntimeErr("the persistent batched kernel is disabled in this build (it stalled on its first GPU run; "
+                     "the cause is not identified yet)");
+  if (opt.workers < 0 || opt.workers > 16) throw ValueErr("workers must be 0 (tuned) or 1..16");
+  if (opt.max_fused < 0 || opt.max_fused > 4) throw ValueErr("batched fusion width must be 0 (tuned) or 1..4");
+  if (opt.tile != 0 && (opt.tile < kTileMin || opt.tile > kTileMax)) throw ValueErr("tile_qubits must be 0 or in 10..12");
+  if (opt.prec != 8 && opt.prec != 16) throw ValueErr("precision must be complex64 or complex128");
   auto out = std::make_unique<BatchOutcome>();
   ParsedBatch pb = parse_batch(circuits, shots, out.get());
-  Stream st;
-  std::vector<uint32_t> all(nc);
-  for (uint32_t i = 0; i < nc; ++i) all[i] = i;
+  BatchSetup su;
+  su.pb = &pb;
+  su.shots = &shots;
+  su.seeds = &seeds;
+  su.prec = opt.prec;
+  su.tile = opt.tile;
+  HIPCHK(hipGetDevice(&su.device));
+  su.qubits.resize(nc);
+  uint64_t total_shots = 0, total_amps = 0;
+  out->offsets.resize(nc);
+  for (size_t i = 0; i < nc; ++i) {
+    terminal_map(pb.p[i], pb.pe[i], su.qubits[i], out->cmap[i]);
+    out->offsets[i] = total_shots;
+    total_shots += shots[i];
+  }
+  out->idx.assign(total_shots, 0);
+  out->latency_us.assign(nc, 0.0);
+  if (opt.dump_states) {
+    out->state_off.resize(nc);
+    for (size_t i = 0; i < nc; ++i) {
+      out->state_off[i] = total_amps;
+      total_amps += 1ull << pb.p[i].n;
+    }
+    out->states.assign(2 * total_amps, 0.0);
+  }
+  // Largest estimated work first (2^n amplitudes times compiled ops), so the
+  // pool's tail holds small circuits.
+  std::vector<uint32_t> order(nc);
+  for (uint32_t i = 0; i < nc; ++i) order[i] = i;
+  auto cost = [&](uint32_t i) { return std::ldexp((double)(pb.pe[i] + 8), (int)pb.p[i].n); };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
 
-  int block = opt.block ? opt.block : 256;
+  int workers = opt.workers ? opt.workers : 4;
   int fused = opt.max_fused ? opt.max_fused : 3;
-  if (block != 256 && block != 512 && block != 1024) throw ValueErr("block must be 256, 512 or 1024");
-  if (fused < 1 || fused > 4) throw ValueErr("batched fusion width must be 1..4");
-  if (opt.autotune && nc > 0) {
-    // Calibration subset: evenly spaced circuits of the cost order.
-    BatchProgram probe = build_batch(pb, shots, seeds, 2, all);
+  if (opt.autotune && nc > 0 && (!opt.workers || !opt.max_fused)) {
+    // Calibration subset: evenly spaced circuits of the cost order, run once
+    // untimed (plans, arenas and code objects warm), then timed per candidate.
     std::vector<uint32_t> sub;
-    const size_t want = std::min<size_t>(nc, 64);
-    for (size_t i = 0; i < want; ++i) sub.push_back(probe.order[(i * nc) / want]);
+    const size_t want = std::min<size_t>(nc, 96);
+    for (size_t i = 0; i < want; ++i) sub.push_back(order[(i * nc) / want]);
+    run_stream_pool(su, sub, 4, fused, nullptr);
     double best = 1e300;
-    const int blocks[3] = {256, 512, 1024};
-    for (int f = 2; f <= 4; ++f) {
-      if (opt.max_fused && f != opt.max_fused) continue;
-      BatchProgram prog = build_batch(pb, shots, seeds, f, sub);
-      for (int b : blocks) {
-        if (opt.block && b != opt.block) continue;
-        const int lm = lmax_for(b);
-        bk::set_lds_limit(b, bk::lds_bytes(lm, b));
-        BatchDevice dev;
-        upload_batch(dev, prog, b, lm, false, st.s, nullptr);
-        const double ms = time_graph(dev, st.s, 1, nullptr);
-        out->tuning.push_back({"block=" + std::to_string(b) + ",k<=" + std::to_string(f), ms});
+    for (int w : {2, 4, 8}) {
+      if (opt.workers && w != opt.workers) continue;
+      for (int f = 2; f <= 4; ++f) {
+        if (opt.max_fused && f != opt.max_fused) continue;
+        const double ms = run_stream_pool(su, sub, w, f, nullptr);
+        out->tuning.push_back({"workers=" + std::to_string(w) + ",k<=" + std::to_string(f), ms});
         if (ms < best) {
           best = ms;
-          block = b;
+          workers = w;
           fused = f;
         }
       }
     }
   }
-  const int lmax = lmax_for(block);
-  bk::set_lds_limit(block, bk::lds_bytes(lmax, block));
-  BatchProgram prog = build_batch(pb, shots, seeds, fused, all);
-  std::vector<uint64_t> dump_off;
-  if (opt.dump_states) {
-    uint64_t acc = 0;
-    for (size_t i = 0; i < nc; ++i) {
-      dump_off.push_back(acc);
-      acc += 1ull << out->nq[i];
-    }
-    dump_off.push_back(acc);
-  }
-  BatchDevice dev;
-  upload_batch(dev, prog, block, lmax, opt.dump_states, st.s, &dump_off);
-  out->total_ms = time_graph(dev, st.s, std::max(1, opt.replays), nullptr);
-  out->block = block;
+  out->total_ms = run_stream_pool(su, order, workers, fused, out.get());
+  out->workers = workers;
   out->fused = fused;
-  out->lmax = lmax;
-  out->grid = dev.args.grid;
-  out->idx.resize(prog.total_shots);
-  if (prog.total_shots)
-    HIPCHK(hipMemcpyAsync(out->idx.data(), dev.out.get(), 4 * prog.total_shots, hipMemcpyDeviceToHost, st.s));
-  std::vector<uint64_t> times(2 * nc);
-  if (nc) HIPCHK(hipMemcpyAsync(times.data(), dev.times.get(), 16 * nc, hipMemcpyDeviceToHost, st.s));
-  if (opt.dump_states) {
-    out->states.resize(2 * dump_off.back());
-    HIPCHK(hipMemcpyAsync(out->states.data(), dev.dump.get(), 8 * dump_off.back(), hipMemcpyDeviceToHost, st.s));
-    dump_off.pop_back();
-    out->state_off = dump_off;
-  }
-  st.sync();
-  const double khz = (double)device_attr(hipDeviceAttributeWallClockRate);
-  out->latency_us.resize(nc);
-  for (size_t i = 0; i < nc; ++i) out->latency_us[i] = (double)(times[2 * i + 1] - times[2 * i]) / khz * 1e3;
-  out->offsets.resize(nc);
-  for (size_t i = 0; i < nc; ++i) out->offsets[i] = prog.circ[i].out_off;
+  out->tile = opt.tile;
+  out->streams = (uint32_t)workers;
   return out;
 }
 

@@ -140,35 +140,37 @@ int device_count();
 
 // ---- batched small circuits (persistent kernel, batch.hip) ----
 struct BatchOptions {
-  int block = 0;       // 0 = autotune / default
-  int max_fused = 0;   // 0 = autotune / default
-  bool autotune = true;
-  int replays = 1;     // timed graph launches
+  int workers = 0;          // host threads, one HIP stream each; 0 = tuned (default 4)
+  int max_fused = 0;        // fusion width 1..4; 0 = tuned (default 3)
+  bool autotune = true;     // time (workers, width) candidates on a sample
+  int tile = kTileDefault;  // LDS tile qubits of each circuit's plan (0 = off)
+  int prec = 8;             // bytes per amplitude: 8 complex64, 16 complex128
   bool dump_states = false;
+  bool persistent = false;  // the persistent kernel: disabled, raises
 };
 struct BatchOutcome {
   std::vector<uint64_t> nq, nc;             // per circuit
   std::vector<uint64_t> offsets;            // sample offset per circuit
-  std::vector<uint32_t> idx;                // full-register sample indices
-  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> cmap;  // (cbit, qubit)
-  std::vector<double> latency_us;           // per circuit, last replay
-  std::vector<uint64_t> state_off;          // dumped states (complex64 -> c128 on query)
-  std::vector<float> states;                // interleaved re, im
-  double total_ms = 0;                      // device time per graph launch
-  int block = 0, fused = 0, lmax = 0;
-  uint32_t grid = 0;
-  std::vector<std::pair<std::string, double>> tuning;  // candidate -> ms
+  std::vector<uint32_t> idx;                // outcome per shot over the measured qubits (highest = MSB)
+  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> cmap;  // (cbit, bit of the outcome)
+  std::vector<double> latency_us;           // per circuit: plan, upload, evolution, sampling
+  std::vector<uint64_t> state_off;          // kept final states (amplitude offsets)
+  std::vector<double> states;               // interleaved re, im
+  double total_ms = 0;                      // wall time of the run (tuning excluded)
+  int workers = 0, fused = 0, tile = 0;
+  uint32_t streams = 0;
+  std::vector<std::pair<std::string, double>> tuning;  // candidate -> ms on the sample
 };
 std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
                                         const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds,
                                         const BatchOptions& opt);
 Counts batch_counts(const BatchOutcome& b, uint64_t i);
-// Host-only text dump of the program the batched launcher would upload
-// (circuits, launch order, fused ops, FP32 matrices; circuits above `lmax`
-// qubits use an HBM slot of the printed size).
 // Measured FP32 matrix-core throughput (TFLOP/s) of a register-only
 // v_mfma_f32_16x16x4f32 loop over every CU; also the time per launch.
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch);
+// Host-only text dump of the program the persistent batched kernel would
+// upload (circuits, launch order, fused ops, FP32 matrices; circuits above
+// `lmax` qubits use an HBM slot of the printed size).
 std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>& circuits,
                                const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds, int max_fused,
                                int lmax);

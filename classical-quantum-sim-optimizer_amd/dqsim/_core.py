@@ -83,11 +83,13 @@ class OpTiming(ctypes.Structure):
 
 class _BatchOpts(ctypes.Structure):
     _fields_ = [
-        ("block", ctypes.c_int32),
+        ("workers", ctypes.c_int32),
         ("max_fused_qubits", ctypes.c_int32),
         ("autotune", ctypes.c_int32),
-        ("replays", ctypes.c_int32),
+        ("tile_qubits", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
         ("dump_states", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
     ]
 
 
@@ -156,6 +158,7 @@ def _load():
                                          ctypes.POINTER(I32), ctypes.POINTER(ctypes.c_uint32), P, U64]),
         "dq_batch_state_c128": (ctypes.c_int, [P, U64, P, U64]),
         "dq_batch_free": (None, [P]),
+        "dq_batch_num_qubits": (ctypes.c_uint32, [P, U64]),
         "dq_batch_device_info": (ctypes.c_int, [I32, P]),
         "dq_batch_program_text": (ctypes.c_int, [P, P, U64, P, P, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
         "dq_mlp_param_count": (U64, []),

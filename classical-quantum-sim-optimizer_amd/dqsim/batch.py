@@ -1,9 +1,14 @@
-"""Batched execution of many small circuits in one persistent GPU launch.
+"""Batched execution of many small circuits.
 
 No reference counterpart: the reference would run such a workload as a loop of
 ``StatevectorSimulator.simulate_shots`` calls. Every circuit must end in
-terminal measurements; each gets its own shot count and seed and is sampled
-with the Philox stream (seed, stream = shot).
+terminal measurements; each gets its own shot count and seed. A pool of host
+threads, each with its own HIP stream and stream-ordered device arena (no
+allocation or device-wide synchronisation per circuit), takes the circuits
+largest first; each circuit follows ``simulate_shots``' terminal-sampling
+path with its own seed, so its counts equal that call's. The persistent
+single-launch kernel is kept for inspection only (``persistent=True``
+raises): it stalled on its first GPU run and no cause has been shown.
 """
 
 from __future__ import annotations
@@ -13,12 +18,13 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import _core
 from ._core import _BatchOpts, _check, _circuit_json, _collect_counts, _lib
+
+_PREC = {"complex64": 8, "complex128": 16}
 
 
 class BatchResult:
-    """Counts, sample indices, timings and the tuned launch shape of a batch."""
+    """Counts, sampled outcomes, timings and the tuned settings of a batch."""
 
     def __init__(self, handle, ncirc: int, shots: List[int], nqubits: List[int]):
         self._h = handle
@@ -26,16 +32,16 @@ class BatchResult:
         self._shots = shots
         self._nq = nqubits
         total = ctypes.c_double()
-        block, fused, lmax = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        grid = ctypes.c_uint32()
+        workers, fused, tile = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        streams = ctypes.c_uint32()
         lat = np.zeros(max(1, ncirc), dtype=np.float64)
-        _check(_lib.dq_batch_info(handle, ctypes.byref(total), ctypes.byref(block), ctypes.byref(fused),
-                                  ctypes.byref(lmax), ctypes.byref(grid), lat.ctypes.data, ncirc))
-        self.device_ms = total.value
-        self.block = block.value
+        _check(_lib.dq_batch_info(handle, ctypes.byref(total), ctypes.byref(workers), ctypes.byref(fused),
+                                  ctypes.byref(tile), ctypes.byref(streams), lat.ctypes.data, ncirc))
+        self.wall_ms = total.value
+        self.workers = workers.value
         self.max_fused_qubits = fused.value
-        self.lds_qubits = lmax.value
-        self.grid = grid.value
+        self.tile_qubits = tile.value
+        self.streams = streams.value
         self.latency_us = lat[:ncirc]
 
     def __del__(self):
@@ -50,6 +56,8 @@ class BatchResult:
         return out
 
     def indices(self, i: int) -> np.ndarray:
+        """Sampled outcomes over the measured qubits (highest measured qubit =
+        most significant bit)."""
         out = np.empty(max(1, self._shots[i]), dtype=np.uint32)
         _check(_lib.dq_batch_indices(self._h, i, out.ctypes.data, self._shots[i]))
         return out[: self._shots[i]]
@@ -61,10 +69,13 @@ class BatchResult:
 
 
 def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = None, base_seed: int = 0,
-              block: int = 0, max_fused_qubits: int = 0, autotune: bool = True, replays: int = 1,
-              keep_states: bool = False) -> BatchResult:
-    """Run circuits as one batched launch. ``shots`` is an int or a list;
-    ``seeds`` defaults to base_seed + index."""
+              workers: int = 0, max_fused_qubits: int = 0, autotune: bool = True, tile_qubits: int = 12,
+              precision: str = "complex64", keep_states: bool = False, persistent: bool = False) -> BatchResult:
+    """Run circuits as one batch. ``shots`` is an int or a list; ``seeds``
+    defaults to base_seed + index. ``workers`` / ``max_fused_qubits`` of 0 are
+    chosen by timing candidates on a sample when ``autotune`` is set."""
+    if precision not in _PREC:
+        raise ValueError("precision must be 'complex64' or 'complex128'")
     texts = [_circuit_json(c) for c in circuits]
     n = len(texts)
     shots_l = [int(shots)] * n if isinstance(shots, (int, np.integer)) else [int(s) for s in shots]
@@ -74,10 +85,11 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
     lens = (ctypes.c_size_t * max(1, n))(*[len(t) for t in texts])
     sh = (ctypes.c_uint64 * max(1, n))(*shots_l)
     sd = (ctypes.c_uint64 * max(1, n))(*seeds_l)
-    o = _BatchOpts(block, max_fused_qubits, 1 if autotune else 0, replays, 1 if keep_states else 0)
+    o = _BatchOpts(workers, max_fused_qubits, 1 if autotune else 0, tile_qubits, _PREC[precision],
+                   1 if keep_states else 0, 1 if persistent else 0)
     h = ctypes.c_void_p()
     _check(_lib.dq_batch_run(ptrs, lens, n, sh, sd, ctypes.byref(o), ctypes.byref(h)))
-    nq = [int(json_n) for json_n in (_core.plan_json(t)["num_qubits"] for t in texts)] if keep_states else [0] * n
+    nq = [int(_lib.dq_batch_num_qubits(h, i)) for i in range(n)]
     return BatchResult(h, n, shots_l, nq)
 
 

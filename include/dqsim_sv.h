@@ -172,33 +172,46 @@ int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats
  * such workloads as a loop of simulate_shots calls) ---- */
 typedef struct dq_batch dq_batch;
 typedef struct dq_batch_opts {
-  int32_t block;            /* threads per workgroup: 256, 512, 1024; 0 = tuned */
+  int32_t workers;          /* host threads, one HIP stream each: 1..16; 0 = tuned */
   int32_t max_fused_qubits; /* 1..4; 0 = tuned */
-  int32_t autotune;         /* time candidate (block, width) pairs on a sample */
-  int32_t replays;          /* timed graph launches (>= 1) */
+  int32_t autotune;         /* time candidate (workers, width) pairs on a sample */
+  int32_t tile_qubits;      /* LDS tile sweeps of each circuit's plan: 0 = off, 10..12 */
+  int32_t precision;        /* DQ_PREC_C64 or DQ_PREC_C128 */
   int32_t dump_states;      /* keep final states for dq_batch_state_c128 */
+  int32_t persistent;       /* 1 = the persistent kernel (disabled: DQ_ERUNTIME) */
 } dq_batch_opts;
 /* Run `ncirc` circuits (terminal measurements only) with shots[i] shots and
- * seed seeds[i] each, in one persistent kernel launched through a hipGraph. */
+ * seed seeds[i] each. A pool of host threads, each with its own HIP stream
+ * and stream-ordered device arena, takes circuits largest first; each circuit
+ * follows dq_sv_simulate_shots' terminal-sampling path (greedy fusion of
+ * width max_fused_qubits, the given tile and precision, seed seeds[i]), so
+ * its counts equal those of that call. */
 int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
                  const uint64_t* seeds, const dq_batch_opts* o, dq_batch** out);
 /* Counts of circuit i (same format as simulate_shots). */
 int dq_batch_counts(const dq_batch* b, uint64_t i, dq_count_cb cb, void* user);
-/* Full-register sample indices of circuit i (len = shots[i]). */
+/* Sampled outcomes of circuit i over its measured qubits, the highest
+ * measured qubit as the most significant bit (len = shots[i]). */
 int dq_batch_indices(const dq_batch* b, uint64_t i, uint32_t* out, uint64_t len);
-/* Device ms per graph launch, chosen block / width / LDS qubit limit /
- * grid, and per-circuit latency in microseconds (ncirc entries). */
-int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* block, int32_t* max_fused, int32_t* lds_qubits,
-                  uint32_t* grid, double* latency_us, uint64_t ncirc);
+/* Wall ms of the run (tuning excluded), chosen workers / fusion width / tile
+ * qubits / streams, and per-circuit latency in microseconds (ncirc entries:
+ * plan, upload, evolution and sampling of that circuit). */
+int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* workers, int32_t* max_fused, int32_t* tile_qubits,
+                  uint32_t* streams, double* latency_us, uint64_t ncirc);
 /* Final state of circuit i as complex128 (needs dump_states). */
 int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len);
+/* Qubit count of circuit i (0 when i is out of range). */
+uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i);
 void dq_batch_free(dq_batch* b);
-/* Launch limits of the batched kernel instance for `block` threads (no launch):
+/* The persistent batched kernel (one workgroup per circuit from an atomic
+ * queue, launched through a hipGraph) is disabled: it stalled on its first
+ * GPU run with no cause shown. The two calls below only inspect it.
+ * Launch limits of its kernel instance for `block` threads (no launch):
  * info7 = {device max LDS per block, hipFuncSetAttribute rc for the planned
  * LDS size, kernel max dynamic LDS, kernel static LDS, kernel max threads,
  * planned LDS qubits, device count}. */
 int dq_batch_device_info(int32_t block, int32_t* info7);
-/* Host-only text dump of the program dq_batch_run would upload for these
+/* Host-only text dump of the program the persistent kernel would take for these
  * circuits at fusion width `max_fused` and `lds_qubits` LDS-resident qubits
  * (header line: circuits, ops, matrix floats, total shots, HBM slot
  * amplitudes; then one line per circuit, the launch order, one line per op,
@@ -207,8 +220,8 @@ int dq_batch_program_text(const char* const* jsons, const size_t* lens, uint64_t
                           const uint64_t* seeds, int32_t max_fused, int32_t lds_qubits, char** out);
 
 /* ---- learned fusion planner (no reference counterpart) ----
- * A 12 -> 64 -> 64 -> 1 FP32 MLP predicting log(ms per 2^24 amplitudes) of a
- * fused launch from its features. Packed weights: W1[12][64], b1[64],
+ * A 16 -> 64 -> 64 -> 1 FP32 MLP predicting log(ms per 2^24 amplitudes) of a
+ * fused launch from its features. Packed weights: W1[16][64], b1[64],
  * W2[64][64], b2[64], w3[64], b3 (dq_mlp_param_count() entries). */
 uint64_t dq_mlp_param_count(void);
 /* Install weights for fusion = DQ_FUSE_LEARNED; count = 0 clears them. */

@@ -1,19 +1,23 @@
-"""Batched small circuits (persistent kernel): each circuit is pinned by the
-FP64 oracle (final state <= 1e-5) and its samples by the oracle's restatement
-of the kernel's sampler on the kernel's own final state (exact indices)."""
+"""Batched small circuits.
+
+The stream-pool runner (one host thread, HIP stream and device arena per
+worker) runs every circuit through simulate_shots' terminal-sampling path with
+the circuit's own seed, so its counts must equal those of the per-circuit call
+exactly, for any worker count; final states are pinned by the FP64 oracle.
+
+The persistent single-launch kernel stays disabled and its test skipped: it
+stalled (no progress for 180 s) on its first GPU run and no cause has been
+shown (a CPU emulation of its logic on that test's program stayed in bounds and
+reproduced the oracle)."""
 
 import numpy as np
 import pytest
 
-from dqsim import batch, workloads
+from dqsim import _core, batch, workloads
+from dqsim.workloads import Circuit
 from oracle import oracle
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the batched persistent launcher stalled (no progress for 180 s) on its first GPU run "
-                            "in test_batch_states_and_samples_match_oracle[256-2]; no cause has been shown, so the "
-                            "launcher raises and these cases stay off the GPU"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def circuits(count, nmin, nmax, dmin=2, dmax=8, base=77):
@@ -21,29 +25,91 @@ def circuits(count, nmin, nmax, dmin=2, dmax=8, base=77):
             for i in range(count)]
 
 
-@pytest.mark.parametrize("block,k", [(256, 2), (512, 3), (1024, 4), (256, 4)])
-def test_batch_states_and_samples_match_oracle(block, k):
-    cs = circuits(12, 3, 12) + circuits(3, 15, 16, base=5)
-    res = batch.run_batch(cs, shots=300, base_seed=11, block=block, max_fused_qubits=k, autotune=False,
+def per_circuit_counts(c, shots, seed, k, tile, precision="complex64"):
+    sim = _core.StatevectorSimulator(seed=seed, precision=precision, fusion="greedy", max_fused_qubits=k,
+                                     tile_qubits=tile)
+    return sim.simulate_shots(c, shots=shots)
+
+
+@pytest.mark.parametrize("workers,k", [(1, 3), (4, 2), (8, 3), (3, 4)])
+def test_batch_counts_equal_per_circuit_calls(workers, k):
+    # 3..12 qubits (one launch per block) and 16..18 qubits (LDS tile sweeps).
+    cs = circuits(10, 3, 12) + circuits(4, 16, 18, base=5)
+    print("batch", workers, k, [c.num_qubits for c in cs], flush=True)
+    res = batch.run_batch(cs, shots=300, base_seed=11, workers=workers, max_fused_qubits=k, autotune=False)
+    assert res.workers == workers and res.max_fused_qubits == k and res.tile_qubits == 12
+    for i, c in enumerate(cs):
+        got = res.counts(i)
+        assert sum(got.values()) == 300
+        assert got == per_circuit_counts(c, 300, 11 + i, k, 12), i
+
+
+def test_batch_states_match_oracle_and_indices_match_counts():
+    cs = circuits(8, 3, 12) + circuits(2, 16, 17, base=5)
+    res = batch.run_batch(cs, shots=200, base_seed=3, workers=4, max_fused_qubits=3, autotune=False,
                           keep_states=True)
-    assert res.block == block and res.max_fused_qubits == k
     for i, c in enumerate(cs):
         n = c.num_qubits
         unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
-        want, _ = oracle.simulate(workloads.Circuit(n, unitary).data, 1)
+        want, _ = oracle.simulate(Circuit(n, unitary).data, 1)
         got = res.statevector(i)
         assert np.abs(got - want).max() < 1e-5, (i, n)
-        idx = oracle.batch_sample(got, 300, 11 + i, block)
-        assert np.array_equal(res.indices(i).astype(np.uint64), idx), i
-        counts = res.counts(i)
-        assert sum(counts.values()) == 300
-        assert counts == oracle.counts_from_indices(idx, n)
+        # Every qubit is measured into the cbit of the same number: the outcome
+        # index is the register value, so the counts are its histogram.
+        idx = res.indices(i)
+        hist = {}
+        for x in idx:
+            key = format(int(x), f"0{n}b")
+            hist[key] = hist.get(key, 0) + 1
+        assert hist == res.counts(i)
+
+
+def test_batch_complex128_counts_equal_per_circuit_calls():
+    cs = circuits(6, 4, 12, base=21) + circuits(2, 16, 16, base=8)
+    res = batch.run_batch(cs, shots=128, base_seed=40, workers=2, max_fused_qubits=2, autotune=False,
+                          precision="complex128", keep_states=True)
+    for i, c in enumerate(cs):
+        assert res.counts(i) == per_circuit_counts(c, 128, 40 + i, 2, 12, precision="complex128"), i
+        unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
+        want, _ = oracle.simulate(Circuit(c.num_qubits, unitary).data, 1)
+        assert np.abs(res.statevector(i) - want).max() < 1e-12, i
+
+
+def test_batch_zero_shots_and_unmeasured_circuit():
+    measured = circuits(2, 3, 6, base=2)
+    bare = Circuit(3, [{"kind": "h", "qubit": 0}, {"kind": "cx", "control": 0, "target": 2}], 0)
+    res = batch.run_batch(measured + [bare], shots=[0, 50, 40], base_seed=1, workers=2, max_fused_qubits=3,
+                          autotune=False)
+    assert res.counts(0) == {}
+    assert sum(res.counts(1).values()) == 50
+    assert res.counts(2) == {"": 40}
 
 
 def test_batch_autotune_and_latency():
     cs = circuits(40, 4, 14, dmin=4, dmax=12, base=9)
     res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True)
-    assert res.block in (256, 512, 1024) and 2 <= res.max_fused_qubits <= 4
-    assert res.device_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
-    for i in range(40):
-        assert sum(res.counts(i).values()) == 64
+    print("tuned", res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
+    assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4
+    assert res.wall_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
+    for i, c in enumerate(cs):
+        assert res.counts(i) == per_circuit_counts(c, 64, 3 + i, res.max_fused_qubits, 12), i
+
+
+@pytest.mark.skip(reason="the persistent batched kernel stalled (no progress for 180 s) on its first GPU run in "
+                         "test_batch_states_and_samples_match_oracle[256-2]; no cause has been shown, so it raises "
+                         "and this case stays off the GPU")
+@pytest.mark.parametrize("block,k", [(256, 2), (512, 3), (1024, 4), (256, 4)])
+def test_batch_states_and_samples_match_oracle(block, k):
+    # The persistent kernel with `block` threads per workgroup and fusion
+    # width k; its chunked sampler is restated by oracle.batch_sample.
+    cs = circuits(12, 3, 12) + circuits(3, 15, 16, base=5)
+    res = batch.run_batch(cs, shots=300, base_seed=11, max_fused_qubits=k, autotune=False, keep_states=True,
+                          persistent=True)
+    for i, c in enumerate(cs):
+        n = c.num_qubits
+        unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
+        want, _ = oracle.simulate(Circuit(n, unitary).data, 1)
+        got = res.statevector(i)
+        assert np.abs(got - want).max() < 1e-5, (i, n)
+        idx = oracle.batch_sample(got, 300, 11 + i, block)
+        assert np.array_equal(res.indices(i).astype(np.uint64), idx), i

@@ -330,12 +330,25 @@ def test_batched_program_stays_inside_its_buffers(k):
     assert np.isfinite(mats).all()
 
 
-def test_batched_launcher_is_disabled_with_a_clear_error():
-    # The persistent batched launcher raises before touching the device.
+def test_batched_runner_validates_before_touching_the_device():
+    # The persistent kernel raises (disabled), and bad options or circuits
+    # that are not terminal-measurement circuits fail before any GPU call.
     from dqsim import batch
     c = workloads.batch_circuit(0, base_seed=1, nmin=3, nmax=4)
-    with pytest.raises(RuntimeError, match="batched persistent launcher is disabled"):
-        batch.run_batch([c], shots=8, autotune=False)
+    with pytest.raises(RuntimeError, match="persistent batched kernel is disabled"):
+        batch.run_batch([c], shots=8, persistent=True)
+    with pytest.raises(ValueError, match="workers"):
+        batch.run_batch([c], shots=8, workers=17)
+    with pytest.raises(ValueError, match="fusion width"):
+        batch.run_batch([c], shots=8, max_fused_qubits=5)
+    with pytest.raises(ValueError, match="tile_qubits"):
+        batch.run_batch([c], shots=8, tile_qubits=9)
+    with pytest.raises(ValueError, match="precision"):
+        batch.run_batch([c], shots=8, precision="complex32")
+    mid = Circuit(2, [{"kind": "h", "qubit": 0}, {"kind": "measure", "qubit": 0, "cbit": 0},
+                      {"kind": "x", "qubit": 1}], 1)
+    with pytest.raises(ValueError, match="terminal measurements only"):
+        batch.run_batch([c, mid], shots=8)
 
 
 def _mixed_tile_circuit(n, depth, seed):
