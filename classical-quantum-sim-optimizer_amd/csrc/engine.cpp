@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -2125,38 +2126,96 @@ struct BatchSetup {
   int prec = 8, tile = kTileDefault, device = 0;
 };
 
-// Run `set` (indices into the batch, in this order) on `workers` host threads,
-// each with its own stream and device arena; every circuit takes the
-// simulate_shots terminal-sampling path (plan, evolve, sample on the device)
-// with the circuit's own seed. With `out` null nothing is recorded (tuning).
-// Returns the wall time in ms.
+// Run `set` (indices into the batch, in this order). Planner threads build
+// the circuits' launch plans in order, at most kAhead ahead of execution;
+// `workers` GPU threads, each with its own stream and device arena, take the
+// circuits in the same order and run each on simulate_shots' terminal-sampling
+// path (upload, evolve, sample on the device) with the circuit's own seed.
+// With `out` null nothing is recorded (tuning). Returns the wall time in ms.
 double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, int workers, int fused,
                        BatchOutcome* out) {
-  std::atomic<size_t> next{0};
+  constexpr size_t kAhead = 64;
+  const size_t total = set.size();
+  const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+  const int planners = (int)std::min<unsigned>(12u, std::max(1u, hw - (unsigned)std::min<int>(workers, (int)hw - 1)));
+  std::vector<std::unique_ptr<Plan>> plans(total);
+  std::vector<double> plan_us(total, 0.0);
+  std::vector<char> ready(total, 0);
+  std::mutex mu;
+  std::condition_variable cv_ready, cv_space;
+  size_t next_plan = 0;
+  std::atomic<size_t> next_job{0};
   std::atomic<bool> stop{false};
-  std::mutex err_mu;
   std::exception_ptr err;
-  const double t0 = now_s();
-  auto work = [&]() {
+  auto fail = [&](std::exception_ptr e) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!err) err = e;
+      stop.store(true);
+    }
+    cv_ready.notify_all();
+    cv_space.notify_all();
+  };
+  auto plan_work = [&]() {
+    try {
+      for (;;) {
+        size_t j;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv_space.wait(lk, [&] { return stop.load() || next_plan >= total || next_plan < next_job.load() + kAhead; });
+          if (stop.load() || next_plan >= total) return;
+          j = next_plan++;
+        }
+        const uint32_t c = set[j];
+        const double t0 = now_s();
+        auto plan = std::make_unique<Plan>(
+            build_plan(su.pb->p[c].ops, 0, su.pb->pe[c], su.prec, FUSE_GREEDY, fused, (int)su.pb->p[c].n, su.tile));
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          plans[j] = std::move(plan);
+          plan_us[j] = (now_s() - t0) * 1e6;
+          ready[j] = 1;
+        }
+        cv_ready.notify_all();
+      }
+    } catch (...) {
+      fail(std::current_exception());
+    }
+  };
+  auto gpu_work = [&]() {
     try {
       HIPCHK(hipSetDevice(su.device));
       DeviceArena arena;
       t_arena = &arena;
       try {
         Stream st;
-        for (size_t i = next.fetch_add(1); i < set.size() && !stop.load(); i = next.fetch_add(1)) {
-          const uint32_t c = set[i];
+        for (;;) {
+          size_t j;
+          {
+            // Under the lock, so a planner checking for space cannot miss it.
+            std::lock_guard<std::mutex> lk(mu);
+            j = next_job.fetch_add(1);
+          }
+          cv_space.notify_all();
+          if (j >= total) break;
+          std::unique_ptr<Plan> plan;
+          {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_ready.wait(lk, [&] { return stop.load() || ready[j]; });
+            if (stop.load()) break;
+            plan = std::move(plans[j]);
+          }
+          const uint32_t c = set[j];
           const double c0 = now_s();
           const Parsed& p = su.pb->p[c];
           const int n = (int)p.n;
           const uint64_t shots = (*su.shots)[c], seed = (*su.seeds)[c];
-          Plan plan = build_plan(p.ops, 0, su.pb->pe[c], su.prec, FUSE_GREEDY, fused, n, su.tile);
-          plan.upload(st.s);
+          plan->upload(st.s);
           Exec ex(su.prec, n, p.nc, 1, st.s);
-          ex.cbase = plan.base();
+          ex.cbase = plan->base();
           k::init_zero(su.prec, ex.sp, ex.stride, 1, n, st.s);
           k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
-          for (const DevOp& d : plan.ops) ex.run(d, 1);
+          for (const DevOp& d : plan->ops) ex.run(d, 1);
           std::vector<uint64_t> ix;
           if (shots && !su.qubits[c].empty()) ix = sample_indices(ex, su.qubits[c], shots, seed);
           else ix.assign(shots, 0);
@@ -2169,13 +2228,13 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
               std::vector<float> tmp(2 * len);
               HIPCHK(hipMemcpyAsync(tmp.data(), ex.sp, 8 * len, hipMemcpyDeviceToHost, st.s));
               HIPCHK(hipStreamSynchronize(st.s));
-              for (uint64_t k2 = 0; k2 < 2 * len; ++k2) dst[k2] = (double)tmp[k2];
+              for (uint64_t q = 0; q < 2 * len; ++q) dst[q] = (double)tmp[q];
             }
           }
           HIPCHK(hipStreamSynchronize(st.s));
           if (out) {
             for (uint64_t s2 = 0; s2 < shots; ++s2) out->idx[out->offsets[c] + s2] = (uint32_t)ix[s2];
-            out->latency_us[c] = (now_s() - c0) * 1e6;
+            out->latency_us[c] = plan_us[j] + (now_s() - c0) * 1e6;
           }
         }
         st.sync();
@@ -2185,13 +2244,13 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
       }
       t_arena = nullptr;
     } catch (...) {
-      stop.store(true);
-      std::lock_guard<std::mutex> lk(err_mu);
-      if (!err) err = std::current_exception();
+      fail(std::current_exception());
     }
   };
+  const double t0 = now_s();
   std::vector<std::thread> pool;
-  for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+  for (int i = 0; i < planners; ++i) pool.emplace_back(plan_work);
+  for (int w = 0; w < workers; ++w) pool.emplace_back(gpu_work);
   for (auto& t : pool) t.join();
   if (err) std::rethrow_exception(err);
   return (now_s() - t0) * 1e3;
