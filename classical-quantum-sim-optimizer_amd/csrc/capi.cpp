@@ -334,6 +334,18 @@ int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len) 
   });
 }
 
+uint64_t dq_batch_tuning_count(const dq_batch* b) { return b->b->tuning.size(); }
+
+int dq_batch_tuning_entry(const dq_batch* b, uint64_t i, char* name, uint64_t cap, double* ms) {
+  return guard([&] {
+    const auto& t = b->b->tuning;
+    if (i >= t.size()) throw dq::ValueErr("tuning entry out of range");
+    if (cap == 0 || t[i].first.size() + 1 > cap) throw dq::ValueErr("name buffer too small");
+    std::memcpy(name, t[i].first.c_str(), t[i].first.size() + 1);
+    *ms = t[i].second;
+  });
+}
+
 uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i) {
   return i < b->b->nq.size() ? (uint32_t)b->b->nq[i] : 0;
 }

@@ -159,6 +159,8 @@ def _load():
         "dq_batch_state_c128": (ctypes.c_int, [P, U64, P, U64]),
         "dq_batch_free": (None, [P]),
         "dq_batch_num_qubits": (ctypes.c_uint32, [P, U64]),
+        "dq_batch_tuning_count": (U64, [P]),
+        "dq_batch_tuning_entry": (ctypes.c_int, [P, U64, ctypes.c_char_p, U64, ctypes.POINTER(D)]),
         "dq_batch_device_info": (ctypes.c_int, [I32, P]),
         "dq_batch_program_text": (ctypes.c_int, [P, P, U64, P, P, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
         "dq_mlp_param_count": (U64, []),

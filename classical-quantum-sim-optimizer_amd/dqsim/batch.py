@@ -43,6 +43,11 @@ class BatchResult:
         self.tile_qubits = tile.value
         self.streams = streams.value
         self.latency_us = lat[:ncirc]
+        self.tuning = {}
+        name, ms = ctypes.create_string_buffer(64), ctypes.c_double()
+        for i in range(int(_lib.dq_batch_tuning_count(handle))):
+            _check(_lib.dq_batch_tuning_entry(handle, i, name, 64, ctypes.byref(ms)))
+            self.tuning[name.value.decode()] = ms.value
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -101,7 +101,7 @@ def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, lo
            "gate_amp_total": work, "latency_us_p50": float(np.percentile(lat, 50)),
            "latency_us_p99": float(np.percentile(lat, 99)), "workers": res.workers,
            "max_fused_qubits": res.max_fused_qubits, "tile_qubits": res.tile_qubits, "streams": res.streams,
-           "host_wall_s_incl_json_and_tuning": wall, "generate_s": gen_s,
+           "tuning_ms_on_sample": res.tuning, "host_wall_s_incl_json_and_tuning": wall, "generate_s": gen_s,
            "path": "stream-pool runner: one host thread + HIP stream + device arena per worker, simulate_shots "
                    "terminal-sampling path per circuit"}
     log(f"[batch] {out}")

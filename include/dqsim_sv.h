@@ -200,6 +200,10 @@ int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* workers, int32_t
                   uint32_t* streams, double* latency_us, uint64_t ncirc);
 /* Final state of circuit i as complex128 (needs dump_states). */
 int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len);
+/* Autotuning record: the number of candidates timed, and candidate i's name
+ * ("workers=W,k<=F") and wall ms on the calibration sample. */
+uint64_t dq_batch_tuning_count(const dq_batch* b);
+int dq_batch_tuning_entry(const dq_batch* b, uint64_t i, char* name, uint64_t cap, double* ms);
 /* Qubit count of circuit i (0 when i is out of range). */
 uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i);
 void dq_batch_free(dq_batch* b);

@@ -90,6 +90,8 @@ def test_batch_autotune_and_latency():
     res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True)
     print("tuned", res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
     assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4
+    assert len(res.tuning) == 9 and all(ms > 0 for ms in res.tuning.values())
+    assert f"workers={res.workers},k<={res.max_fused_qubits}" in res.tuning
     assert res.wall_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
     for i, c in enumerate(cs):
         assert res.counts(i) == per_circuit_counts(c, 64, 3 + i, res.max_fused_qubits, 12), i
