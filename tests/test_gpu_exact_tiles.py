@@ -51,6 +51,18 @@ def test_mixed_circuits_bit_exact(n, seed):
     assert np.array_equal(exact_state(insts, n), want)
 
 
+def test_headline_family_at_24_qubits_bit_exact():
+    # The benchmark's circuit family (Haar SU(2) layers + random CZ/CX
+    # matchings) at 24 qubits: 2^13 tiles per sweep, every sweep kind of the
+    # 32-qubit run (register chains, dense pairs, diagonals, swaps of pairs).
+    circ = workloads.random_su2(n=24, depth=10, seed=77)
+    insts = circ.data["instructions"]
+    print("[exact] random_su2 24q depth 10", flush=True)
+    assert has_exact_tiles(insts, 24)
+    want = oracle.simulate(circ.data, 60)[0]
+    assert np.array_equal(exact_state(insts, 24), want)
+
+
 def test_qft_and_brickwork_bit_exact():
     for circ in (workloads.qft(17, seed=4, measure=False)[0], workloads.brickwork(n=18, layers=8, measure=False)):
         n = circ.num_qubits

@@ -22,6 +22,7 @@ CASES = [
     ("su2", 16, 10, 2), ("su2", 16, 10, 3), ("su2", 16, 10, 4), ("su2", 16, 10, 5),
     ("su2", 17, 11, 4), ("su2", 18, 12, 4), ("su2", 20, 12, 5), ("su2", 20, 12, 3),
     ("qft", 16, 12, 4), ("mixed", 17, 12, 3), ("mixed", 18, 11, 5), ("mixed", 16, 10, 4),
+    ("su2", 24, 12, 3), ("mixed", 22, 12, 3),
 ]
 
 
