@@ -591,17 +591,31 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
   return m;
 }
 
-// Tile-sweep matrix form of a block: kind 1 (k = 3; narrower blocks are
-// padded to 3 qubits) permuted 16x16 real embedding for the 16x16x4 MFMA,
+// Tile-sweep matrix form of a block: kind 3 (k = 2; 1-qubit blocks are
+// padded to 2 qubits) the 8x8 real embedding as A fragments of the 4x4x1
+// MFMA, kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA,
 // kind 2 (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
 // (kind 0, a complex FP32 matrix, serves no tile block.)
-int tile_kind(int k) { return k <= 2 ? 0 : k == 3 ? 1 : 2; }
+int tile_kind(int k) { return k <= 2 ? 3 : k == 3 ? 1 : 2; }
 size_t tile_matrix_bytes(int k) {
-  const size_t d = (size_t)1 << k;
-  return tile_kind(k) == 0 ? 8 * d * d : 4 * (2 * d) * (2 * d);
+  const size_t d = (size_t)1 << std::max(k, 2);
+  return 4 * (2 * d) * (2 * d);
 }
 std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
   const int kind = tile_kind(k);
+  if (kind == 3) {
+    // E[2c+p][2c'+s]: output c part p, input c' part s (0 real, 1 imaginary).
+    // Stored as A fragments: [i][h][kk] = E[4 h + i][kk].
+    std::vector<float> f(64, 0.f);
+    for (int row = 0; row < 8; ++row)
+      for (int col = 0; col < 8; ++col) {
+        const int c = row >> 1, p = row & 1, cin = col >> 1, q = col & 1;
+        const cplx x = m[(size_t)c * 4 + cin];
+        const double val = p == 0 ? (q == 0 ? x.re : -x.im) : (q == 0 ? x.im : x.re);
+        f[(size_t)(row & 3) * 16 + (row >> 2) * 8 + col] = (float)val;
+      }
+    return f;
+  }
   if (kind == 2) {
     // Fragment-major: [t][q][lane] = E[32 t + lane % 32][2 q + lane / 32].
     const int R = 2 << k, T = R / 32, STEPS = R / 2;
@@ -664,15 +678,14 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   std::vector<float> mats;
   const size_t head = blocks.size() * sizeof(k::TileBlock);
   for (FusedBlock* orig : blocks) {
-    // Blocks of 1 or 2 qubits become 3-qubit blocks (U kron I on the highest
-    // free tile qubits): the 16x16x4 MFMA applies them at the cost of a
-    // 3-qubit block, off the vector ALU.
+    // 1-qubit blocks become 2-qubit blocks (U kron I on the highest free
+    // tile qubit) for the 4x4x1 MFMA path.
     FusedBlock padded;
     FusedBlock* b = orig;
-    if (orig->qubits.size() < 3) {
+    if (orig->qubits.size() < 2) {
       padded.ops = orig->ops;
       std::vector<uint32_t> q = orig->qubits;
-      for (int i = nl - 1; i >= 0 && q.size() < 3; --i)
+      for (int i = nl - 1; i >= 0 && q.size() < 2; --i)
         if (std::find(q.begin(), q.end(), (uint32_t)d.targs.spos[i]) == q.end()) q.push_back(d.targs.spos[i]);
       std::sort(q.rbegin(), q.rend());
       padded.qubits = q;
@@ -705,8 +718,14 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
   d.low = 0;
-  int n3 = 0, n4 = 0;
-  for (const k::TileBlock& t : tb) (t.kind == 2 ? n4 : n3) += 1;
+  // Matrix-core work in units of a 3-qubit block (a 2-qubit block on the
+  // 4x4x1 path costs half of one, a 4-qubit block counts apart).
+  float n3 = 0.f;
+  int n4 = 0;
+  for (const k::TileBlock& t : tb) {
+    if (t.kind == 2) ++n4;
+    else n3 += t.kind == 3 ? 0.5f : 1.f;
+  }
   d.feat[4] = d.gates / 32.f;
   d.feat[8] = n / 34.f;
   d.feat[12] = 1.f;
@@ -822,7 +841,7 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
     while (i < blocks.size()) {
       size_t blob = 0, j = i;
       while (j < blocks.size() && j - i < kTileBlocksMax) {
-        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes(std::max(3, (int)blocks[j].qubits.size()));
+        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
         if (blob + bb > kTileBlobMax) break;
         blob += bb;
         ++j;

@@ -334,9 +334,10 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // one tile are applied in LDS, the loads of its next tile are in flight (the
 // block arithmetic reads only LDS, so it never waits on those loads).
 // Block kinds (TileBlock.kind):
-//   1  k = 3 (narrower blocks arrive padded to 3 qubits): permuted 16x16 real
-//      embedding on v_mfma_f32_16x16x4f32, each lane holding whole complex
-//      inputs and outputs;
+//   3  k = 2 (1-qubit blocks arrive padded to 2 qubits): 8x8 real embedding
+//      on v_mfma_f32_4x4x1f32, each lane computing one whole group;
+//   1  k = 3: permuted 16x16 real embedding on v_mfma_f32_16x16x4f32, each
+//      lane holding whole complex inputs and outputs;
 //   2  k = 4: 32x32 real embedding on v_mfma_f32_32x32x2f32 with the
 //      operand layout of k_fused_mfma.
 // 5-qubit blocks stay out of sweeps (ordinary launches): their operands would
@@ -433,6 +434,79 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
     for (int j = 0; j < U; ++j) {
       tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
       tile[bs[j] ^ o1] = make_float2(acc[j][2], acc[j][3]);
+    }
+  }
+}
+
+// k = 2 (1-qubit blocks arrive padded to 2) on v_mfma_f32_4x4x1f32: 16
+// blocks b of 4x4x1 per instruction (A lane 4b+i: A_b[i][0]; B lane 4b+j:
+// B_b[0][j]; D lane 4b+j, register i: D_b[i][j]). Every lane owns one group
+// of 4 amplitudes and feeds its 8 real inputs, one per K step, as column j of
+// block b = lane/4; its A operand at step kk is E[4h + lane%4][kk] of the 8x8
+// real embedding E (the same in every block), so D_b[i][j] accumulates output
+// row 4h + i of the lane's own group. Two accumulators (h = 0, 1) give the
+// lane its group's 4 outputs: 0.5 MFMA cycles per amplitude, half of a
+// 3-qubit block's. `e` holds the A fragments: e[16 i + 8 h + kk] = E[4h+i][kk].
+template <int L>
+__device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& b, const float* e) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float a[16];
+  const float4* e4 = (const float4*)(e + 16 * (lane & 3));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 v = e4[q];
+    a[4 * q] = v.x;
+    a[4 * q + 1] = v.y;
+    a[4 * q + 2] = v.z;
+    a[4 * q + 3] = v.w;
+  }
+  uint32_t srt[2];
+  uniform_bits<2>(b, srt);
+  const uint32_t lane_part = insert_zeros32(lane, srt, 2);
+  uint32_t off[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) off[j] = local_offset<2>(b, j) ^ lane_part;
+  // 64-group rows per wave, U of them per iteration with independent chains.
+  constexpr int NR = (1 << L) / 4 / 64;  // 64-group rows of the block
+  constexpr int PERW = NR / (TPB / 64);  // L >= 10: at least 1 per wave
+  constexpr int U = 1;
+  static_assert(NR % (TPB / 64) == 0 && PERW % U == 0, "64-group rows split evenly over the waves");
+  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t step = insert_zeros32(64 * (TPB / 64), srt, 2);
+  uint32_t base = insert_zeros32(w * 64, srt, 2);
+#pragma unroll
+  for (int it = 0; it < PERW; it += U) {
+    uint32_t bs[U];
+    float2 x[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bs[u] = base;
+      base = ((base | mask) + step) & ~mask;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x[u][j] = tile[bs[u] ^ off[j]];
+    }
+    floatx4 lo[U], hi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      lo[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+      hi[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float bv = (kk & 1) ? x[u][kk >> 1].y : x[u][kk >> 1].x;
+        lo[u] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[kk], bv, lo[u], 0, 0, 0);
+        hi[u] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[8 + kk], bv, hi[u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      tile[bs[u] ^ off[0]] = make_float2(lo[u][0], lo[u][1]);
+      tile[bs[u] ^ off[1]] = make_float2(lo[u][2], lo[u][3]);
+      tile[bs[u] ^ off[2]] = make_float2(hi[u][0], hi[u][1]);
+      tile[bs[u] ^ off[3]] = make_float2(hi[u][2], hi[u][3]);
     }
   }
 }
@@ -558,7 +632,8 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
       const TileBlock& b = blocks[bi];
       const float* m = (const float*)(meta + b.mat_off);
-      if (!K4 || b.kind == 1) tile_block_mfma16<L>(tile, b, m);
+      if (b.kind == 3) tile_block_mfma4<L>(tile, b, m);
+      else if (!K4 || b.kind == 1) tile_block_mfma16<L>(tile, b, m);
       else tile_block_mfma32<L>(tile, b, m);
       __syncthreads();
     }

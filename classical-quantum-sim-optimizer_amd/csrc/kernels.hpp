@@ -81,8 +81,8 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, co
 // descriptors, then the block matrices (16-byte aligned).
 struct TileBlock {
   uint32_t k;        // 1..5 qubits
-  uint32_t kind;     // 0: complex FP32 matrix (k <= 2); 1: permuted 16x16 embedding (k = 3);
-                     // 2: 2^(k+1) real embedding (k = 4, 5)
+  uint32_t kind;     // 3: 8x8 real embedding on 4x4x1 (k <= 2); 1: permuted 16x16
+                     // embedding (k = 3); 2: 2^(k+1) real embedding (k = 4, 5)
   uint32_t mat_off;  // byte offset of the matrix in the blob
   uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
   uint32_t srt[5];   // the same local bits sorted ascending
