@@ -81,8 +81,8 @@ def profiled_mfma_busy(kernel, workload):
 
 def mfma_flops_per_step(circ, precision, fusion, max_fused, tile):
     """FP32 matrix-core flops of one step from the launch plan: a k-qubit
-    block pass is 8 * 2^k * 2^n (one complex MAC = 8 real flops); blocks of
-    fewer than 3 qubits run padded to 3 in tile sweeps."""
+    block pass is 8 * 2^k * 2^n (one complex MAC = 8 real flops); 1-qubit
+    blocks run padded to 2 qubits in tile sweeps."""
     from dqsim import _core
 
     plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile)
@@ -91,7 +91,7 @@ def mfma_flops_per_step(circ, precision, fusion, max_fused, tile):
     for op in plan["launches"]:
         if op["kind"] == "tile" and not op.get("exact"):
             for b in op["blocks"]:
-                flops += 8.0 * (1 << max(3, len(b["qubits"]))) * float(1 << n)
+                flops += 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
         elif op["kind"] == "fused_mfma":
             flops += 8.0 * (1 << len(op["qubits"])) * float(1 << n)
     return flops
