@@ -1,5 +1,9 @@
 // Persistent batched-circuit kernel (complex64): thousands of small, uneven
-// circuits in one launch.
+// circuits in one launch. DISABLED: it stalled on its first GPU run and no
+// cause has been shown, so nothing launches it; the kernel stays for its
+// launch-limit probe (func_info, set_lds_limit) and the host-only program dump
+// that a CPU emulation of its logic runs on. Batched circuits run on the
+// stream-pool runner (engine.cpp, run_stream_pool).
 //
 // Each workgroup pulls circuits from an atomic work queue (host order:
 // largest estimated cost first) and runs a whole circuit by itself:
@@ -177,8 +181,6 @@ __global__ __launch_bounds__(BS) void k_batch(const BatchCircuit* __restrict__ c
   }
 }
 
-__global__ void k_fill_int(int* p, int v) { *p = v; }
-
 }  // namespace
 
 size_t lds_bytes(int lmax, int block) { return ((size_t)8 << lmax) + (size_t)8 * block; }
@@ -206,19 +208,6 @@ int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads
   *static_lds = (int)fa.sharedSizeBytes;
   *max_threads = fa.maxThreadsPerBlock;
   return 0;
-}
-
-void launch(const LaunchArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(1), 0, st, a.queue, 0);
-  const size_t shmem = lds_bytes(a.lmax, a.block);
-  dim3 grid(a.grid);
-#define L(B)                                                                                                   \
-  hipLaunchKernelGGL(k_batch<B>, grid, dim3(B), shmem, st, a.circ, a.ncirc, a.order, a.ops, a.mats, a.scratch, \
-                     a.slot_amps, a.lmax, a.queue, a.out, a.times, a.dump, a.dump_off)
-  if (a.block == 256) L(256);
-  else if (a.block == 512) L(512);
-  else L(1024);
-#undef L
 }
 
 }  // namespace bk

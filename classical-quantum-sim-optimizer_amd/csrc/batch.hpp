@@ -1,4 +1,4 @@
-// Device structures and launcher of the persistent batched-circuit kernel.
+// Device structures and launch-limit probes of the (disabled) persistent batched-circuit kernel.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -24,31 +24,11 @@ struct BatchOp {
   uint32_t s[5];     // the same qubits sorted ascending
 };
 
-struct LaunchArgs {
-  const BatchCircuit* circ;
-  uint32_t ncirc;
-  const uint32_t* order;
-  const BatchOp* ops;
-  const float* mats;
-  float2* scratch;
-  uint64_t slot_amps;
-  int lmax;
-  int* queue;
-  uint32_t* out;
-  uint64_t* times;
-  float2* dump;
-  const uint64_t* dump_off;
-  uint32_t grid;
-  int block;  // 256, 512 or 1024
-};
-
 size_t lds_bytes(int lmax, int block);
 // hipFuncSetAttribute(MaxDynamicSharedMemorySize); returns the hipError_t.
 int set_lds_limit(int block, size_t bytes);
 // hipFuncGetAttributes of the kernel instance for `block`; returns hipError_t.
 int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads);
-// Queue reset (a fill kernel, graph-capturable) + the persistent kernel.
-void launch(const LaunchArgs& a, hipStream_t st);
 
 }  // namespace bk
 }  // namespace dq

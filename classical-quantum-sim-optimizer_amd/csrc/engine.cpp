@@ -1948,57 +1948,11 @@ BatchProgram build_batch(const ParsedBatch& pb, const std::vector<uint64_t>& sho
   return prog;
 }
 
-struct BatchDevice {
-  DeviceMem circ, order, ops, mats, scratch, queue, out, times, dump, dump_off;
-  bk::LaunchArgs args{};
-};
-
 int device_attr(hipDeviceAttribute_t a) {
   int dev = 0, v = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&v, a, dev));
   return v;
-}
-
-void upload_batch(BatchDevice& d, const BatchProgram& prog, int block, int lmax, bool dump, hipStream_t st,
-                  const std::vector<uint64_t>* dump_off) {
-  auto up = [&](DeviceMem& m, const void* p, size_t bytes) {
-    m.alloc(bytes);
-    if (bytes) HIPCHK(hipMemcpyAsync(m.get(), p, bytes, hipMemcpyHostToDevice, st));
-  };
-  up(d.circ, prog.circ.data(), prog.circ.size() * sizeof(bk::BatchCircuit));
-  up(d.order, prog.order.data(), prog.order.size() * sizeof(uint32_t));
-  up(d.ops, prog.ops.data(), prog.ops.size() * sizeof(bk::BatchOp));
-  up(d.mats, prog.mats.data(), prog.mats.size() * sizeof(float));
-  const int cus = device_attr(hipDeviceAttributeMultiprocessorCount);
-  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(prog.circ.size(), (uint64_t)cus * 2));
-  uint64_t slot = 0;
-  for (auto& c : prog.circ)
-    if ((int)c.n > lmax) slot = std::max<uint64_t>(slot, 1ull << c.n);
-  d.scratch.alloc(std::max<uint64_t>(8, 8 * slot * grid));
-  d.queue.alloc(sizeof(int));
-  d.out.alloc(4 * std::max<uint64_t>(1, prog.total_shots));
-  d.times.alloc(16 * std::max<size_t>(1, prog.circ.size()));
-  if (dump) {
-    up(d.dump_off, dump_off->data(), dump_off->size() * sizeof(uint64_t));
-    d.dump.alloc(8 * std::max<uint64_t>(1, dump_off->back()));
-  }
-  bk::LaunchArgs& a = d.args;
-  a.circ = d.circ.as<bk::BatchCircuit>();
-  a.ncirc = (uint32_t)prog.circ.size();
-  a.order = d.order.as<uint32_t>();
-  a.ops = d.ops.as<bk::BatchOp>();
-  a.mats = d.mats.as<float>();
-  a.scratch = d.scratch.as<float2>();
-  a.slot_amps = slot;
-  a.lmax = lmax;
-  a.queue = d.queue.as<int>();
-  a.out = d.out.as<uint32_t>();
-  a.times = d.times.as<uint64_t>();
-  a.dump = dump ? d.dump.as<float2>() : nullptr;
-  a.dump_off = dump ? d.dump_off.as<uint64_t>() : nullptr;
-  a.grid = grid;
-  a.block = block;
 }
 
 }  // namespace
@@ -2013,37 +1967,6 @@ void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_
   *max_dynamic = *static_lds = *max_threads = -1;
   bk::func_info(block, max_dynamic, static_lds, max_threads);
 }
-
-namespace {
-int lmax_for(int block) {
-  const int lim = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
-  int l = 0;
-  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= lim) ++l;
-  return l;
-}
-
-// Time one launch of a captured graph (warm-up launch first).
-double time_graph(const BatchDevice& d, hipStream_t st, int reps, hipGraphExec_t* keep) {
-  hipGraph_t g;
-  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  bk::launch(d.args, st);
-  HIPCHK(hipStreamEndCapture(st, &g));
-  hipGraphExec_t ge;
-  HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  HIPCHK(hipGraphDestroy(g));
-  HIPCHK(hipGraphLaunch(ge, st));
-  Event a, b;
-  HIPCHK(hipEventRecord(a.e, st));
-  for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge, st));
-  HIPCHK(hipEventRecord(b.e, st));
-  HIPCHK(hipStreamSynchronize(st));
-  double ms = elapsed_ms(a, b) / std::max(1, reps);
-  if (keep) *keep = ge;
-  else HIPCHK(hipGraphExecDestroy(ge));
-  return ms;
-}
-
-}  // namespace
 
 namespace {
 ParsedBatch parse_batch(const std::vector<std::pair<const char*, size_t>>& circuits,
