@@ -997,6 +997,51 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
       chains.back().cbits = cb;
     }
   }
+  // A diagonal inside a chain gets its multiplier per chain register
+  // (register j holds chain-bit pattern j): bit j of q[0] says whether
+  // amplitude j is multiplied, and the (re, im) pairs follow the op's data at
+  // byte q[1]. The kernel then multiplies without matching patterns.
+  for (const k::TileChain& ch : chains) {
+    if (ch.nq == 0) continue;
+    uint32_t c[3] = {ch.cbits & 0xFF, (ch.cbits >> 8) & 0xFF, (ch.cbits >> 16) & 0xFF};
+    auto cidx = [&](uint32_t m) {
+      uint32_t r = 0;
+      for (uint32_t b = 0; b < ch.nq; ++b)
+        if ((m >> c[b]) & 1) r |= 1u << b;
+      return r;
+    };
+    for (uint32_t o = ch.first; o < ch.first + ch.count; ++o) {
+      k::TileOp& t = tops[o];
+      if (t.kind != 2) {
+        // Pair update or swap: bit p of q[0] marks the register pairs it
+        // applies to, pairs (0,1) (2,3) (0,2) (1,3) of the chain registers.
+        static const uint32_t pa[4] = {0, 2, 0, 1}, pb[4] = {1, 3, 2, 3};
+        const uint32_t tj = cidx(t.bbit), cm = cidx(t.qmask & ~t.bbit), cv = cidx(t.fixed);
+        uint32_t pm = 0;
+        for (uint32_t pp = 0; pp < 4; ++pp)
+          if (pb[pp] < (1u << ch.nq) && pb[pp] == (pa[pp] | tj) && (pa[pp] & tj) == 0 && (pa[pp] & cm) == cv)
+            pm |= 1u << pp;
+        t.q[0] = pm;
+        continue;
+      }
+      const uint32_t* pats = (const uint32_t*)datas[o].data();
+      const double* vals = (const double*)(datas[o].data() + 32 * sizeof(uint32_t));
+      const uint32_t qm = cidx(t.qmask);
+      double dl[8] = {};
+      uint32_t mask = 0;
+      for (uint32_t jj = 0; jj < (1u << ch.nq); ++jj)
+        for (uint32_t e = 0; e < t.k; ++e)
+          if (cidx(pats[e]) == (jj & qm)) {
+            mask |= 1u << jj;
+            dl[2 * jj] = vals[2 * e];
+            dl[2 * jj + 1] = vals[2 * e + 1];
+          }
+      t.q[0] = mask;
+      t.q[1] = (uint32_t)datas[o].size();
+      const uint8_t* b = (const uint8_t*)dl;
+      datas[o].insert(datas[o].end(), b, b + sizeof(dl));
+    }
+  }
   const size_t chead = chains.size() * sizeof(k::TileChain);
   const size_t head = chead + tops.size() * sizeof(k::TileOp);
   size_t off = head;

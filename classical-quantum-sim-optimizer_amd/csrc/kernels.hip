@@ -656,14 +656,19 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 // ---------------------------------------------------------------------------
 // Group index p -> local index: zeros inserted at the op's sorted positions
 // (held in registers; loops fully unrolled so nothing is indexed at run time).
+// Op descriptors sit in LDS and are the same for every lane: reading a field
+// through readfirstlane keeps it in a scalar register, so decoding it runs on
+// the scalar unit instead of costing vector instructions per lane and group.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 struct TileGroup {
   uint32_t low[8];  // (1 << pos[i]) - 1 for i < nb
   uint32_t nb, fixed;
   __device__ __forceinline__ explicit TileGroup(const TileOp& op) {
-    nb = __builtin_amdgcn_readfirstlane(op.nbits);
+    nb = uni(op.nbits);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) low[i] = (uint32_t)i < nb ? (1u << op.pos[i]) - 1 : 0xFFFFFFFFu;
-    fixed = op.fixed;
+    for (int i = 0; i < 8; ++i) low[i] = (uint32_t)i < nb ? (1u << uni(op.pos[i])) - 1 : 0xFFFFFFFFu;
+    fixed = uni(op.fixed);
   }
   __device__ __forceinline__ uint32_t operator()(uint32_t p) const {
 #pragma unroll
@@ -675,38 +680,40 @@ struct TileGroup {
 
 template <int L>
 __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, const unsigned char* meta) {
-  const uint32_t count = (1u << L) >> op.nbits;
-  const double* d = (const double*)(meta + op.data_off);
+  const uint32_t count = (1u << L) >> uni(op.nbits);
+  const double* d = (const double*)(meta + uni(op.data_off));
   const TileGroup tile_group(op);
-  if (op.kind == 0) {
+  const uint32_t kind = uni(op.kind), abit = uni(op.abit), bbit = uni(op.bbit);
+  if (kind == 0) {
     double u[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) u[i] = d[i];
     for (uint32_t p = threadIdx.x; p < count; p += TPB) {
       const uint32_t i = tile_group(p);
-      const double2 x = tile[i], y = tile[i | op.bbit];
+      const double2 x = tile[i], y = tile[i | bbit];
       double2 na, nc;
       na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
       na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
       nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
       nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
       tile[i] = na;
-      tile[i | op.bbit] = nc;
+      tile[i | bbit] = nc;
     }
-  } else if (op.kind == 1) {
+  } else if (kind == 1) {
     for (uint32_t p = threadIdx.x; p < count; p += TPB) {
       const uint32_t i = tile_group(p);
-      const double2 a = tile[i | op.abit], c = tile[i | op.bbit];
-      tile[i | op.abit] = c;
-      tile[i | op.bbit] = a;
+      const double2 a = tile[i | abit], c = tile[i | bbit];
+      tile[i | abit] = c;
+      tile[i | bbit] = a;
     }
-  } else if (op.kind == 2) {
+  } else if (kind == 2) {
     const uint32_t* pat = (const uint32_t*)d;
     const double* val = d + 16;  // patterns occupy the first 128 bytes
+    const uint32_t nk = uni(op.k);
     for (uint32_t p = threadIdx.x; p < count; p += TPB) {
       const uint32_t i0 = tile_group(p);
-      for (uint32_t t = 0; t < op.k; ++t) {
-        const uint32_t i = i0 | pat[t];
+      for (uint32_t t = 0; t < nk; ++t) {
+        const uint32_t i = i0 | uni(pat[t]);
         const double2 a = tile[i];
         const double dr = val[2 * t], di = val[2 * t + 1];
         double2 r;
@@ -717,10 +724,10 @@ __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, cons
     }
   } else {
     // Dense, k <= 3 in sweeps.
-    const uint32_t kk = op.k, D = 1u << kk;
+    const uint32_t kk = uni(op.k), D = 1u << kk;
     uint32_t qb[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) qb[q] = (uint32_t)q < kk ? 1u << op.q[q] : 0u;
+    for (int q = 0; q < 3; ++q) qb[q] = (uint32_t)q < kk ? 1u << uni(op.q[q]) : 0u;
     uint32_t off[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -757,59 +764,27 @@ __device__ __forceinline__ void tile128_op(double2* tile, const TileOp& op, cons
 }
 
 // Register chains: consecutive 1-qubit, X and diagonal ops on <= 2 qubits in
-// total (the host also supports 3 in this code; it issues at most 2),
-// applied by one thread to the 2^NQ amplitudes of its group held in
-// registers, op after op with each op's own expression (register j holds
-// local index base | combo(j), bit i of j <-> chain qubit c[i]). Every
-// amplitude goes through the same arithmetic as with one LDS pass per op;
-// pair and pattern indices are resolved at compile time per target bit.
-__device__ __forceinline__ uint32_t chain_index(uint32_t m, const uint32_t* c, int nq) {
-  uint32_t j = 0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (i < nq && ((m >> c[i]) & 1)) j |= 1u << i;
-  return j;
-}
-
-template <int NQ, int TJ>
-__device__ __forceinline__ void chain_one(double2* v, uint32_t cm, uint32_t cv, const double* u) {
-  constexpr int D = 1 << NQ;
-#pragma unroll
-  for (int j0 = 0; j0 < D; ++j0) {
-    if (j0 & TJ) continue;
-    if (((uint32_t)j0 & cm) != cv) continue;  // wave-uniform
-    const double2 x = v[j0], y = v[j0 | TJ];
-    double2 na, nc;
-    na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
-    na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
-    nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
-    nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
-    v[j0] = na;
-    v[j0 | TJ] = nc;
-  }
-}
-
-template <int NQ, int TJ>
-__device__ __forceinline__ void chain_x(double2* v, uint32_t cm, uint32_t cv) {
-  constexpr int D = 1 << NQ;
-#pragma unroll
-  for (int j0 = 0; j0 < D; ++j0) {
-    if (j0 & TJ) continue;
-    if (((uint32_t)j0 & cm) != cv) continue;
-    const double2 a = v[j0];
-    v[j0] = v[j0 | TJ];
-    v[j0 | TJ] = a;
-  }
-}
+// total, applied by one thread to the 2^NQ amplitudes of its group held in
+// registers (register j holds local index base | combo(j), bit i of j <->
+// chain qubit c[i]), op after op with each op's own expression, so every
+// amplitude goes through the same arithmetic as with one LDS pass per op.
+// The host resolves each op to the register pairs (or, for a diagonal, the
+// registers and multipliers) it touches, so the thread runs straight-line
+// updates selected by wave-uniform mask bits.
+// Register pairs of a chain: bit p of an op's pair mask selects pair
+// (kPairA[p], kPairB[p]) of the 2^NQ chain registers.
+__device__ constexpr int kPairA[4] = {0, 2, 0, 1};
+__device__ constexpr int kPairB[4] = {1, 3, 2, 3};
 
 template <int L, int NQ>
 __device__ __forceinline__ void tile128_chain(double2* tile, const TileChain& ch, const TileOp* ops,
                                               const unsigned char* meta) {
   constexpr int D = 1 << NQ;
   uint32_t c[3], low[3];
+  const uint32_t cbits = uni(ch.cbits), first = uni(ch.first), count = uni(ch.count);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    c[i] = (ch.cbits >> (8 * i)) & 0xFF;
+    c[i] = (cbits >> (8 * i)) & 0xFF;
     low[i] = (1u << c[i]) - 1;
   }
   uint32_t off[D];
@@ -829,49 +804,58 @@ __device__ __forceinline__ void tile128_chain(double2* tile, const TileChain& ch
     double2 v[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) v[j] = tile[base | off[j]];
-    for (uint32_t o = 0; o < ch.count; ++o) {
-      const TileOp& op = ops[ch.first + o];
-      const double* d = (const double*)(meta + op.data_off);
-      if (op.kind == 2) {
-        const uint32_t* pat = (const uint32_t*)d;
-        const double* val = d + 16;
-        const uint32_t qm = chain_index(op.qmask, c, NQ);
-        uint32_t key[8];
+    for (uint32_t o = 0; o < count; ++o) {
+      const TileOp& op = ops[first + o];
+      const double* d = (const double*)(meta + uni(op.data_off));
+      const uint32_t kind = uni(op.kind);
+      if (kind == 2) {
+        // Per-register multipliers resolved on the host: bit j of q[0] marks
+        // the registers a diagonal entry applies to, values at byte q[1].
+        const uint32_t m = uni(op.q[0]);
+        const double* dl = d + uni(op.q[1]) / sizeof(double);
+        double dv[2 * D];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) key[t] = (uint32_t)t < op.k ? chain_index(pat[t], c, NQ) : 0xFFFFFFFFu;
+        for (int i = 0; i < 2 * D; ++i) dv[i] = dl[i];
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-          const uint32_t kj = (uint32_t)j & qm;
-#pragma unroll
-          for (int t = 0; t < 8; ++t)
-            if (key[t] == kj) {
-              const double2 a = v[j];
-              const double dr = val[2 * t], di = val[2 * t + 1];
-              double2 r;
-              r.x = a.x * dr - a.y * di;
-              r.y = a.x * di + a.y * dr;
-              v[j] = r;
-            }
-        }
+        for (int j = 0; j < D; ++j)
+          if ((m >> j) & 1) {
+            const double2 a = v[j];
+            const double dr = dv[2 * j], di = dv[2 * j + 1];
+            double2 r;
+            r.x = a.x * dr - a.y * di;
+            r.y = a.x * di + a.y * dr;
+            v[j] = r;
+          }
         continue;
       }
-      const uint32_t tj = chain_index(op.bbit, c, NQ);
-      const uint32_t cm = chain_index(op.qmask & ~op.bbit, c, NQ), cv = chain_index(op.fixed, c, NQ);
-      if (op.kind == 0) {
+      // Pair update (kind 0) or swap (kind 1) on the pairs the host selected
+      // (target bit and control values resolved per register pair).
+      const uint32_t pm = uni(op.q[0]);
+      constexpr int NP = NQ == 1 ? 1 : 4;
+      if (kind == 0) {
         double u[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) u[i] = d[i];
-        if (tj == 1) chain_one<NQ, 1>(v, cm, cv, u);
-        if constexpr (NQ >= 2)
-          if (tj == 2) chain_one<NQ, 2>(v, cm, cv, u);
-        if constexpr (NQ >= 3)
-          if (tj == 4) chain_one<NQ, 4>(v, cm, cv, u);
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp)
+          if ((pm >> pp) & 1) {
+            const double2 x = v[kPairA[pp]], y = v[kPairB[pp]];
+            double2 na, nc;
+            na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
+            na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
+            nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
+            nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
+            v[kPairA[pp]] = na;
+            v[kPairB[pp]] = nc;
+          }
       } else {
-        if (tj == 1) chain_x<NQ, 1>(v, cm, cv);
-        if constexpr (NQ >= 2)
-          if (tj == 2) chain_x<NQ, 2>(v, cm, cv);
-        if constexpr (NQ >= 3)
-          if (tj == 4) chain_x<NQ, 4>(v, cm, cv);
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp)
+          if ((pm >> pp) & 1) {
+            const double2 t = v[kPairA[pp]];
+            v[kPairA[pp]] = v[kPairB[pp]];
+            v[kPairB[pp]] = t;
+          }
       }
     }
 #pragma unroll

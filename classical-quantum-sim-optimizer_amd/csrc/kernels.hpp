@@ -108,7 +108,11 @@ struct TileOp {
   uint32_t data_off;  // byte offset of the op's data in the blob
   uint32_t qmask;     // local bits of all the op's qubits (targets and controls)
   uint32_t pos[8];    // sorted local positions of the op's qubits (targets and controls)
-  uint32_t q[8];      // dense: local bit of each block qubit, q[0] = MSB of the block index
+  uint32_t q[8];      // dense: local bit of each block qubit, q[0] = MSB of the block index;
+                      // in a register chain: pair update or swap: q[0] = register pairs it
+                      // applies to (bit p: pair (0,1) (2,3) (0,2) (1,3)); diagonal: q[0] =
+                      // registers it multiplies (bit j), q[1] = byte offset of their
+                      // (re, im) pairs in the op's data
 };
 static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
 // A chain of consecutive ops applied by one thread to the 2^nq amplitudes of
