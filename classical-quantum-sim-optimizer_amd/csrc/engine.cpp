@@ -299,6 +299,7 @@ struct DevOp {
   std::vector<int> tile_mfma;
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
   int chains = 0;      // exact sweeps: register chains / single-op passes
+  int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -1042,6 +1043,38 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
       datas[o].insert(datas[o].end(), b, b + sizeof(dl));
     }
   }
+  // Chains whose op sequence is one of the kernel's compiled signatures
+  // (two qubits, k::kChainSigs: updates on either register bit, a controlled
+  // swap of registers 1,3 or 2,3, a multiplier on register 3; or one qubit
+  // and a single update) run code with every register index fixed at compile
+  // time; the signature number goes in bits 8.. of nq.
+  int fixed_chains = 0;
+  for (k::TileChain& ch : chains) {
+    if (ch.nq == 1 && ch.count == 1 && tops[ch.first].kind == 0 && tops[ch.first].q[0] == 0x1) {
+      ch.nq = 1u | (1u << 8);
+      ++fixed_chains;
+      continue;
+    }
+    if (ch.nq != 2 || ch.count > 3) continue;
+    int code[3] = {0, 0, 0};
+    bool ok = true;
+    for (uint32_t o = 0; o < ch.count && ok; ++o) {
+      const k::TileOp& t = tops[ch.first + o];
+      if (t.kind == 0 && t.q[0] == 0x3) code[o] = k::CH_U1;
+      else if (t.kind == 0 && t.q[0] == 0xC) code[o] = k::CH_U2;
+      else if (t.kind == 1 && t.q[0] == 0x8) code[o] = k::CH_S13;
+      else if (t.kind == 1 && t.q[0] == 0x2) code[o] = k::CH_S23;
+      else if (t.kind == 2 && t.q[0] == 0x8) code[o] = k::CH_D3;
+      else ok = false;
+    }
+    if (!ok) continue;
+    for (uint32_t sgn = 1; sgn < k::kChainSigCount; ++sgn)
+      if (k::kChainSigs[sgn][0] == code[0] && k::kChainSigs[sgn][1] == code[1] && k::kChainSigs[sgn][2] == code[2]) {
+        ch.nq = 2u | (sgn << 8);
+        ++fixed_chains;
+        break;
+      }
+  }
   const size_t chead = chains.size() * sizeof(k::TileChain);
   const size_t head = chead + tops.size() * sizeof(k::TileOp);
   size_t off = head;
@@ -1060,6 +1093,7 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
   d.targs.L = (uint32_t)L;
   d.targs.nblocks = (uint32_t)chains.size();
   d.chains = (int)chains.size();
+  d.fixed_chains = fixed_chains;
   d.targs.blob16 = (uint32_t)(blob.size() / 16);
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
@@ -1918,7 +1952,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
     for (size_t t = 0; t < d.src.size(); ++t) js << (t ? ", " : "") << d.src[t];
     js << "]";
     if (d.kind == D_TILE) {
-      js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"chains\": " << d.chains;
+      js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"chains\": " << d.chains
+         << ", \"fixed_chains\": " << d.fixed_chains;
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";

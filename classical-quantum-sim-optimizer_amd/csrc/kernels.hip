@@ -863,6 +863,109 @@ __device__ __forceinline__ void tile128_chain(double2* tile, const TileChain& ch
   }
 }
 
+// One op of a compiled two-qubit chain signature: every register index is
+// fixed at compile time (a swap is a renaming), the arithmetic is the generic
+// chain's expression for the same op.
+template <int OP>
+__device__ __forceinline__ void chain_fixed_op(double2* v, const TileOp& op, const unsigned char* meta) {
+  if constexpr (OP == CH_U1 || OP == CH_U2) {
+    constexpr int P0A = OP == CH_U1 ? 0 : 0, P0B = OP == CH_U1 ? 1 : 2;
+    constexpr int P1A = OP == CH_U1 ? 2 : 1, P1B = 3;
+    const double* d = (const double*)(meta + uni(op.data_off));
+    double u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u[i] = d[i];
+    auto upd = [&](int a, int b) {
+      const double2 x = v[a], y = v[b];
+      double2 na, nc;
+      na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
+      na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
+      nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
+      nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
+      v[a] = na;
+      v[b] = nc;
+    };
+    upd(P0A, P0B);
+    upd(P1A, P1B);
+  } else if constexpr (OP == CH_S13 || OP == CH_S23) {
+    constexpr int A = OP == CH_S13 ? 1 : 2;
+    const double2 t = v[A];
+    v[A] = v[3];
+    v[3] = t;
+  } else if constexpr (OP == CH_D3) {
+    const double* dl = (const double*)(meta + uni(op.data_off) + uni(op.q[1]));
+    const double dr = dl[6], di = dl[7];
+    const double2 a = v[3];
+    double2 r;
+    r.x = a.x * dr - a.y * di;
+    r.y = a.x * di + a.y * dr;
+    v[3] = r;
+  }
+}
+
+template <int L, int O0, int O1, int O2>
+__device__ __forceinline__ void tile128_chain_fixed(double2* tile, const TileChain& ch, const TileOp* ops,
+                                                    const unsigned char* meta) {
+  const uint32_t cbits = uni(ch.cbits), first = uni(ch.first);
+  const uint32_t c0 = cbits & 0xFF, c1 = (cbits >> 8) & 0xFF;
+  const uint32_t low0 = (1u << c0) - 1, low1 = (1u << c1) - 1;
+  const uint32_t off[4] = {0u, 1u << c0, 1u << c1, (1u << c0) | (1u << c1)};
+  constexpr uint32_t groups = (1u << L) >> 2;
+  for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
+    uint32_t base = g;
+    base = (base & low0) | ((base & ~low0) << 1);
+    base = (base & low1) | ((base & ~low1) << 1);
+    double2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = tile[base | off[j]];
+    chain_fixed_op<O0>(v, ops[first], meta);
+    if constexpr (O1 != CH_NONE) chain_fixed_op<O1>(v, ops[first + 1], meta);
+    if constexpr (O2 != CH_NONE) chain_fixed_op<O2>(v, ops[first + 2], meta);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tile[base | off[j]] = v[j];
+  }
+}
+
+// One-qubit chain with one update: register pair (0,1) of each group.
+template <int L>
+__device__ __forceinline__ void tile128_chain1_fixed(double2* tile, const TileChain& ch, const TileOp* ops,
+                                                     const unsigned char* meta) {
+  const uint32_t c0 = uni(ch.cbits) & 0xFF, first = uni(ch.first);
+  const uint32_t low0 = (1u << c0) - 1, off1 = 1u << c0;
+  const TileOp& op = ops[first];
+  const double* d = (const double*)(meta + uni(op.data_off));
+  constexpr uint32_t groups = (1u << L) >> 1;
+  for (uint32_t g = threadIdx.x; g < groups; g += TPB) {
+    const uint32_t base = (g & low0) | ((g & ~low0) << 1);
+    double u[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) u[i] = d[i];
+    const double2 x = tile[base], y = tile[base | off1];
+    double2 na, nc;
+    na.x = (u[0] * x.x - u[1] * x.y) + (u[2] * y.x - u[3] * y.y);
+    na.y = (u[0] * x.y + u[1] * x.x) + (u[2] * y.y + u[3] * y.x);
+    nc.x = (u[4] * x.x - u[5] * x.y) + (u[6] * y.x - u[7] * y.y);
+    nc.y = (u[4] * x.y + u[5] * x.x) + (u[6] * y.y + u[7] * y.x);
+    tile[base] = na;
+    tile[base | off1] = nc;
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void tile128_chain_sig(double2* tile, const TileChain& ch, uint32_t sig,
+                                                  const TileOp* ops, const unsigned char* meta) {
+#define SIG(N)                                                                                           \
+  case N:                                                                                                \
+    tile128_chain_fixed<L, kChainSigs[N][0], kChainSigs[N][1], kChainSigs[N][2]>(tile, ch, ops, meta); \
+    break
+  switch (sig) {
+    SIG(1); SIG(2); SIG(3); SIG(4); SIG(5); SIG(6); SIG(7); SIG(8); SIG(9); SIG(10);
+    default: break;
+  }
+#undef SIG
+}
+static_assert(kChainSigCount == 11, "one case per compiled chain signature");
+
 template <int L>
 __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,
                                                  TileArgs a, const float4* __restrict__ blob) {
@@ -929,11 +1032,12 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
     }
     for (uint32_t ci = 0; ci < a.nblocks; ++ci) {
       const TileChain& ch = chains[ci];
-      switch (ch.nq) {
-        case 0: tile128_op<L>(tile, ops[ch.first], meta); break;
-        case 1: tile128_chain<L, 1>(tile, ch, ops, meta); break;
-        default: tile128_chain<L, 2>(tile, ch, ops, meta); break;
-      }
+      const uint32_t nq = uni(ch.nq);
+      if ((nq >> 8) && (nq & 0xFF) == 1) tile128_chain1_fixed<L>(tile, ch, ops, meta);
+      else if (nq >> 8) tile128_chain_sig<L>(tile, ch, nq >> 8, ops, meta);
+      else if (nq == 0) tile128_op<L>(tile, ops[uni(ch.first)], meta);
+      else if (nq == 1) tile128_chain<L, 1>(tile, ch, ops, meta);
+      else tile128_chain<L, 2>(tile, ch, ops, meta);
       __syncthreads();
     }
 #pragma unroll

@@ -118,8 +118,20 @@ static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
 // A chain of consecutive ops applied by one thread to the 2^nq amplitudes of
 // its group held in registers (nq = 1..2 chain qubits), or one op applied
 // through LDS (nq = 0). Blob: nchains TileChain, then the TileOps, then data.
+// Register-chain op letters of the compiled two-qubit chain signatures:
+// update on register bit 0 (pairs (0,1),(2,3)) or bit 1 (pairs (0,2),(1,3)),
+// swap of registers 1,3 or 2,3, multiplier on register 3.
+enum ChainOp { CH_NONE = 0, CH_U1 = 1, CH_U2 = 2, CH_S13 = 3, CH_S23 = 4, CH_D3 = 5 };
+constexpr uint32_t kChainSigCount = 11;
+constexpr int kChainSigs[kChainSigCount][3] = {
+    {0, 0, 0},                      // 0: not compiled
+    {CH_U1, CH_U2, CH_S13}, {CH_U1, CH_U2, CH_S23}, {CH_U2, CH_U1, CH_S13}, {CH_U2, CH_U1, CH_S23},
+    {CH_U1, CH_U2, CH_D3},  {CH_U2, CH_U1, CH_D3},  {CH_U1, CH_D3, 0},      {CH_U1, CH_S23, 0},
+    {CH_U2, CH_D3, 0},      {CH_U2, CH_S13, 0}};
+// A one-qubit chain holding a single update is compiled too (signature 1 in
+// bits 8.. of nq = 1).
 struct TileChain {
-  uint32_t nq;     // chain qubits (0: single op through LDS)
+  uint32_t nq;     // chain qubits (0: single op through LDS); bits 8..: compiled signature
   uint32_t first;  // index of the chain's first op
   uint32_t count;  // ops in the chain
   uint32_t cbits;  // local bits of the chain qubits, ascending, 8 bits each
