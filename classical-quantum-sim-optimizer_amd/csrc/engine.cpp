@@ -862,7 +862,7 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
 
 // Exact complex128 sweeps: maximal runs of consecutive compiled ops (program
 // order, nothing reordered) whose qubits fit in L tile qubits together with
-// qubits 0..3; each op keeps its per-op kernel's arithmetic, so results are
+// qubits 0..2; each op keeps its per-op kernel's arithmetic, so results are
 // bit-identical to one launch per op. Dense ops wider than 3 qubits and
 // diagonals with more than 32 entries run as their own launches.
 bool exact_tileable(const Op& op) {
@@ -1120,7 +1120,10 @@ void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, siz
       plain(i++);
       continue;
     }
-    uint64_t smask = 0xFull | qubit_mask(op_qubits(ops[i]));
+    // Qubits 0..2 are always tile qubits: 8 lanes move one contiguous 128 B
+    // run. Fixing 0..3 (256 B runs) needed 171 sweeps instead of 150 at 32
+    // qubits and measured 3.6 % slower.
+    uint64_t smask = 0x7ull | qubit_mask(op_qubits(ops[i]));
     size_t blob = exact_op_bytes(ops[i]);
     size_t j = i + 1;
     while (j < end && j - i < kOpsMax && exact_tileable(ops[j])) {

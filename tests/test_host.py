@@ -434,7 +434,7 @@ def test_tile_sweeps_off_and_limits():
 @pytest.mark.parametrize("kind", ["su2", "mixed", "qft"])
 def test_exact_sweeps_keep_program_order(kind):
     # complex128 without fusion: sweeps are runs of consecutive compiled ops
-    # (nothing reordered), each within L tile qubits that include 0..3.
+    # (nothing reordered), each within L tile qubits that include 0..2.
     n = 18
     if kind == "su2":
         c = workloads.random_su2(n=n, depth=6, seed=4)
@@ -450,7 +450,7 @@ def test_exact_sweeps_keep_program_order(kind):
         assert tiles and len(plan["launches"]) < plan["compiled_ops"]
         for t in tiles:
             s = set(t["qubits"])
-            assert len(s) == min(L, 11) and {0, 1, 2, 3} <= s
+            assert len(s) == min(L, 11) and {0, 1, 2} <= s
             assert all(set(b["qubits"]) <= s for b in t["blocks"])
     assert not any(x["kind"] == "tile" for x in _core.plan_json(c, tile_qubits=0)["launches"])
 
