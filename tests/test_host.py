@@ -455,6 +455,20 @@ def test_exact_sweeps_keep_program_order(kind):
     assert not any(x["kind"] == "tile" for x in _core.plan_json(c, tile_qubits=0)["launches"])
 
 
+def test_exact_sweep_chains_use_compiled_signatures():
+    # Compiled ops of the benchmark family come as One(a), One(b), CX/CZ(a, b)
+    # triples: nearly every register chain matches a compiled signature, while
+    # circuits with controlled and multi-qubit ops keep the generic chains.
+    c = workloads.random_su2(n=24, depth=10, seed=5)
+    tiles = [x for x in _core.plan_json(c)["launches"] if x["kind"] == "tile"]
+    chains = sum(t["chains"] for t in tiles)
+    fixed = sum(t["fixed_chains"] for t in tiles)
+    assert chains > 0 and fixed >= 0.95 * chains
+    m = _mixed_tile_circuit(18, 5, seed=9)
+    tiles = [x for x in _core.plan_json(m)["launches"] if x["kind"] == "tile"]
+    assert sum(t["fixed_chains"] for t in tiles) < sum(t["chains"] for t in tiles)
+
+
 def test_learned_plan_selection_follows_the_model():
     # complex64 tile plans under fusion="learned" are chosen among (block
     # width, tile size) candidates by the model's predicted total time.
