@@ -108,7 +108,7 @@ def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, lo
     return out
 
 
-def planner_report(train_circuits: int = 240, train_n=(24, 30), eval_n=(28, 30), seed: int = 5, epochs: int = 300,
+def planner_report(train_circuits: int = 1200, train_n=(24, 30), eval_n=(28, 30), seed: int = 5, epochs: int = 300,
                    log=print) -> dict:
     """Train the planner MLP on engine-recorded launch timings (per-block and
     tile plans of four circuit families), then on held-out circuits compare
