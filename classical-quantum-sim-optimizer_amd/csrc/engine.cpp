@@ -1045,9 +1045,10 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
   }
   // Chains whose op sequence is one of the kernel's compiled signatures
   // (two qubits, k::kChainSigs: updates on either register bit, a controlled
-  // swap of registers 1,3 or 2,3, a multiplier on register 3; or one qubit
-  // and a single update) run code with every register index fixed at compile
-  // time; the signature number goes in bits 8.. of nq.
+  // swap of registers 1,3 or 2,3, a multiplier on register 3 such as a
+  // controlled phase; or one qubit and a single update) run code with every
+  // register index fixed at compile time; the signature number goes in bits
+  // 8.. of nq.
   int fixed_chains = 0;
   for (k::TileChain& ch : chains) {
     if (ch.nq == 1 && ch.count == 1 && tops[ch.first].kind == 0 && tops[ch.first].q[0] == 0x1) {

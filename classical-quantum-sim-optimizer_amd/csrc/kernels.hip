@@ -959,12 +959,12 @@ __device__ __forceinline__ void tile128_chain_sig(double2* tile, const TileChain
     tile128_chain_fixed<L, kChainSigs[N][0], kChainSigs[N][1], kChainSigs[N][2]>(tile, ch, ops, meta); \
     break
   switch (sig) {
-    SIG(1); SIG(2); SIG(3); SIG(4); SIG(5); SIG(6); SIG(7); SIG(8); SIG(9); SIG(10);
+    SIG(1); SIG(2); SIG(3); SIG(4); SIG(5); SIG(6); SIG(7); SIG(8); SIG(9); SIG(10); SIG(11);
     default: break;
   }
 #undef SIG
 }
-static_assert(kChainSigCount == 11, "one case per compiled chain signature");
+static_assert(kChainSigCount == 12, "one case per compiled chain signature");
 
 template <int L>
 __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uint64_t stride, uint64_t ntiles,

@@ -122,12 +122,12 @@ static_assert(sizeof(TileOp) == 96, "TileOp is 96 bytes");
 // update on register bit 0 (pairs (0,1),(2,3)) or bit 1 (pairs (0,2),(1,3)),
 // swap of registers 1,3 or 2,3, multiplier on register 3.
 enum ChainOp { CH_NONE = 0, CH_U1 = 1, CH_U2 = 2, CH_S13 = 3, CH_S23 = 4, CH_D3 = 5 };
-constexpr uint32_t kChainSigCount = 11;
+constexpr uint32_t kChainSigCount = 12;
 constexpr int kChainSigs[kChainSigCount][3] = {
     {0, 0, 0},                      // 0: not compiled
     {CH_U1, CH_U2, CH_S13}, {CH_U1, CH_U2, CH_S23}, {CH_U2, CH_U1, CH_S13}, {CH_U2, CH_U1, CH_S23},
     {CH_U1, CH_U2, CH_D3},  {CH_U2, CH_U1, CH_D3},  {CH_U1, CH_D3, 0},      {CH_U1, CH_S23, 0},
-    {CH_U2, CH_D3, 0},      {CH_U2, CH_S13, 0}};
+    {CH_U2, CH_D3, 0},      {CH_U2, CH_S13, 0},     {CH_D3, 0, 0}};
 // A one-qubit chain holding a single update is compiled too (signature 1 in
 // bits 8.. of nq = 1).
 struct TileChain {
