@@ -2320,6 +2320,11 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
       out->state_off[i] = total_amps;
       total_amps += 1ull << pb.p[i].n;
     }
+    // Kept states are complex128 host copies; refuse what would not fit.
+    constexpr uint64_t kKeepMax = 64ull << 30;
+    if (total_amps > kKeepMax / 16)
+      throw MemErr("keeping the final states needs " + std::to_string(16 * total_amps >> 20) +
+                   " MiB of host memory, above the 65536 MiB limit");
     out->states.assign(2 * total_amps, 0.0);
   }
   // Largest estimated work first (2^n amplitudes times compiled ops), so the
