@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--precision", default="complex64", choices=["complex64", "complex128"],
                     help="precision of the timed evolution (complex128 with --fusion none times the exact sweeps)")
     ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
+    ap.add_argument("--relabel", action="store_true",
+                    help="relabelled tile sweeps (measured slower here; default: qubits 0..3 pinned in every tile)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=26)
@@ -210,6 +212,7 @@ def main():
     import dqsim
     from dqsim import workloads
 
+    dqsim._core.set_sweep_relabel(args.relabel)
     copy_gbs = device_copy_gbs()
     log(f"[bench] device copy (4 GiB, torch copy_): {copy_gbs:.0f} GB/s read+write")
     mfma_peak_tflops, _ = dqsim._core.mfma_fp32_peak(3)
@@ -221,7 +224,7 @@ def main():
     fused = dqsim.StatevectorSimulator(seed=1, precision=args.precision, fusion=args.fusion,
                                        max_fused_qubits=args.max_fused, max_memory_mb=mem_mb, tile_qubits=args.tile)
     log(f"[bench] n={n} depth={depth} gates={gates} {args.precision} fusion={args.fusion} k<={args.max_fused} "
-        f"tile={args.tile}")
+        f"tile={args.tile} relabel={args.relabel}")
     for w in range(args.warmup):
         t = time.perf_counter()
         fused.evolve_timed(circ, repeats=1, record=False)
@@ -323,7 +326,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
-                   "parallelism": "single GPU"},
+                   "relabelled_sweeps": args.relabel, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "frac_of_device_copy": achieved / copy_gbs,

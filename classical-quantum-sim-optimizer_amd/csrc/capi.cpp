@@ -234,6 +234,8 @@ void dq_string_free(char* s) { std::free(s); }
 
 int dq_device_count(void) { return dq::device_count(); }
 
+int dq_set_sweep_relabel(int32_t on) { return dq::set_sweep_relabel(on != 0) ? 1 : 0; }
+
 int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch) {
   return guard([&] { *tflops = dq::mfma_fp32_peak_tflops(reps, ms_per_launch); });
 }

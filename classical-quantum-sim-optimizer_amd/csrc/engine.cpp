@@ -23,6 +23,7 @@
 #include <cstddef>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <random>
@@ -217,6 +218,10 @@ int device_count() {
   return n;
 }
 
+// Relabelled tile sweeps (emit_sweeps, emit_exact_sweeps).
+std::atomic<bool> g_sweep_relabel{false};
+bool set_sweep_relabel(bool on) { return g_sweep_relabel.exchange(on); }
+
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   int dev = 0, cus = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -300,6 +305,7 @@ struct DevOp {
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
   int chains = 0;      // exact sweeps: register chains / single-op passes
   int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
+  std::vector<uint32_t> tile_lq;  // qubit of each local bit at the load (plan description)
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -368,7 +374,7 @@ struct Plan {
   size_t push(const void* p, size_t bytes) {
     size_t off = (blob.size() + 255) & ~(size_t)255;
     blob.resize(off + bytes);
-    std::memcpy(blob.data() + off, p, bytes);
+    if (bytes) std::memcpy(blob.data() + off, p, bytes);
     return off;
   }
   void upload(hipStream_t st) {
@@ -654,26 +660,250 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Relabelled tile sweeps
+// ---------------------------------------------------------------------------
+// During a run of tile sweeps a bit position of the state may hold another
+// qubit than its own: a sweep may store its tile with the positions of some
+// of its qubits exchanged (in place: the same addresses it read), and the
+// next sweep's tile then has to contain whatever qubits sit at positions
+// 0..F-1 (the contiguous runs its loads move). Each sweep can bring up to two
+// of its qubits into those low positions, chosen as the ones the next blocks
+// need first, so the low positions stop pinning qubits 0..F-1 into every
+// tile. A run ends with the identity layout (restore passes if needed).
+// Off by default: at 32 qubits it cuts the sweeps by 9 % (67 -> 61 complex64,
+// 150 -> 138 exact complex128), but the sweeps it leaves carry more blocks
+// on kernels whose arithmetic and HBM time are nearly equal, and the lane
+// swaps add VALU work, so the whole run measured about 5 % slower.
+// Invariant kept by every relabelling: low position j holds qubit j or a
+// qubit >= F (a qubit < F only ever returns to its own position).
+
+struct BitLayout {
+  uint32_t pos[64];  // bit position holding qubit q
+  uint32_t at[64];   // qubit held at bit position p
+  int n;
+  explicit BitLayout(int n_) : n(n_) {
+    for (uint32_t q = 0; q < 64; ++q) pos[q] = at[q] = q;
+  }
+  bool identity() const {
+    for (int q = 0; q < n; ++q)
+      if (pos[q] != (uint32_t)q) return false;
+    return true;
+  }
+  uint64_t low_mask(int F) const {
+    uint64_t m = 0;
+    for (int p = 0; p < F; ++p) m |= 1ull << at[p];
+    return m;
+  }
+};
+
+// A sweep's tile as the kernel sees it (k::TileArgs): load-order local bit
+// i holds qubit lq[i], loaded from position spos[i]; after the lane swaps
+// (as the tile goes into LDS) local bit i holds qubit cq[i], which the
+// blocks address and which is stored at position opos[i].
+struct TileMap {
+  int L = 0, F = 0;
+  uint32_t lq[12] = {}, cq[12] = {}, spos[12] = {}, opos[12] = {}, srt[12] = {};
+  uint32_t nswap = 0, sbit[2] = {0, 0};
+  bool relabels() const {
+    if (nswap) return true;
+    for (int i = 0; i < L; ++i)
+      if (opos[i] != spos[i]) return true;
+    return false;
+  }
+  TileMap plain() const {  // same tile, stored where it was read
+    TileMap t = *this;
+    std::copy(spos, spos + 12, t.opos);
+    std::copy(lq, lq + 12, t.cq);
+    t.nswap = 0;
+    return t;
+  }
+  void to_args(k::TileArgs& a) const {
+    a.L = (uint32_t)L;
+    std::copy(spos, spos + 12, a.spos);
+    std::copy(opos, opos + 12, a.opos);
+    std::copy(srt, srt + 12, a.srt);
+    a.nswap = nswap;
+    a.sbit[0] = sbit[0];
+    a.sbit[1] = sbit[1];
+  }
+};
+
+// Tile of the qubits in `tmask` (exactly L, including those at positions
+// 0..F-1) in layout `lay`, stored with `swaps` = (qubit x, low position j):
+// x (at a position >= F) sits at local bit 8 + s and trades places with lane
+// bit j. Then every other content of the tile's high positions goes to its
+// own position when that is in the tile, else keeps its position when free,
+// else takes a free one. Updates `lay` to the layout after the store.
+TileMap sweep_map(BitLayout& lay, uint64_t tmask, int L, int F,
+                  const std::vector<std::pair<uint32_t, uint32_t>>& swaps) {
+  if (__builtin_popcountll(tmask) != L || (tmask & lay.low_mask(F)) != lay.low_mask(F) || swaps.size() > 2 ||
+      (int)swaps.size() > L - 8)
+    throw RuntimeErr("internal: bad tile for a sweep");
+  TileMap tm;
+  tm.L = L;
+  tm.F = F;
+  std::vector<uint32_t> rest;
+  for (int p = F; p < lay.n; ++p) {
+    const uint32_t q = lay.at[p];
+    bool sw = false;
+    for (auto& x : swaps) sw |= x.first == q;
+    if (((tmask >> q) & 1) && !sw) rest.push_back(q);
+  }
+  for (auto& x : swaps)
+    if (lay.pos[x.first] < (uint32_t)F || !((tmask >> x.first) & 1) || x.second >= (uint32_t)F)
+      throw RuntimeErr("internal: bad lane swap for a sweep");
+  if (rest.size() + swaps.size() != (size_t)(L - F)) throw RuntimeErr("internal: tile size mismatch");
+  for (int b = 0; b < F; ++b) tm.lq[b] = lay.at[b];
+  for (size_t s = 0; s < swaps.size(); ++s) tm.lq[8 + s] = swaps[s].first;
+  size_t r = 0;
+  for (int b = F; b < L; ++b)
+    if (b < 8 || b >= 8 + (int)swaps.size()) tm.lq[b] = rest[r++];
+  for (int b = 0; b < L; ++b) tm.spos[b] = lay.pos[tm.lq[b]];
+  std::copy(tm.spos, tm.spos + L, tm.srt);
+  std::sort(tm.srt, tm.srt + L);
+  // Contents after the lane swaps.
+  uint32_t content[12];
+  std::copy(tm.lq, tm.lq + L, content);
+  tm.nswap = (uint32_t)swaps.size();
+  for (size_t s = 0; s < swaps.size(); ++s) {
+    const uint32_t j = swaps[s].second;
+    tm.sbit[s] = j;
+    std::swap(content[j], content[8 + s]);
+  }
+  std::vector<char> high(64, 0), used(64, 0);
+  for (int b = F; b < L; ++b) high[tm.spos[b]] = 1;
+  std::vector<int> dest(L, -1);
+  for (int b = 0; b < F; ++b) dest[b] = b;
+  for (int b = F; b < L; ++b)
+    if (high[content[b]] && !used[content[b]]) {
+      dest[b] = (int)content[b];
+      used[content[b]] = 1;
+    }
+  for (int b = F; b < L; ++b) {
+    const uint32_t cur = lay.pos[content[b]];
+    if (dest[b] < 0 && high[cur] && !used[cur]) {
+      dest[b] = (int)cur;
+      used[cur] = 1;
+    }
+  }
+  for (int b = F; b < L; ++b)
+    if (dest[b] < 0)
+      for (int c = F; c < L; ++c)
+        if (!used[tm.spos[c]]) {
+          dest[b] = (int)tm.spos[c];
+          used[tm.spos[c]] = 1;
+          break;
+        }
+  for (int b = 0; b < L; ++b) {
+    tm.cq[b] = content[b];
+    tm.opos[b] = (uint32_t)dest[b];
+    lay.pos[content[b]] = tm.opos[b];
+    lay.at[tm.opos[b]] = content[b];
+  }
+  return tm;
+}
+
+// Low-position choice for the next sweep: the tile's qubits at positions >= F
+// whose first use (`first[q]`, smaller is sooner) comes before that of a low
+// occupant replace the latest-used occupants, at most two per sweep. A qubit
+// < F may only take its own position. At the end of a run (`last`) qubits
+// 0..F-1 in the tile return to their positions instead.
+std::vector<std::pair<uint32_t, uint32_t>> choose_swaps(const BitLayout& lay, uint64_t tmask, int L, int F,
+                                                         const std::vector<uint32_t>& first, bool last) {
+  std::vector<std::pair<uint32_t, uint32_t>> sw;
+  const size_t cap = (size_t)std::min(2, L - 8);
+  if (last) {
+    for (int j = 0; j < F && sw.size() < cap; ++j)
+      if (lay.at[j] != (uint32_t)j && ((tmask >> j) & 1)) sw.push_back({(uint32_t)j, (uint32_t)j});
+    return sw;
+  }
+  std::vector<uint32_t> cand;
+  for (int p = F; p < lay.n; ++p) {
+    const uint32_t q = lay.at[p];
+    if (((tmask >> q) & 1) && first[q] != UINT32_MAX) cand.push_back(q);
+  }
+  std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return first[a] < first[b]; });
+  std::vector<char> slot_used(F, 0);
+  for (uint32_t x : cand) {
+    if (sw.size() >= cap) break;
+    int best = -1;
+    for (int j = 0; j < F; ++j) {
+      if (slot_used[j] || first[lay.at[j]] <= first[x]) continue;
+      if (x < (uint32_t)F && (uint32_t)j != x) continue;
+      if (best < 0 || first[lay.at[j]] > first[lay.at[best]]) best = j;
+    }
+    if (best < 0) continue;
+    slot_used[best] = 1;
+    sw.push_back({x, (uint32_t)best});
+  }
+  return sw;
+}
+
+// Passes of tile sweeps with no blocks that bring the state back to the
+// identity layout. Each pass fixes up to two low positions (the qubit of
+// position j trades places with the occupant, which goes to its own position)
+// and sends every other qubit whose current and own positions both fit in
+// the tile to its own position; no qubit at its own position moves, so every
+// pass places at least one more qubit and the loop ends.
+void restore_layout(BitLayout& lay, int L, int F, const std::function<void(const TileMap&)>& emit) {
+  for (int pass = 0; !lay.identity(); ++pass) {
+    if (pass >= 64) throw RuntimeErr("internal: tile layout restore did not converge");
+    std::vector<char> inP(64, 0);
+    int room = L - F;
+    std::vector<std::pair<uint32_t, uint32_t>> sw;
+    auto fits = [&](uint32_t a, uint32_t b) {
+      const int need = (a >= (uint32_t)F && !inP[a]) + (b >= (uint32_t)F && !inP[b] && b != a);
+      return need <= room;
+    };
+    auto add = [&](uint32_t p) {
+      if (p >= (uint32_t)F && !inP[p]) {
+        inP[p] = 1;
+        --room;
+      }
+    };
+    for (int j = 0; j < F && (int)sw.size() < std::min(2, L - 8); ++j) {
+      if (lay.at[j] == (uint32_t)j) continue;
+      const uint32_t from = lay.pos[j], occ = lay.at[j];
+      if (from < (uint32_t)F || occ < (uint32_t)F) throw RuntimeErr("internal: low-position invariant broken");
+      if (!fits(from, occ)) continue;
+      add(from);
+      add(occ);
+      sw.push_back({(uint32_t)j, (uint32_t)j});
+    }
+    for (int q = F; q < lay.n; ++q) {
+      const uint32_t cur = lay.pos[q];
+      if (cur == (uint32_t)q || cur < (uint32_t)F) continue;
+      if (fits(cur, (uint32_t)q)) {
+        add(cur);
+        add((uint32_t)q);
+      }
+    }
+    for (int p = F; p < lay.n && room > 0; ++p) add((uint32_t)p);
+    uint64_t tmask = lay.low_mask(F);
+    for (int p = F; p < lay.n; ++p)
+      if (inP[p]) tmask |= 1ull << lay.at[p];
+    emit(sweep_map(lay, tmask, L, F, sw));
+  }
+}
+
 // Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
 // read and write of the state for all of them. The blob (descriptors, then
 // matrices) is what every workgroup copies into LDS.
-void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int L, const std::vector<Op>& ops,
+void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, const std::vector<Op>& ops,
                int n) {
   DevOp d;
   d.kind = D_TILE;
   d.category = 1;
   d.gates = 0;
-  // Pad the tile to exactly L qubits with the lowest free qubits.
-  for (uint32_t q = 0; __builtin_popcountll(smask) < L; ++q) smask |= 1ull << q;
+  const int L = tm.L;
   std::vector<int> local(64, -1);
-  int nl = 0;  // tile qubits so far
-  for (uint32_t q = 0; q < 64; ++q)
-    if ((smask >> q) & 1) {
-      d.targs.spos[nl] = q;
-      local[q] = nl++;
-      d.touched.push_back(q);
-    }
-  d.targs.L = (uint32_t)L;
+  const int nl = L;
+  for (int b = 0; b < L; ++b) local[tm.cq[b]] = b;  // LDS order (after the lane swaps)
+  d.touched.assign(tm.lq, tm.lq + L);
+  std::sort(d.touched.begin(), d.touched.end());
+  d.tile_lq.assign(tm.lq, tm.lq + L);
+  tm.to_args(d.targs);
   d.targs.nblocks = (uint32_t)blocks.size();
   std::vector<k::TileBlock> tb;
   std::vector<float> mats;
@@ -687,7 +917,7 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
       padded.ops = orig->ops;
       std::vector<uint32_t> q = orig->qubits;
       for (int i = nl - 1; i >= 0 && q.size() < 2; --i)
-        if (std::find(q.begin(), q.end(), (uint32_t)d.targs.spos[i]) == q.end()) q.push_back(d.targs.spos[i]);
+        if (std::find(q.begin(), q.end(), tm.cq[i]) == q.end()) q.push_back(tm.cq[i]);
       std::sort(q.rbegin(), q.rend());
       padded.qubits = q;
       b = &padded;
@@ -738,22 +968,29 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, uint64_t smask, int
 
 // Tile sweeps of a unitary run ops[begin, end):
 //  1. fuse to blocks of <= 2 qubits (wider compiled ops stay single blocks);
-//  2. group the blocks into sweeps: a sweep's qubits always include {0,1,2,3}
-//     (a tile row is then a 128-byte run); it repeatedly takes, among the
-//     blocks whose earlier untaken blocks share no qubit with them (so the
-//     taken order is a valid reordering), the one adding the fewest new
-//     qubits, while the union stays within L qubits. Each sweep tries this
-//     from {0..3} alone and seeded with each of the first 8 available blocks,
-//     and keeps the trial that takes the most blocks;
+//  2. group the blocks into sweeps: a sweep's qubits always include those at
+//     bit positions 0..3 (a tile row is then a 128-byte run); it repeatedly
+//     takes, among the blocks whose earlier untaken blocks share no qubit
+//     with them (so the taken order is a valid reordering), the one adding
+//     the fewest new qubits, while the union stays within L qubits. Each
+//     sweep tries this from the low qubits alone and seeded with each of the
+//     first 8 available blocks, and keeps the trial that takes the most
+//     blocks. The tile is padded to L qubits with the qubits the next blocks
+//     use first, and (relabelling on) the sweep's store brings up to two of
+//     them into positions 0..3;
 //  3. fuse each sweep's ops again into blocks of <= kt qubits (fewer passes
 //     over the tile in LDS) and emit one tile launch per sweep, split where the
-//     blob would exceed its LDS budget. A sweep of one block is an ordinary
-//     launch; blocks wider than 4 qubits always are.
+//     blob would exceed its LDS budget. A sweep of one block that leaves the
+//     layout alone is an ordinary launch; blocks wider than 4 qubits always
+//     are, after the identity layout is restored.
 void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int prec, int n, int L, int kt,
                  const MergeScorer& scorer) {
   constexpr size_t kWindow = 512;
+  constexpr int F = 4;
   // LDS per workgroup = 8 * 2^L + blob; the blob cap keeps it within 64 KiB.
   constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = 64;
+  const bool relabel = g_sweep_relabel;
+  BitLayout lay(n);
   std::vector<FusedBlock> pre = greedy_fuse(ops, begin, end, 2);
   std::vector<uint64_t> mask(pre.size());
   for (size_t b = 0; b < pre.size(); ++b) mask[b] = qubit_mask(pre[b].qubits);
@@ -763,12 +1000,12 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
   std::vector<char> taken(pre.size(), 0);
   std::vector<int> mark(pre.size(), -1);  // trial number that took the block
   int trial_no = 0;
-  // One min-growth trial from `seed` (-1: qubits 0..3 only); returns the
+  // One min-growth trial from `seed` (-1: the low qubits only); returns the
   // blocks it would take, in a valid execution order.
   auto trial = [&](size_t head, int seed, uint64_t& smask) {
     const int gen = ++trial_no;
     std::vector<int> take;
-    smask = 0xFull;
+    smask = lay.low_mask(F);
     if (seed >= 0) {
       mark[seed] = gen;
       take.push_back(seed);
@@ -799,17 +1036,24 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
     }
     return take;
   };
+  auto restore = [&]() {
+    restore_layout(lay, L, F, [&](const TileMap& tm) {
+      std::vector<FusedBlock*> none;
+      emit_tile(plan, none, tm, ops, n);
+    });
+  };
   size_t head = 0;
   while (head < rem.size()) {
     // rem[head..] holds the untaken blocks in program order.
     const int h = rem[head];
     if (!tileable((size_t)h)) {
+      restore();
       emit_block(plan, pre[h], ops, prec, n);
       taken[h] = 1;
       ++head;
       continue;
     }
-    // Multi-start: grow from qubits 0..3 alone and from each of the first
+    // Multi-start: grow from the low qubits alone and from each of the first
     // kStarts available blocks; keep the trial that takes the most blocks.
     constexpr int kStarts = 8;
     uint64_t smask = 0;
@@ -820,7 +1064,8 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
       for (size_t r = head; r < rem.size() && starts < kStarts; ++r) {
         const int b = rem[r];
         if (taken[b]) continue;
-        if (!(mask[b] & blocked) && tileable((size_t)b)) {
+        if (!(mask[b] & blocked) && tileable((size_t)b) &&
+            __builtin_popcountll(lay.low_mask(F) | mask[b]) <= L) {
           ++starts;
           uint64_t m2 = 0;
           std::vector<int> t = trial(head, b, m2);
@@ -833,8 +1078,37 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
       }
     }
     for (int b : sweep) taken[b] = 1;
-    // The head block is always available and fits (<= 4 new qubits), so the
-    // sweep is never empty.
+    // The head block is always available and fits (its <= 4 qubits with the
+    // 4 low ones), so the sweep is never empty.
+    // First use of each qubit among the blocks still to come (window).
+    std::vector<uint32_t> first(64, UINT32_MAX);
+    bool more = false;
+    {
+      size_t seen = 0;
+      for (size_t r = head; r < rem.size() && seen < kWindow; ++r) {
+        const int b = rem[r];
+        if (taken[b]) continue;
+        more = true;
+        for (uint32_t q : pre[b].qubits)
+          if (first[q] == UINT32_MAX) first[q] = (uint32_t)seen;
+        ++seen;
+      }
+    }
+    if (relabel) {
+      std::vector<uint32_t> cand;
+      for (int p = 0; p < n; ++p) cand.push_back(lay.at[p]);
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return first[a] < first[b]; });
+      for (uint32_t q : cand) {
+        if (__builtin_popcountll(smask) >= L) break;
+        smask |= 1ull << q;
+      }
+    } else {
+      for (int p = 0; __builtin_popcountll(smask) < L; ++p) smask |= 1ull << lay.at[p];
+    }
+    const bool was_identity = lay.identity();
+    const TileMap tm =
+        sweep_map(lay, smask, L, F, relabel ? choose_swaps(lay, smask, L, F, first, !more)
+                                            : std::vector<std::pair<uint32_t, uint32_t>>{});
     std::vector<int> order;
     for (int b : sweep) order.insert(order.end(), pre[b].ops.begin(), pre[b].ops.end());
     std::vector<FusedBlock> blocks = greedy_fuse_list(ops, order, std::max(1, std::min(kt, 4)), scorer);
@@ -847,17 +1121,20 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
         blob += bb;
         ++j;
       }
-      if (j == i + 1) {
+      // Only the sweep's last launch relabels.
+      const TileMap& m = j == blocks.size() ? tm : tm.plain();
+      if (j == i + 1 && was_identity && !m.relabels()) {
         emit_block(plan, blocks[i], ops, prec, n);
       } else {
         std::vector<FusedBlock*> bp;
         for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
-        emit_tile(plan, bp, smask, L, ops, n);
+        emit_tile(plan, bp, m, ops, n);
       }
       i = j;
     }
     while (head < rem.size() && taken[rem[head]]) ++head;
   }
+  restore();
 }
 
 // Exact complex128 sweeps: maximal runs of consecutive compiled ops (program
@@ -889,21 +1166,19 @@ size_t exact_op_bytes(const Op& op) {
   return sizeof(k::TileOp) + ((data + 15) & ~(size_t)15);
 }
 
-void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, uint64_t smask, int L) {
+void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, const TileMap& tm) {
   DevOp d;
   d.kind = D_TILE;
   d.exact = true;
   d.category = 1;
   d.gates = 0;
-  for (uint32_t q = 0; __builtin_popcountll(smask) < L; ++q) smask |= 1ull << q;
+  const int L = tm.L;
   std::vector<int> local(64, -1);
-  int nl = 0;
-  for (uint32_t q = 0; q < 64; ++q)
-    if ((smask >> q) & 1) {
-      d.targs.spos[nl] = q;
-      local[q] = nl++;
-      d.touched.push_back(q);
-    }
+  for (int b = 0; b < L; ++b) local[tm.cq[b]] = b;  // LDS order (after the lane swaps)
+  d.touched.assign(tm.lq, tm.lq + L);
+  std::sort(d.touched.begin(), d.touched.end());
+  d.tile_lq.assign(tm.lq, tm.lq + L);
+  tm.to_args(d.targs);
   auto lmask = [&](uint64_t g) {
     uint32_t m = 0;
     for (uint32_t q = 0; q < 64; ++q)
@@ -925,6 +1200,7 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
       t.pos[b] = (uint32_t)local[x.e.pos[b]];
       t.qmask |= 1u << t.pos[b];
     }
+    std::sort(t.pos, t.pos + t.nbits);  // local bits need not follow qubit order
     t.fixed = lmask(x.e.fixed);
     std::vector<uint8_t> dd;
     auto put = [&](const void* p, size_t bytes) {
@@ -1108,23 +1384,30 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, ui
 
 void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int n, int L) {
   // LDS per workgroup = 16 * 2^L + blob; the blob cap keeps it within 64 KiB.
-  constexpr size_t kBlobMax = 32768, kOpsMax = 128;
+  constexpr size_t kBlobMax = 32768, kOpsMax = 128, kWindow = 512;
+  // The qubits at positions 0..2 are always tile qubits: 8 lanes move one
+  // contiguous 128 B run. Fixing 0..3 (256 B runs) needed 171 sweeps instead
+  // of 150 at 32 qubits and measured 3.6 % slower.
+  constexpr int F = 3;
+  const bool relabel = g_sweep_relabel;
+  BitLayout lay(n);
   auto plain = [&](size_t o) {
     plan.ops.push_back(lower(ops[o], plan, 16));
     plan.ops.back().touched = op_qubits(ops[o]);
     block_features(plan.ops.back().touched, {(int)o}, ops, n, plan.ops.back().feat);
     plan.ops.back().src = {(int)o};
   };
+  auto restore = [&]() {
+    restore_layout(lay, L, F, [&](const TileMap& tm) { emit_tile128(plan, ops, 0, 0, tm); });
+  };
   size_t i = begin;
   while (i < end) {
     if (!exact_tileable(ops[i])) {
+      restore();
       plain(i++);
       continue;
     }
-    // Qubits 0..2 are always tile qubits: 8 lanes move one contiguous 128 B
-    // run. Fixing 0..3 (256 B runs) needed 171 sweeps instead of 150 at 32
-    // qubits and measured 3.6 % slower.
-    uint64_t smask = 0x7ull | qubit_mask(op_qubits(ops[i]));
+    uint64_t smask = lay.low_mask(F) | qubit_mask(op_qubits(ops[i]));
     size_t blob = exact_op_bytes(ops[i]);
     size_t j = i + 1;
     while (j < end && j - i < kOpsMax && exact_tileable(ops[j])) {
@@ -1135,10 +1418,32 @@ void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, siz
       blob += bb;
       ++j;
     }
-    if (j - i == 1) plain(i);
-    else emit_tile128(plan, ops, i, j, smask, L);
+    // First use of each qubit among the ops that follow (window).
+    std::vector<uint32_t> first(64, UINT32_MAX);
+    for (size_t o = j; o < end && o < j + kWindow; ++o)
+      for (uint32_t q : op_qubits(ops[o]))
+        if (first[q] == UINT32_MAX) first[q] = (uint32_t)(o - j);
+    const bool last = j == end || !exact_tileable(ops[j]);
+    if (relabel) {
+      std::vector<uint32_t> cand;
+      for (int p = 0; p < n; ++p) cand.push_back(lay.at[p]);
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return first[a] < first[b]; });
+      for (uint32_t q : cand) {
+        if (__builtin_popcountll(smask) >= L) break;
+        smask |= 1ull << q;
+      }
+    } else {
+      for (int p = 0; __builtin_popcountll(smask) < L; ++p) smask |= 1ull << lay.at[p];
+    }
+    const bool was_identity = lay.identity();
+    const TileMap tm =
+        sweep_map(lay, smask, L, F, relabel ? choose_swaps(lay, smask, L, F, first, last)
+                                            : std::vector<std::pair<uint32_t, uint32_t>>{});
+    if (j - i == 1 && was_identity && !tm.relabels()) plain(i);
+    else emit_tile128(plan, ops, i, j, tm);
     i = j;
   }
+  restore();
 }
 
 Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n,
@@ -1958,6 +2263,18 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
     if (d.kind == D_TILE) {
       js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"chains\": " << d.chains
          << ", \"fixed_chains\": " << d.fixed_chains;
+      // Layout of the sweep: load and store bit position of each local bit
+      // (local order), and the low positions that trade places with local
+      // bits 8, 9 before the store.
+      auto list = [&](const char* key, const uint32_t* v, uint32_t cnt) {
+        js << ", \"" << key << "\": [";
+        for (uint32_t t = 0; t < cnt; ++t) js << (t ? ", " : "") << v[t];
+        js << "]";
+      };
+      list("local_qubits", d.tile_lq.data(), (uint32_t)d.tile_lq.size());
+      list("load_positions", d.targs.spos, d.targs.L);
+      list("store_positions", d.targs.opos, d.targs.L);
+      list("swaps", d.targs.sbit, d.targs.nswap);
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";

@@ -168,6 +168,8 @@ Counts batch_counts(const BatchOutcome& b, uint64_t i);
 // Measured FP32 matrix-core throughput (TFLOP/s) of a register-only
 // v_mfma_f32_16x16x4f32 loop over every CU; also the time per launch.
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch);
+// Relabelled tile sweeps on or off (process-wide); returns the previous setting.
+bool set_sweep_relabel(bool on);
 // Host-only text dump of the program the persistent batched kernel would
 // upload (circuits, launch order, fused ops, FP32 matrices; circuits above
 // `lmax` qubits use an HBM slot of the printed size).

@@ -326,9 +326,11 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // ---------------------------------------------------------------------------
 // LDS-tile sweep: several fused blocks per HBM pass
 // ---------------------------------------------------------------------------
-// Local index t of a tile: bit i <-> global qubit spos[i]; the tile's other
-// global bits come from the tile number (zeros inserted at S). spos[0..3] are
-// qubits 0..3, so 16 consecutive threads move one contiguous 128-byte run.
+// Local index t of a tile: bit i <-> bit position spos[i] of the state; the
+// tile's other bits come from the tile number (zeros inserted at the tile's
+// positions). spos[0..3] are positions 0..3, so 16 consecutive threads move
+// one contiguous 128-byte run. A sweep may relabel positions within its
+// tile: lane swaps on the loaded registers, then store positions opos.
 // A workgroup first copies the sweep's blob (block descriptors, then their
 // matrices) into LDS, then walks its tiles persistently: while the blocks of
 // one tile are applied in LDS, the loads of its next tile are in flight (the
@@ -344,6 +346,114 @@ __global__ __launch_bounds__(TPB) void k_fused_mfma(float2* __restrict__ s, uint
 // raise the kernel's register use past three workgroups per CU.
 // MFMA matrices are stored fragment-major (the A value of K step q for lane l
 // at [q][l]), so every operand read is one conflict-free 256-byte LDS row.
+
+// Value held by lane (lane ^ (1 << j)) of the same 16-lane row, j < 4 and
+// wave-uniform. Every DPP pattern used is its own inverse, so none depends on
+// the direction convention of a shift: quad_perm [1,0,3,2] (xor 1),
+// quad_perm [2,3,0,1] (xor 2), quad_perm [3,2,1,0] then row_half_mirror
+// (xor 3, then xor 7: xor 4), row_ror:8 (xor 8).
+__device__ __forceinline__ int lane_xor(int v, uint32_t j) {
+  switch (j) {
+    case 0: return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);
+    case 1: return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);
+    case 2: {
+      const int t = __builtin_amdgcn_update_dpp(v, v, 0x1B, 0xF, 0xF, false);
+      return __builtin_amdgcn_update_dpp(t, t, 0x141, 0xF, 0xF, false);
+    }
+    default: return __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false);
+  }
+}
+
+// Exchange of lane bit j with a register bit: `a` holds register-bit value
+// 0, `b` value 1; `h` is lane bit j of this lane. Afterwards the lane with
+// bit j = 0 holds (a, partner's a) and the lane with bit j = 1 holds
+// (partner's b, b): lane bit and register bit have traded places. Each lane
+// sends the one value its partner needs (one DPP move per word).
+__device__ __forceinline__ void lane_swap(float& ax, float& ay, float& bx, float& by, uint32_t j, bool h) {
+  const float sx = h ? ax : bx, sy = h ? ay : by;
+  const float rx = __int_as_float(lane_xor(__float_as_int(sx), j));
+  const float ry = __int_as_float(lane_xor(__float_as_int(sy), j));
+  if (h) {
+    ax = rx;
+    ay = ry;
+  } else {
+    bx = rx;
+    by = ry;
+  }
+}
+__device__ __forceinline__ double lane_xor_d(double v, uint32_t j) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)lane_xor((int)(uint32_t)u, j), hi = (uint32_t)lane_xor((int)(uint32_t)(u >> 32), j);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ void lane_swap(double& ax, double& ay, double& bx, double& by, uint32_t j, bool h) {
+  const double sx = h ? ax : bx, sy = h ? ay : by;
+  const double rx = lane_xor_d(sx, j), ry = lane_xor_d(sy, j);
+  if (h) {
+    ax = rx;
+    ay = ry;
+  } else {
+    bx = rx;
+    by = ry;
+  }
+}
+
+// Lane swaps of a loaded tile (registers of local index tid + TPB i, real
+// and imaginary parts in x[i], y[i]) before it goes into LDS: for s <
+// nswap, lane bit sbit[s] trades places with register bit s. In-register
+// (DPP), so the relabelling costs no LDS traffic; the sweep's blocks address
+// the swapped local order.
+template <int PER, typename T>
+__device__ __forceinline__ void tile_lane_swaps(T (&x)[PER], T (&y)[PER], const TileArgs& a) {
+  const uint32_t nswap = __builtin_amdgcn_readfirstlane(a.nswap);
+  if (nswap == 0) return;
+  auto pass = [&](int bit, uint32_t j) {
+    const bool h = (threadIdx.x >> j) & 1;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (!(i & bit)) lane_swap(x[i], y[i], x[i + bit], y[i + bit], j, h);
+  };
+  pass(1, __builtin_amdgcn_readfirstlane(a.sbit[0]));
+  if (nswap == 2) pass(2, __builtin_amdgcn_readfirstlane(a.sbit[1]));
+}
+template <int PER>
+__device__ __forceinline__ void tile_lane_swaps(float2 (&r)[PER], const TileArgs& a) {
+  if (__builtin_amdgcn_readfirstlane(a.nswap) == 0) return;
+  float x[PER], y[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    x[i] = r[i].x;
+    y[i] = r[i].y;
+  }
+  tile_lane_swaps(x, y, a);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) r[i] = make_float2(x[i], y[i]);
+}
+
+// Store of a tile from LDS: local bit i goes to bit position opos[i] (equal
+// to spos[i] unless the sweep relabels).
+template <int L, typename V>
+__device__ __forceinline__ void tile_store(V* __restrict__ st, const V* tile, uint64_t base, const TileArgs& a) {
+  constexpr int PER = (1 << L) / TPB;
+  uint64_t olo = 0;
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit)
+    if ((threadIdx.x >> bit) & 1) olo |= 1ull << a.opos[bit];
+  // Opaque to the optimiser: otherwise the PER per-lane offsets olo | ohi(i)
+  // are hoisted out of the tile loop into 2 * PER more VGPRs.
+  asm volatile("" : "+v"(olo));
+  uint64_t om[L - 8];
+#pragma unroll
+  for (int bit = 0; bit < L - 8; ++bit) om[bit] = 1ull << a.opos[8 + bit];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int bit = 0; bit < L - 8; ++bit)
+      if ((i >> bit) & 1) o |= om[bit];
+    st[base | olo | o] = tile[threadIdx.x + TPB * i];
+  }
+}
 
 __device__ __forceinline__ uint32_t insert_zeros32(uint32_t p, const uint32_t* srt, int k) {
   for (int i = 0; i < k; ++i) {
@@ -587,20 +697,22 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
 #pragma unroll
   for (int bit = 0; bit < 8; ++bit)
     if ((threadIdx.x >> bit) & 1) lo |= 1ull << a.spos[bit];
-  uint64_t hi[PER];
+  asm volatile("" : "+v"(lo));  // keeps the PER offsets lo | hi(i) out of VGPRs
+  uint64_t hm[L - 8];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
+  for (int bit = 0; bit < L - 8; ++bit) hm[bit] = 1ull << a.spos[8 + bit];
+  auto hi = [&](int i) {
     uint64_t o = 0;
 #pragma unroll
     for (int bit = 0; bit < L - 8; ++bit)
-      if ((i >> bit) & 1) o |= 1ull << a.spos[8 + bit];
-    hi[i] = o;
-  }
+      if ((i >> bit) & 1) o |= hm[bit];
+    return o;
+  };
   auto tile_base = [&](uint64_t tau) {
     uint64_t base = tau;
 #pragma unroll
     for (int i = 0; i < L; ++i) {
-      const uint64_t low = (1ull << a.spos[i]) - 1;
+      const uint64_t low = (1ull << a.srt[i]) - 1;
       base = (base & low) | ((base & ~low) << 1);
     }
     return base;
@@ -615,7 +727,8 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
   if (tau < ntiles) {
     const uint64_t b0 = tile_base(tau);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi[i]];
+    for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi(i)];
+    tile_lane_swaps(r, a);
 #pragma unroll
     for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
   }
@@ -627,7 +740,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     if (more) {
       const uint64_t nb = tile_base(nxt);
 #pragma unroll
-      for (int i = 0; i < PER; ++i) r[i] = st[nb | lo | hi[i]];
+      for (int i = 0; i < PER; ++i) r[i] = st[nb | lo | hi(i)];
     }
     for (uint32_t bi = 0; bi < a.nblocks; ++bi) {
       const TileBlock& b = blocks[bi];
@@ -637,9 +750,9 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       else tile_block_mfma32<L>(tile, b, m);
       __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    tile_store<L>(st, tile, base, a);
     if (more) {
+      tile_lane_swaps(r, a);
       __syncthreads();  // every wave has read this tile out of LDS
 #pragma unroll
       for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
@@ -981,20 +1094,22 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
 #pragma unroll
   for (int bit = 0; bit < 8; ++bit)
     if ((threadIdx.x >> bit) & 1) lo |= 1ull << a.spos[bit];
-  uint64_t hi[PER];
+  asm volatile("" : "+v"(lo));  // keeps the PER offsets lo | hi(i) out of VGPRs
+  uint64_t hm[L - 8];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
+  for (int bit = 0; bit < L - 8; ++bit) hm[bit] = 1ull << a.spos[8 + bit];
+  auto hi = [&](int i) {
     uint64_t o = 0;
 #pragma unroll
     for (int bit = 0; bit < L - 8; ++bit)
-      if ((i >> bit) & 1) o |= 1ull << a.spos[8 + bit];
-    hi[i] = o;
-  }
+      if ((i >> bit) & 1) o |= hm[bit];
+    return o;
+  };
   auto tile_base = [&](uint64_t tau) {
     uint64_t base = tau;
 #pragma unroll
     for (int i = 0; i < L; ++i) {
-      const uint64_t low = (1ull << a.spos[i]) - 1;
+      const uint64_t low = (1ull << a.srt[i]) - 1;
       base = (base & low) | ((base & ~low) << 1);
     }
     return base;
@@ -1009,10 +1124,11 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
     const uint64_t b0 = tile_base(tau);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const double2 v = st[b0 | lo | hi[i]];
+      const double2 v = st[b0 | lo | hi(i)];
       rx[i] = v.x;
       ry[i] = v.y;
     }
+    tile_lane_swaps(rx, ry, a);
 #pragma unroll
     for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = make_double2(rx[i], ry[i]);
   }
@@ -1025,7 +1141,7 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
       const uint64_t nb = tile_base(nxt);
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
-        const double2 v = st[nb | lo | hi[i]];
+        const double2 v = st[nb | lo | hi(i)];
         rx[i] = v.x;
         ry[i] = v.y;
       }
@@ -1040,9 +1156,9 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
       else tile128_chain<L, 2>(tile, ch, ops, meta);
       __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) st[base | lo | hi[i]] = tile[threadIdx.x + TPB * i];
+    tile_store<L>(st, tile, base, a);
     if (more) {
+      tile_lane_swaps(rx, ry, a);
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = make_double2(rx[i], ry[i]);

@@ -75,7 +75,7 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, co
 
 // ---- LDS-tile sweep (complex64) ----
 // One sweep applies several fused blocks whose qubits all lie in a tile set S
-// (|S| = L <= 12, S always holds qubits 0..3). Each workgroup loads one
+// (|S| = L <= 12, S always holds the qubits at bit positions 0..3). Each workgroup loads one
 // 2^L-amplitude tile into LDS, applies every block there, and writes it back.
 // Blob of one sweep (copied to LDS by every workgroup): nblocks TileBlock
 // descriptors, then the block matrices (16-byte aligned).
@@ -89,12 +89,24 @@ struct TileBlock {
   uint32_t pad[3];
 };
 static_assert(sizeof(TileBlock) == 64, "TileBlock is 64 bytes");
+// A sweep may also relabel the state's bit positions within its tile (the
+// store writes the same addresses the load read, so the sweep stays in
+// place): as a loaded tile goes into LDS, lane bit sbit[s] and register bit
+// s (local bit 8 + s) exchange contents for s < nswap; the blocks address
+// that swapped local order, and local bit i is stored at bit position
+// opos[i]. Local bits 0..F-1 (F = 4 complex64, 3 complex128) are positions
+// 0..F-1 both ways, so loads and stores move contiguous runs.
 struct TileArgs {
   uint32_t L;         // tile qubits
   uint32_t nblocks;
   uint32_t blob16;    // blob size in 16-byte units
   uint32_t kmax;      // widest block of the sweep (3 or 4)
-  uint32_t spos[12];  // S sorted ascending (global qubit of local bit i)
+  uint32_t spos[12];  // bit position of load-order local bit i
+  uint32_t opos[12];  // bit position of LDS local bit i (after the lane swaps) at the store
+  uint32_t srt[12];   // the tile's bit positions sorted ascending
+  uint32_t nswap;     // lane swaps between the load and LDS (0..2)
+  uint32_t sbit[2];   // lane bit of swap s (< F)
+  uint32_t pad;
 };
 // Exact complex128 sweep (no fusion): the blob holds nops TileOp descriptors,
 // then each op's FP64 data (16-byte aligned). Ops are applied in program

@@ -147,6 +147,7 @@ def _load():
         "dq_string_free": (None, [P]),
         "dq_device_count": (ctypes.c_int, []),
         "dq_mfma_fp32_peak": (ctypes.c_int, [I32, ctypes.POINTER(D), ctypes.POINTER(D)]),
+        "dq_set_sweep_relabel": (ctypes.c_int, [I32]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
@@ -555,6 +556,13 @@ def mfma_fp32_peak(reps: int = 5):
     tf, ms = ctypes.c_double(), ctypes.c_double()
     _check(_lib.dq_mfma_fp32_peak(reps, ctypes.byref(tf), ctypes.byref(ms)))
     return tf.value, ms.value
+
+
+def set_sweep_relabel(on: bool) -> bool:
+    """Relabelled tile sweeps on or off for this process (default off: fewer
+    sweeps, but slower sweeps); returns the previous setting. Results do not
+    depend on it (complex128 exact sweeps stay bit-identical)."""
+    return bool(_lib.dq_set_sweep_relabel(1 if on else 0))
 
 
 def set_planner_weights(weights) -> None:

@@ -87,3 +87,30 @@ def test_shots_with_mid_circuit_measurement_match_oracle(terminal):
     got = sim.simulate_shots(circ, shots=40)
     ref, _ = oracle.run_shots(circ.data, 40, 21, budget_mb=4096, sample_terminal=terminal, max_parallel_shots=16)
     assert got == ref
+
+
+def test_relabelled_exact_sweeps_bit_identical():
+    # Exact sweeps that relabel bit positions (lane swaps, permuted store
+    # positions, restore passes) change where amplitudes are stored, never
+    # their arithmetic: the state equals the oracle's and the fixed-low
+    # sweeps' bit for bit, for 11- and 10-qubit tiles.
+    n = 20
+    circ = workloads.random_su2(n=n, depth=30, seed=52)
+    insts = circ.data["instructions"]
+    print("[exact] relabel random_su2 20q depth 30", flush=True)
+    prev = dqsim._core.set_sweep_relabel(True)
+    try:
+        plan = dqsim._core.plan_json(circ)
+    finally:
+        dqsim._core.set_sweep_relabel(prev)
+    tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+    assert any(len(x["swaps"]) == 2 for x in tiles) and any(x["gates"] == 0 for x in tiles)
+    want = oracle.simulate(circ.data, 60)[0]
+    for tile in (11, 10):
+        for on in (True, False):
+            prev = dqsim._core.set_sweep_relabel(on)
+            try:
+                got = exact_state(insts, n, tile=tile)
+            finally:
+                dqsim._core.set_sweep_relabel(prev)
+            assert np.array_equal(got, want), (tile, on)

@@ -69,3 +69,33 @@ def test_tile_sweeps_with_trajectory_batches():
     # where a draw falls within that distance of a CDF boundary.
     common = sum(min(res[0].get(b, 0), res[12].get(b, 0)) for b in set(res[0]) | set(res[12]))
     assert common >= 60
+
+
+@pytest.mark.parametrize("L,k", [(12, 3), (12, 4), (11, 3), (10, 2)])
+def test_relabelled_sweeps_match_oracle_and_fixed_low(L, k):
+    # Sweeps that store their tile with bit positions exchanged (lane swaps
+    # and permuted store positions) and the restore passes that end each run:
+    # same state as sweeps that pin qubits 0..3, within the FP32 tolerance.
+    n = 20
+    circ = workloads.random_su2(n=n, depth=24, seed=40 + L + k)
+    print(f"[tiles] relabel su2 n={n} L={L} k={k}", flush=True)
+    prev = dqsim._core.set_sweep_relabel(True)
+    try:
+        plan = dqsim._core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                     tile_qubits=L)
+    finally:
+        dqsim._core.set_sweep_relabel(prev)
+    tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
+    assert any(len(x["swaps"]) == 2 for x in tiles) and any(x["gates"] == 0 for x in tiles)
+    want = oracle.simulate(circ.data, 1)[0]
+    states = {}
+    for on in (True, False):
+        prev = dqsim._core.set_sweep_relabel(on)
+        try:
+            sim = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                             tile_qubits=L)
+            states[on] = sim.simulate(circ).statevector
+        finally:
+            dqsim._core.set_sweep_relabel(prev)
+        assert np.abs(states[on] - want).max() < 1e-5
+    assert np.abs(states[True] - states[False]).max() < 1e-5

@@ -374,12 +374,57 @@ def _mixed_tile_circuit(n, depth, seed):
     return Circuit(n, ins)
 
 
+def _replay_layout(plan, F):
+    """Replays the state's bit layout through a plan: each tile reads every
+    local qubit from the position the layout holds it at, local bits 0..F-1
+    are positions 0..F-1 at the load and at the store, the store is a
+    permutation of the tile's own positions (in place), low position j only
+    ever holds qubit j or a qubit >= F, launches other than tiles run in the
+    identity layout, and the plan ends in the identity layout. Returns the
+    number of tiles that relabel."""
+    n = plan["num_qubits"]
+    pos = list(range(n))
+    relabels = 0
+    for x in plan["launches"]:
+        if x["kind"] != "tile":
+            assert pos == list(range(n)), "launch outside a tile in a relabelled layout"
+            continue
+        lq, sp, op, sw = x["local_qubits"], x["load_positions"], x["store_positions"], x["swaps"]
+        L = len(lq)
+        assert L == x["k"] and len(set(lq)) == L and sorted(lq) == sorted(x["qubits"])
+        assert [pos[q] for q in lq] == sp
+        assert sp[:F] == list(range(F)) and op[:F] == list(range(F))
+        assert sorted(op) == sorted(sp)
+        assert len(sw) <= 2 and all(0 <= j < F for j in sw)
+        content = list(lq)
+        for s, j in enumerate(sw):
+            content[j], content[8 + s] = content[8 + s], content[j]
+        for b in range(L):
+            pos[content[b]] = op[b]
+        assert sorted(pos) == list(range(n))
+        at = {p: q for q, p in enumerate(pos)}
+        assert all(at[j] == j or at[j] >= F for j in range(F))
+        relabels += bool(sw) or op != sp
+        for b in x["blocks"]:
+            assert set(b["qubits"]) <= set(lq)
+    assert pos == list(range(n)), "plan does not end in the identity layout"
+    return relabels
+
+
+@pytest.fixture(params=[True, False], ids=["relabel", "fixed-low"])
+def relabel(request):
+    prev = _core.set_sweep_relabel(request.param)
+    yield request.param
+    _core.set_sweep_relabel(prev)
+
+
 @pytest.mark.parametrize("L,k", [(10, 2), (10, 5), (11, 4), (12, 3), (12, 4), (12, 5)])
 @pytest.mark.parametrize("kind", ["su2", "mixed", "qft"])
-def test_tile_sweep_plan_is_a_valid_reordering(kind, L, k):
+def test_tile_sweep_plan_is_a_valid_reordering(kind, L, k, relabel):
     # Every compiled op is applied exactly once, ops that share a qubit keep
-    # their order, and each sweep's blocks fit its L tile qubits (which always
-    # include qubits 0..3).
+    # their order, each sweep's blocks fit its L tile qubits (which always
+    # include the qubits at positions 0..3), and the bit layout stays
+    # consistent and ends as the identity.
     n = 20
     if kind == "su2":
         c = workloads.random_su2(n=n, depth=12, seed=L * 10 + k)
@@ -402,10 +447,13 @@ def test_tile_sweep_plan_is_a_valid_reordering(kind, L, k):
             last[q] = o
     tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
     assert tiles
+    relabels = _replay_layout(plan, 4)
+    assert (relabels > 0) == relabel
     for t in tiles:
         s = set(t["qubits"])
-        assert len(s) == L and {0, 1, 2, 3} <= s and t["k"] == L
-        assert len(t["blocks"]) >= 2
+        assert len(s) == L and t["k"] == L
+        if not relabel:
+            assert {0, 1, 2, 3} <= s and len(t["blocks"]) >= 2
         for b in t["blocks"]:
             assert set(b["qubits"]) <= s and 1 <= len(b["qubits"]) <= 5
             assert b["mfma"]  # k <= 3 (padded to 3) on 16x16x4, k = 4 on 32x32x2
@@ -432,9 +480,10 @@ def test_tile_sweeps_off_and_limits():
 
 
 @pytest.mark.parametrize("kind", ["su2", "mixed", "qft"])
-def test_exact_sweeps_keep_program_order(kind):
+def test_exact_sweeps_keep_program_order(kind, relabel):
     # complex128 without fusion: sweeps are runs of consecutive compiled ops
-    # (nothing reordered), each within L tile qubits that include 0..2.
+    # (nothing reordered), each within L tile qubits that include the qubits
+    # at positions 0..2, with a consistent bit layout ending as the identity.
     n = 18
     if kind == "su2":
         c = workloads.random_su2(n=n, depth=6, seed=4)
@@ -448,11 +497,30 @@ def test_exact_sweeps_keep_program_order(kind):
         assert order == list(range(plan["compiled_ops"]))
         tiles = [x for x in plan["launches"] if x["kind"] == "tile"]
         assert tiles and len(plan["launches"]) < plan["compiled_ops"]
+        assert (_replay_layout(plan, 3) > 0) == relabel
         for t in tiles:
             s = set(t["qubits"])
-            assert len(s) == min(L, 11) and {0, 1, 2} <= s
+            assert len(s) == min(L, 11) and ({0, 1, 2} <= s or relabel)
             assert all(set(b["qubits"]) <= s for b in t["blocks"])
     assert not any(x["kind"] == "tile" for x in _core.plan_json(c, tile_qubits=0)["launches"])
+
+
+def test_relabelled_sweeps_take_fewer_passes():
+    # On the depth-40 random SU(2) family, relabelling the low positions
+    # (restore passes included) needs fewer sweeps than pinning qubits 0..3
+    # (complex64) or 0..2 (complex128) into every tile.
+    c = workloads.random_su2(n=26, depth=40, seed=1234)
+    counts = {}
+    for on in (True, False):
+        prev = _core.set_sweep_relabel(on)
+        try:
+            c64 = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=3, tile_qubits=12)
+            c128 = _core.plan_json(c, tile_qubits=11)
+        finally:
+            _core.set_sweep_relabel(prev)
+        counts[on] = (len(c64["launches"]), len(c128["launches"]))
+        assert _replay_layout(c64, 4) >= 0 and _replay_layout(c128, 3) >= 0
+    assert counts[True][0] < counts[False][0] and counts[True][1] < counts[False][1], counts
 
 
 def test_exact_sweep_chains_use_compiled_signatures():
@@ -507,4 +575,5 @@ def test_exact_sweeps_group_ops_into_register_chains():
     assert tiles and all(t["exact"] for t in tiles)
     assert sum(t["chains"] for t in tiles) < sum(len(t["ops"]) for t in tiles)
     for t in tiles:
-        assert 1 <= t["chains"] <= len(t["ops"])
+        # Restore passes (layout only) apply no ops.
+        assert (1 <= t["chains"] <= len(t["ops"])) or (t["chains"] == 0 and not t["ops"] and t["gates"] == 0)
