@@ -1,11 +1,13 @@
 // C ABI (include/dqsim_sv.h): error mapping, handles and thread-local errors.
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
 #include <random>
 #include <string>
 
 #include "../../include/dqsim_sv.h"
 #include "engine.hpp"
+#include "pblock.hpp"
 #include "planner.hpp"
 
 struct dq_sv {
@@ -61,6 +63,38 @@ void check_query(const dq::Result& r, const uint64_t* qubits, uint64_t k) {
     for (uint64_t j = 0; j < i && !bad; ++j) bad = qubits[j] == qubits[i];
     if (bad) throw dq::ValueErr("Query qubits must be distinct and within the statevector's range");
   }
+}
+}  // namespace
+
+// ---- P-block backend ----
+struct dq_pb_result {
+  std::unique_ptr<dq::pb::Result> r;
+};
+
+namespace {
+dq::pb::Options pb_options(const dq_pb_opts* o) {
+  dq::pb::Options p;
+  if (o->max_memory_mb == 0 || o->max_memory_mb > (uint64_t)INT64_MAX / (1024 * 1024))
+    throw dq::ValueErr("max_memory_mb must be positive and representable");
+  p.memory = o->max_memory_mb * 1024 * 1024;
+  p.has_max_block = o->has_max_block_qubits != 0;
+  p.max_block_qubits = o->max_block_qubits;
+  p.max_parallel_shots = o->max_parallel_shots;
+  p.sample_terminal = o->sample_terminal != 0;
+  p.split_separable = o->split_separable != 0;
+  p.max_split_qubits = o->max_split_qubits;
+  p.separation_tolerance = o->separation_tolerance;
+  p.profile = o->profile != 0;
+  p.has_seed = o->has_seed != 0;
+  p.seed = o->seed;
+  dq::pb::check_options(p);
+  return p;
+}
+char* dup_string(const std::string& s) {
+  char* p = (char*)std::malloc(s.size() + 1);
+  if (!p) throw std::bad_alloc();
+  std::memcpy(p, s.c_str(), s.size() + 1);
+  return p;
 }
 }  // namespace
 
@@ -378,6 +412,121 @@ int dq_mlp_grad(const float* w, const float* X, const float* y, int32_t n, float
 
 int dq_mlp_predict(const float* w, const float* X, uint64_t n, float* out) {
   return guard([&] { dq::mlp_predict(w, X, n, out); });
+}
+
+
+void dq_pb_opts_default(dq_pb_opts* o) {
+  *o = dq_pb_opts{};
+  o->max_memory_mb = 1024;
+  o->max_parallel_shots = 1;
+  o->sample_terminal = 1;
+  o->max_split_qubits = 12;
+}
+
+int dq_pb_check_opts(const dq_pb_opts* o) {
+  return guard([&] { (void)pb_options(o); });
+}
+
+int dq_pb_simulate(const dq_pb_opts* o, const char* json, size_t len, dq_pb_result** out) {
+  return guard([&] {
+    const dq::pb::Options p = pb_options(o);
+    auto* r = new dq_pb_result;
+    try {
+      r->r = dq::pb::simulate(p, json, len);
+    } catch (...) {
+      delete r;
+      throw;
+    }
+    *out = r;
+  });
+}
+
+int dq_pb_simulate_shots(const dq_pb_opts* o, const char* json, size_t len, uint64_t shots, int32_t profile,
+                         dq_count_cb cb, void* user) {
+  return guard([&] {
+    const dq::pb::Options p = pb_options(o);
+    dq::Counts c = dq::pb::simulate_shots(p, json, len, shots, profile != 0);
+    emit(c, cb, user);
+  });
+}
+
+void dq_pb_result_free(dq_pb_result* r) { delete r; }
+
+int dq_pb_result_json(const dq_pb_result* r, int32_t what, char** out) {
+  return guard([&] {
+    std::ostringstream js;
+    const dq::pb::Result& res = *r->r;
+    if (what == 0) {
+      js << "[";
+      for (size_t i = 0; i < res.physical.size(); ++i) js << (i ? ", " : "") << res.physical[i];
+      js << "]";
+    } else if (what == 1) {
+      js << "[";
+      for (size_t i = 0; i < res.cbits.size(); ++i)
+        js << (i ? ", " : "") << "[" << res.cbits[i].first << ", " << res.cbits[i].second << "]";
+      js << "]";
+    } else if (what == 2) {
+      js << "[";
+      auto bq = dq::pb::block_qubits(res);
+      for (size_t b = 0; b < bq.size(); ++b) {
+        js << (b ? ", " : "") << "[";
+        for (size_t i = 0; i < bq[b].size(); ++i) js << (i ? ", " : "") << bq[b][i];
+        js << "]";
+      }
+      js << "]";
+    } else if (what == 3) {
+      js << dq::pb::diagnostics_json(res, false);
+    } else if (what == 4) {
+      js << (dq::pb::profiled(res) ? dq::pb::diagnostics_json(res, true) : std::string("null"));
+    } else {
+      throw dq::ValueErr("unknown result field");
+    }
+    *out = dup_string(js.str());
+  });
+}
+
+int dq_pb_statevector_len(const dq_pb_result* r, uint64_t* len) {
+  return guard([&] { *len = dq::pb::statevector_len(*r->r); });
+}
+
+int dq_pb_statevector(const dq_pb_result* r, void* out, uint64_t len) {
+  return guard([&] {
+    if (len != dq::pb::statevector_len(*r->r)) throw dq::ValueErr("statevector length mismatch");
+    dq::pb::statevector(*r->r, out);
+  });
+}
+
+int dq_pb_amplitude(const dq_pb_result* r, const char* bits, size_t len, double* out) {
+  return guard([&] { dq::pb::amplitude(*r->r, std::string(bits, len), out); });
+}
+
+int dq_pb_probabilities(const dq_pb_result* r, const uint64_t* qubits, uint64_t k, int32_t all, char** out) {
+  return guard([&] {
+    std::vector<uint64_t> qs(qubits, qubits + k);
+    auto p = dq::pb::probabilities(*r->r, all ? nullptr : &qs);
+    std::ostringstream js;
+    js << "[";
+    char buf[40];
+    for (size_t i = 0; i < p.size(); ++i) {
+      std::snprintf(buf, sizeof buf, "%.17g", p[i].second);
+      js << (i ? ", " : "") << "[" << p[i].first << ", " << buf << "]";
+    }
+    js << "]";
+    *out = dup_string(js.str());
+  });
+}
+
+int dq_pb_counts(const dq_pb_result* r, uint64_t shots, const uint64_t* qubits, uint64_t k, int32_t all, uint64_t seed,
+                 dq_count_cb cb, void* user) {
+  return guard([&] {
+    std::vector<uint64_t> qs(qubits, qubits + k);
+    dq::Counts c = dq::pb::counts(*r->r, shots, all ? nullptr : &qs, seed);
+    emit(c, cb, user);
+  });
+}
+
+int dq_pb_expectation(const dq_pb_result* r, const char* pauli, size_t len, double* out) {
+  return guard([&] { *out = dq::pb::expectation_value(*r->r, std::string(pauli, len)); });
 }
 
 }  // extern "C"

@@ -662,4 +662,22 @@ std::vector<Op> compile(const std::vector<Instruction>& insts) {
   return result;
 }
 
+// Helpers shared with the P-block backend (pblock.cpp).
+std::vector<std::pair<uint64_t, uint64_t>> quantum_ranges(const Circuit& c) {
+  return register_ranges(c.qregs, "quantum");
+}
+void validate_circuit_rules(const Circuit& c) {
+  Ranges q = register_ranges(c.qregs, "quantum");
+  Ranges b = register_ranges(c.cregs, "classical");
+  for (size_t i = 0; i < c.instructions.size(); ++i) {
+    try {
+      validate_instruction(c.instructions[i], q, b);
+    } catch (const ValueErr& e) {
+      throw ValueErr("Instruction " + std::to_string(i) + ": " + e.msg);
+    }
+  }
+}
+bool one_qubit_matrix(const Instruction& in, uint32_t& q, Mat2& m) { return matrix_1q(in, q, m); }
+bool compile_instruction(const Instruction& in, Op& op) { return compile_one(in, op); }
+
 }  // namespace dq

@@ -93,6 +93,15 @@ struct Op {
 
 std::vector<Op> compile(const std::vector<Instruction>& insts);
 
+// Pieces of the above for the P-block backend: sorted quantum register
+// ranges, the reference validation without the statevector qubit-count
+// limit, gate_matrix_1q (src/types.rs:303-344) and one instruction compiled
+// alone (execution.rs:66-274 per-instruction rules; false for no-ops).
+std::vector<std::pair<uint64_t, uint64_t>> quantum_ranges(const Circuit& c);
+void validate_circuit_rules(const Circuit& c);
+bool one_qubit_matrix(const Instruction& in, uint32_t& q, Mat2& m);
+bool compile_instruction(const Instruction& in, Op& op);
+
 // Gate library.
 namespace gates {
 Mat2 X();

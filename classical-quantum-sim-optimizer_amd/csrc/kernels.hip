@@ -1948,5 +1948,276 @@ void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st)
 #undef L
 }
 
+// ---------------------------------------------------------------------------
+// P-block (product of dense blocks): complex128 block buffers. Complex
+// products and sums are written out as in the reference (no contraction).
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double2 pb_mul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double pb_nsq(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ uint64_t pb_insert(uint64_t i, uint32_t q, uint64_t bit) {
+  const uint64_t low = (1ull << q) - 1;
+  return (i & low) | ((i & ~low) << 1) | (bit << q);
+}
+
+// Merge (model.rs:199-244): out[r * nl + j] = left[j] * right[r].
+__global__ __launch_bounds__(TPB) void k_pb_kron(double2* __restrict__ out, const double2* __restrict__ left,
+                                                 uint64_t nl, const double2* __restrict__ right, uint64_t total) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t r = i / nl, j = i - r * nl;
+  out[i] = pb_mul(left[j], right[r]);
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_scale(double2* __restrict__ s, uint64_t count, double2 c) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < count) s[i] = pb_mul(s[i], c);
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_scale_real(double2* __restrict__ s, uint64_t count, double f) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < count) s[i] = make_double2(s[i].x * f, s[i].y * f);
+}
+
+// Amplitudes with local bit `q` equal to `bit`, in order (model.rs:486-489).
+__global__ __launch_bounds__(TPB) void k_pb_extract(double2* __restrict__ out, const double2* __restrict__ in,
+                                                    uint64_t half, uint32_t q, uint64_t bit) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < half) out[i] = in[pb_insert(i, q, bit)];
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_norms(double* __restrict__ out, const double2* __restrict__ s,
+                                                  uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i < count) out[i] = pb_nsq(s[i]);
+}
+
+// Separation test of local qubit q (model.rs:497-556), one workgroup: the
+// pivot pair of largest weight |x0|^2 + |x1|^2 (the last one among equals),
+// the normalised single-qubit factor s, the remaining amplitudes
+// conj(s0) x0 + conj(s1) x1 and the sums loss = sum |x - s value|^2,
+// total = sum |x|^2, kept = sum |value|^2 (pairwise, not sequential).
+// res: s0.re, s0.im, s1.re, s1.im, loss, total, kept, norm, pivot, bad.
+__global__ __launch_bounds__(TPB) void k_pb_separate(const double2* __restrict__ s, uint64_t half, uint32_t q,
+                                                     double2* __restrict__ rem, double* __restrict__ res) {
+  __shared__ double sw[TPB];
+  __shared__ uint64_t si[TPB];
+  __shared__ double2 single[2];
+  __shared__ double sums[3][TPB];
+  double bw = -1.0;
+  uint64_t bi = 0;
+  for (uint64_t i = threadIdx.x; i < half; i += TPB) {
+    const double w = pb_nsq(s[pb_insert(i, q, 0)]) + pb_nsq(s[pb_insert(i, q, 1)]);
+    if (w >= bw) {
+      bw = w;
+      bi = i;
+    }
+  }
+  sw[threadIdx.x] = bw;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int h = TPB / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < (unsigned)h) {
+      const double w2 = sw[threadIdx.x + h];
+      const uint64_t i2 = si[threadIdx.x + h];
+      if (w2 > sw[threadIdx.x] || (w2 == sw[threadIdx.x] && i2 > si[threadIdx.x])) {
+        sw[threadIdx.x] = w2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t p = si[0];
+    double2 a = s[pb_insert(p, q, 0)], b = s[pb_insert(p, q, 1)];
+    const double norm = sqrt(pb_nsq(a) + pb_nsq(b));
+    a = make_double2(a.x / norm, a.y / norm);
+    b = make_double2(b.x / norm, b.y / norm);
+    single[0] = a;
+    single[1] = b;
+    res[0] = a.x;
+    res[1] = a.y;
+    res[2] = b.x;
+    res[3] = b.y;
+    res[7] = norm;
+    res[8] = (double)p;
+    res[9] = (norm == 0.0 || !isfinite(norm)) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double2 c0 = make_double2(single[0].x, -single[0].y), c1 = make_double2(single[1].x, -single[1].y);
+  double loss = 0.0, total = 0.0, kept = 0.0;
+  for (uint64_t i = threadIdx.x; i < half; i += TPB) {
+    const double2 x0 = s[pb_insert(i, q, 0)], x1 = s[pb_insert(i, q, 1)];
+    const double2 p0 = pb_mul(c0, x0), p1 = pb_mul(c1, x1);
+    const double2 v = make_double2(p0.x + p1.x, p0.y + p1.y);
+    rem[i] = v;
+    const double2 m0 = pb_mul(single[0], v), m1 = pb_mul(single[1], v);
+    loss = loss + pb_nsq(make_double2(x0.x - m0.x, x0.y - m0.y));
+    total = total + pb_nsq(x0);
+    loss = loss + pb_nsq(make_double2(x1.x - m1.x, x1.y - m1.y));
+    total = total + pb_nsq(x1);
+    kept = kept + pb_nsq(v);
+  }
+  sums[0][threadIdx.x] = loss;
+  sums[1][threadIdx.x] = total;
+  sums[2][threadIdx.x] = kept;
+  __syncthreads();
+  for (int h = TPB / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < (unsigned)h)
+      for (int c = 0; c < 3; ++c) sums[c][threadIdx.x] = sums[c][threadIdx.x] + sums[c][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[4] = sums[0][0];
+    res[5] = sums[1][0];
+    res[6] = sums[2][0];
+  }
+}
+
+__device__ __forceinline__ uint64_t pb_index(uint64_t b, const PbTable& t, const uint32_t* __restrict__ shifts) {
+  uint64_t idx = 0;
+  for (uint32_t j = 0; j < t.k; ++j) idx |= ((b >> shifts[t.shift_off + j]) & 1ull) << j;
+  return idx;
+}
+
+// Product over blocks of one value each (result.rs:118-140, 179-195):
+// out[b] = prod_t data_t[index_t(b)], folded from 1 in table order.
+__global__ __launch_bounds__(TPB) void k_pb_product_c128(double2* __restrict__ out, uint64_t count,
+                                                         const PbTable* __restrict__ tabs, uint32_t ntab,
+                                                         const uint32_t* __restrict__ shifts) {
+  const uint64_t b = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (b >= count) return;
+  double2 acc = make_double2(1.0, 0.0);
+  for (uint32_t t = 0; t < ntab; ++t) acc = pb_mul(acc, ((const double2*)tabs[t].data)[pb_index(b, tabs[t], shifts)]);
+  out[b] = acc;
+}
+__global__ __launch_bounds__(TPB) void k_pb_product_f64(double* __restrict__ out, uint64_t count,
+                                                        const PbTable* __restrict__ tabs, uint32_t ntab,
+                                                        const uint32_t* __restrict__ shifts) {
+  const uint64_t b = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (b >= count) return;
+  double acc = 1.0;
+  for (uint32_t t = 0; t < ntab; ++t) acc = acc * ((const double*)tabs[t].data)[pb_index(b, tabs[t], shifts)];
+  out[b] = acc;
+}
+
+// Product sampling (model.rs:606-617): one Philox draw per block per shot,
+// draw = u * cdf[last], outcome = first j with cdf[j] > draw (clamped).
+__global__ __launch_bounds__(TPB) void k_pb_sample(uint32_t* __restrict__ out, uint64_t shots,
+                                                   const PbTable* __restrict__ tabs, uint32_t ntab, uint64_t seed,
+                                                   uint64_t stream0) {
+  const uint64_t sh = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (sh >= shots) return;
+  for (uint32_t t = 0; t < ntab; ++t) {
+    const double* cdf = (const double*)tabs[t].data;
+    const uint64_t len = tabs[t].len;
+    const double draw = uniform53(seed, stream0 + sh, t) * cdf[len - 1];
+    uint64_t lo = 0, hi = len;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (cdf[mid] <= draw) lo = mid + 1;
+      else hi = mid;
+    }
+    out[sh * ntab + t] = (uint32_t)(lo < len ? lo : len - 1);
+  }
+}
+
+// Pauli expectation of one block (result.rs:253-293): terms
+// conj(s[b ^ flip]) * phase(b) * s[b], partial sums per workgroup.
+__global__ __launch_bounds__(TPB) void k_pb_pauli(const double2* __restrict__ s, uint64_t size, uint64_t xmask,
+                                                  uint64_t ymask, uint64_t zmask, uint32_t nq,
+                                                  double2* __restrict__ partial) {
+  __shared__ double2 red[TPB];
+  double2 acc = make_double2(0.0, 0.0);
+  for (uint64_t b = (uint64_t)blockIdx.x * TPB + threadIdx.x; b < size; b += (uint64_t)gridDim.x * TPB) {
+    double2 phase = make_double2(1.0, 0.0);
+    for (uint32_t i = 0; i < nq; ++i) {
+      const uint64_t bit = (b >> i) & 1;
+      if ((ymask >> i) & 1) phase = pb_mul(phase, make_double2(0.0, bit == 0 ? 1.0 : -1.0));
+      else if (((zmask >> i) & 1) && bit == 1) phase = make_double2(-phase.x, -phase.y);
+    }
+    const double2 o = s[b ^ (xmask | ymask)];
+    const double2 t = pb_mul(pb_mul(make_double2(o.x, -o.y), phase), s[b]);
+    acc = make_double2(acc.x + t.x, acc.y + t.y);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = TPB / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < (unsigned)h)
+      red[threadIdx.x] = make_double2(red[threadIdx.x].x + red[threadIdx.x + h].x,
+                                      red[threadIdx.x].y + red[threadIdx.x + h].y);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_sum2(const double2* __restrict__ in, uint64_t count,
+                                                 double2* __restrict__ out) {
+  __shared__ double2 red[TPB];
+  double2 acc = make_double2(0.0, 0.0);
+  for (uint64_t i = threadIdx.x; i < count; i += TPB) acc = make_double2(acc.x + in[i].x, acc.y + in[i].y);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = TPB / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < (unsigned)h)
+      red[threadIdx.x] = make_double2(red[threadIdx.x].x + red[threadIdx.x + h].x,
+                                      red[threadIdx.x].y + red[threadIdx.x + h].y);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+}  // namespace
+
+void pb_kron(void* out, const void* left, uint64_t nl, const void* right, uint64_t nr, hipStream_t st) {
+  const uint64_t total = nl * nr;
+  hipLaunchKernelGGL(k_pb_kron, dim3(blocks_for(total, TPB)), dim3(TPB), 0, st, (double2*)out,
+                     (const double2*)left, nl, (const double2*)right, total);
+}
+void pb_scale(void* s, uint64_t count, double re, double im, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_scale, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, (double2*)s, count,
+                     make_double2(re, im));
+}
+void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_scale_real, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, (double2*)s, count, f);
+}
+void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_extract, dim3(blocks_for(half, TPB)), dim3(TPB), 0, st, (double2*)out,
+                     (const double2*)in, half, q, (uint64_t)bit);
+}
+void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_norms, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, out, (const double2*)s, count);
+}
+void pb_separate(const void* s, uint64_t half, uint32_t q, void* rem, double* res, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_separate, dim3(1), dim3(TPB), 0, st, (const double2*)s, half, q, (double2*)rem, res);
+}
+void pb_product_c128(void* out, uint64_t count, const PbTable* tabs, uint32_t ntab, const uint32_t* shifts,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_product_c128, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, (double2*)out, count, tabs,
+                     ntab, shifts);
+}
+void pb_product_f64(double* out, uint64_t count, const PbTable* tabs, uint32_t ntab, const uint32_t* shifts,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_product_f64, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, out, count, tabs, ntab,
+                     shifts);
+}
+void pb_sample(uint32_t* out, uint64_t shots, const PbTable* tabs, uint32_t ntab, uint64_t seed, uint64_t stream0,
+               hipStream_t st) {
+  if (shots == 0) return;
+  hipLaunchKernelGGL(k_pb_sample, dim3(blocks_for(shots, TPB)), dim3(TPB), 0, st, out, shots, tabs, ntab, seed,
+                     stream0);
+}
+uint32_t pb_pauli_blocks(uint64_t size) { return (uint32_t)std::min<uint64_t>(blocks_for(size, TPB), 1024); }
+void pb_pauli(const void* s, uint64_t size, uint64_t xmask, uint64_t ymask, uint64_t zmask, uint32_t nq,
+              void* partial, void* out, hipStream_t st) {
+  const uint32_t g = pb_pauli_blocks(size);
+  hipLaunchKernelGGL(k_pb_pauli, dim3(g), dim3(TPB), 0, st, (const double2*)s, size, xmask, ymask, zmask, nq,
+                     (double2*)partial);
+  hipLaunchKernelGGL(k_pb_sum2, dim3(1), dim3(TPB), 0, st, (const double2*)partial, (uint64_t)g, (double2*)out);
+}
+
 }  // namespace k
 }  // namespace dq

@@ -219,5 +219,38 @@ void gather(int prec, const void* s, const uint64_t* idx, uint64_t count, void* 
 // ---- conversion ----
 void to_c128(int prec, const void* s, uint64_t count, void* out, hipStream_t st);
 
+// ---- P-block: product of dense complex128 blocks ----
+// Merge: out[r * nl + j] = left[j] * right[r].
+void pb_kron(void* out, const void* left, uint64_t nl, const void* right, uint64_t nr, hipStream_t st);
+void pb_scale(void* s, uint64_t count, double re, double im, hipStream_t st);  // s[i] *= (re, im)
+void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st);         // s[i] *= f
+// out[i] = in[i with a bit `bit` inserted at local position q], i < half.
+void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st);
+void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st);  // |s[i]|^2
+// Separation test of local qubit q (one workgroup); writes the remaining
+// amplitudes to rem and res[10] = s0.re, s0.im, s1.re, s1.im, loss, total,
+// kept, norm, pivot, bad-norm flag.
+void pb_separate(const void* s, uint64_t half, uint32_t q, void* rem, double* res, hipStream_t st);
+// One factor table of a product: index bit j of an output index b is bit
+// shifts[shift_off + j] of b; len entries (double2 or double).
+struct PbTable {
+  const void* data;
+  uint64_t len;
+  uint32_t k, shift_off;
+};
+void pb_product_c128(void* out, uint64_t count, const PbTable* tabs_dev, uint32_t ntab, const uint32_t* shifts_dev,
+                     hipStream_t st);
+void pb_product_f64(double* out, uint64_t count, const PbTable* tabs_dev, uint32_t ntab, const uint32_t* shifts_dev,
+                    hipStream_t st);
+// Product sampling from per-table CDFs: out[s * ntab + t] = outcome index of
+// table t in shot s (Philox stream stream0 + s, draw t).
+void pb_sample(uint32_t* out, uint64_t shots, const PbTable* tabs_dev, uint32_t ntab, uint64_t seed, uint64_t stream0,
+               hipStream_t st);
+// Pauli expectation of one block: partial needs pb_pauli_blocks(size)
+// double2, out receives the block's complex sum.
+uint32_t pb_pauli_blocks(uint64_t size);
+void pb_pauli(const void* s, uint64_t size, uint64_t xmask, uint64_t ymask, uint64_t zmask, uint32_t nq,
+              void* partial, void* out, hipStream_t st);
+
 }  // namespace k
 }  // namespace dq

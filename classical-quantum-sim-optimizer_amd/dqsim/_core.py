@@ -513,22 +513,6 @@ class StabilizerResult:
     __init__ = _not_here("StabilizerResult")
 
 
-class PBlockSimulator:
-    __init__ = _not_here("PBlockSimulator")
-
-
-class PBlockResult:
-    __init__ = _not_here("PBlockResult")
-
-
-def simulate_distributed(distributed, mode="p_block", seed=None, profile=False, **options):
-    raise NotImplementedError("The P-block backend is not part of this dense statevector engine")
-
-
-def simulate_distributed_shots(distributed, mode="p_block", shots=1000, seed=None, profile=False, **options):
-    raise NotImplementedError("The P-block backend is not part of this dense statevector engine")
-
-
 # ---------------------------------------------------------------------------
 # Host-only helpers
 # ---------------------------------------------------------------------------
@@ -578,3 +562,8 @@ __all__ = [
     "StabilizerSimulator", "StabilizerResult", "StatevectorSimulator", "simulate_distributed",
     "simulate_distributed_shots", "simulate_monolithic", "simulate_monolithic_shots",
 ]
+
+
+# P-block backend (product of dense blocks): defined in _pblock, which binds
+# its own C entry points on this module's library handle.
+from ._pblock import PBlockResult, PBlockSimulator, simulate_distributed, simulate_distributed_shots  # noqa: E402

@@ -153,6 +153,48 @@ int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion
                  int32_t tile_qubits, char** out);
 void dq_string_free(char* s);
 
+/* ---- P-block backend: the state as a product of dense blocks ----
+ * Replaces PBlockSimulator / PBlockResult (src/distributed/pblock/
+ * simulator.rs:24-322, result.rs:21-293). Inputs are circuit JSON in the
+ * transport schema; a distributed input is first merged into one circuit by
+ * the host wrapper (input.rs:54-224), with one quantum register per
+ * physical qubit. */
+typedef struct dq_pb_result dq_pb_result;
+typedef struct dq_pb_opts {
+  uint64_t max_memory_mb;        /* default 1024 */
+  int32_t has_max_block_qubits;  /* 0: no limit */
+  uint64_t max_block_qubits;
+  uint64_t max_parallel_shots;   /* default 1 */
+  int32_t sample_terminal;       /* default 1 */
+  int32_t split_separable;       /* default 0 */
+  uint64_t max_split_qubits;     /* default 12 */
+  double separation_tolerance;   /* default 0 */
+  int32_t profile;
+  int32_t has_seed;
+  uint64_t seed;
+} dq_pb_opts;
+void dq_pb_opts_default(dq_pb_opts* o);
+/* The constructor's checks (simulator.rs:58-95): DQ_EVALUE with its message. */
+int dq_pb_check_opts(const dq_pb_opts* o);
+int dq_pb_simulate(const dq_pb_opts* o, const char* json, size_t len, dq_pb_result** out);
+int dq_pb_simulate_shots(const dq_pb_opts* o, const char* json, size_t len, uint64_t shots, int32_t profile,
+                         dq_count_cb cb, void* user);
+void dq_pb_result_free(dq_pb_result* r);
+/* JSON text of a result field; free with dq_string_free. what: 0 physical
+ * qubits, 1 classical bits [[cbit, value]...], 2 block qubits (physical
+ * labels), 3 diagnostics, 4 profile (diagnostics + total_time; "null" when
+ * the simulator did not profile). */
+int dq_pb_result_json(const dq_pb_result* r, int32_t what, char** out);
+int dq_pb_statevector_len(const dq_pb_result* r, uint64_t* len);  /* DQ_EMEMORY beyond the budget */
+int dq_pb_statevector(const dq_pb_result* r, void* out_c128, uint64_t len);
+int dq_pb_amplitude(const dq_pb_result* r, const char* bits, size_t len, double* out_re_im);
+/* [[index, probability]...] for probabilities > 0; all != 0: every qubit,
+ * highest label first. */
+int dq_pb_probabilities(const dq_pb_result* r, const uint64_t* qubits, uint64_t k, int32_t all, char** out_json);
+int dq_pb_counts(const dq_pb_result* r, uint64_t shots, const uint64_t* qubits, uint64_t k, int32_t all,
+                 uint64_t seed, dq_count_cb cb, void* user);
+int dq_pb_expectation(const dq_pb_result* r, const char* pauli, size_t len, double* out);
+
 /* Process-wide switch for relabelled tile sweeps (default off): a sweep may
  * store its tile with some bit positions exchanged so that the next sweep's
  * contiguous low positions hold the qubits it needs; every run of sweeps
