@@ -466,6 +466,49 @@ int64_t or_compile_kinds(const or_inst* insts, int64_t count, int32_t* kinds, in
   return n;
 }
 
+/* One instruction compiled alone, for the P-block oracle (pblock.py):
+ * execution.rs's per-instruction rules without the statevector's
+ * single-qubit fusion. Returns 1 and fills the descriptor (matrix entries
+ * re, im interleaved into mat: One 4, Diagonal 2^k, Dense 4^k), 0 for a
+ * no-op, -1 on error. Conditions are those of the flattened instruction. */
+
+int or_compile_one(const or_inst* in, or_opdesc* d, double* mat, int64_t mat_cap) {
+  op_t o;
+  memset(&o, 0, sizeof o);
+  const int r = compile_inst(in, &o);
+  if (r < 0) return -1;
+  if (r == 1) return 0;
+  memset(d, 0, sizeof *d);
+  d->kind = o.kind;
+  d->target = o.target;
+  d->a = o.a;
+  d->b = o.b;
+  d->qubit = o.qubit;
+  d->cbit = o.cbit;
+  d->nctl = o.nctl;
+  for (int i = 0; i < 5; ++i) {
+    d->cq[i] = o.cq[i];
+    d->cv[i] = o.cv[i];
+    d->qubits[i] = o.qubits[i];
+  }
+  d->k = o.k;
+  int nm = 0;
+  if (o.kind == K_ONE) nm = 4;
+  else if (o.kind == K_DIAG) nm = 1 << o.k;
+  else if (o.kind == K_DENSE) nm = 1 << (2 * o.k);
+  d->nmat = nm;
+  if (nm > mat_cap / 2) {
+    free(o.mat);
+    return -1;
+  }
+  for (int i = 0; i < nm; ++i) {
+    mat[2 * i] = o.mat[i].re;
+    mat[2 * i + 1] = o.mat[i].im;
+  }
+  free(o.mat);
+  return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Kernels (src/engine.rs:64-453)                                           */
 /* ------------------------------------------------------------------------ */

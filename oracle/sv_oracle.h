@@ -78,6 +78,20 @@ int or_gate_1q(int kind, double theta, double phi, double lam, double out[8]);
 /* Number of compiled operations and their kinds (for compile tests):
  * kinds: 0 one,1 x,2 swap,3 diagonal,4 dense,5 measure,6 reset,7 conditional. */
 int64_t or_compile_kinds(const or_inst* insts, int64_t count, int32_t* kinds, int64_t cap);
+/* One instruction compiled alone (execution.rs rules, no 1q fusion), for
+ * the P-block restatement: 1 compiled, 0 no-op, -1 error. */
+typedef struct {
+  int32_t kind; /* 0 one, 1 x, 2 swap, 3 diagonal, 4 dense, 5 measure, 6 reset, 7 no-op */
+  uint32_t target, a, b, qubit;
+  uint64_t cbit;
+  int32_t nctl;
+  uint32_t cq[5];
+  uint8_t cv[5];
+  int32_t k;
+  uint32_t qubits[5];
+  int32_t nmat;
+} or_opdesc;
+int or_compile_one(const or_inst* in, or_opdesc* d, double* mat, int64_t mat_cap);
 
 /* simulate(): one trajectory from |0..0>, RNG stream 0.  state: 2^n complex128. */
 int or_simulate(const or_inst* insts, int64_t count, int n, uint64_t nc, uint64_t seed, double* state,
