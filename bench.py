@@ -160,6 +160,43 @@ def brickwork_cpu_baseline(shots=1000, seed=42, repeats=5):
     return out
 
 
+def pblock_cpu_baseline():
+    """The CPU P-block restatement (oracle/pblock.py: Python bookkeeping over
+    the C FP64 kernels, OpenMP inside the kernels) on the same cases, one
+    timed run each after a warm-up of the smallest case."""
+    import os
+
+    from dqsim import reports
+    from oracle import oracle, pblock
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    out = {"unit": "s per case", "cores": threads, "kind": "port", "seconds": {}}
+    cases = reports.pblock_cases()
+    for name, opts, circ, shots in cases:
+        data = pblock_transport_data(circ)
+        kw = {"memory_mb": opts.get("max_memory_mb", 1024), "sample_terminal": opts.get("sample_terminal", True)}
+        t0 = time.perf_counter()
+        counts, strategy = pblock.simulate_shots(data, shots, opts["seed"], **kw)
+        out["seconds"][name] = time.perf_counter() - t0
+        log(f"[pblock-cpu] {name}: {out['seconds'][name] * 1e3:.1f} ms ({strategy})")
+        if sum(counts.values()) != shots:
+            raise RuntimeError(name)
+    out["sample"] = "every case of the P-block report once"
+    return out
+
+
+def pblock_transport_data(circ) -> dict:
+    """Plain circuit JSON of a report case (distributed inputs merged by the
+    engine's transport, whose qubit labels are already 0..n-1 here)."""
+    from dqsim import _pblock
+
+    data = json.loads(_pblock._transport(circ, 1 << 40))
+    n = sum(r["size"] for r in data["qregs"].values())
+    data["qregs"] = {"q": {"name": "q", "base": 0, "size": n}}
+    return data
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,7 +217,8 @@ def main():
     ap.add_argument("--cpu-depth", type=int, default=40)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--report", choices=["precision", "batch", "planner", "brickwork", "batch-loop"], default=None,
+    ap.add_argument("--report", choices=["precision", "batch", "planner", "brickwork", "batch-loop", "pblock"],
+                    default=None,
                     help="run one secondary report instead of the headline benchmark")
     args = ap.parse_args()
     if args.report:
@@ -188,10 +226,12 @@ def main():
 
         fn = {"precision": reports.precision_report, "batch": reports.batch_report,
               "planner": reports.planner_report, "brickwork": reports.brickwork_report,
-              "batch-loop": reports.batch_loop_report}[args.report]
+              "batch-loop": reports.batch_loop_report, "pblock": reports.pblock_report}[args.report]
         res = fn(log=log)
         if args.report == "brickwork":
             res["cpu_baseline"] = brickwork_cpu_baseline()
+        if args.report == "pblock":
+            res["cpu_baseline"] = pblock_cpu_baseline()
         print(json.dumps({"report": args.report, **res}), flush=True)
         return
     if args.copy_calibration:

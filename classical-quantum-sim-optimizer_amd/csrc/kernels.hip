@@ -1989,6 +1989,13 @@ __global__ __launch_bounds__(TPB) void k_pb_extract(double2* __restrict__ out, c
   if (i < half) out[i] = in[pb_insert(i, q, bit)];
 }
 
+__global__ void k_pb_set2(double2* __restrict__ s, double2 a, double2 b) {
+  if (threadIdx.x == 0) {
+    s[0] = a;
+    s[1] = b;
+  }
+}
+
 __global__ __launch_bounds__(TPB) void k_pb_norms(double* __restrict__ out, const double2* __restrict__ s,
                                                   uint64_t count) {
   const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
@@ -2187,6 +2194,10 @@ void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st) {
 void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_extract, dim3(blocks_for(half, TPB)), dim3(TPB), 0, st, (double2*)out,
                      (const double2*)in, half, q, (uint64_t)bit);
+}
+void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_set2, dim3(1), dim3(64), 0, st, (double2*)s, make_double2(a_re, a_im),
+                     make_double2(b_re, b_im));
 }
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_norms, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, out, (const double2*)s, count);

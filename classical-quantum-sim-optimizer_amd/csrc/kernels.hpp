@@ -227,6 +227,7 @@ void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st);         /
 // out[i] = in[i with a bit `bit` inserted at local position q], i < half.
 void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st);
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st);  // |s[i]|^2
+void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st);  // s[0], s[1]
 // Separation test of local qubit q (one workgroup); writes the remaining
 // amplitudes to rem and res[10] = s0.re, s0.im, s1.re, s1.im, loss, total,
 // kept, norm, pivot, bad-norm flag.

@@ -340,10 +340,11 @@ class Pool {
     b->qubits = std::move(qubits);
     return b;
   }
+  // A single-wire block holding (v4[0] + i v4[1], v4[2] + i v4[3]), written
+  // by a kernel (no host copy to wait for).
   std::unique_ptr<Block> basis_block(uint64_t q, const double* v4) {
     auto b = new_block(2, {q});
-    PBCHK(hipMemcpyAsync(b->amps, v4, 32, hipMemcpyHostToDevice, st));
-    sync();
+    k::pb_set2(b->amps, v4[0], v4[1], v4[2], v4[3], st);
     return b;
   }
 
@@ -679,7 +680,9 @@ class Pool {
     blocks[slot] = basis_block(q, single4);
     positions[q] = {slot, 0};
     stats.split_calls += 1;
-    refresh_known(q);
+    // refresh_known(q) on the factor just written, from its host values.
+    const cplx a0(single4[0], single4[1]), a1(single4[2], single4[3]), zero(0, 0);
+    known[q] = ceq(a1, zero) ? 0 : ceq(a0, zero) ? 1 : -1;
     for (uint64_t w : qs) refresh_known(w);
     record();
   }
@@ -690,8 +693,8 @@ class Pool {
     const int n = (int)b.nq();
     const int seg1 = n < 11 ? n : 11;
     const uint64_t c1 = 1ull << (n - seg1);
-    red_a.alloc(16 * c1);
-    red_b.alloc(16 * c1);
+    if (red_a.size() < 16 * c1) red_a.alloc(16 * c1);
+    if (red_b.size() < 16 * c1) red_b.alloc(16 * c1);
     k::ProbSpec ps;
     ps.branch = branch;
     k::seg_reduce_prob(16, b.amps, b.size, 1, n, ps, seg1, red_a.get(), c1, st);

@@ -227,3 +227,72 @@ def batch_loop_report(count: int = 4096, shots: int = 1024, base_seed: int = 202
            "path": "one simulate_shots call per circuit (complex64, greedy <=3-qubit fusion, tile sweeps)"}
     log(f"[batch-loop] {out}")
     return out
+
+
+# ---------------------------------------------------------------------------
+# P-block backend: the reference's benchmarks/pblock_benchmark.py cases
+# ---------------------------------------------------------------------------
+def pblock_cases():
+    """(name, simulator options, circuit or distributed input, shots): the six
+    cases of the reference's P-block benchmark (max_memory_mb=64, seed 71)
+    and one 24-qubit entangled block (GHZ chain then a U3 layer) sampled 1000
+    times, where each gate updates 2^24 amplitudes."""
+    from types import SimpleNamespace
+
+    def pairs(n):
+        return [op for q in range(0, n, 2) for op in ({"kind": "h", "qubit": q},
+                                                      {"kind": "cx", "control": q, "target": q + 1})]
+
+    def meas(n):
+        return [{"kind": "measure", "qubit": q, "cbit": q} for q in range(n)]
+
+    def distributed(n, ops, singleton):
+        groups = {q: [q] for q in range(n)} if singleton else {0: list(range(n))}
+        buckets = {node: [] for node in groups}
+        for op in ops:
+            q = op.get("qubit", op.get("control", 0))
+            buckets[q if singleton else 0].append(op)
+        circuits = {node: workloads.Circuit(n, b, n) for node, b in buckets.items()}
+        for c in circuits.values():
+            c.instructions = c.data["instructions"]
+        return SimpleNamespace(circuits=circuits, qubits_per_node=groups,
+                               _instruction_index={id(op): i for i, op in enumerate(ops)})
+
+    product = [{"kind": "h", "qubit": q} for q in range(16)] + meas(16)
+    dynamic = pairs(20) + [{"kind": "measure", "qubit": 0, "cbit": 0},
+                           {"kind": "conditional", "condition": {"creg_base": 0, "creg_size": 1, "creg_value": 1},
+                            "op": {"kind": "x", "qubit": 1}}, {"kind": "reset", "qubit": 2}] + meas(20)
+    rng = np.random.default_rng(71)
+    ghz = [{"kind": "h", "qubit": 0}] + [{"kind": "cx", "control": q, "target": q + 1} for q in range(23)]
+    ghz += [{"kind": "u3", "qubit": q, "theta": float(rng.uniform(0, np.pi)), "phi": float(rng.uniform(0, 6.28)),
+             "lam": float(rng.uniform(0, 6.28))} for q in range(24)] + meas(24)
+    base = {"seed": 71, "max_memory_mb": 64}
+    return [
+        ("product_16q_one_node_16shots", base, distributed(16, product, False), 16),
+        ("product_16q_sixteen_nodes_16shots", base, distributed(16, product, True), 16),
+        ("bell_pairs_20q_terminal_1000shots", base, workloads.Circuit(20, pairs(20) + meas(20), 20), 1000),
+        ("bell_pairs_20q_forced_1000shots", dict(base, sample_terminal=False),
+         workloads.Circuit(20, pairs(20) + meas(20), 20), 1000),
+        ("bell_pairs_100q_1000shots", base, workloads.Circuit(100, pairs(100) + meas(100), 100), 1000),
+        ("dynamic_20q_100shots", base, workloads.Circuit(20, dynamic, 20), 100),
+        ("ghz_u3_24q_block_1000shots", {"seed": 71, "max_memory_mb": 2048}, workloads.Circuit(24, ghz, 24), 1000),
+    ]
+
+
+def pblock_report(repeats: int = 5, log=print) -> dict:
+    """Median wall seconds of PBlockSimulator.simulate_shots per case (one
+    warm-up call first), as the reference's benchmark reports them."""
+    out = {"repeats": repeats, "seconds": {}}
+    for name, opts, circ, shots in pblock_cases():
+        sim = _core.PBlockSimulator(**opts)
+        sim.simulate_shots(circ, shots)
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            counts = sim.simulate_shots(circ, shots)
+            times.append(time.perf_counter() - t0)
+            if sum(counts.values()) != shots:
+                raise RuntimeError(f"{name}: {sum(counts.values())} shots returned")
+        out["seconds"][name] = {"median": float(np.median(times)), "samples": times}
+        log(f"[pblock] {name}: median {np.median(times) * 1e3:.2f} ms")
+    return out
