@@ -229,7 +229,7 @@ double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   const uint32_t grid = (uint32_t)cus * 8;  // 8 waves per SIMD
   const int iters = 20000;
   DeviceMem out;
-  out.alloc((size_t)grid * 256 * sizeof(float));
+  out.alloc((size_t)k::mfma_peak_out_floats(grid) * sizeof(float));
   Stream st;
   k::mfma_peak(out.as<float>(), grid, iters, st.s);  // warm-up
   Event a, b;

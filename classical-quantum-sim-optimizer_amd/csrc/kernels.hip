@@ -1757,6 +1757,8 @@ __global__ __launch_bounds__(TPB) void k_mfma_peak(float* __restrict__ out, int 
 }
 }  // namespace
 
+uint64_t mfma_peak_out_floats(uint32_t grid) { return (uint64_t)grid * TPB; }
+
 uint64_t mfma_peak_flops_per_launch(uint32_t grid, int iters) {
   // 2 * 16 * 16 * 4 flops per instruction, 8 per round, one wave per 64 threads.
   return (uint64_t)grid * (TPB / 64) * (uint64_t)iters * 8ull * 2048ull;

@@ -158,8 +158,10 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
 
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
-// FP32 matrix-core peak microbenchmark (`out` holds grid * 256 floats).
+// FP32 matrix-core peak microbenchmark (`out` holds mfma_peak_out_floats(grid)
+// floats: one per thread).
 void mfma_peak(float* out, uint32_t grid, int iters, hipStream_t st);
+uint64_t mfma_peak_out_floats(uint32_t grid);
 uint64_t mfma_peak_flops_per_launch(uint32_t grid, int iters);
 void broadcast(int prec, void* dst, uint64_t stride, uint32_t batch, const void* src, int n, hipStream_t st);
 
