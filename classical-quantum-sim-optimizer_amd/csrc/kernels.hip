@@ -1998,6 +1998,34 @@ __global__ void k_pb_set2(double2* __restrict__ s, double2 a, double2 b) {
   }
 }
 
+// Batched forms (slot b at b * stride): merge, and the measured half and
+// basis factor of each slot's outcome (read from its MeasState).
+__global__ __launch_bounds__(TPB) void k_pb_kron_b(double2* __restrict__ out, const double2* __restrict__ left,
+                                                   uint64_t nl, const double2* __restrict__ right, uint64_t total,
+                                                   uint64_t nr) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t b = blockIdx.y;
+  const uint64_t r = i / nl, j = i - r * nl;
+  out[b * total + i] = pb_mul(left[b * nl + j], right[b * nr + r]);
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_extract_ms(double2* __restrict__ out, const double2* __restrict__ in,
+                                                       uint64_t half, uint32_t q, const MeasState* __restrict__ ms) {
+  const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+  const uint64_t b = blockIdx.y;
+  if (i < half) out[b * half + i] = in[2 * half * b + pb_insert(i, q, (uint64_t)ms[b].outcome)];
+}
+
+__global__ __launch_bounds__(TPB) void k_pb_basis_ms(double2* __restrict__ out, const MeasState* __restrict__ ms,
+                                                     uint32_t batch) {
+  const uint32_t b = blockIdx.x * TPB + threadIdx.x;
+  if (b >= batch) return;
+  const int v = ms[b].outcome;
+  out[2 * (uint64_t)b] = make_double2(v == 0 ? 1.0 : 0.0, 0.0);
+  out[2 * (uint64_t)b + 1] = make_double2(v == 1 ? 1.0 : 0.0, 0.0);
+}
+
 __global__ __launch_bounds__(TPB) void k_pb_norms(double* __restrict__ out, const double2* __restrict__ s,
                                                   uint64_t count) {
   const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
@@ -2196,6 +2224,20 @@ void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st) {
 void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_extract, dim3(blocks_for(half, TPB)), dim3(TPB), 0, st, (double2*)out,
                      (const double2*)in, half, q, (uint64_t)bit);
+}
+void pb_kron_batch(void* out, const void* left, uint64_t nl, const void* right, uint64_t nr, uint32_t batch,
+                   hipStream_t st) {
+  const uint64_t total = nl * nr;
+  hipLaunchKernelGGL(k_pb_kron_b, dim3(blocks_for(total, TPB), batch), dim3(TPB), 0, st, (double2*)out,
+                     (const double2*)left, nl, (const double2*)right, total, nr);
+}
+void pb_extract_ms(void* out, const void* in, uint64_t half, uint32_t q, const MeasState* ms, uint32_t batch,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_extract_ms, dim3(blocks_for(half, TPB), batch), dim3(TPB), 0, st, (double2*)out,
+                     (const double2*)in, half, q, ms);
+}
+void pb_basis_ms(void* out, const MeasState* ms, uint32_t batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_pb_basis_ms, dim3(blocks_for(batch, TPB)), dim3(TPB), 0, st, (double2*)out, ms, batch);
 }
 void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_set2, dim3(1), dim3(64), 0, st, (double2*)s, make_double2(a_re, a_im),

@@ -228,6 +228,14 @@ void pb_scale(void* s, uint64_t count, double re, double im, hipStream_t st);  /
 void pb_scale_real(void* s, uint64_t count, double f, hipStream_t st);         // s[i] *= f
 // out[i] = in[i with a bit `bit` inserted at local position q], i < half.
 void pb_extract(void* out, const void* in, uint64_t half, uint32_t q, uint32_t bit, hipStream_t st);
+// Batched (slot b at b * its size): merge of each slot's pair of blocks;
+// half of each slot's block at the slot's measured outcome (ms[b].outcome);
+// each slot's one-qubit basis factor |outcome>.
+void pb_kron_batch(void* out, const void* left, uint64_t nl, const void* right, uint64_t nr, uint32_t batch,
+                   hipStream_t st);
+void pb_extract_ms(void* out, const void* in, uint64_t half, uint32_t q, const MeasState* ms, uint32_t batch,
+                   hipStream_t st);
+void pb_basis_ms(void* out, const MeasState* ms, uint32_t batch, hipStream_t st);
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st);  // |s[i]|^2
 void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st);  // s[0], s[1]
 // Separation test of local qubit q (one workgroup); writes the remaining

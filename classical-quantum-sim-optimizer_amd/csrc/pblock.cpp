@@ -853,6 +853,489 @@ class Pool {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Lockstep trajectories. `batch` shots share one block structure: block i
+// keeps batch slots of blocks[i]->size amplitudes (slot b at amps + 2 * size
+// * b), so each gate, merge and measurement is one launch for every shot. The
+// host decisions a lone pool takes from its state (known controls, trivial
+// controlled gates, reduced diagonals, merges, extractions) are taken once
+// for the batch; a shot whose own decision differs from the batch's leaves it
+// (`alive` false) and is run again afterwards on a pool of its own from the
+// start. Shot s draws from Philox stream s in both places, so every count is
+// the one a lone trajectory gives. Slots that left keep evolving unitarily
+// (or are flagged off) and are never read.
+class BatchPool {
+ public:
+  using Mask = std::vector<char>;  // per slot: the current op applies
+  uint32_t B;
+  std::vector<std::unique_ptr<Block>> blocks;
+  std::vector<std::pair<uint64_t, uint64_t>> positions;
+  std::vector<uint64_t> free;
+  uint64_t live = 0;
+  std::vector<int8_t> known;  // slot-major, B x wires
+  Mask alive;
+  std::vector<std::map<uint64_t, int>> cbits;
+  Options opt;
+  hipStream_t st;
+  DeviceMem rng, ms, err, red_a, red_b, flags;
+  std::vector<std::shared_ptr<DeviceMem>> mats;
+  std::map<const Op*, const void*> mat_of;
+  std::vector<std::shared_ptr<DeviceMem>> graveyard;
+  std::vector<std::shared_ptr<std::vector<int>>> host_keep;  // sources of queued uploads
+
+  // Every slot starts as `base`; slot b is shot stream0 + b.
+  BatchPool(const Pool& base, uint32_t batch, const Options& o, uint64_t seed, uint64_t stream0)
+      : B(batch), positions(base.positions), free(base.free), live(base.live), alive(batch, 1), cbits(batch),
+        opt(o), st(base.st), mats(base.mats), mat_of(base.mat_of) {
+    const uint64_t n = positions.size();
+    for (auto& b : base.blocks) {
+      if (!b) {
+        blocks.emplace_back();
+        continue;
+      }
+      auto c = new_block(b->size, b->qubits);
+      k::broadcast(16, c->amps, b->size, B, b->amps, (int)b->nq(), st);
+      blocks.push_back(std::move(c));
+    }
+    known.resize((size_t)B * n);
+    for (uint32_t s = 0; s < B; ++s) std::copy(base.known.begin(), base.known.end(), known.begin() + (size_t)s * n);
+    rng.alloc(sizeof(k::RngState) * B);
+    ms.alloc(sizeof(k::MeasState) * B);
+    err.alloc(sizeof(int));
+    flags.alloc(sizeof(int) * B);
+    PBCHK(hipMemsetAsync(err.get(), 0, sizeof(int), st));
+    PBCHK(hipMemsetAsync(ms.get(), 0, sizeof(k::MeasState) * B, st));
+    k::rng_init(rng.as<k::RngState>(), B, seed, stream0, st);
+  }
+
+  void sync() {
+    PBCHK(hipStreamSynchronize(st));
+    graveyard.clear();
+    host_keep.clear();
+  }
+  uint64_t wires() const { return positions.size(); }
+  int8_t& kn(uint32_t s, uint64_t q) { return known[(size_t)s * wires() + q]; }
+  uint64_t bytes() const { return live * 16 + wires() * 256; }
+  void reserve(uint64_t extra) const {
+    if (add_checked(bytes(), extra) > opt.memory) fail("P-block working allocation exceeds max_memory_mb");
+  }
+  std::unique_ptr<Block> new_block(uint64_t size, std::vector<uint64_t> qubits) const {
+    auto b = std::make_unique<Block>();
+    b->mem = std::make_shared<DeviceMem>(16 * size * B);
+    b->amps = b->mem->as<double>();
+    b->size = size;
+    b->qubits = std::move(qubits);
+    return b;
+  }
+
+  bool any(const Mask& m) const { return std::find(m.begin(), m.end(), 1) != m.end(); }
+  bool covers(const Mask& m) const {
+    for (uint32_t s = 0; s < B; ++s)
+      if (alive[s] && !m[s]) return false;
+    return true;
+  }
+  // Slots of m whose decision differs from the first one's leave the batch.
+  template <class D>
+  void agree(Mask& m, const std::vector<D>& d) {
+    int r = -1;
+    for (uint32_t s = 0; s < B; ++s) {
+      if (!m[s]) continue;
+      if (r < 0) r = (int)s;
+      else if (!(d[s] == d[r])) m[s] = alive[s] = 0;
+    }
+  }
+  int first(const Mask& m) const {
+    for (uint32_t s = 0; s < B; ++s)
+      if (m[s]) return (int)s;
+    return -1;
+  }
+  // A structural change applies to every slot or to none: the slots on the
+  // other side of the first live slot leave. Returns whether it applies.
+  bool unanimous(Mask& m) {
+    const int r = first(alive);
+    const bool apply = r >= 0 && m[r];
+    for (uint32_t s = 0; s < B; ++s) {
+      if (alive[s] && (bool)m[s] != apply) alive[s] = 0;
+      m[s] = apply && alive[s];
+    }
+    return apply;
+  }
+  k::Flag flag(const Mask& m) {
+    if (covers(m)) return k::Flag{};
+    auto h = std::make_shared<std::vector<int>>(m.begin(), m.end());
+    PBCHK(hipMemcpyAsync(flags.get(), h->data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+    host_keep.push_back(h);
+    return k::Flag{flags.as<int>(), 1};
+  }
+  // (a0, a1) of every slot of a one-wire block.
+  std::vector<cplx> read_single(const Block& b) {
+    std::vector<double> v(4 * (size_t)B);
+    PBCHK(hipMemcpyAsync(v.data(), b.amps, 32 * (size_t)B, hipMemcpyDeviceToHost, st));
+    sync();
+    std::vector<cplx> out(2 * (size_t)B);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = cplx(v[2 * i], v[2 * i + 1]);
+    return out;
+  }
+  // Pool::refresh_known for the slots of m.
+  void refresh(const std::vector<uint64_t>& ws, const Mask& m) {
+    for (uint64_t w : ws) {
+      const Block& b = *blocks[positions[w].first];
+      if (b.nq() != 1) {
+        for (uint32_t s = 0; s < B; ++s)
+          if (m[s]) kn(s, w) = -1;
+        continue;
+      }
+      const std::vector<cplx> v = read_single(b);
+      const cplx zero(0, 0);
+      for (uint32_t s = 0; s < B; ++s)
+        if (m[s]) kn(s, w) = ceq(v[2 * s + 1], zero) ? 0 : ceq(v[2 * s], zero) ? 1 : -1;
+    }
+  }
+
+  uint64_t merge(uint64_t a, uint64_t b) {
+    if (a == b) return a;
+    const uint64_t n = blocks[a]->nq() + blocks[b]->nq();
+    if (opt.has_max_block && n > opt.max_block_qubits)
+      fail("P-block merge exceeds max_block_qubits; no entanglement was discarded");
+    const uint64_t dim = dimension(n);
+    reserve(nbytes(dim, 16));
+    const Block& left = *blocks[a];
+    const Block& right = *blocks[b];
+    std::vector<uint64_t> qubits = left.qubits;
+    qubits.insert(qubits.end(), right.qubits.begin(), right.qubits.end());
+    auto out = new_block(dim, qubits);
+    k::pb_kron_batch(out->amps, left.amps, left.size, right.amps, right.size, B, st);
+    live = live - left.size - right.size + dim;
+    for (size_t i = 0; i < qubits.size(); ++i) positions[qubits[i]] = {a, i};
+    graveyard.push_back(blocks[a]->mem);
+    graveyard.push_back(blocks[b]->mem);
+    blocks[a] = std::move(out);
+    blocks[b].reset();
+    free.push_back(b);
+    return a;
+  }
+  bool needs_merge(const std::vector<uint64_t>& qs) const {
+    for (uint64_t q : qs)
+      if (positions[q].first != positions[qs[0]].first) return true;
+    return false;
+  }
+  // Pool::ensure; the merges are structural, so m must be unanimous first.
+  uint64_t ensure(const std::vector<uint64_t>& qs) {
+    std::vector<uint64_t> bl;
+    for (uint64_t q : qs) bl.push_back(positions[q].first);
+    std::sort(bl.begin(), bl.end());
+    bl.erase(std::unique(bl.begin(), bl.end()), bl.end());
+    std::stable_sort(bl.begin(), bl.end(), [&](uint64_t x, uint64_t y) { return blocks[x]->nq() < blocks[y]->nq(); });
+    for (size_t i = 1; i < bl.size(); ++i) merge(bl[0], bl[i]);
+    return bl[0];
+  }
+  // Per slot: -1 when a known control fails (the gate is skipped), else the
+  // set of controls still unknown, as a bit mask over cs.
+  std::vector<int64_t> control_codes(const std::vector<std::pair<uint64_t, bool>>& cs, const Mask& m) {
+    std::vector<int64_t> code(B, 0);
+    for (uint32_t s = 0; s < B; ++s) {
+      if (!m[s]) continue;
+      for (size_t i = 0; i < cs.size(); ++i) {
+        const int8_t v = kn(s, cs[i].first);
+        if (v < 0) code[s] |= 1ll << i;
+        else if ((v != 0) != cs[i].second) {
+          code[s] = -1;
+          break;
+        }
+      }
+    }
+    return code;
+  }
+  // Controls, trivial-gate skip and merges shared by one() and the
+  // controlled swap; returns the active controls or nullopt (nothing runs).
+  std::optional<std::vector<std::pair<uint64_t, bool>>> controls(const std::vector<std::pair<uint64_t, bool>>& cs,
+                                                                 Mask& m) {
+    if (!any(m)) return std::nullopt;
+    const std::vector<int64_t> code = control_codes(cs, m);
+    agree(m, code);
+    const int r = first(m);
+    if (r < 0 || code[r] < 0) return std::nullopt;
+    std::vector<std::pair<uint64_t, bool>> active;
+    for (size_t i = 0; i < cs.size(); ++i)
+      if ((code[r] >> i) & 1) active.push_back(cs[i]);
+    return active;
+  }
+  k::Expand local_expand(const std::vector<std::pair<uint64_t, bool>>& cs, std::vector<uint32_t> extra) const {
+    uint64_t fixed = 0;
+    std::vector<uint32_t> pos = std::move(extra);
+    for (auto& c : cs) {
+      const uint32_t l = (uint32_t)positions[c.first].second;
+      pos.push_back(l);
+      if (c.second) fixed |= 1ull << l;
+    }
+    return expand(pos, fixed);
+  }
+  // Merge the blocks of qs when needed (only if every slot applies).
+  std::optional<uint64_t> gather(const std::vector<uint64_t>& qs, Mask& m) {
+    if (needs_merge(qs) && !unanimous(m)) return std::nullopt;
+    if (!any(m)) return std::nullopt;
+    return ensure(qs);
+  }
+
+  // Pool::one
+  void one(uint64_t q, const std::vector<cplx>& mat, const std::vector<std::pair<uint64_t, bool>>& cs, Mask& m) {
+    auto active = controls(cs, m);
+    if (!active) return;
+    if (!active->empty() && blocks[positions[q].first]->nq() == 1) {
+      const std::vector<cplx> v = read_single(*blocks[positions[q].first]);
+      std::vector<char> trivial(B, 0);
+      for (uint32_t s = 0; s < B; ++s) {
+        if (!m[s]) continue;
+        const cplx a = v[2 * s], z = v[2 * s + 1];
+        trivial[s] = ceq(cadd(cmul(mat[0], a), cmul(mat[1], z)), a) && ceq(cadd(cmul(mat[2], a), cmul(mat[3], z)), z);
+      }
+      agree(m, trivial);
+      const int r = first(m);
+      if (r < 0 || trivial[r]) return;
+    }
+    std::vector<uint64_t> qs;
+    for (auto& c : *active) qs.push_back(c.first);
+    qs.push_back(q);
+    const auto idx = gather(qs, m);
+    if (!idx) return;
+    Block& b = *blocks[*idx];
+    const uint32_t t = (uint32_t)positions[q].second;
+    const k::Expand e = local_expand(*active, {t});
+    const Mat2 xm = gates::X();
+    bool is_x = true;
+    for (int i = 0; i < 4; ++i) is_x = is_x && ceq(mat[i], xm[i]);
+    if (is_x) {
+      k::xgate(16, b.amps, b.size, B, (int)b.nq(), e, t, flag(m), st);
+    } else {
+      double u[8];
+      for (int i = 0; i < 4; ++i) {
+        u[2 * i] = mat[i].re;
+        u[2 * i + 1] = mat[i].im;
+      }
+      k::one(16, b.amps, b.size, B, (int)b.nq(), e, t, u, flag(m), st);
+    }
+    refresh(qs, m);
+  }
+
+  const void* dense_matrix(const Op& op) {
+    auto it = mat_of.find(&op);
+    if (it != mat_of.end()) return it->second;
+    auto dm = std::make_shared<DeviceMem>(16 * op.mat.size());
+    std::vector<double> h;
+    for (auto& v : op.mat) {
+      h.push_back(v.re);
+      h.push_back(v.im);
+    }
+    PBCHK(hipMemcpyAsync(dm->get(), h.data(), 8 * h.size(), hipMemcpyHostToDevice, st));
+    sync();
+    mats.push_back(dm);
+    mat_of[&op] = dm->get();
+    return dm->get();
+  }
+
+  // Pool::gate
+  void gate(const Op& op, Mask& m) {
+    std::vector<std::pair<uint64_t, bool>> cs;
+    for (auto& c : op.controls) cs.push_back({c.first, c.second});
+    switch (op.kind) {
+      case OpKind::One: one(op.target, op.mat, cs, m); break;
+      case OpKind::X: {
+        const Mat2 x = gates::X();
+        one(op.target, std::vector<cplx>(x.begin(), x.end()), cs, m);
+        break;
+      }
+      case OpKind::Swap: {
+        auto active = controls(cs, m);
+        if (!active) return;
+        if (active->empty()) {
+          if (!unanimous(m)) return;
+          const auto pa = positions[op.a], pbp = positions[op.b];
+          blocks[pa.first]->qubits[pa.second] = op.b;
+          blocks[pbp.first]->qubits[pbp.second] = op.a;
+          std::swap(positions[op.a], positions[op.b]);
+          for (uint32_t s = 0; s < B; ++s) std::swap(kn(s, op.a), kn(s, op.b));
+          return;
+        }
+        std::vector<uint64_t> qs;
+        for (auto& c : *active) qs.push_back(c.first);
+        qs.push_back(op.a);
+        qs.push_back(op.b);
+        const auto idx = gather(qs, m);
+        if (!idx) return;
+        Block& b = *blocks[*idx];
+        const uint32_t la = (uint32_t)positions[op.a].second, lb = (uint32_t)positions[op.b].second;
+        k::swap(16, b.amps, b.size, B, (int)b.nq(), local_expand(*active, {la, lb}), la, lb, flag(m), st);
+        refresh(qs, m);
+        break;
+      }
+      case OpKind::Diagonal: {
+        if (!any(m)) return;
+        const std::vector<uint32_t>& qs = op.qubits;
+        // Per slot: the unknown wires (mask over qs) and the reduced values.
+        std::vector<std::vector<uint64_t>> key(B);
+        for (uint32_t s = 0; s < B; ++s) {
+          if (!m[s]) continue;
+          uint64_t act = 0, na = 0;
+          for (size_t i = 0; i < qs.size(); ++i)
+            if (kn(s, qs[i]) < 0) act |= 1ull << i, ++na;
+          key[s].push_back(act);
+          for (uint64_t basis = 0; basis < (1ull << na); ++basis) {
+            uint64_t index = 0, cursor = 0;
+            for (uint32_t q : qs) {
+              uint64_t bit;
+              if (kn(s, q) >= 0) bit = (uint64_t)kn(s, q);
+              else bit = (basis >> (na - 1 - cursor++)) & 1;
+              index = (index << 1) | bit;
+            }
+            uint64_t re, im;
+            std::memcpy(&re, &op.mat[index].re, 8);
+            std::memcpy(&im, &op.mat[index].im, 8);
+            key[s].push_back(re);
+            key[s].push_back(im);
+          }
+        }
+        agree(m, key);
+        const int r = first(m);
+        if (r < 0) return;
+        std::vector<uint64_t> active;
+        for (size_t i = 0; i < qs.size(); ++i)
+          if ((key[r][0] >> i) & 1) active.push_back(qs[i]);
+        std::vector<cplx> reduced;
+        for (size_t j = 1; j < key[r].size(); j += 2) {
+          double re, im;
+          std::memcpy(&re, &key[r][j], 8);
+          std::memcpy(&im, &key[r][j + 1], 8);
+          reduced.push_back(cplx(re, im));
+        }
+        bool uniform = true;
+        for (auto& v : reduced) uniform = uniform && ceq(v, reduced[0]);
+        if (active.empty() || uniform) {
+          Block& b = *blocks[positions[qs[0]].first];
+          if (covers(m)) {
+            k::pb_scale(b.amps, b.size * B, reduced[0].re, reduced[0].im, st);
+          } else {
+            for (uint32_t s = 0; s < B; ++s)
+              if (m[s]) k::pb_scale(b.amps + 2 * b.size * s, b.size, reduced[0].re, reduced[0].im, st);
+          }
+        } else {
+          const auto idx = gather(active, m);
+          if (!idx) return;
+          Block& b = *blocks[*idx];
+          std::vector<uint32_t> local;
+          for (uint64_t q : active) local.push_back((uint32_t)positions[q].second);
+          const int kq = (int)local.size();
+          std::vector<uint64_t> pats;
+          std::vector<double> vals;
+          for (int j = 0; j < (1 << kq); ++j) {
+            if (reduced[j].re == 1.0 && reduced[j].im == 0.0) continue;
+            uint64_t pat = 0;
+            for (int i = 0; i < kq; ++i)
+              if ((j >> (kq - 1 - i)) & 1) pat |= 1ull << local[i];
+            pats.push_back(pat);
+            vals.push_back(reduced[j].re);
+            vals.push_back(reduced[j].im);
+          }
+          k::diag(16, b.amps, b.size, B, (int)b.nq(), expand(local, 0), (int)pats.size(), pats.data(), vals.data(),
+                  flag(m), st);
+        }
+        break;
+      }
+      case OpKind::Dense: {
+        std::vector<uint64_t> qs(op.qubits.begin(), op.qubits.end());
+        const auto idx = gather(qs, m);
+        if (!idx) return;
+        Block& b = *blocks[*idx];
+        std::vector<uint32_t> local;
+        for (uint64_t q : qs) local.push_back((uint32_t)positions[q].second);
+        const int kq = (int)local.size();
+        uint64_t offsets[32] = {};
+        for (int j = 0; j < (1 << kq); ++j) {
+          uint64_t off = 0;
+          for (int i = 0; i < kq; ++i)
+            if ((j >> (kq - 1 - i)) & 1) off |= 1ull << local[i];
+          offsets[j] = off;
+        }
+        k::dense(16, b.amps, b.size, B, (int)b.nq(), kq, expand(local, 0), dense_matrix(op), offsets, flag(m), st);
+        refresh(qs, m);
+        break;
+      }
+      case OpKind::Noop: break;
+      default: fail("Internal error: nonunitary operation in a P-block gate");
+    }
+  }
+
+  // Pool::measure for the slots of m; returns the outcome per slot. A
+  // measurement that splits its block is structural (unanimous).
+  std::vector<int> measure(uint64_t q, Mask& m) {
+    std::vector<int> out(B, 0);
+    const auto pos = positions[q];
+    Block& b = *blocks[pos.first];
+    const uint64_t size = b.size;
+    if (size > 2 && !unanimous(m)) return out;
+    if (!any(m)) return out;
+    if (size > 2) reserve(nbytes(size / 2 + 2, 16));
+    const int n = (int)b.nq();
+    const int seg1 = n < 11 ? n : 11;
+    const uint64_t c1 = 1ull << (n - seg1);
+    if (red_a.size() < 16 * c1 * B) red_a.alloc(16 * c1 * B);
+    if (red_b.size() < 16 * c1 * B) red_b.alloc(16 * c1 * B);
+    k::ProbSpec ps;
+    ps.branch = (int)pos.second;
+    k::seg_reduce_prob(16, b.amps, b.size, B, n, ps, seg1, red_a.get(), c1, st);
+    uint64_t count = c1;
+    void* src = red_a.get();
+    void* dst = red_b.get();
+    while (count > 1) {
+      int lg = 0;
+      while ((1ull << lg) < count) ++lg;
+      const int sg = lg < 11 ? lg : 11;
+      k::seg_reduce_f64(src, c1, count, B, sg, dst, c1, st);
+      count >>= sg;
+      std::swap(src, dst);
+    }
+    k::meas_decide(src, c1, B, ms.as<k::MeasState>(), rng.as<k::RngState>(), nullptr, 0, -1, flag(m),
+                   err.as<int>(), st);
+    k::collapse(16, b.amps, b.size, B, n, (uint32_t)pos.second, ms.as<k::MeasState>(), st);
+    std::vector<k::MeasState> h(B);
+    int e = 0;
+    PBCHK(hipMemcpyAsync(h.data(), ms.get(), sizeof(k::MeasState) * B, hipMemcpyDeviceToHost, st));
+    PBCHK(hipMemcpyAsync(&e, err.get(), sizeof e, hipMemcpyDeviceToHost, st));
+    sync();
+    if (e & 1) {
+      // Only a slot's own run can tell whether the norm was its own: every
+      // slot is run again alone, where the error is raised.
+      std::fill(alive.begin(), alive.end(), 0);
+      std::fill(m.begin(), m.end(), 0);
+      return out;
+    }
+    for (uint32_t s = 0; s < B; ++s) out[s] = h[s].outcome;
+    if (size > 2) {
+      auto rem = new_block(size / 2, {});
+      k::pb_extract_ms(rem->amps, b.amps, size / 2, (uint32_t)pos.second, ms.as<k::MeasState>(), B, st);
+      const uint64_t idx = pos.first;
+      std::unique_ptr<Block> old = std::move(blocks[idx]);
+      graveyard.push_back(old->mem);
+      if (free.empty()) fail("Internal error: a multi-qubit block leaves no free slot");
+      const uint64_t slot = free.back();
+      free.pop_back();
+      std::vector<uint64_t> qs = old->qubits;
+      qs.erase(qs.begin() + (long)pos.second);
+      live = live - old->size + rem->size + 2;
+      for (size_t i = 0; i < qs.size(); ++i) positions[qs[i]] = {idx, i};
+      rem->qubits = qs;
+      blocks[idx] = std::move(rem);
+      auto basis = new_block(2, {q});
+      k::pb_basis_ms(basis->amps, ms.as<k::MeasState>(), B, st);
+      blocks[slot] = std::move(basis);
+      positions[q] = {slot, 0};
+      refresh(qs, m);
+    }
+    for (uint32_t s = 0; s < B; ++s)
+      if (m[s]) kn(s, q) = (int8_t)out[s];
+    return out;
+  }
+};
+
 Result::Result() = default;
 Result::~Result() = default;
 
@@ -879,6 +1362,42 @@ void run_op(const POp& op, Pool& pool, std::map<uint64_t, int>& cbits) {
       }
       break;
     case PK::Gate: pool.gate(op.gate); break;
+    case PK::Noop: break;
+  }
+}
+
+// run_op for the live slots of a batch (m: the slots the op reaches).
+void run_batch(const POp& op, BatchPool& p, BatchPool::Mask& m) {
+  switch (op.kind) {
+    case PK::Conditional: {
+      BatchPool::Mask inner(p.B, 0);
+      for (uint32_t s = 0; s < p.B; ++s) {
+        if (!m[s]) continue;
+        uint64_t actual = 0;
+        for (uint64_t bit = 0; bit < op.cond.creg_size; ++bit) {
+          auto it = p.cbits[s].find(op.cond.creg_base + bit);
+          actual |= (uint64_t)(it == p.cbits[s].end() ? 0 : it->second) << bit;
+        }
+        inner[s] = actual == op.cond.creg_value;
+      }
+      run_batch(*op.inner, p, inner);
+      break;
+    }
+    case PK::Measure: {
+      const std::vector<int> out = p.measure(op.q, m);
+      for (uint32_t s = 0; s < p.B; ++s)
+        if (m[s]) p.cbits[s][op.c] = out[s];
+      break;
+    }
+    case PK::Reset: {
+      const std::vector<int> out = p.measure(op.q, m);
+      BatchPool::Mask x(p.B, 0);
+      for (uint32_t s = 0; s < p.B; ++s) x[s] = m[s] && out[s] == 1;
+      const Mat2 xm = gates::X();
+      p.one(op.q, std::vector<cplx>(xm.begin(), xm.end()), {}, x);
+      break;
+    }
+    case PK::Gate: p.gate(op.gate, m); break;
     case PK::Noop: break;
   }
 }
@@ -1040,31 +1559,63 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
   std::unique_ptr<Pool> kept;
   if (retain) kept = std::move(prefix);
   prefix.reset();
-  // Workers run one after another on this stream; each keeps its budget
-  // share and its own restarted pool, and shot s always draws from Philox
-  // stream s, so counts do not depend on the worker count.
-  for (uint64_t w = 0; w < workers; ++w) {
-    std::unique_ptr<Pool> pool;
+  // Without profiling or separation, shots run in lockstep batches (one
+  // launch sequence per batch); the shots a batch evicts, or every shot
+  // otherwise, run one after another on a restarted pool. Shot s always
+  // draws from Philox stream s, so counts depend neither on the batching nor
+  // on the worker count.
+  const uint64_t start_op = retain ? prefix_len : 0;
+  std::vector<uint64_t> todo;
+  if (!profile && !wopt.split_separable && shots > 1) {
+    std::unique_ptr<Pool> zero;
     if (kept) {
       if (kept->bytes() > wopt.memory)
         fail("P-block prefix exceeds worker memory budget; reduce max_parallel_shots");
-      pool = kept->clone();
-      pool->opt = wopt;
     } else {
-      pool = std::make_unique<Pool>(n, wopt, stream.s);
+      zero = std::make_unique<Pool>(n, wopt, stream.s);
     }
-    uint64_t iteration = 0;
-    for (uint64_t s = w; s < shots; s += workers, ++iteration) {
-      if (iteration > 0) {
-        if (kept) pool->restore(*kept);
-        else pool->reset_zero();
+    const Pool& base = kept ? *kept : *zero;
+    // Every slot stays within the worker budget, so a batch holds at most
+    // 32 GiB of blocks.
+    const uint64_t cap = std::max<uint64_t>(1, (32ull << 30) / std::max<uint64_t>(wopt.memory, 1));
+    const uint64_t bmax = std::min<uint64_t>(1024, cap);
+    for (uint64_t s0 = 0; s0 < shots; s0 += bmax) {
+      const uint32_t bsz = (uint32_t)std::min<uint64_t>(bmax, shots - s0);
+      BatchPool bp(base, bsz, wopt, seed, s0);
+      for (size_t i = start_op; i < ops.size() && bp.any(bp.alive); ++i) {
+        BatchPool::Mask m = bp.alive;
+        run_batch(ops[i], bp, m);
       }
-      pool->seed_rng(seed, s);
-      std::map<uint64_t, int> cb;
-      for (size_t i = retain ? prefix_len : 0; i < ops.size(); ++i) run_op(ops[i], *pool, cb);
-      counts[format_cbits(cb, nc)] += 1;
-      stats.add(pool->stats);
+      bp.sync();
+      for (uint32_t s = 0; s < bsz; ++s) {
+        if (bp.alive[s]) counts[format_cbits(bp.cbits[s], nc)] += 1;
+        else todo.push_back(s0 + s);
+      }
     }
+  } else {
+    for (uint64_t s = 0; s < shots; ++s) todo.push_back(s);
+  }
+  std::unique_ptr<Pool> pool;
+  for (uint64_t s : todo) {
+    if (!pool) {
+      if (kept) {
+        if (kept->bytes() > wopt.memory)
+          fail("P-block prefix exceeds worker memory budget; reduce max_parallel_shots");
+        pool = kept->clone();
+        pool->opt = wopt;
+      } else {
+        pool = std::make_unique<Pool>(n, wopt, stream.s);
+      }
+    } else if (kept) {
+      pool->restore(*kept);
+    } else {
+      pool->reset_zero();
+    }
+    pool->seed_rng(seed, s);
+    std::map<uint64_t, int> cb;
+    for (size_t i = start_op; i < ops.size(); ++i) run_op(ops[i], *pool, cb);
+    counts[format_cbits(cb, nc)] += 1;
+    stats.add(pool->stats);
   }
   strategy = retain ? "prefix_trajectories" : "trajectories";
   prefix_used = retain ? prefix_len : 0;

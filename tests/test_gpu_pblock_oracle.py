@@ -87,3 +87,23 @@ def test_result_counts_match_oracle():
     o = opb.simulate(c.data, 5)
     assert g.counts(500, qubits=[0, 5, 6], seed=19) == o.counts(500, qubits=[0, 5, 6], seed=19)
     assert g.counts(200, seed=3) == o.counts(200, seed=3)
+
+
+@pytest.mark.parametrize("trial", range(4))
+def test_lockstep_trajectories_match_oracle_and_lone_runs(trial, tmp_path, monkeypatch):
+    # Dynamic circuits whose shots part ways in structure (controls on
+    # measured wires, conditional gates that merge blocks, resets): the
+    # lockstep batches give the oracle's counts draw for draw, and so do the
+    # lone per-shot runs (a profiled call runs every shot on its own pool)
+    # and batches of two slots (a large per-worker budget shrinks them).
+    c = dynamic_circuit(6, 40, 3100 + trial)
+    c = Circuit(6, c.data["instructions"] + [inst("measure", qubit=q, cbit=q) for q in range(6)], 6)
+    seed = 41 + trial
+    print(f"[pblock-oracle] lockstep trial={trial}", flush=True)
+    g = PBlockSimulator(seed=seed, sample_terminal=False).simulate_shots(c, 120)
+    o, _ = opb.simulate_shots(c.data, 120, seed, sample_terminal=False)
+    assert g == o
+    small = PBlockSimulator(seed=seed, sample_terminal=False, max_memory_mb=16384, max_parallel_shots=1)
+    assert small.simulate_shots(c, 120) == g
+    monkeypatch.chdir(tmp_path)
+    assert PBlockSimulator(seed=seed, sample_terminal=False).simulate_shots(c, 120, profile=True) == g
