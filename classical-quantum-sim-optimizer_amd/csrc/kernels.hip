@@ -2026,6 +2026,20 @@ __global__ __launch_bounds__(TPB) void k_pb_basis_ms(double2* __restrict__ out, 
   out[2 * (uint64_t)b + 1] = make_double2(v == 1 ? 1.0 : 0.0, 0.0);
 }
 
+__global__ __launch_bounds__(64) void k_pb_small_tables(const PbSmallTable* __restrict__ t, uint32_t count,
+                                                       double* __restrict__ totals) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= count) return;
+  const PbSmallTable d = t[i];
+  const double2* a = (const double2*)d.amps;
+  double acc = 0.0;
+  for (uint64_t j = 0; j < d.len; ++j) {
+    acc = acc + pb_nsq(a[j]);
+    d.cdf[j] = acc;
+  }
+  totals[d.slot] = acc;
+}
+
 __global__ __launch_bounds__(TPB) void k_pb_norms(double* __restrict__ out, const double2* __restrict__ s,
                                                   uint64_t count) {
   const uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x;
@@ -2242,6 +2256,10 @@ void pb_basis_ms(void* out, const MeasState* ms, uint32_t batch, hipStream_t st)
 void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_set2, dim3(1), dim3(64), 0, st, (double2*)s, make_double2(a_re, a_im),
                      make_double2(b_re, b_im));
+}
+void pb_small_tables(const PbSmallTable* tabs_dev, uint32_t count, double* totals, hipStream_t st) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_pb_small_tables, dim3(blocks_for(count, 64)), dim3(64), 0, st, tabs_dev, count, totals);
 }
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st) {
   hipLaunchKernelGGL(k_pb_norms, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, out, (const double2*)s, count);

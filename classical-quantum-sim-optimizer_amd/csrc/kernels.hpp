@@ -237,6 +237,16 @@ void pb_extract_ms(void* out, const void* in, uint64_t half, uint32_t q, const M
                    hipStream_t st);
 void pb_basis_ms(void* out, const MeasState* ms, uint32_t batch, hipStream_t st);
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st);  // |s[i]|^2
+// CDF tables of blocks of at most 1024 amplitudes in one launch: cdf[j] is
+// the sequential inclusive prefix of |amps[i]|^2 (cdf_build's value for a
+// single group, bit for bit) and totals[slot] its last entry.
+struct PbSmallTable {
+  const void* amps;
+  double* cdf;
+  uint64_t len;
+  uint64_t slot;
+};
+void pb_small_tables(const PbSmallTable* tabs_dev, uint32_t count, double* totals, hipStream_t st);
 void pb_set2(void* s, double a_re, double a_im, double b_re, double b_im, hipStream_t st);  // s[0], s[1]
 // Separation test of local qubit q (one workgroup); writes the remaining
 // amplitudes to rem and res[10] = s0.re, s0.im, s1.re, s1.im, loss, total,

@@ -786,8 +786,27 @@ class Pool {
     DeviceMem totals(8 * std::max<size_t>(live_blocks.size(), 1));
     DeviceMem errs(4);
     PBCHK(hipMemsetAsync(errs.get(), 0, 4, st));
+    // Blocks of at most 1024 amplitudes (one CDF group): one slab, one launch.
+    uint64_t small_len = 0;
+    for (const Block* b : live_blocks)
+      if (b->size <= 1024) small_len += b->size;
+    std::shared_ptr<DeviceMem> slab;
+    if (small_len > 0) {
+      slab = std::make_shared<DeviceMem>(8 * small_len);
+      keep.push_back(slab);
+    }
+    std::vector<k::PbSmallTable> small;
+    tabs.resize(live_blocks.size());
+    uint64_t off = 0;
     for (size_t t = 0; t < live_blocks.size(); ++t) {
       const Block& b = *live_blocks[t];
+      if (b.size <= 1024) {
+        double* cdf = slab->as<double>() + off;
+        off += b.size;
+        small.push_back({b.amps, cdf, b.size, t});
+        tabs[t] = {cdf, b.size, (uint32_t)b.nq(), 0};
+        continue;
+      }
       auto norms = std::make_shared<DeviceMem>(8 * b.size);
       auto cdf = std::make_shared<DeviceMem>(8 * b.size);
       auto gt = std::make_shared<DeviceMem>(8 * ((b.size + 1023) / 1024 + 1));
@@ -797,7 +816,14 @@ class Pool {
       keep.push_back(norms);
       keep.push_back(cdf);
       keep.push_back(gt);
-      tabs.push_back({cdf->get(), b.size, (uint32_t)b.nq(), 0});
+      tabs[t] = {cdf->get(), b.size, (uint32_t)b.nq(), 0};
+    }
+    if (!small.empty()) {
+      auto desc = std::make_shared<DeviceMem>(sizeof(k::PbSmallTable) * small.size());
+      PBCHK(hipMemcpyAsync(desc->get(), small.data(), sizeof(k::PbSmallTable) * small.size(), hipMemcpyHostToDevice,
+                           st));
+      k::pb_small_tables(desc->as<k::PbSmallTable>(), (uint32_t)small.size(), totals.as<double>(), st);
+      keep.push_back(desc);
     }
     std::vector<double> tot(live_blocks.size());
     if (!tot.empty())
