@@ -1607,14 +1607,25 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
     const uint64_t bmax = std::min<uint64_t>(1024, cap);
     for (uint64_t s0 = 0; s0 < shots; s0 += bmax) {
       const uint32_t bsz = (uint32_t)std::min<uint64_t>(bmax, shots - s0);
-      BatchPool bp(base, bsz, wopt, seed, s0);
-      for (size_t i = start_op; i < ops.size() && bp.any(bp.alive); ++i) {
-        BatchPool::Mask m = bp.alive;
-        run_batch(ops[i], bp, m);
+      std::vector<std::map<uint64_t, int>> done;
+      BatchPool::Mask alive;
+      try {
+        BatchPool bp(base, bsz, wopt, seed, s0);
+        for (size_t i = start_op; i < ops.size() && bp.any(bp.alive); ++i) {
+          BatchPool::Mask m = bp.alive;
+          run_batch(ops[i], bp, m);
+        }
+        bp.sync();
+        done = std::move(bp.cbits);
+        alive = std::move(bp.alive);
+      } catch (const std::exception&) {
+        // The lone runs raise the error of the first failing shot, as the
+        // per-shot path does: this and every later shot run alone.
+        for (uint64_t s = s0; s < shots; ++s) todo.push_back(s);
+        break;
       }
-      bp.sync();
       for (uint32_t s = 0; s < bsz; ++s) {
-        if (bp.alive[s]) counts[format_cbits(bp.cbits[s], nc)] += 1;
+        if (alive[s]) counts[format_cbits(done[s], nc)] += 1;
         else todo.push_back(s0 + s);
       }
     }

@@ -286,6 +286,15 @@ def test_sampling_and_prefix_fall_back_when_memory_is_tight(tmp_path):
     assert report["deterministic_prefix_ops"] == 0
     assert report["parallel_shots"] == 1
     assert report["peak_working_bytes"] <= 3 * 1024 ** 2
+    # Unprofiled, the shots run in one lockstep batch under the same budget.
+    assert PBlockSimulator(seed=3, max_memory_mb=3, max_parallel_shots=4).simulate_shots(Circuit(32, ops, 2), 4) \
+        == counts
+    # A trajectory that outgrows max_block_qubits raises what a lone shot raises.
+    # (ry states: a CX onto |+> would be skipped as trivial and never merge)
+    ops = [inst("ry", qubit=q, theta=0.3 + 0.1 * q) for q in range(6)] + [inst("measure", qubit=5, cbit=0)]
+    ops += [inst("cx", control=q, target=q + 1) for q in range(4)]
+    with pytest.raises(MemoryError, match="max_block_qubits"):
+        PBlockSimulator(seed=3, max_block_qubits=4).simulate_shots(Circuit(6, ops, 1), 50)
 
 
 def test_profiles_confirm_shared_evolution(tmp_path):
