@@ -51,10 +51,13 @@ void hip_check(hipError_t e, const char* what) {
 
 struct Stream {
   hipStream_t s = nullptr;
-  Stream() { HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  int dev = 0;
+  Stream() { s = acquire_stream(&dev); }
   ~Stream() {
-    if (s) (void)hipStreamDestroy(s);
+    if (s) release_stream(s, dev);
   }
+  Stream(const Stream&) = delete;
+  Stream& operator=(const Stream&) = delete;
   void sync() { HIPCHK(hipStreamSynchronize(s)); }
 };
 
@@ -157,6 +160,48 @@ size_t arena_class(size_t bytes) {
   return c;
 }
 }  // namespace
+
+namespace {
+struct StreamPool {
+  std::mutex mu;
+  std::map<int, std::vector<hipStream_t>> idle;  // device -> streams
+};
+StreamPool& stream_pool() {
+  static StreamPool* p = new StreamPool;
+  return *p;
+}
+constexpr size_t kIdleStreams = 32;
+}  // namespace
+
+hipStream_t acquire_stream(int* dev) {
+  HIPCHK(hipGetDevice(dev));
+  {
+    StreamPool& p = stream_pool();
+    std::lock_guard<std::mutex> lk(p.mu);
+    auto& v = p.idle[*dev];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+void release_stream(hipStream_t s, int dev) {
+  {
+    StreamPool& p = stream_pool();
+    std::lock_guard<std::mutex> lk(p.mu);
+    auto& v = p.idle[dev];
+    if (v.size() < kIdleStreams) {
+      v.push_back(s);
+      return;
+    }
+  }
+  (void)hipStreamDestroy(s);
+}
 
 DeviceMem::~DeviceMem() { reset(); }
 void DeviceMem::reset() {

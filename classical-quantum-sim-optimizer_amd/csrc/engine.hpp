@@ -58,6 +58,12 @@ struct SimProfile {
   double oq_time = 0, nq_time = 0, mq_time = 0, total_time = 0;
 };
 
+// Streams are reused per device (creating and destroying one costs about
+// 0.1 ms, a visible share of a small circuit's call). The pool is never torn
+// down: the HIP runtime may be gone by static destruction.
+hipStream_t acquire_stream(int* dev);
+void release_stream(hipStream_t s, int dev);
+
 class DeviceMem {
  public:
   DeviceMem() = default;

@@ -1421,32 +1421,78 @@ __global__ void k_rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_
 // ---------------------------------------------------------------------------
 constexpr uint64_t kGroup = 1024;
 
-__global__ void k_cdf_local(const double* __restrict__ bins, uint64_t bstride, uint64_t count,
-                            double* __restrict__ C, double* __restrict__ gt) {
-  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t lo = g * kGroup;
-  if (lo >= count) return;
-  const uint64_t hi = lo + kGroup < count ? lo + kGroup : count;
-  double acc = 0.0;
-  for (uint64_t j = lo; j < hi; ++j) {
-    acc = acc + bins[j * bstride];
-    C[j] = acc;
+// In-place sequential prefix of v[0..len) in LDS (inclusive: v[i] = acc
+// after adding v[i]; exclusive: v[i] = acc before), one addition at a time
+// in index order; returns the final acc. Loads run eight ahead of the adds.
+template <bool Inclusive>
+__device__ __forceinline__ double lds_seq_scan(double* v, uint32_t len, double acc) {
+  uint32_t i = 0;
+  for (; i + 8 <= len; i += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = v[i + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (Inclusive) {
+        acc = acc + x[u];
+        v[i + u] = acc;
+      } else {
+        v[i + u] = acc;
+        acc = acc + x[u];
+      }
+    }
   }
-  gt[g] = acc;
+  for (; i < len; ++i) {
+    const double x = v[i];
+    if (Inclusive) {
+      acc = acc + x;
+      v[i] = acc;
+    } else {
+      v[i] = acc;
+      acc = acc + x;
+    }
+  }
+  return acc;
 }
 
-__global__ void k_cdf_offsets(double* __restrict__ gt, uint64_t ngroups, double* __restrict__ total,
-                              int* __restrict__ err) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  double off = 0.0, last = 0.0;
-  for (uint64_t g = 0; g < ngroups; ++g) {
-    double tg = gt[g];
-    gt[g] = off;
-    last = off + tg;
-    off = last;
+// One 64-lane workgroup per group of kGroup bins: coalesced loads into LDS,
+// one lane's sequential inclusive prefix (from 0.0), coalesced stores;
+// gt[g] = the group total.
+__global__ __launch_bounds__(64) void k_cdf_local(const double* __restrict__ bins, uint64_t bstride, uint64_t count,
+                                                  double* __restrict__ C, double* __restrict__ gt) {
+  __shared__ double v[kGroup];
+  const uint64_t g = blockIdx.x;
+  const uint64_t lo = g * kGroup;
+  const uint32_t len = (uint32_t)(lo + kGroup < count ? kGroup : count - lo);
+  for (uint32_t i = threadIdx.x; i < len; i += 64) v[i] = bins[(lo + i) * bstride];
+  __syncthreads();
+  if (threadIdx.x == 0) gt[g] = lds_seq_scan<true>(v, len, 0.0);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < len; i += 64) C[lo + i] = v[i];
+}
+
+// Group totals -> sequential exclusive prefix in place, in chunks staged
+// through LDS (one workgroup); *total = the sum of every group.
+constexpr uint32_t kOffChunk = 4096;
+__global__ __launch_bounds__(64) void k_cdf_offsets(double* __restrict__ gt, uint64_t ngroups,
+                                                    double* __restrict__ total, int* __restrict__ err) {
+  __shared__ double v[kOffChunk];
+  __shared__ double carry;
+  if (threadIdx.x == 0) carry = 0.0;
+  for (uint64_t c0 = 0; c0 < ngroups; c0 += kOffChunk) {
+    const uint32_t len = (uint32_t)(ngroups - c0 < kOffChunk ? ngroups - c0 : kOffChunk);
+    for (uint32_t i = threadIdx.x; i < len; i += 64) v[i] = gt[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) carry = lds_seq_scan<false>(v, len, carry);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < len; i += 64) gt[c0 + i] = v[i];
+    __syncthreads();
   }
-  *total = last;
-  if (!(isfinite(last) && last > 0.0)) atomicOr(err, 2);
+  if (threadIdx.x == 0) {
+    const double last = carry;
+    *total = last;
+    if (!(isfinite(last) && last > 0.0)) atomicOr(err, 2);
+  }
 }
 
 __global__ __launch_bounds__(TPB) void k_cdf_add(double* __restrict__ C, uint64_t count,
@@ -1894,8 +1940,9 @@ void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hi
 void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* gt, double* total, int* err,
                hipStream_t st) {
   const uint64_t ngroups = (count + kGroup - 1) / kGroup;
-  hipLaunchKernelGGL(k_cdf_local, dim3(blocks_for(ngroups, 64)), dim3(64), 0, st, bins, bstride, count, C, gt);
-  hipLaunchKernelGGL(k_cdf_offsets, dim3(1), dim3(1), 0, st, gt, ngroups, total, err);
+  if (ngroups > 0)
+    hipLaunchKernelGGL(k_cdf_local, dim3((uint32_t)ngroups), dim3(64), 0, st, bins, bstride, count, C, gt);
+  hipLaunchKernelGGL(k_cdf_offsets, dim3(1), dim3(64), 0, st, gt, ngroups, total, err);
   hipLaunchKernelGGL(k_cdf_add, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, C, count, (const double*)gt);
 }
 

@@ -79,9 +79,10 @@ uint64_t random_seed() {
 
 struct Stream {
   hipStream_t s = nullptr;
-  Stream() { PBCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  int dev = 0;
+  Stream() { s = acquire_stream(&dev); }
   ~Stream() {
-    if (s) (void)hipStreamDestroy(s);
+    if (s) release_stream(s, dev);
   }
   Stream(const Stream&) = delete;
   Stream& operator=(const Stream&) = delete;
