@@ -3,7 +3,9 @@
 #include <cstring>
 #include <sstream>
 #include <random>
+#include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/dqsim_sv.h"
 #include "engine.hpp"
@@ -98,7 +100,42 @@ char* dup_string(const std::string& s) {
 }
 }  // namespace
 
+struct dq_counts {
+  std::string keys;
+  std::vector<uint64_t> offsets{0};
+  std::vector<uint64_t> values;
+  bool failed = false;
+};
+
 extern "C" {
+
+dq_counts* dq_counts_new(void) { return new (std::nothrow) dq_counts; }
+
+int dq_counts_append(void* table, const char* bits, size_t len, uint64_t count) {
+  auto* t = static_cast<dq_counts*>(table);
+  try {
+    t->keys.append(bits, len);
+    t->offsets.push_back(t->keys.size());
+    t->values.push_back(count);
+    return 0;
+  } catch (...) {
+    t->failed = true;
+    return 1;
+  }
+}
+
+int dq_counts_view(const dq_counts* t, uint64_t* n, const char** keys, const uint64_t** offsets,
+                   const uint64_t** values) {
+  return guard([&] {
+    if (t->failed) throw std::bad_alloc();
+    *n = t->values.size();
+    *keys = t->keys.data();
+    *offsets = t->offsets.data();
+    *values = t->values.data();
+  });
+}
+
+void dq_counts_free(dq_counts* t) { delete t; }
 
 void dq_sv_opts_default(dq_sv_opts* o) {
   std::memset(o, 0, sizeof *o);

@@ -129,6 +129,10 @@ def _load():
         "dq_sv_create": (ctypes.c_int, [ctypes.POINTER(_Opts), ctypes.POINTER(P)]),
         "dq_sv_free": (None, [P]),
         "dq_sv_simulate_shots": (ctypes.c_int, [P, ctypes.c_char_p, SZ, U64, I32, _COUNT_CB, P]),
+        "dq_counts_new": (P, []),
+        "dq_counts_view": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
+        "dq_counts_free": (None, [P]),
         "dq_sv_simulate": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(P)]),
         "dq_result_num_qubits": (ctypes.c_uint32, [P]),
         "dq_result_num_cbits": (U64, [P]),
@@ -239,14 +243,33 @@ def _u64_array(values: List[int]):
     return arr
 
 
-def _collect_counts():
-    out: Dict[str, int] = {}
+class _CountTable:
+    """Counts received into the library's packed table: `cb`/`user` go to an
+    entry point taking a dq_count_cb, `result()` reads every outcome at once
+    (no Python call per outcome)."""
 
-    def cb(_user, bits, length, count):
-        out[ctypes.string_at(bits, length).decode("ascii")] = int(count)
-        return 0
+    def __init__(self):
+        self.user = _lib.dq_counts_new()
+        if not self.user:
+            raise MemoryError("count table allocation failed")
+        self.cb = _COUNT_CB(ctypes.cast(_lib.dq_counts_append, ctypes.c_void_p).value)
 
-    return out, _COUNT_CB(cb)
+    def result(self) -> Dict[str, int]:
+        n, keys, offs, vals = ctypes.c_uint64(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(_lib.dq_counts_view(self.user, ctypes.byref(n), ctypes.byref(keys), ctypes.byref(offs),
+                                   ctypes.byref(vals)))
+        m = n.value
+        if m == 0:
+            return {}
+        o = list((ctypes.c_uint64 * (m + 1)).from_address(offs.value))
+        v = list((ctypes.c_uint64 * m).from_address(vals.value))
+        text = ctypes.string_at(keys.value, o[m]).decode("ascii")
+        return {text[o[i]:o[i + 1]]: v[i] for i in range(m)}
+
+    def __del__(self):
+        if getattr(self, "user", None):
+            _lib.dq_counts_free(self.user)
+            self.user = None
 
 
 # ---------------------------------------------------------------------------
@@ -337,10 +360,10 @@ class SimulationResult:
         qs = list(range(self._n))[::-1] if qubits is None else _qubit_list(qubits)
         seed = _opt_u64(seed, "seed")
         arr = _u64_array(qs)
-        out, cb = _collect_counts()
+        tab = _CountTable()
         _check(_lib.dq_result_counts(self._h, shots, arr, len(qs), 0 if seed is None else 1,
-                                     0 if seed is None else seed, cb, None))
-        return out
+                                     0 if seed is None else seed, tab.cb, tab.user))
+        return tab.result()
 
     def fidelity(self, other) -> float:
         if not isinstance(other, np.ndarray) or other.dtype != np.complex128 or other.ndim != 1:
@@ -420,9 +443,9 @@ class StatevectorSimulator:
         shots = _as_u64(shots, "shots")
         profile = _as_bool(profile, "profile")
         data = _circuit_json(circuit)
-        out, cb = _collect_counts()
-        _check(_lib.dq_sv_simulate_shots(self._h, data, len(data), shots, 1 if profile else 0, cb, None))
-        return out
+        tab = _CountTable()
+        _check(_lib.dq_sv_simulate_shots(self._h, data, len(data), shots, 1 if profile else 0, tab.cb, tab.user))
+        return tab.result()
 
     def simulate(self, circuit) -> SimulationResult:
         """Execute one trajectory. Measurements collapse its returned state."""

@@ -17,7 +17,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import _core
-from ._core import _as_bool, _as_u64, _check, _circuit_json, _collect_counts, _lib, _u64_array
+from ._core import _as_bool, _as_u64, _check, _circuit_json, _CountTable, _lib, _u64_array
 
 _USIZE_MAX = (1 << 64) - 1
 
@@ -255,9 +255,9 @@ class PBlockResult:
         qs = [] if qubits is None else _core._qubit_list(qubits)
         s = random.getrandbits(64) if seed is None else _as_u64(seed, "seed")
         arr = _u64_array(qs)
-        out, cb = _collect_counts()
-        _check(_lib.dq_pb_counts(self._h, shots, arr, len(qs), 1 if qubits is None else 0, s, cb, None))
-        return out
+        tab = _CountTable()
+        _check(_lib.dq_pb_counts(self._h, shots, arr, len(qs), 1 if qubits is None else 0, s, tab.cb, tab.user))
+        return tab.result()
 
     def expectation_value(self, pauli: str) -> float:
         if not isinstance(pauli, str):
@@ -302,10 +302,10 @@ class PBlockSimulator:
         shots = _as_u64(shots, "shots")
         profile = _as_bool(profile, "profile")
         data = _transport(distributed, self._o.max_memory_mb << 20)
-        out, cb = _collect_counts()
-        _check(_lib.dq_pb_simulate_shots(ctypes.byref(self._o), data, len(data), shots, 1 if profile else 0, cb,
-                                         None))
-        return out
+        tab = _CountTable()
+        _check(_lib.dq_pb_simulate_shots(ctypes.byref(self._o), data, len(data), shots, 1 if profile else 0, tab.cb,
+                                         tab.user))
+        return tab.result()
 
 
 _PB_OPTIONS = ("max_memory_mb", "max_block_qubits", "max_parallel_shots", "sample_terminal", "split_separable",

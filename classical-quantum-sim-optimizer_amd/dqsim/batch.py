@@ -18,7 +18,7 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from ._core import _BatchOpts, _check, _circuit_json, _collect_counts, _lib
+from ._core import _BatchOpts, _check, _circuit_json, _CountTable, _lib
 
 _PREC = {"complex64": 8, "complex128": 16}
 
@@ -56,9 +56,9 @@ class BatchResult:
             self._h = None
 
     def counts(self, i: int) -> Dict[str, int]:
-        out, cb = _collect_counts()
-        _check(_lib.dq_batch_counts(self._h, i, cb, None))
-        return out
+        tab = _CountTable()
+        _check(_lib.dq_batch_counts(self._h, i, tab.cb, tab.user))
+        return tab.result()
 
     def indices(self, i: int) -> np.ndarray:
         """Sampled outcomes over the measured qubits (highest measured qubit =

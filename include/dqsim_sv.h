@@ -84,6 +84,20 @@ typedef struct dq_sim_profile {
 /* One (bitstring, count) entry; return nonzero to abort the iteration. */
 typedef int (*dq_count_cb)(void* user, const char* bits, size_t len, uint64_t count);
 
+/* Packed count table. No counterpart in the reference: a faster way to take
+ * counts from any entry point with a dq_count_cb. Pass dq_counts_append as
+ * `cb` and a table as `user`; the library appends every outcome itself, with
+ * no call into the host language per outcome. dq_counts_view gives n
+ * entries in emission order: key i is keys[offsets[i] .. offsets[i+1]), its
+ * count values[i]; the pointers stay valid until the next append or
+ * dq_counts_free. DQ_EMEMORY from the view if an append ran out of memory. */
+typedef struct dq_counts dq_counts;
+dq_counts* dq_counts_new(void);
+int dq_counts_append(void* table, const char* bits, size_t len, uint64_t count);
+int dq_counts_view(const dq_counts* t, uint64_t* n, const char** keys, const uint64_t** offsets,
+                   const uint64_t** values);
+void dq_counts_free(dq_counts* t);
+
 /* Per-launch device timing (engine-recorded). */
 typedef struct dq_op_timing {
   int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA),

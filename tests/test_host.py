@@ -577,3 +577,17 @@ def test_exact_sweeps_group_ops_into_register_chains():
     for t in tiles:
         # Restore passes (layout only) apply no ops.
         assert (1 <= t["chains"] <= len(t["ops"])) or (t["chains"] == 0 and not t["ops"] and t["gates"] == 0)
+
+
+def test_packed_count_table_round_trip():
+    # The library's own dq_count_cb appends into a packed table that Python
+    # reads once; keys keep emission order, any length, full uint64 counts.
+    import ctypes
+    from dqsim import _core
+    t = _core._CountTable()
+    entries = [("0101", 7), ("1", 1), ("", 0), ("1110", 2 ** 40)]
+    for key, count in entries:
+        b = key.encode()
+        assert t.cb(t.user, ctypes.cast(ctypes.c_char_p(b), ctypes.POINTER(ctypes.c_char)), len(b), count) == 0
+    assert list(t.result().items()) == entries
+    assert _core._CountTable().result() == {}
