@@ -1281,6 +1281,9 @@ __device__ __forceinline__ double2 shfl_xor2(double2 v, int m) {
 // Block = 2048 consecutive virtual elements; element j*256 + t is loaded by
 // thread t (coalesced). Lane butterflies give the pairwise sums of aligned
 // runs of up to 64 elements; the 32 run sums per block are combined in LDS.
+// The indexing below (8 loads of 256 threads per 2048-element block, 4 waves
+// of 64 lanes, run j * 4 + wave) holds only for 256-thread workgroups.
+static_assert(TPB == 256, "k_seg_reduce assumes 256 threads: 4 waves x 8 loads per 2048-element block");
 template <class Src>
 __global__ __launch_bounds__(TPB) void k_seg_reduce(Src src, uint64_t count, int seg, double2* __restrict__ out,
                                                     uint64_t out_stride) {

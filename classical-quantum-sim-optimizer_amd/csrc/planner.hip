@@ -2,7 +2,7 @@
 // the GPU by these kernels.
 //
 // Parameters are one packed FP32 vector (kMlpParams entries):
-//   W1[F][H], b1[H], W2[H][H], b2[H], w3[H], b3      (F = 12 inputs, H = 64)
+//   W1[F][H], b1[H], W2[H][H], b2[H], w3[H], b3      (F = kFeatures = 16 inputs, H = 64)
 // Model: h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = h2 . w3 + b3.
 // Loss: mean squared error over the batch.
 // Every sum runs in a fixed sequential order (one thread per output), so a

@@ -110,6 +110,8 @@ def _load():
     lib.or_apply_dense.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
     lib.or_measure.restype = ctypes.c_int
     lib.or_measure.argtypes = [P, ctypes.c_int, ctypes.c_uint32, D, ctypes.POINTER(D), ctypes.POINTER(D)]
+    lib.or_measure_f32.restype = ctypes.c_int
+    lib.or_measure_f32.argtypes = [P, ctypes.c_int, ctypes.c_uint32, D, ctypes.POINTER(D), ctypes.POINTER(D)]
     lib.or_ptree.restype = D
     lib.or_ptree.argtypes = [P, U64]
     lib.or_marginal.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
@@ -326,11 +328,35 @@ def apply_dense(state, qubits, mat):
     lib().or_apply_dense(state.ctypes.data, n, q, len(qubits), m.ctypes.data)
 
 
-def measure(state, qubit, r):
+def measure(state, qubit, r, f32=False):
+    """measure_qubit with draw r (in place). f32: the state holds complex64
+    values and the collapse rounds the kept amplitudes to FP32."""
     n = state.size.bit_length() - 1
     p0, p1 = ctypes.c_double(), ctypes.c_double()
-    out = lib().or_measure(state.ctypes.data, n, qubit, r, ctypes.byref(p0), ctypes.byref(p1))
+    fn = lib().or_measure_f32 if f32 else lib().or_measure
+    out = fn(state.ctypes.data, n, qubit, r, ctypes.byref(p0), ctypes.byref(p1))
     return out, p0.value, p1.value
+
+
+def terminal_trajectories(prefix: np.ndarray, measures: Sequence[Tuple[int, int]], nc: int, shots: int, seed: int,
+                          f32: bool = False) -> Dict[str, int]:
+    """Trajectory shots (shots.rs:178-259) of a circuit whose unitary prefix
+    has already been evolved to `prefix` and whose remaining instructions are
+    the unconditional measurements `measures` = [(qubit, cbit), ...]: shot s
+    copies the prefix and measures in order with draws (seed, stream s,
+    draw 0, 1, ...). f32 collapses as a complex64 state does."""
+    counts: Dict[str, int] = {}
+    for s in range(shots):
+        st = np.array(prefix, dtype=np.complex128, copy=True)
+        bits = [0] * max(1, nc)
+        for d, (q, c) in enumerate(measures):
+            out, _, _ = measure(st, q, uniform53(seed, s, d), f32=f32)
+            if out < 0:
+                raise RuntimeError("oracle: invalid normalisation")
+            bits[c] = out
+        key = "".join(str(b) for b in bits[:nc][::-1])
+        counts[key] = counts.get(key, 0) + 1
+    return counts
 
 
 def ptree(x: np.ndarray) -> float:

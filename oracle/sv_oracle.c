@@ -638,7 +638,10 @@ double or_ptree(const double* x, uint64_t len) {
 static inline double prob(const cx* st, uint64_t i) { return st[i].re * st[i].re + st[i].im * st[i].im; }
 
 /* measure_qubit (engine.rs:367-435) with the pairwise-tree branch sums. */
-int or_measure(double* s, int n, uint32_t qubit, double r, double* p0o, double* p1o) {
+/* f32 != 0: the state holds complex64 values (exactly upcast) and the kept
+ * amplitudes are rounded back to FP32 after scaling, as a complex64 state is
+ * collapsed (scale computed in FP64, product rounded to float). */
+static int measure_impl(double* s, int n, uint32_t qubit, double r, double* p0o, double* p1o, int f32) {
   const uint64_t len = 1ull << n, bit = 1ull << qubit;
   const uint64_t chunk = len < 2048 ? len : 2048, nch = len / chunk;
   double* c0 = (double*)malloc(sizeof(double) * nch);
@@ -675,11 +678,23 @@ int or_measure(double* s, int n, uint32_t qubit, double r, double* p0o, double* 
     if ((int)((i & bit) != 0) == outcome) {
       st[i].re = st[i].re * scale;
       st[i].im = st[i].im * scale;
+      if (f32) {
+        st[i].re = (double)(float)st[i].re;
+        st[i].im = (double)(float)st[i].im;
+      }
     } else {
       st[i] = C(0, 0);
     }
   }
   return outcome;
+}
+
+int or_measure(double* s, int n, uint32_t qubit, double r, double* p0o, double* p1o) {
+  return measure_impl(s, n, qubit, r, p0o, p1o, 0);
+}
+
+int or_measure_f32(double* s, int n, uint32_t qubit, double r, double* p0o, double* p1o) {
+  return measure_impl(s, n, qubit, r, p0o, p1o, 1);
 }
 
 /* Virtual index map of a marginal: low bits walk the unmeasured qubits in

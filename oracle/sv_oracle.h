@@ -114,6 +114,9 @@ void or_apply_diagonal(double* s, int n, const uint32_t* qubits, int k, const do
 void or_apply_dense(double* s, int n, const uint32_t* qubits, int k, const double* mat);
 /* measure_qubit with an explicit uniform r in [0,1): returns the outcome. */
 int or_measure(double* s, int n, uint32_t qubit, double r, double* p0, double* p1);
+/* The same on a complex64 state held as complex128: kept amplitudes are
+ * scaled in FP64 and rounded to FP32 (a complex64 collapse). */
+int or_measure_f32(double* s, int n, uint32_t qubit, double r, double* p0, double* p1);
 /* Pairwise-tree sum of a power-of-two array (in place on a copy). */
 double or_ptree(const double* x, uint64_t len);
 /* Marginal probabilities, qubits[0] = MSB of the output (2^k entries). */

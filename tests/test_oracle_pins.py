@@ -307,3 +307,95 @@ def test_mlp_oracle_gradient_matches_finite_differences():
         e[i] = 1e-6
         fd = (mlp.loss(w + e, X, y) - mlp.loss(w - e, X, y)) / 2e-6
         assert abs(fd - g[i]) < 1e-6 + 1e-5 * abs(g[i]), i
+
+
+# ---------------------------------------------------------------------------
+# Result queries and samplers that the reference's tests do not cover
+# ---------------------------------------------------------------------------
+def _random_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def test_fidelity_matches_vdot_and_closed_forms():
+    # |<a|b>|^2 (simulator.rs:178-190): numpy's vdot conjugates its first
+    # argument; a dropped conjugate or a swapped real/imaginary part fails.
+    rng = np.random.default_rng(2024)
+    for n in (1, 3, 8, 11):
+        a, b = _random_state(n, rng), _random_state(n, rng)
+        want = abs(np.vdot(a, b)) ** 2
+        assert abs(oracle.fidelity(a, b) - want) <= 1e-14 + 1e-12 * want
+        assert abs(oracle.fidelity(a, a) - 1.0) < 1e-13
+        assert abs(oracle.fidelity(a, np.exp(0.731j) * a) - 1.0) < 1e-13  # global phase
+        assert abs(oracle.fidelity(a, b) - oracle.fidelity(b, a)) < 1e-15
+    zero, plus = np.array([1, 0], dtype=complex), np.array([1, 1], dtype=complex) / math.sqrt(2)
+    assert oracle.fidelity(zero, np.array([0, 1], dtype=complex)) == 0.0  # orthogonal
+    assert abs(oracle.fidelity(zero, plus) - 0.5) < 1e-15
+    # (1, i)/sqrt2 with itself: 1 with the conjugate, 0 without it.
+    y = np.array([1, 1j]) / math.sqrt(2)
+    assert abs(oracle.fidelity(y, y) - 1.0) < 1e-15
+    assert abs(oracle.fidelity(y, y.conj())) < 1e-15
+
+
+def _inverse_cdf(p, shots, seed):
+    # Textbook inverse-CDF sampling over the full sequential CDF.
+    cdf = np.cumsum(p) / p.sum()
+    cdf[-1] = 1.0
+    r = np.array([oracle.uniform53(seed, s, 0) for s in range(shots)])
+    return np.searchsorted(cdf, r, side="right")
+
+
+def test_batch_sample_matches_inverse_cdf_on_dyadic_states():
+    # Probabilities that are multiples of 2^-16 make every partial sum exact,
+    # so the chunked search (chunk totals, binary search, in-chunk scan) must
+    # pick exactly the outcome of plain inverse-CDF sampling, for any chunking.
+    rng = np.random.default_rng(7)
+    for n in (1, 4, 7, 10):
+        amp = rng.integers(0, 3, size=1 << n).astype(float)
+        amp[rng.integers(1 << n)] = 2.0  # at least one non-zero outcome
+        w = amp * amp  # 0, 1 or 4: |a|^2 of amp * 2^-8 is w * 2^-16 exactly
+        amp = amp * 2.0 ** -8
+        want = _inverse_cdf(w, 600, 31 + n)
+        for block in (1, 2, 16, 256, 1024):
+            got = oracle.batch_sample(amp.astype(complex), 600, 31 + n, block)
+            assert np.array_equal(got, want), (n, block)
+    # Zero-probability outcomes are never drawn, and a basis state always
+    # gives its own index.
+    st = np.zeros(64, dtype=complex)
+    st[37] = 1.0
+    assert set(oracle.batch_sample(st, 100, 3, 16).tolist()) == {37}
+
+
+def test_measure_f32_rounds_the_collapse_like_complex64():
+    # Kept amplitudes are scaled in FP64 by 1/sqrt(p) and rounded to FP32;
+    # restated here with numpy's float32 conversion.
+    rng = np.random.default_rng(12)
+    for n in (2, 6, 12):
+        st = _random_state(n, rng).astype(np.complex64).astype(np.complex128)
+        for q, r in ((0, 0.3), (n - 1, 0.8), (n // 2, 0.5)):
+            ref = st.copy()
+            got = st.copy()
+            out, p0, p1 = oracle.measure(got, q, r, f32=True)
+            out64, q0, q1 = oracle.measure(ref, q, r)
+            assert (out, p0, p1) == (out64, q0, q1)
+            keep = ((np.arange(1 << n) >> q) & 1) == out
+            scale = 1.0 / math.sqrt(p1 if out else p0)
+            want = np.where(keep, st, 0)
+            want = (want.real * scale).astype(np.float32) + 1j * (want.imag * scale).astype(np.float32)
+            assert np.array_equal(got, want.astype(np.complex128))
+            assert np.array_equal(ref[keep] * 0 + (st[keep].real * scale + 1j * st[keep].imag * scale), ref[keep])
+            st = got
+
+
+def test_terminal_trajectories_match_oracle_run_shots():
+    # The trajectory helper (prefix + ordered measurements, stream = shot,
+    # draw = measurement index) against the oracle's own trajectory driver.
+    n = 7
+    unitary = workloads.random_su2(n=n, depth=4, seed=3).data["instructions"]
+    measures = [(3, 0), (0, 1), (6, 2), (3, 3), (1, 1)]
+    ins = unitary + [{"kind": "measure", "qubit": q, "cbit": c} for q, c in measures]
+    circ = {"qregs": {"q": {"name": "q", "base": 0, "size": n}}, "cregs": {"c": {"name": "c", "base": 0, "size": 4}},
+            "instructions": ins}
+    prefix, _ = oracle.simulate({**circ, "instructions": unitary, "cregs": {}}, 1)
+    want, _ = oracle.run_shots(circ, 300, 19, sample_terminal=False)
+    assert oracle.terminal_trajectories(prefix, measures, 4, 300, 19) == want
