@@ -200,6 +200,10 @@ int dq_sv_simulate(dq_sv* sim, const char* json, size_t len, dq_result** out) {
   });
 }
 
+int dq_sv_evolve_noalloc(dq_sv* sim, const char* json, size_t len, void* state, uint64_t state_len, void* stream) {
+  return guard([&] { dq::evolve_into(sim->o, json, len, state, state_len, (hipStream_t)stream); });
+}
+
 uint32_t dq_result_num_qubits(const dq_result* r) { return r->r->n; }
 uint64_t dq_result_num_cbits(const dq_result* r) { return r->r->nc; }
 

@@ -1564,6 +1564,33 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
 // ---------------------------------------------------------------------------
 // Executor
 // ---------------------------------------------------------------------------
+// Launches of a unitary plan op on `nb` states of `stride` amplitudes at `s`
+// (plan constants at `cbase`); false for measure, reset and no-op, which need
+// the executor's buffers or do nothing.
+bool run_unitary(const DevOp& d, int prec, void* s, uint64_t stride, uint32_t nb, int n, const uint8_t* cbase,
+                 k::Flag f, bool mfma_enabled, hipStream_t st) {
+  switch (d.kind) {
+    case D_ONE: k::one(prec, s, stride, nb, n, d.e, d.target, d.u, f, st); return true;
+    case D_X: k::xgate(prec, s, stride, nb, n, d.e, d.target, f, st); return true;
+    case D_SWAP: k::swap(prec, s, stride, nb, n, d.e, d.a, d.b, f, st); return true;
+    case D_DIAG:
+      k::diag(prec, s, stride, nb, n, d.e, (int)d.pat.size(), d.pat.data(), d.dval.data(), f, st);
+      return true;
+    case D_DENSE: k::dense(prec, s, stride, nb, n, d.kq, d.e, cbase + d.mat_off, d.offsets, f, st); return true;
+    case D_FUSED:
+      if (d.mfma && mfma_enabled &&
+          k::fused_block_mfma(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.emb_off), d.offsets, st))
+        return true;
+      k::fused_block(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.mat_off), d.offsets, st);
+      return true;
+    case D_TILE:
+      if (d.exact) k::tile_sweep_c128(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
+      else k::tile_sweep(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
+      return true;
+    default: return false;
+  }
+}
+
 struct Exec {
   int prec, n;
   uint64_t nc, ncs, stride, c1;
@@ -1645,20 +1672,8 @@ struct Exec {
       f.stride = 1;
     }
     void* s = sp;
+    if (run_unitary(d, prec, s, stride, nb, n, cbase, f, mfma_enabled, st)) return;
     switch (d.kind) {
-      case D_ONE: k::one(prec, s, stride, nb, n, d.e, d.target, d.u, f, st); break;
-      case D_X: k::xgate(prec, s, stride, nb, n, d.e, d.target, f, st); break;
-      case D_SWAP: k::swap(prec, s, stride, nb, n, d.e, d.a, d.b, f, st); break;
-      case D_DIAG:
-        k::diag(prec, s, stride, nb, n, d.e, (int)d.pat.size(), d.pat.data(), d.dval.data(), f, st);
-        break;
-      case D_DENSE: k::dense(prec, s, stride, nb, n, d.kq, d.e, cbase + d.mat_off, d.offsets, f, st); break;
-      case D_FUSED:
-        if (d.mfma && mfma_enabled &&
-            k::fused_block_mfma(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.emb_off), d.offsets, st))
-          break;
-        k::fused_block(s, stride, nb, n, d.kq, d.e, (const float*)(cbase + d.mat_off), d.offsets, st);
-        break;
       case D_MEAS: measure(d.qubit, d.cbit, f, nb); break;
       case D_RESET: {
         measure(d.qubit, -1, f, nb);
@@ -1669,11 +1684,7 @@ struct Exec {
         k::xgate(prec, s, stride, nb, n, e, d.qubit, xf, st);
         break;
       }
-      case D_TILE:
-        if (d.exact) k::tile_sweep_c128(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
-        else k::tile_sweep(s, stride, nb, n, d.targs, cbase + d.tile_off, st);
-        break;
-      case D_NOOP: break;
+      default: break;
     }
   }
 };
@@ -1938,6 +1949,40 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   st.sync();
   res->state = std::move(ex.owned);
   return res;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free evolution into a caller's device state
+// ---------------------------------------------------------------------------
+// simulate()'s launch plan for a unitary circuit, with the plan constants
+// copied into this GPU's static scratch array (noalloc.hip) and the state in
+// the caller's buffer: no device allocation. Every check runs before any
+// device call. Returns after the work on `stream` has finished.
+void evolve_into(const Options& o, const char* json, size_t len, void* state, uint64_t state_len,
+                 hipStream_t stream) {
+  Parsed p = read_and_validate(json, len);
+  p.ops = compile(p.c.instructions);
+  for (const Op& op : p.ops)
+    if (!op.is_unitary())
+      throw ValueErr("the allocation-free evolution takes unitary circuits only (no measure, reset or conditional)");
+  if (p.n >= 64 || state_len != (1ull << p.n))
+    throw ValueErr("state buffer holds " + std::to_string(state_len) + " amplitudes; the circuit needs 2^" +
+                   std::to_string(p.n));
+  if (state == nullptr) throw ValueErr("state buffer is null");
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.tile);
+  uint64_t scratch_bytes = 0;
+  void* scratch = noalloc_scratch(&scratch_bytes);
+  if (plan.blob.size() > scratch_bytes)
+    throw ValueErr("the circuit's plan constants need " + std::to_string(plan.blob.size()) +
+                   " bytes; the allocation-free evolution holds at most " + std::to_string(scratch_bytes));
+  if (!plan.blob.empty())
+    HIPCHK(hipMemcpyAsync(scratch, plan.blob.data(), plan.blob.size(), hipMemcpyHostToDevice, stream));
+  const uint64_t stride = 1ull << p.n;
+  k::init_zero(o.prec, state, stride, 1, (int)p.n, stream);
+  const uint8_t* cbase = (const uint8_t*)scratch;
+  for (const DevOp& d : plan.ops) run_unitary(d, o.prec, state, stride, 1, (int)p.n, cbase, k::Flag{}, true, stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -2293,7 +2338,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
   static const char* names[] = {"one", "x", "swap", "diagonal", "dense", "fused", "measure", "reset", "noop", "tile"};
   std::ostringstream js;
   js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
-     << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions) << ", \"launches\": [";
+     << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions)
+     << ", \"blob_bytes\": " << plan.blob.size() << ", \"launches\": [";
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     const DevOp& d = plan.ops[i];
     js << (i ? ", " : "") << "{\"kind\": \"" << names[d.kind] << "\", \"k\": " << d.kq

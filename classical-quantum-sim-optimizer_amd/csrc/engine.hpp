@@ -121,6 +121,15 @@ struct EvolveStats {
 };
 
 std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len);
+// simulate() of a unitary circuit into a caller's device state (state_len =
+// 2^n amplitudes), plan constants in the static scratch array of
+// noalloc_scratch(); allocates no device memory and returns after `stream`
+// has finished.
+void evolve_into(const Options& o, const char* json, size_t len, void* state, uint64_t state_len,
+                 hipStream_t stream);
+// Device address of this GPU's allocation-free scratch array (resolved once
+// per device, with that device current) and its size.
+void* noalloc_scratch(uint64_t* bytes);
 Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t shots, bool profile);
 
 void result_probabilities(const Result& r, const std::vector<uint64_t>& qubits, double* out);

@@ -134,6 +134,7 @@ def _load():
                                           ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]),
         "dq_counts_free": (None, [P]),
         "dq_sv_simulate": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(P)]),
+        "dq_sv_evolve_noalloc": (ctypes.c_int, [P, ctypes.c_char_p, SZ, P, U64, P]),
         "dq_result_num_qubits": (ctypes.c_uint32, [P]),
         "dq_result_num_cbits": (U64, [P]),
         "dq_result_classical_bits": (ctypes.c_int, [P, P, U64]),

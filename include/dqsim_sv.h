@@ -128,6 +128,24 @@ int dq_sv_simulate_shots(dq_sv* sim, const char* json, size_t len, uint64_t shot
 /* StatevectorSimulator::simulate (simulator.rs:319-371). */
 int dq_sv_simulate(dq_sv* sim, const char* json, size_t len, dq_result** out);
 
+/* Allocation-free variant of dq_sv_simulate. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Evolves a unitary circuit (no measure, reset or conditional: DQ_EVALUE)
+ * from |0...0> into the caller's device buffer `state` of `state_len` = 2^n
+ * amplitudes in the simulator's precision (complex128 or complex64,
+ * interleaved re/im), with the launch plan and arithmetic of dq_sv_simulate
+ * on the same options, so the state equals that call's statevector bit for
+ * bit. Work runs on `stream` (a hipStream_t; NULL = the null stream) and the
+ * call returns when it has finished. The only other device memory used is a
+ * static per-GPU scratch array of DQ_NOALLOC_SCRATCH_BYTES for the plan's
+ * constants; a circuit whose constants need more is rejected with DQ_EVALUE
+ * before any device work. The array is shared by every call on a GPU and
+ * keeps the last call's contents: calls that overlap on one GPU race.
+ * max_memory_mb does not apply (the library allocates nothing). */
+#define DQ_NOALLOC_SCRATCH_BYTES (16ull << 20)
+int dq_sv_evolve_noalloc(dq_sv* sim, const char* json, size_t len, void* state, uint64_t state_len, void* stream);
+
 /* SimulationResult (simulator.rs:85-191). */
 uint32_t dq_result_num_qubits(const dq_result* r);
 uint64_t dq_result_num_cbits(const dq_result* r);

@@ -591,3 +591,20 @@ def test_packed_count_table_round_trip():
         assert t.cb(t.user, ctypes.cast(ctypes.c_char_p(b), ctypes.POINTER(ctypes.c_char)), len(b), count) == 0
     assert list(t.result().items()) == entries
     assert _core._CountTable().result() == {}
+
+
+def test_noalloc_evolution_validates_before_touching_the_device():
+    # dq_sv_evolve_noalloc: circuit, unitarity and buffer length are checked
+    # before any device call (a null buffer is never dereferenced).
+    sim = dqsim.StatevectorSimulator(seed=1)
+    measured = Circuit(3, [{"kind": "h", "qubit": 0}, {"kind": "measure", "qubit": 0, "cbit": 0}], 1)
+    bare = Circuit(3, [{"kind": "h", "qubit": 1}])
+    cond = Circuit(3, [{"kind": "conditional", "condition": {"creg_base": 0, "creg_size": 1, "creg_value": 1},
+                        "op": {"kind": "x", "qubit": 0}}], 1)
+    for circ, length, text in ((measured, 8, "unitary circuits only"), (cond, 8, "unitary circuits only"),
+                               (bare, 4, "holds 4 amplitudes"), (bare, 8, "null")):
+        js = _core._circuit_json(circ)
+        with pytest.raises(ValueError, match=text):
+            _core._check(_core._lib.dq_sv_evolve_noalloc(sim._h, js, len(js), None, length, None))
+    with pytest.raises(ValueError):
+        _core._check(_core._lib.dq_sv_evolve_noalloc(sim._h, b"{", 1, None, 8, None))
