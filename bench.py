@@ -22,10 +22,10 @@ sys.path.insert(0, os.path.join(ROOT, "classical-quantum-sim-optimizer_amd"))
 sys.path.insert(0, ROOT)
 
 KIND_NAMES = {0: "one", 1: "x", 2: "swap", 3: "diag", 4: "dense", 5: "fused_valu", 6: "measure", 7: "reset",
-              8: "fused_mfma", 9: "tile", 10: "tile128"}
+              8: "fused_mfma", 9: "tile", 10: "tile128", 11: "xsweep"}
 KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag": "k_diag", "dense": "k_dense",
                 "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse",
-                "tile": "k_tile", "tile128": "k_tile128"}
+                "tile": "k_tile", "tile128": "k_tile128", "xsweep": "k_xsweep"}
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
@@ -79,13 +79,14 @@ def profiled_mfma_busy(kernel, workload):
     return None
 
 
-def mfma_flops_per_step(circ, precision, fusion, max_fused, tile):
+def mfma_flops_per_step(circ, precision, fusion, max_fused, tile, out_of_place):
     """FP32 matrix-core flops of one step from the launch plan: a k-qubit
     block pass is 8 * 2^k * 2^n (one complex MAC = 8 real flops); 1-qubit
     blocks run padded to 2 qubits in tile sweeps."""
     from dqsim import _core
 
-    plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile)
+    plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile,
+                           out_of_place=out_of_place)
     n = plan["num_qubits"]
     flops = 0.0
     for op in plan["launches"]:
@@ -251,6 +252,10 @@ def main():
     ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
     ap.add_argument("--relabel", action="store_true",
                     help="relabelled tile sweeps (measured slower here; default: qubits 0..3 pinned in every tile)")
+    ap.add_argument("--sweep-mode", type=int, default=None, choices=[0, 1, 2, 3, 4],
+                    help="complex64 fused sweeps: 0 in place; out of place with 1 persistent 256-thread, "
+                         "2 one-shot 256-thread, 3 one-shot 512-thread, 4 persistent 512-thread workgroups "
+                         "(default: the library's)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=24)
@@ -293,6 +298,10 @@ def main():
     from dqsim import workloads
 
     dqsim._core.set_sweep_relabel(args.relabel)
+    if args.sweep_mode is not None:
+        dqsim._core.set_sweep_mode(args.sweep_mode)
+    sweep_mode = dqsim._core.set_sweep_mode(1)
+    dqsim._core.set_sweep_mode(sweep_mode)
     copy_gbs = device_copy_gbs()
     log(f"[bench] device copy (4 GiB, torch copy_): {copy_gbs:.0f} GB/s read+write")
     mfma_peak_tflops, _ = dqsim._core.mfma_fp32_peak(3)
@@ -326,7 +335,7 @@ def main():
     total_bytes = sum(a["bytes"] for a in agg.values()) / args.steps
     value = gates * float(1 << n) / (ms_per_step / 1e3)
 
-    mflops = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused, args.tile)
+    mflops = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused, args.tile, sweep_mode != 0)
     mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "fused_mfma")) / args.steps
     mfma = None
     if mflops > 0 and mfma_ms > 0:
@@ -401,7 +410,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
-                   "relabelled_sweeps": args.relabel, "parallelism": "single GPU"},
+                   "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "frac_of_device_copy": achieved / copy_gbs,

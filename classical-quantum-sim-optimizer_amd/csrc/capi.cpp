@@ -279,11 +279,11 @@ void dq_result_free(dq_result* r) { delete r; }
 const char* dq_last_error(void) { return g_err.c_str(); }
 
 int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
-                 int32_t tile_qubits, char** out) {
+                 int32_t tile_qubits, int32_t out_of_place, char** out) {
   return guard([&] {
     if (tile_qubits != 0 && (tile_qubits < dq::kTileMin || tile_qubits > dq::kTileMax))
       throw dq::ValueErr("tile_qubits must be 0 or in 10..12");
-    std::string s = dq::plan_description(json, len, precision, fusion, max_fused, tile_qubits);
+    std::string s = dq::plan_description(json, len, precision, fusion, max_fused, tile_qubits, out_of_place != 0);
     char* p = (char*)std::malloc(s.size() + 1);
     if (!p) throw std::bad_alloc();
     std::memcpy(p, s.c_str(), s.size() + 1);
@@ -310,6 +310,8 @@ void dq_string_free(char* s) { std::free(s); }
 int dq_device_count(void) { return dq::device_count(); }
 
 int dq_set_sweep_relabel(int32_t on) { return dq::set_sweep_relabel(on != 0) ? 1 : 0; }
+
+int dq_set_sweep_mode(int32_t mode) { return dq::set_sweep_mode(mode); }
 
 int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch) {
   return guard([&] { *tflops = dq::mfma_fp32_peak_tflops(reps, ms_per_launch); });

@@ -267,6 +267,12 @@ int device_count() {
 std::atomic<bool> g_sweep_relabel{false};
 bool set_sweep_relabel(bool on) { return g_sweep_relabel.exchange(on); }
 
+// Complex64 fused sweeps when the budget holds a second state: 0 in place
+// (k_tile), out of place (k_xsweep) with 1 persistent 256-thread, 2 one-shot
+// 256-thread, 3 one-shot 512-thread, 4 persistent 512-thread workgroups.
+std::atomic<int> g_sweep_mode{2};
+int set_sweep_mode(int mode) { return g_sweep_mode.exchange(mode < 0 || mode > 4 ? 2 : mode); }
+
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   int dev = 0, cus = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -318,7 +324,7 @@ uint64_t random_seed() {
 // ---------------------------------------------------------------------------
 // Device plan
 // ---------------------------------------------------------------------------
-enum DKind { D_ONE, D_X, D_SWAP, D_DIAG, D_DENSE, D_FUSED, D_MEAS, D_RESET, D_NOOP, D_TILE };
+enum DKind { D_ONE, D_X, D_SWAP, D_DIAG, D_DENSE, D_FUSED, D_MEAS, D_RESET, D_NOOP, D_TILE, D_XSWEEP };
 
 struct DevOp {
   DKind kind = D_NOOP;
@@ -351,6 +357,10 @@ struct DevOp {
   int chains = 0;      // exact sweeps: register chains / single-op passes
   int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
   std::vector<uint32_t> tile_lq;  // qubit of each local bit at the load (plan description)
+  // D_XSWEEP: out-of-place sweep from the current state buffer to the other
+  // one (blob at tile_off, as for D_TILE); `persist` picks the launch shape.
+  k::XSweepArgs xargs{};
+  bool persist = true;
 };
 
 // Planner features of a block of compiled ops (also used for single ops):
@@ -416,6 +426,7 @@ struct Plan {
   std::vector<DevOp> ops;
   std::vector<uint8_t> blob;  // matrices and condition chains
   DeviceMem dev;
+  bool uses_alt = false;  // out-of-place sweeps: the executor needs a second state buffer
   size_t push(const void* p, size_t bytes) {
     size_t off = (blob.size() + 255) & ~(size_t)255;
     blob.resize(off + bytes);
@@ -932,24 +943,19 @@ void restore_layout(BitLayout& lay, int L, int F, const std::function<void(const
   }
 }
 
-// Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
-// read and write of the state for all of them. The blob (descriptors, then
-// matrices) is what every workgroup copies into LDS.
-void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, const std::vector<Op>& ops,
-               int n) {
-  DevOp d;
-  d.kind = D_TILE;
-  d.category = 1;
-  d.gates = 0;
-  const int L = tm.L;
+// Blob of a tile sweep (TileBlock descriptors, then the block matrices) for
+// the tile whose LDS local bit b holds qubit cq[b]; fills the launch's block
+// bookkeeping (gates, source ops, features) and returns the blob's offset in
+// the plan, its 16-byte units and the widest block.
+struct TileBlob {
+  size_t off = 0;
+  uint32_t blob16 = 0, nblocks = 0, kmax = 0;
+};
+TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const uint32_t* cq, int L,
+                   const std::vector<Op>& ops, int n) {
+  TileBlob out;
   std::vector<int> local(64, -1);
-  const int nl = L;
-  for (int b = 0; b < L; ++b) local[tm.cq[b]] = b;  // LDS order (after the lane swaps)
-  d.touched.assign(tm.lq, tm.lq + L);
-  std::sort(d.touched.begin(), d.touched.end());
-  d.tile_lq.assign(tm.lq, tm.lq + L);
-  tm.to_args(d.targs);
-  d.targs.nblocks = (uint32_t)blocks.size();
+  for (int b = 0; b < L; ++b) local[cq[b]] = b;
   std::vector<k::TileBlock> tb;
   std::vector<float> mats;
   const size_t head = blocks.size() * sizeof(k::TileBlock);
@@ -961,8 +967,8 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, 
     if (orig->qubits.size() < 2) {
       padded.ops = orig->ops;
       std::vector<uint32_t> q = orig->qubits;
-      for (int i = nl - 1; i >= 0 && q.size() < 2; --i)
-        if (std::find(q.begin(), q.end(), tm.cq[i]) == q.end()) q.push_back(tm.cq[i]);
+      for (int i = L - 1; i >= 0 && q.size() < 2; --i)
+        if (std::find(q.begin(), q.end(), cq[i]) == q.end()) q.push_back(cq[i]);
       std::sort(q.rbegin(), q.rend());
       padded.qubits = q;
       b = &padded;
@@ -981,7 +987,7 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, 
     }
     std::sort(t.srt, t.srt + t.k);
     tb.push_back(t);
-    d.targs.kmax = std::max(d.targs.kmax, t.k);
+    out.kmax = std::max(out.kmax, t.k);
     d.gates += (int)b->ops.size();
     d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
     d.tile_blocks.push_back(orig->qubits);
@@ -990,10 +996,9 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, 
   std::vector<uint8_t> blob(head + mats.size() * sizeof(float));
   std::memcpy(blob.data(), tb.data(), head);
   std::memcpy(blob.data() + head, mats.data(), mats.size() * sizeof(float));
-  d.targs.blob16 = (uint32_t)(blob.size() / 16);
-  d.tile_off = plan.push(blob.data(), blob.size());
-  d.kq = L;
-  d.low = 0;
+  out.blob16 = (uint32_t)(blob.size() / 16);
+  out.nblocks = (uint32_t)blocks.size();
+  out.off = plan.push(blob.data(), blob.size());
   // Matrix-core work in units of a 3-qubit block (a 2-qubit block on the
   // 4x4x1 path costs half of one, a 4-qubit block counts apart).
   float n3 = 0.f;
@@ -1008,6 +1013,30 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, 
   d.feat[13] = L / 12.f;
   d.feat[14] = n3 / 16.f;
   d.feat[15] = n4 / 16.f;
+  return out;
+}
+
+// Several fused blocks applied to 2^L-amplitude tiles held in LDS: one HBM
+// read and write of the state for all of them. The blob (descriptors, then
+// matrices) is what every workgroup copies into LDS.
+void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, const std::vector<Op>& ops,
+               int n) {
+  DevOp d;
+  d.kind = D_TILE;
+  d.category = 1;
+  d.gates = 0;
+  const int L = tm.L;
+  d.touched.assign(tm.lq, tm.lq + L);
+  std::sort(d.touched.begin(), d.touched.end());
+  d.tile_lq.assign(tm.lq, tm.lq + L);
+  tm.to_args(d.targs);
+  const TileBlob tbl = tile_blob(plan, d, blocks, tm.cq, L, ops, n);  // LDS order (after the lane swaps)
+  d.targs.nblocks = tbl.nblocks;
+  d.targs.kmax = tbl.kmax;
+  d.targs.blob16 = tbl.blob16;
+  d.tile_off = tbl.off;
+  d.kq = L;
+  d.low = 0;
   plan.ops.push_back(std::move(d));
 }
 
@@ -1180,6 +1209,258 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
     while (head < rem.size() && taken[rem[head]]) ++head;
   }
   restore();
+}
+
+// One out-of-place sweep launch: tile of the qubits local[0..L) (local bit j
+// holds local[j]), read from the layout `in_pos` (qubit -> bit position;
+// contiguous when local bit j sits at position j) and written to the layout
+// `out_pos`, with `blocks` applied in LDS.
+void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* local, int L, const uint32_t* in_pos,
+                  const uint32_t* out_pos, int n, const std::vector<Op>& ops, bool persist, int threads) {
+  DevOp d;
+  d.kind = D_XSWEEP;
+  d.category = 1;
+  d.gates = 0;
+  d.persist = persist;
+  k::XSweepArgs& a = d.xargs;
+  a.L = (uint32_t)L;
+  a.ntb = (uint32_t)(n - L);
+  a.threads = (uint32_t)threads;
+  std::vector<int> in_at(64, -1);
+  for (int q = 0; q < n; ++q) in_at[in_pos[q]] = q;
+  std::vector<char> is_local(64, 0), src_local(64, 0);
+  bool contiguous = true;
+  for (int j = 0; j < L; ++j) {
+    is_local[local[j]] = 1;
+    src_local[in_pos[local[j]]] = 1;
+    a.rpos[j] = in_pos[local[j]];
+    contiguous = contiguous && in_pos[local[j]] == (uint32_t)j;
+  }
+  a.gather = contiguous ? 0u : 1u;
+  std::copy(a.rpos, a.rpos + L, a.rsrt);
+  std::sort(a.rsrt, a.rsrt + L);
+  // Store order: the tile's destination positions ascending.
+  std::vector<std::pair<uint32_t, uint32_t>> dst;  // (destination position, local bit)
+  for (int j = 0; j < L; ++j) dst.push_back({out_pos[local[j]], (uint32_t)j});
+  std::sort(dst.begin(), dst.end());
+  for (int j = 0; j < L; ++j) {
+    a.wpos[j] = dst[j].first;
+    a.wlocal[j] = dst[j].second;
+  }
+  // Tile-index bit k: the k-th lowest source position outside the tile.
+  uint32_t k = 0;
+  for (int p = 0; p < n; ++p)
+    if (!src_local[p]) a.wtile[k++] = out_pos[in_at[p]];
+  // Both maps must be bijections onto positions 0..n-1 (the kernel's reads
+  // and writes then stay inside the state and cover it once).
+  uint64_t rseen = 0, wseen = 0;
+  for (int j = 0; j < L; ++j) {
+    rseen |= 1ull << a.rpos[j];
+    wseen |= 1ull << a.wpos[j];
+  }
+  for (uint32_t t = 0; t < k; ++t) wseen |= 1ull << a.wtile[t];
+  const uint64_t all = n == 64 ? ~0ull : (1ull << n) - 1;
+  if ((int)k != n - L || __builtin_popcountll(rseen) != L || (rseen & ~all) || wseen != all)
+    throw RuntimeErr("internal: bad out-of-place sweep layout");
+  d.touched.assign(local, local + L);
+  std::sort(d.touched.begin(), d.touched.end());
+  d.tile_lq.assign(local, local + L);
+  const TileBlob tbl = tile_blob(plan, d, blocks, local, L, ops, n);
+  a.nblocks = tbl.nblocks;
+  a.kmax = tbl.kmax;
+  a.blob16 = tbl.blob16;
+  d.tile_off = tbl.off;
+  d.kq = L;
+  d.low = 0;
+  plan.uses_alt = true;
+  plan.ops.push_back(std::move(d));
+}
+
+// Out-of-place (permuting) tile sweeps of a unitary run ops[begin, end),
+// complex64 with fusion. Sweeps read their tiles from one state buffer and
+// write them to the other, so the bit layout (qubit -> position) may change
+// at every sweep: each sweep writes its tile so that the next sweep's qubits
+// sit at positions 0..L-1, whose tiles are then single contiguous runs of
+// 2^L amplitudes. The anchor qubits 0..3 stay at positions 0..3 throughout
+// (every tile holds them, as in emit_sweeps): a store then moves runs of at
+// least 16 amplitudes (128 bytes), longer by the qubits that the next sweep
+// keeps (those go right above the anchor), and its LDS reads stay free of
+// bank conflicts. A run starts and ends in the identity layout: its first
+// sweep gathers its tile (runs of 16 amplitudes), its last stores to the
+// qubits' own positions. Grouping of blocks into sweeps follows emit_sweeps;
+// the tile is padded with the qubits the next blocks use first, which keeps
+// consecutive sweeps overlapping.
+void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int prec, int n, int L, int kt,
+                  const MergeScorer& scorer, bool persist, int threads) {
+  constexpr size_t kWindow = 512;
+  constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = 64;
+  const uint64_t anchor = (1ull << std::min(4, n)) - 1;
+  std::vector<FusedBlock> pre = greedy_fuse(ops, begin, end, 2);
+  std::vector<uint64_t> mask(pre.size());
+  for (size_t b = 0; b < pre.size(); ++b) mask[b] = qubit_mask(pre[b].qubits);
+  auto tileable = [&](size_t b) { return !pre[b].qubits.empty() && pre[b].qubits.size() <= 4; };
+  std::vector<int> rem(pre.size());
+  for (size_t b = 0; b < pre.size(); ++b) rem[b] = (int)b;
+  std::vector<char> taken(pre.size(), 0);
+  std::vector<int> mark(pre.size(), -1);
+  int trial_no = 0;
+  auto trial = [&](size_t head, int seed, uint64_t& smask) {
+    const int gen = ++trial_no;
+    std::vector<int> take;
+    smask = anchor;
+    if (seed >= 0) {
+      mark[seed] = gen;
+      take.push_back(seed);
+      smask |= mask[seed];
+    }
+    for (;;) {
+      uint64_t blocked = 0;
+      int best = -1, best_grow = 99;
+      size_t seen = 0;
+      for (size_t r = head; r < rem.size() && seen < kWindow; ++r) {
+        const int b = rem[r];
+        if (taken[b] || mark[b] == gen) continue;
+        ++seen;
+        if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(smask | mask[b]) <= L) {
+          const int grow = __builtin_popcountll(mask[b] & ~smask);
+          if (grow < best_grow) {
+            best = b;
+            best_grow = grow;
+            if (grow == 0) break;
+          }
+        }
+        blocked |= mask[b];
+      }
+      if (best < 0 || take.size() >= 4 * kTileBlocksMax) break;
+      mark[best] = gen;
+      take.push_back(best);
+      smask |= mask[best];
+    }
+    return take;
+  };
+  struct Group {
+    std::vector<int> blocks;  // pre-fused blocks in execution order
+    uint64_t q = 0;           // tile qubits (exactly L)
+    int standalone = -1;      // a block run as its own launch in the identity layout
+  };
+  std::vector<Group> groups;
+  size_t head = 0;
+  while (head < rem.size()) {
+    const int h = rem[head];
+    if (!tileable((size_t)h)) {
+      Group g;
+      g.standalone = h;
+      groups.push_back(g);
+      taken[h] = 1;
+      ++head;
+      continue;
+    }
+    constexpr int kStarts = 8;
+    uint64_t smask = 0;
+    std::vector<int> sweep = trial(head, -1, smask);
+    {
+      uint64_t blocked = 0;
+      int starts = 0;
+      for (size_t r = head; r < rem.size() && starts < kStarts; ++r) {
+        const int b = rem[r];
+        if (taken[b]) continue;
+        if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(anchor | mask[b]) <= L) {
+          ++starts;
+          uint64_t m2 = 0;
+          std::vector<int> t = trial(head, b, m2);
+          if (t.size() > sweep.size()) {
+            sweep = std::move(t);
+            smask = m2;
+          }
+        }
+        blocked |= mask[b];
+      }
+    }
+    for (int b : sweep) taken[b] = 1;
+    // Pad the tile with the qubits the next blocks use first.
+    std::vector<uint32_t> first(64, UINT32_MAX);
+    {
+      size_t seen = 0;
+      for (size_t r = head; r < rem.size() && seen < kWindow; ++r) {
+        const int b = rem[r];
+        if (taken[b]) continue;
+        for (uint32_t q : pre[b].qubits)
+          if (first[q] == UINT32_MAX) first[q] = (uint32_t)seen;
+        ++seen;
+      }
+    }
+    std::vector<uint32_t> cand;
+    for (int q = 0; q < n; ++q) cand.push_back((uint32_t)q);
+    std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return first[a] < first[b]; });
+    for (uint32_t q : cand) {
+      if (__builtin_popcountll(smask) >= L) break;
+      smask |= 1ull << q;
+    }
+    Group g;
+    g.blocks = std::move(sweep);
+    g.q = smask;
+    groups.push_back(std::move(g));
+    while (head < rem.size() && taken[rem[head]]) ++head;
+  }
+  // Layouts and launches, one run of sweeps at a time.
+  std::vector<uint32_t> ident(64);
+  for (uint32_t q = 0; q < 64; ++q) ident[q] = q;
+  size_t gi = 0;
+  while (gi < groups.size()) {
+    if (groups[gi].standalone >= 0) {
+      emit_block(plan, pre[groups[gi].standalone], ops, prec, n);
+      ++gi;
+      continue;
+    }
+    size_t gj = gi;
+    while (gj < groups.size() && groups[gj].standalone < 0) ++gj;
+    // Sweep gi reads the identity layout: local order = its qubits ascending
+    // (the anchor first).
+    std::vector<uint32_t> cur_pos = ident, order;
+    for (int q = 0; q < n; ++q)
+      if ((groups[gi].q >> q) & 1) order.push_back((uint32_t)q);
+    for (size_t s = gi; s < gj; ++s) {
+      const Group& g = groups[s];
+      std::vector<uint32_t> next_pos = ident, next_order;
+      if (s + 1 < gj) {
+        const uint64_t nq = groups[s + 1].q;
+        for (uint32_t q : order)
+          if ((nq >> q) & 1) next_order.push_back(q);  // kept qubits, anchor first, in this sweep's order
+        for (int q = 0; q < n; ++q)
+          if (((nq >> q) & 1) && !((g.q >> q) & 1)) next_order.push_back((uint32_t)q);
+        if ((int)next_order.size() != L) throw RuntimeErr("internal: out-of-place sweep tile size");
+        uint32_t p = 0;
+        for (uint32_t q : next_order) next_pos[q] = p++;
+        for (int q = 0; q < n; ++q)
+          if (!((nq >> q) & 1)) next_pos[q] = p++;
+      }
+      // The sweep's ops fused again into blocks of <= kt qubits, split where
+      // the blob would exceed its LDS budget; only the last launch changes
+      // the layout.
+      std::vector<int> opsorder;
+      for (int b : g.blocks) opsorder.insert(opsorder.end(), pre[b].ops.begin(), pre[b].ops.end());
+      std::vector<FusedBlock> blocks = greedy_fuse_list(ops, opsorder, std::max(1, std::min(kt, 4)), scorer);
+      size_t i = 0;
+      do {
+        size_t blob = 0, j = i;
+        while (j < blocks.size() && j - i < kTileBlocksMax) {
+          const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
+          if (blob + bb > kTileBlobMax) break;
+          blob += bb;
+          ++j;
+        }
+        std::vector<FusedBlock*> bp;
+        for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
+        const bool last = j >= blocks.size();
+        emit_xlaunch(plan, bp, order.data(), L, cur_pos.data(), last ? next_pos.data() : cur_pos.data(), n, ops,
+                     persist, threads);
+        i = j;
+      } while (i < blocks.size());
+      cur_pos = next_pos;
+      order = next_order;
+    }
+    gi = gj;
+  }
 }
 
 // Exact complex128 sweeps: maximal runs of consecutive compiled ops (program
@@ -1491,9 +1772,14 @@ void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, siz
   restore();
 }
 
+// `permute`: complex64 fused sweeps may run out of place (emit_xsweeps) when
+// the process-wide sweep mode allows it; the caller then gives the executor
+// room for a second state buffer.
 Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, int fusion, int max_fused, int n,
-                int tile) {
+                int tile, bool permute = false) {
   Plan plan;
+  const int mode = g_sweep_mode.load();
+  permute = permute && mode != 0;
   size_t i = begin;
   const int lc = std::min(tile, 11);  // complex128 tiles: 16 * 2^L bytes of LDS
   while (i < end) {
@@ -1549,6 +1835,9 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
           best_l = def_l;
         }
         emit_sweeps(plan, ops, i, j, prec, n, best_l, best_k, nullptr);
+      } else if (permute) {
+        emit_xsweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer, mode == 1 || mode == 4,
+                     mode >= 3 && tile >= 11 ? 512 : 256);
       } else {
         emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
       }
@@ -1597,9 +1886,30 @@ struct Exec {
   uint32_t batch;
   hipStream_t st;
   DeviceMem owned, cbits, rng, ms, flags, err, red_a, red_b;
-  void* sp = nullptr;  // state of slot 0; slots follow at `stride` amplitudes
+  DeviceMem alt_mem;      // second state buffer of out-of-place sweeps
+  void* sp = nullptr;     // state of slot 0; slots follow at `stride` amplitudes
+  void* alt = nullptr;    // the other buffer (out-of-place sweeps write here, then the two trade roles)
   const uint8_t* cbase = nullptr;
   bool mfma_enabled = true;
+
+  // Plan constants, and the second state buffer when the plan sweeps out of
+  // place (only for states the executor owns).
+  void prepare(const Plan& plan) {
+    cbase = plan.base();
+    if (plan.uses_alt && alt == nullptr) {
+      if (!owned.get()) throw RuntimeErr("internal: out-of-place sweeps on a borrowed state");
+      alt_mem.alloc(stride * (uint64_t)prec * batch);
+      alt = alt_mem.get();
+    }
+  }
+  // After the plan: `owned` holds the current state again.
+  void settle() {
+    if (alt != nullptr && sp == alt_mem.get()) {
+      std::swap(owned, alt_mem);
+      sp = owned.get();
+      alt = alt_mem.get();
+    }
+  }
 
   // With `external` set, the executor works on a caller-owned state.
   Exec(int prec_, int n_, uint64_t nc_, uint32_t batch_, hipStream_t st_, void* external = nullptr)
@@ -1674,6 +1984,11 @@ struct Exec {
     void* s = sp;
     if (run_unitary(d, prec, s, stride, nb, n, cbase, f, mfma_enabled, st)) return;
     switch (d.kind) {
+      case D_XSWEEP:
+        if (!d.conds.empty() || alt == nullptr) throw RuntimeErr("internal: out-of-place sweep without its buffers");
+        k::xsweep(sp, alt, stride, nb, n, d.xargs, cbase + d.tile_off, d.persist, st);
+        std::swap(sp, alt);
+        break;
       case D_MEAS: measure(d.qubit, d.cbit, f, nb); break;
       case D_RESET: {
         measure(d.qubit, -1, f, nb);
@@ -1698,7 +2013,8 @@ double op_bytes(const DevOp& d, int n, int prec) {
     case D_DIAG: return 2.0 * prec * len / std::ldexp(1.0, d.kq) * (double)d.pat.size();
     case D_DENSE:
     case D_FUSED:
-    case D_TILE: return 2.0 * prec * len;
+    case D_TILE:
+    case D_XSWEEP: return 2.0 * prec * len;
     case D_MEAS:
     case D_RESET: return 3.0 * prec * len;
     default: return 0.0;
@@ -1707,6 +2023,7 @@ double op_bytes(const DevOp& d, int n, int prec) {
 
 int op_kind_code(const DevOp& d) {
   switch (d.kind) {
+    case D_XSWEEP: return 11;
     case D_ONE: return 0;
     case D_X: return 1;
     case D_SWAP: return 2;
@@ -1892,10 +2209,13 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   Stream st;
   // A profiled run times each compiled op by its reference category, so it
   // runs one launch per op.
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.profile ? 0 : o.tile);
+  // Out-of-place sweeps need room for a second state within the budget.
+  const bool permute = !o.profile && lay.bytes <= o.budget - (lay.bytes + p.nc);
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, (int)p.n, o.profile ? 0 : o.tile,
+                         permute);
   plan.upload(st.s);
   Exec ex(o.prec, (int)p.n, p.nc, 1, st.s);
-  ex.cbase = plan.base();
+  ex.prepare(plan);
   k::init_zero(o.prec, ex.sp, ex.stride, 1, (int)p.n, st.s);
   k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
   auto res = std::make_unique<Result>();
@@ -1947,6 +2267,7 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   res->cbits.assign(p.nc, 0);
   if (p.nc) HIPCHK(hipMemcpyAsync(res->cbits.data(), ex.cbits.get(), p.nc, hipMemcpyDeviceToHost, st.s));
   st.sync();
+  ex.settle();
   res->state = std::move(ex.owned);
   return res;
 }
@@ -2046,10 +2367,12 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
     const uint64_t cb = cdf_bytes(m);
     if (worker_bytes <= UINT64_MAX - cb && worker_bytes + cb <= o.budget) {
       Stream st;
-      Plan plan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n, o.tile);
+      const uint64_t used = worker_bytes + cb;
+      Plan plan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n, o.tile,
+                             lay.bytes <= o.budget - used);
       plan.upload(st.s);
       Exec ex(o.prec, n, nc, 1, st.s);
-      ex.cbase = plan.base();
+      ex.prepare(plan);
       k::init_zero(o.prec, ex.sp, ex.stride, 1, n, st.s);
       k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
       for (const DevOp& d : plan.ops) ex.run(d, 1);
@@ -2081,28 +2404,28 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
   workers = std::max<uint64_t>(workers, 1);
 
   Stream st;
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), o.prec, o.fusion, o.max_fused, n, o.tile);
+  // With a cached prefix, the unitary prefix ops[0, prefix_end) has its own
+  // plan, run once on one state (out of place when the budget holds a second
+  // state while the prefix runs); the batched tail sweeps in place.
+  Plan plan = build_plan(p.ops, cache_prefix ? prefix_end : 0, p.ops.size(), o.prec, o.fusion, o.max_fused, n,
+                         o.tile);
   plan.upload(st.s);
-  // Fused runs never cross a non-unitary op, so the plan's leading unitary
-  // launches cover exactly ops[0, prefix_end).
-  size_t plan_prefix = 0;
-  while (plan_prefix < plan.ops.size()) {
-    const DevOp& d = plan.ops[plan_prefix];
-    if (!d.conds.empty() || d.kind == D_MEAS || d.kind == D_RESET) break;
-    ++plan_prefix;
-  }
   DeviceMem prefix;
   if (cache_prefix) {
+    Plan pplan = build_plan(p.ops, 0, prefix_end, o.prec, o.fusion, o.max_fused, n, o.tile,
+                            lay.bytes <= budget - worker_bytes);
+    pplan.upload(st.s);
     Exec pex(o.prec, n, nc, 1, st.s);
-    pex.cbase = plan.base();
+    pex.prepare(pplan);
     k::init_zero(o.prec, pex.sp, pex.stride, 1, n, st.s);
-    for (size_t i = 0; i < plan_prefix; ++i) pex.run(plan.ops[i], 1);
+    for (const DevOp& d : pplan.ops) pex.run(d, 1);
     st.sync();
+    pex.settle();
     prefix = std::move(pex.owned);
   }
-  const size_t tail_begin = cache_prefix ? plan_prefix : 0;
+  const size_t tail_begin = 0;
   Exec ex(o.prec, n, nc, (uint32_t)workers, st.s);
-  ex.cbase = plan.base();
+  ex.prepare(plan);
   std::vector<uint8_t> all(shots * ex.ncs);
   if (profile) shot_times.assign(shots, 0.0);
   for (uint64_t s0 = 0; s0 < shots; s0 += workers) {
@@ -2276,10 +2599,11 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
   const size_t pe = prefix_end_of(p.ops);
   const int n = (int)p.n;
   Stream st;
-  Plan plan = build_plan(p.ops, 0, pe, o.prec, o.fusion, o.max_fused, n, o.tile);
+  Plan plan = build_plan(p.ops, 0, pe, o.prec, o.fusion, o.max_fused, n, o.tile,
+                         lay.bytes <= o.budget - (lay.bytes + p.nc));
   plan.upload(st.s);
   Exec ex(o.prec, n, p.nc, 1, st.s);
-  ex.cbase = plan.base();
+  ex.prepare(plan);
   EvolveStats stats;
   stats.gates = count_gates(p.c.instructions);
   stats.launches = plan.ops.size();
@@ -2321,6 +2645,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
     res->n = (uint32_t)n;
     res->nc = p.nc;
     res->cbits.assign(p.nc, 0);
+    ex.settle();
     res->state = std::move(ex.owned);
     *keep = std::move(res);
   }
@@ -2330,12 +2655,14 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
 // ---------------------------------------------------------------------------
 // Plan description (host only)
 // ---------------------------------------------------------------------------
-std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile) {
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile,
+                             bool permute) {
   Parsed p = read_and_validate(json, len);
   p.ops = compile(p.c.instructions);
   const size_t pe = prefix_end_of(p.ops);
-  Plan plan = build_plan(p.ops, 0, p.ops.size(), prec, fusion, max_fused, (int)p.n, tile);
-  static const char* names[] = {"one", "x", "swap", "diagonal", "dense", "fused", "measure", "reset", "noop", "tile"};
+  Plan plan = build_plan(p.ops, 0, p.ops.size(), prec, fusion, max_fused, (int)p.n, tile, permute);
+  static const char* names[] = {"one",     "x",     "swap", "diagonal", "dense", "fused",
+                                "measure", "reset", "noop", "tile",     "tile"};
   std::ostringstream js;
   js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
      << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions)
@@ -2366,6 +2693,32 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       list("load_positions", d.targs.spos, d.targs.L);
       list("store_positions", d.targs.opos, d.targs.L);
       list("swaps", d.targs.sbit, d.targs.nswap);
+      js << ", \"blocks\": [";
+      for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
+        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";
+        for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
+        js << "]}";
+      }
+      js << "]";
+    }
+    if (d.kind == D_XSWEEP) {
+      // Out-of-place sweep: read position of each local bit (contiguous:
+      // local bit j at position j), destination position of each local bit
+      // and of each tile-index bit, and the blocks.
+      const k::XSweepArgs& a = d.xargs;
+      auto list = [&](const char* key, const uint32_t* v, uint32_t cnt) {
+        js << ", \"" << key << "\": [";
+        for (uint32_t t = 0; t < cnt; ++t) js << (t ? ", " : "") << v[t];
+        js << "]";
+      };
+      uint32_t store[12] = {};
+      for (uint32_t j = 0; j < a.L; ++j) store[a.wlocal[j]] = a.wpos[j];
+      js << ", \"exact\": false, \"out_of_place\": true, \"gather\": " << (a.gather ? "true" : "false")
+         << ", \"persist\": " << (d.persist ? "true" : "false");
+      list("local_qubits", d.tile_lq.data(), (uint32_t)d.tile_lq.size());
+      list("load_positions", a.rpos, a.L);
+      list("store_positions", store, a.L);
+      list("tile_store_positions", a.wtile, a.ntb);
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";

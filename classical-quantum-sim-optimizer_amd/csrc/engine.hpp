@@ -149,7 +149,12 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
 void result_compare(const Result& a, const Result& b, double* max_abs, double* l2);
 
 // Compile-only plan description (no device use), as JSON.
-std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile);
+std::string plan_description(const char* json, size_t len, int prec, int fusion, int max_fused, int tile,
+                             bool permute);
+// Sweep mode of complex64 fused plans whose budget holds a second state
+// (0 in place, 1 out of place with persistent workgroups, 2 out of place with
+// one workgroup per tile); returns the previous mode.
+int set_sweep_mode(int mode);
 
 int device_count();
 

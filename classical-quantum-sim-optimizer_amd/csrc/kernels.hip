@@ -495,7 +495,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // so lane l (u = l/16) feeds the four K steps from inputs 2u and 2u+1 of
 // group l%16 and receives outputs 2u and 2u+1 of that group whole.
 // `e` holds the A fragments: e[64 s + l] = embedding[l % 16][4 s + l / 16].
-template <int L>
+template <int L, int NTH = TPB>
 __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock& b, const float* e) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, u = lane >> 4;
@@ -509,14 +509,14 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
   // 16-group tiles per wave, U of them per iteration with independent MFMA
   // chains interleaved.
   constexpr int NT = (1 << L) / 8 / 16;  // 16-group tiles of the block
-  constexpr int PERW = NT / (TPB / 64);   // L >= 10: at least 2 per wave
+  constexpr int PERW = NT / (NTH / 64);   // L >= 10: at least 2 per wave
   constexpr int U = PERW >= 4 ? 4 : PERW;
-  static_assert(NT % (TPB / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
+  static_assert(NT % (NTH / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
   // Masked-carry stepping: with the block's bits set, adding IZ(step) carries
   // over them, so base = IZ(mt * 16) advances by 3 scalar operations.
   const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(16 * (TPB / 64), srt, 3);
+  const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
   uint32_t base = insert_zeros32(w * 16, srt, 3);
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
@@ -557,7 +557,7 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
 // row 4h + i of the lane's own group. Two accumulators (h = 0, 1) give the
 // lane its group's 4 outputs: 0.5 MFMA cycles per amplitude, half of a
 // 3-qubit block's. `e` holds the A fragments: e[16 i + 8 h + kk] = E[4h+i][kk].
-template <int L>
+template <int L, int NTH = TPB>
 __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& b, const float* e) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float a[16];
@@ -578,12 +578,12 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
   for (int j = 0; j < 4; ++j) off[j] = local_offset<2>(b, j) ^ lane_part;
   // 64-group rows per wave, U of them per iteration with independent chains.
   constexpr int NR = (1 << L) / 4 / 64;  // 64-group rows of the block
-  constexpr int PERW = NR / (TPB / 64);  // L >= 10: at least 1 per wave
+  constexpr int PERW = NR / (NTH / 64);  // L >= 10: at least 1 per wave
   constexpr int U = 1;
-  static_assert(NR % (TPB / 64) == 0 && PERW % U == 0, "64-group rows split evenly over the waves");
+  static_assert(NR % (NTH / 64) == 0 && PERW % U == 0, "64-group rows split evenly over the waves");
   const uint32_t mask = (1u << srt[0]) | (1u << srt[1]);
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(64 * (TPB / 64), srt, 2);
+  const uint32_t step = insert_zeros32(64 * (NTH / 64), srt, 2);
   uint32_t base = insert_zeros32(w * 64, srt, 2);
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
@@ -623,7 +623,7 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
 
 // k = 4 on v_mfma_f32_32x32x2f32 with the operand layout of k_fused_mfma.
 // `emb` holds the A fragments: emb[64 q + l] = E[l % 32][2 q + l / 32].
-template <int L>
+template <int L, int NTH = TPB>
 __device__ __forceinline__ void tile_block_mfma32(float2* tile, const TileBlock& b, const float* emb) {
   constexpr int K = 4, STEPS = 16, NIN = 8, NOUT = 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -639,14 +639,14 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, const TileBlock&
   // 32-group tiles per wave (at L = 10 only waves 0 and 1 have one), U per
   // iteration with independent MFMA chains sharing each A operand read.
   constexpr int NT = (1 << L) / 16 / 32;
-  constexpr int PERW = (NT + TPB / 64 - 1) / (TPB / 64);
+  constexpr int PERW = (NT + NTH / 64 - 1) / (NTH / 64);
   constexpr int U = PERW >= 2 ? 2 : 1;
   static_assert(PERW % U == 0, "32-group tiles per wave split evenly");
   const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]) | (1u << srt[3]);
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(32 * (TPB / 64), srt, K);
+  const uint32_t step = insert_zeros32(32 * (NTH / 64), srt, K);
   uint32_t base = insert_zeros32(w * 32, srt, K);
-  if (NT < TPB / 64 && (int)w >= NT) return;  // wave-uniform: no 32-group tile for this wave
+  if (NT < NTH / 64 && (int)w >= NT) return;  // wave-uniform: no 32-group tile for this wave
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
     uint32_t bs[U];
@@ -756,6 +756,167 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       __syncthreads();  // every wave has read this tile out of LDS
 #pragma unroll
       for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Out-of-place (permuting) tile sweep, complex64 (XSweepArgs in kernels.hpp).
+// ---------------------------------------------------------------------------
+// Bit scatter of the low `nb` bits of v to positions pos[0..nb).
+__device__ __forceinline__ uint64_t scatter_bits(uint64_t v, const uint32_t* pos, uint32_t nb) {
+  uint64_t o = 0;
+  for (uint32_t k = 0; k < nb; ++k) o |= ((v >> k) & 1ull) << pos[k];
+  return o;
+}
+
+// The blocks of one sweep applied to the LDS tile, a barrier after each.
+template <int L, bool K4, int NTH>
+__device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock* blocks, const unsigned char* meta,
+                                                  uint32_t nblocks) {
+  for (uint32_t bi = 0; bi < nblocks; ++bi) {
+    const TileBlock& b = blocks[bi];
+    const float* m = (const float*)(meta + b.mat_off);
+    if (b.kind == 3) tile_block_mfma4<L, NTH>(tile, b, m);
+    else if (!K4 || b.kind == 1) tile_block_mfma16<L, NTH>(tile, b, m);
+    else tile_block_mfma32<L, NTH>(tile, b, m);
+    __syncthreads();
+  }
+}
+
+// Store of a tile from LDS in output order: store index u = tid + TPB * i
+// reads local index lo_l | hi_l(i) and writes destination offset lo_w | hi_w(i)
+// (plus the tile's base), so each wave's stores cover runs of consecutive
+// addresses as long as the low destination positions are tile bits.
+template <int L, int NTH>
+__device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const float2* tile, uint64_t wbase,
+                                             const XSweepArgs& a) {
+  constexpr int PER = (1 << L) / NTH;
+  constexpr int LB = NTH == 512 ? 9 : 8;  // thread-index bits
+  uint32_t lo_l = 0;
+  uint64_t lo_w = 0;
+#pragma unroll
+  for (int bit = 0; bit < LB; ++bit)
+    if ((threadIdx.x >> bit) & 1) {
+      lo_l |= 1u << a.wlocal[bit];
+      lo_w |= 1ull << a.wpos[bit];
+    }
+  // Opaque to the optimiser, so the PER per-store offsets are formed per use
+  // instead of being hoisted out of the tile loop into 3 * PER VGPRs.
+  asm volatile("" : "+v"(lo_l), "+v"(lo_w));
+  uint32_t hl[L - LB];
+  uint64_t hw[L - LB];
+#pragma unroll
+  for (int bit = 0; bit < L - LB; ++bit) {
+    hl[bit] = 1u << a.wlocal[LB + bit];
+    hw[bit] = 1ull << a.wpos[LB + bit];
+  }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    uint32_t l = lo_l;
+    uint64_t w = wbase | lo_w;
+#pragma unroll
+    for (int bit = 0; bit < L - LB; ++bit)
+      if ((i >> bit) & 1) {
+        l |= hl[bit];
+        w |= hw[bit];
+      }
+    dst[w] = tile[l];
+  }
+}
+
+// Source offsets of local index tid + TPB * i: contiguous (the tile is one
+// run starting at tau << L) or gathered from positions rpos.
+template <int L, int NTH>
+struct XSweepRead {
+  static constexpr int LB = NTH == 512 ? 9 : 8;  // thread-index bits
+  uint64_t lo;
+  uint64_t hm[L - LB];
+  __device__ __forceinline__ explicit XSweepRead(const XSweepArgs& a) {
+    lo = 0;
+    if (a.gather) {
+#pragma unroll
+      for (int bit = 0; bit < LB; ++bit)
+        if ((threadIdx.x >> bit) & 1) lo |= 1ull << a.rpos[bit];
+#pragma unroll
+      for (int bit = 0; bit < L - LB; ++bit) hm[bit] = 1ull << a.rpos[LB + bit];
+    } else {
+      lo = threadIdx.x;
+#pragma unroll
+      for (int bit = 0; bit < L - LB; ++bit) hm[bit] = (uint64_t)NTH << bit;
+    }
+    asm volatile("" : "+v"(lo));
+  }
+  __device__ __forceinline__ uint64_t base(uint64_t tau, const XSweepArgs& a) const {
+    if (!a.gather) return tau << L;
+    uint64_t b = tau;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const uint64_t low = (1ull << a.rsrt[i]) - 1;
+      b = (b & low) | ((b & ~low) << 1);
+    }
+    return b;
+  }
+  __device__ __forceinline__ uint64_t off(int i) const {
+    uint64_t o = lo;
+#pragma unroll
+    for (int bit = 0; bit < L - LB; ++bit)
+      if ((i >> bit) & 1) o |= hm[bit];
+    return o;
+  }
+};
+
+template <int L, bool K4, bool PERSIST, int NTH>
+__global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                uint64_t stride, uint64_t ntiles, XSweepArgs a,
+                                                const float4* __restrict__ blob) {
+  extern __shared__ float4 lds4[];
+  constexpr int PER = (1 << L) / NTH;
+  float2* tile = (float2*)lds4;
+  const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L) / 2);
+  for (uint32_t i = threadIdx.x; i < a.blob16; i += NTH) lds4[(1 << L) / 2 + i] = blob[i];
+  const TileBlock* blocks = (const TileBlock*)meta;
+  const float2* sb = src + (uint64_t)blockIdx.y * stride;
+  float2* db = dst + (uint64_t)blockIdx.y * stride;
+  const XSweepRead<L, NTH> rd(a);
+  float2 r[PER];
+  uint64_t tau = blockIdx.x;
+  if (!PERSIST) {
+    const uint64_t b0 = rd.base(tau, a);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
+    __syncthreads();  // the tile and the blob are in LDS
+    tile_apply_blocks<L, K4, NTH>(tile, blocks, meta, a.nblocks);
+    xsweep_store<L, NTH>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
+    return;
+  }
+  // Persistent: the next tile's loads are in flight while this tile's blocks
+  // run in LDS (as in k_tile).
+  if (tau < ntiles) {
+    const uint64_t b0 = rd.base(tau, a);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
+  }
+  __syncthreads();
+  for (; tau < ntiles; tau += gridDim.x) {
+    const uint64_t nxt = tau + gridDim.x;
+    const bool more = nxt < ntiles;
+    if (more) {
+      const uint64_t nb = rd.base(nxt, a);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) r[i] = sb[nb | rd.off(i)];
+    }
+    tile_apply_blocks<L, K4, NTH>(tile, blocks, meta, a.nblocks);
+    xsweep_store<L, NTH>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
+    if (more) {
+      __syncthreads();  // every wave has read this tile out of LDS
+#pragma unroll
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
     }
     __syncthreads();
   }
@@ -1850,6 +2011,35 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
     case 11: if (k4) L(11, true); else L(11, false); break;
     default: if (k4) L(12, true); else L(12, false); break;
   }
+#undef L
+}
+
+size_t xsweep_lds_bytes(const XSweepArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
+
+void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
+            const void* blob, bool persist, hipStream_t st) {
+  if (a.L < 10 || a.L > 12 || (int)a.L > n || a.ntb != (uint32_t)(n - (int)a.L))
+    throw std::runtime_error("xsweep: tile size does not match the state");
+  if (a.threads != 256 && !(a.threads == 512 && a.L >= 11))
+    throw std::runtime_error("xsweep: 256 threads, or 512 for tiles of 11-12 qubits");
+  const uint64_t ntiles = 1ull << (n - (int)a.L);
+  const size_t lds = xsweep_lds_bytes(a);
+  dim3 g((uint32_t)(persist ? tile_grid(ntiles) : ntiles), batch);
+#define L(LL, K4, P, NT)                                                                                          \
+  hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \
+                     ntiles, a, (const float4*)blob)
+#define LP(LL, K4, NT) \
+  if (persist) L(LL, K4, true, NT); else L(LL, K4, false, NT)
+#define LT(LL, K4) \
+  if (a.threads == 512) { LP(LL, K4, 512); } else { LP(LL, K4, 256); }
+  const bool k4 = a.kmax >= 4;
+  switch (a.L) {
+    case 10: if (k4) { LP(10, true, 256); } else { LP(10, false, 256); } break;
+    case 11: if (k4) { LT(11, true); } else { LT(11, false); } break;
+    default: if (k4) { LT(12, true); } else { LT(12, false); } break;
+  }
+#undef LT
+#undef LP
 #undef L
 }
 

@@ -156,6 +156,38 @@ size_t tile_lds_bytes(const TileArgs& a);
 void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob_dev,
                 hipStream_t st);
 
+// ---- out-of-place (permuting) tile sweep (complex64) ----
+// Reads every tile from `src` and writes it to `dst` (distinct buffers), so
+// the state may change its bit layout (qubit -> bit position) from sweep to
+// sweep. Tile tau's local bit j is read from position j (contiguous mode: the
+// tile is one contiguous run of 2^L amplitudes) or, in gather mode, from
+// position rpos[j] with the tile index spread over the remaining positions in
+// ascending order. The blocks (TileBlock array, as in tile_sweep) address
+// local bits. The store walks the tile in output order: store index u (bit j
+// of u < L) takes local index sum bit_j(u) << wlocal[j] and goes to position
+// wpos[j]; tile-index bit k goes to position wtile[k]. With wpos ascending,
+// consecutive lanes write consecutive addresses.
+struct XSweepArgs {
+  uint32_t L;          // tile qubits
+  uint32_t nblocks;
+  uint32_t blob16;     // blob size in 16-byte units
+  uint32_t kmax;       // widest block (3 or 4)
+  uint32_t gather;     // 0: contiguous read; 1: read positions rpos
+  uint32_t ntb;        // tile-index bits (n - L)
+  uint32_t threads;    // workgroup size: 256, or 512 for L >= 11
+  uint32_t pad[3];
+  uint32_t rpos[12];   // gather: source position of local bit j
+  uint32_t rsrt[12];   // gather: rpos sorted ascending
+  uint32_t wlocal[12]; // store index bit j -> local bit
+  uint32_t wpos[12];   // store index bit j -> destination position
+  uint32_t wtile[52];  // tile-index bit k -> destination position
+};
+size_t xsweep_lds_bytes(const XSweepArgs& a);
+// persist: workgroups walk tiles with a one-tile prefetch (grid 64 per CU);
+// otherwise one workgroup per tile.
+void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
+            const void* blob_dev, bool persist, hipStream_t st);
+
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);
 // FP32 matrix-core peak microbenchmark (`out` holds mfma_peak_out_floats(grid)

@@ -148,7 +148,9 @@ def _load():
         "dq_result_gather": (ctypes.c_int, [P, P, U64, P]),
         "dq_result_compare": (ctypes.c_int, [P, P, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_last_error": (ctypes.c_char_p, []),
-        "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, I32, ctypes.POINTER(ctypes.c_void_p)]),
+        "dq_plan_json": (ctypes.c_int, [ctypes.c_char_p, SZ, I32, I32, I32, I32, I32,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+        "dq_set_sweep_mode": (ctypes.c_int, [I32]),
         "dq_string_free": (None, [P]),
         "dq_device_count": (ctypes.c_int, []),
         "dq_mfma_fp32_peak": (ctypes.c_int, [I32, ctypes.POINTER(D), ctypes.POINTER(D)]),
@@ -540,14 +542,16 @@ class StabilizerResult:
 # ---------------------------------------------------------------------------
 # Host-only helpers
 # ---------------------------------------------------------------------------
-def plan_json(circuit_or_json, precision="complex128", fusion="none", max_fused_qubits=4, tile_qubits=12) -> dict:
-    """Launch plan of a circuit (parse, validate, compile, fuse; no GPU)."""
+def plan_json(circuit_or_json, precision="complex128", fusion="none", max_fused_qubits=4, tile_qubits=12,
+              out_of_place=False) -> dict:
+    """Launch plan of a circuit (parse, validate, compile, fuse; no GPU).
+    out_of_place: the plan of a call whose budget holds a second state."""
     data = circuit_or_json if isinstance(circuit_or_json, (bytes, str)) else _circuit_json(circuit_or_json)
     if isinstance(data, str):
         data = data.encode("utf-8")
     out = ctypes.c_void_p()
     _check(_lib.dq_plan_json(data, len(data), _PRECISIONS[precision], _FUSIONS[fusion], int(max_fused_qubits),
-                             int(tile_qubits), ctypes.byref(out)))
+                             int(tile_qubits), 1 if out_of_place else 0, ctypes.byref(out)))
     try:
         return json.loads(ctypes.string_at(out).decode("utf-8"))
     finally:
@@ -571,6 +575,14 @@ def set_sweep_relabel(on: bool) -> bool:
     sweeps, but slower sweeps); returns the previous setting. Results do not
     depend on it (complex128 exact sweeps stay bit-identical)."""
     return bool(_lib.dq_set_sweep_relabel(1 if on else 0))
+
+
+def set_sweep_mode(mode: int) -> int:
+    """Sweep mode of complex64 fused plans for this process: 0 in place,
+    1 out of place with persistent workgroups (default), 2 out of place with
+    one workgroup per tile. Out-of-place sweeps run only when a call's budget
+    holds a second state. Returns the previous mode."""
+    return int(_lib.dq_set_sweep_mode(int(mode)))
 
 
 def set_planner_weights(weights) -> None:

@@ -102,7 +102,7 @@ void dq_counts_free(dq_counts* t);
 typedef struct dq_op_timing {
   int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA),
                     9 tile sweep (k = tile qubits, gates = ops applied),
-                    10 exact complex128 tile sweep */
+                    10 exact complex128 tile sweep, 11 out-of-place tile sweep */
   int32_t k;     /* qubits touched */
   int32_t low;   /* lowest qubit touched */
   int32_t gates; /* compiled ops folded into the launch */
@@ -180,9 +180,11 @@ const char* dq_last_error(void);
 /* ---- no reference counterpart: host-only inspection and measurement ---- */
 
 /* Parse, validate, compile and fuse without touching the GPU; writes a JSON
- * description of the launch plan. Free it with dq_string_free. */
+ * description of the launch plan. out_of_place != 0 describes the plan of a
+ * call whose budget holds a second state (complex64 fused sweeps may then run
+ * out of place, see dq_set_sweep_mode). Free it with dq_string_free. */
 int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
-                 int32_t tile_qubits, char** out);
+                 int32_t tile_qubits, int32_t out_of_place, char** out);
 void dq_string_free(char* s);
 
 /* ---- P-block backend: the state as a product of dense blocks ----
@@ -236,6 +238,17 @@ int dq_pb_expectation(const dq_pb_result* r, const char* pauli, size_t len, doub
  * whole run measured about 5 % slower, hence off by default. No counterpart in
  * the reference: a tuning and A/B switch. Returns the previous setting. */
 int dq_set_sweep_relabel(int32_t on);
+
+/* Process-wide sweep mode of complex64 fused plans (no counterpart in the
+ * reference: a tuning and A/B switch). When a call's max_memory_mb holds a
+ * second state, its tile sweeps may run out of place: each sweep reads its
+ * tiles from one state buffer as contiguous runs and writes them to the other
+ * in the bit layout the next sweep reads contiguously; the last sweep of a
+ * run restores the qubits' own positions. 0 = always in place, 1 = out of
+ * place with persistent workgroups (default), 2 = out of place with one
+ * workgroup per tile. Results stay within complex64 rounding of each other
+ * (blocks may group differently). Returns the previous mode. */
+int dq_set_sweep_mode(int32_t mode);
 
 /* Number of visible GPUs (0 on a host without one). */
 int dq_device_count(void);

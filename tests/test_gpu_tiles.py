@@ -99,3 +99,26 @@ def test_relabelled_sweeps_match_oracle_and_fixed_low(L, k):
             dqsim._core.set_sweep_relabel(prev)
         assert np.abs(states[on] - want).max() < 1e-5
     assert np.abs(states[True] - states[False]).max() < 1e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("kind,n,L,k", [("su2", 20, 12, 3), ("su2", 18, 11, 4), ("mixed", 17, 12, 3),
+                                        ("qft", 16, 10, 2)], ids=lambda v: str(v))
+def test_sweep_modes_match_oracle(mode, kind, n, L, k):
+    # In-place sweeps (0) and out-of-place sweeps (1-4: persistent or
+    # one-shot workgroups of 256 or 512 threads; 512 only for 11-12 qubit
+    # tiles, 256 otherwise) all within the complex64 tolerance of the oracle,
+    # with and without room for the second state in the budget.
+    print(f"[sweep-mode] {mode} {kind} n={n} L={L} k={k}", flush=True)
+    circ = build(kind, n)
+    want = oracle.simulate(circ.data, 1)[0]
+    old = dqsim._core.set_sweep_mode(mode)
+    try:
+        state_mb = (1 << n) * 8 >> 20
+        for budget in (max(1, state_mb) + 1, 4 * state_mb + 64):
+            sim = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                             tile_qubits=L, max_memory_mb=budget)
+            got = sim.simulate(circ).statevector
+            assert np.abs(got - want).max() < 1e-5, budget
+    finally:
+        dqsim._core.set_sweep_mode(old)

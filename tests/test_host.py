@@ -608,3 +608,77 @@ def test_noalloc_evolution_validates_before_touching_the_device():
             _core._check(_core._lib.dq_sv_evolve_noalloc(sim._h, js, len(js), None, length, None))
     with pytest.raises(ValueError):
         _core._check(_core._lib.dq_sv_evolve_noalloc(sim._h, b"{", 1, None, 8, None))
+
+
+def _xsweep_move(state, launch, n, L):
+    """Data movement of one out-of-place sweep launch (blocks left out): the
+    kernel's read and write index maps restated with numpy."""
+    t = np.arange(1 << L, dtype=np.uint64)
+    tau = np.arange(1 << (n - L), dtype=np.uint64)
+    rpos, spos = launch["load_positions"], launch["store_positions"]
+    if launch["gather"]:
+        local = sum(((t >> np.uint64(j)) & np.uint64(1)) << np.uint64(p) for j, p in enumerate(rpos))
+        free = [p for p in range(n) if p not in rpos]
+        base = sum(((tau >> np.uint64(k)) & np.uint64(1)) << np.uint64(p) for k, p in enumerate(free))
+    else:
+        assert rpos == list(range(L))
+        local, base = t, tau << np.uint64(L)
+    wlocal = sum(((t >> np.uint64(j)) & np.uint64(1)) << np.uint64(p) for j, p in enumerate(spos))
+    wbase = sum(((tau >> np.uint64(k)) & np.uint64(1)) << np.uint64(p)
+                for k, p in enumerate(launch["tile_store_positions"]))
+    src = (base[:, None] | local[None, :]).ravel()
+    dst = (wbase[:, None] | wlocal[None, :]).ravel()
+    assert src.max() < (1 << n) and dst.max() < (1 << n)
+    assert np.unique(src).size == src.size and np.unique(dst).size == dst.size
+    out = np.empty_like(state)
+    out[dst] = state[src]
+    return out
+
+
+@pytest.mark.parametrize("kind,n,L,k", [("su2", 16, 12, 3), ("su2", 18, 12, 4), ("su2", 17, 11, 3),
+                                        ("mixed", 17, 12, 3), ("qft", 16, 10, 2), ("su2", 20, 12, 2)])
+def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k):
+    # Tag every amplitude with its index and push the tags through the
+    # launches' read/write maps: each launch's tile qubits must sit where the
+    # current layout holds them, the blocks must act on tile qubits, and a run
+    # of sweeps must end with every amplitude back at its own index.
+    c = {"su2": lambda: workloads.random_su2(n=n, depth=10, seed=n + k),
+         "mixed": lambda: workloads.mixed(n, 5, seed=n),
+         "qft": lambda: workloads.qft(n, seed=2, measure=False)[0]}[kind]()
+    plan = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=L,
+                           out_of_place=True)
+    launches = plan["launches"]
+    assert any(x.get("out_of_place") for x in launches)
+    # Each run of sweeps (between standalone launches) gathers its first tile only.
+    for i, x in enumerate(launches):
+        if x.get("out_of_place"):
+            assert x["gather"] == (i == 0 or not launches[i - 1].get("out_of_place")), i
+    tags = np.arange(1 << n, dtype=np.uint64)
+    pos = list(range(n))  # qubit -> bit position
+    for x in plan["launches"]:
+        if not x.get("out_of_place"):
+            assert pos == list(range(n))  # other launches see the identity layout
+            continue
+        lq = x["local_qubits"]
+        assert [pos[q] for q in lq] == x["load_positions"]
+        assert all(set(b["qubits"]) <= set(lq) for b in x["blocks"])
+        assert {0, 1, 2, 3} <= set(lq)
+        tags = _xsweep_move(tags, x, n, L)
+        free = [q for q in sorted(range(n), key=lambda q: pos[q]) if q not in lq]
+        new = [None] * n
+        for j, q in enumerate(lq):
+            new[q] = x["store_positions"][j]
+        for kk, q in enumerate(free):
+            new[q] = x["tile_store_positions"][kk]
+        pos = new
+        # A tag (original index) now sits at the index its bits map to.
+        probe = np.array([5, (1 << n) - 3, 12345 % (1 << n)], dtype=np.uint64)
+        moved = sum(((probe >> np.uint64(q)) & np.uint64(1)) << np.uint64(pos[q]) for q in range(n))
+        assert np.array_equal(tags[moved.astype(np.int64)], probe)
+        # Stores move runs of at least 16 amplitudes (positions 0..3 are tile bits).
+        assert set(range(4)) <= set(x["store_positions"])
+    assert pos == list(range(n))
+    assert np.array_equal(tags, np.arange(1 << n, dtype=np.uint64))
+    # Same ops as the in-place plan, each once.
+    ops = sorted(o for x in plan["launches"] for o in x["ops"])
+    assert ops == list(range(plan["compiled_ops"]))
