@@ -2135,7 +2135,7 @@ std::vector<uint64_t> sample_indices(Exec& ex, const std::vector<uint64_t>& qubi
   if (m <= kMatMax) {
     DeviceMem bins = reduce_virtual(ex, map, n - m);
     const uint64_t count = 1ull << m;
-    DeviceMem C(8 * count), gt(8 * ((count + 1023) / 1024));
+    DeviceMem C(8 * count), gt(8 * k::cdf_tmp_entries(count));
     k::cdf_build(bins.as<double>(), 2, count, C.as<double>(), gt.as<double>(), tot.as<double>(), ex.err.as<int>(),
                  ex.st);
     k::sample_cdf(C.as<double>(), count, tot.as<double>(), seed, 0, shots, idx.as<uint64_t>(), ex.st);
@@ -2144,7 +2144,7 @@ std::vector<uint64_t> sample_indices(Exec& ex, const std::vector<uint64_t>& qubi
     if (n - m > 11) throw ValueErr("marginal sampling over more than 2^11 amplitudes per outcome is not supported");
     DeviceMem blk = reduce_virtual(ex, map, n - m + 11);
     const uint64_t nblocks = 1ull << (m - 11);
-    DeviceMem C(8 * nblocks), gt(8 * ((nblocks + 1023) / 1024));
+    DeviceMem C(8 * nblocks), gt(8 * k::cdf_tmp_entries(nblocks));
     k::cdf_build(blk.as<double>(), 2, nblocks, C.as<double>(), gt.as<double>(), tot.as<double>(), ex.err.as<int>(),
                  ex.st);
     k::sample_blocked(ex.prec, ex.sp, n, map.data(), n - m, C.as<double>(), nblocks, tot.as<double>(), seed,

@@ -1581,89 +1581,98 @@ __global__ void k_rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_
 }
 
 // ---------------------------------------------------------------------------
-// CDF (grouped sequential prefix) and sampling
+// CDF (block scan) and sampling
 // ---------------------------------------------------------------------------
-constexpr uint64_t kGroup = 1024;
+// Canonical inclusive prefix P of bins x[0..N): blocks of kScanBlock bins,
+// each scanned by one workgroup (block_scan below); above one block, the
+// block totals T[g] (each block's last output) are scanned the same way,
+// recursively, and P[j] = O[g] + L[j] with O[g] = 0 for g = 0, else the scan
+// of T at g - 1. block_scan of y[0..m), thread t owning y[8t .. 8t+8):
+//   p[e]   sequential: p[0] = y[8t], p[e] = p[e-1] + y[8t+e] (0 past m);
+//   R      the thread totals p[7] scanned in rows of 16 lanes (Hillis-Steele:
+//          for d = 1, 2, 4, 8 lane i adds lane i-d of its row, DPP row_shr);
+//   S[i]   = RO[row] + R[i], RO = 0, R[15], RO[1] + R[31], RO[2] + R[47];
+//   TO[t]  = WO[w] + E[t], E = S of the previous lane (0 for lane 0), WO the
+//          sequential prefix of the four wave totals S[63] (WO[0] = 0);
+//   out    L[8t+e] = TO[t] + p[e].
+// Every addition of zero is exact, so the bins' order of addition is fixed
+// and the CPU restatement reproduces P bit for bit.
+constexpr uint32_t kScanBlock = 2048, kScanPer = 8;
 
-// In-place sequential prefix of v[0..len) in LDS (inclusive: v[i] = acc
-// after adding v[i]; exclusive: v[i] = acc before), one addition at a time
-// in index order; returns the final acc. Loads run eight ahead of the adds.
-template <bool Inclusive>
-__device__ __forceinline__ double lds_seq_scan(double* v, uint32_t len, double acc) {
-  uint32_t i = 0;
-  for (; i + 8 <= len; i += 8) {
-    double x[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = v[i + u];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (Inclusive) {
-        acc = acc + x[u];
-        v[i + u] = acc;
-      } else {
-        v[i + u] = acc;
-        acc = acc + x[u];
-      }
-    }
-  }
-  for (; i < len; ++i) {
-    const double x = v[i];
-    if (Inclusive) {
-      acc = acc + x;
-      v[i] = acc;
-    } else {
-      v[i] = acc;
-      acc = acc + x;
-    }
-  }
-  return acc;
+// LDS slot of bin i of a block: one pad slot per 8 bins, so a thread's 8
+// consecutive bins sit 9 slots after its neighbour's (conflict-free).
+__device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + (i >> 3); }
+
+// Value held by lane i-d of the same 16-lane row (0.0 for the row's first d
+// lanes): DPP row_shr:d on both 32-bit halves.
+template <int D>
+__device__ __forceinline__ double row_shr_or_zero(double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x110 + D, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x110 + D, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// One 64-lane workgroup per group of kGroup bins: coalesced loads into LDS,
-// one lane's sequential inclusive prefix (from 0.0), coalesced stores;
-// gt[g] = the group total.
-__global__ __launch_bounds__(64) void k_cdf_local(const double* __restrict__ bins, uint64_t bstride, uint64_t count,
-                                                  double* __restrict__ C, double* __restrict__ gt) {
-  __shared__ double v[kGroup];
-  const uint64_t g = blockIdx.x;
-  const uint64_t lo = g * kGroup;
-  const uint32_t len = (uint32_t)(lo + kGroup < count ? kGroup : count - lo);
-  for (uint32_t i = threadIdx.x; i < len; i += 64) v[i] = bins[(lo + i) * bstride];
+// One workgroup per block: x[(lo + i) * xstride] for i < m -> L[lo + i];
+// bt[block] = L[lo + m - 1].
+__global__ __launch_bounds__(256) void k_scan_block(const double* __restrict__ x, uint64_t xstride, uint64_t count,
+                                                    double* __restrict__ L, double* __restrict__ bt) {
+  __shared__ double v[kScanBlock + kScanBlock / 8];
+  __shared__ double wt[4];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t lo = (uint64_t)blockIdx.x * kScanBlock;
+  const uint32_t m = (uint32_t)(count - lo < kScanBlock ? count - lo : kScanBlock);
+  for (uint32_t i = t; i < kScanBlock; i += 256) v[scan_slot(i)] = i < m ? x[(lo + i) * xstride] : 0.0;
   __syncthreads();
-  if (threadIdx.x == 0) gt[g] = lds_seq_scan<true>(v, len, 0.0);
+  double p[kScanPer];
+  double acc = v[scan_slot(kScanPer * t)];
+  p[0] = acc;
+#pragma unroll
+  for (uint32_t e = 1; e < kScanPer; ++e) {
+    acc = acc + v[scan_slot(kScanPer * t + e)];
+    p[e] = acc;
+  }
+  double r = acc;
+  r = r + row_shr_or_zero<1>(r);
+  r = r + row_shr_or_zero<2>(r);
+  r = r + row_shr_or_zero<4>(r);
+  r = r + row_shr_or_zero<8>(r);
+  const double rt0 = lane_value(r, 15), rt1 = lane_value(r, 31), rt2 = lane_value(r, 47);
+  const double ro1 = rt0, ro2 = ro1 + rt1, ro3 = ro2 + rt2;
+  const uint32_t row = lane >> 4;
+  const double sv = (row == 0 ? 0.0 : row == 1 ? ro1 : row == 2 ? ro2 : ro3) + r;
+  double ex = __shfl_up(sv, 1, 64);
+  if (lane == 0) ex = 0.0;
+  if (lane == 63) wt[w] = sv;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < len; i += 64) C[lo + i] = v[i];
+  const double wo1 = wt[0], wo2 = wo1 + wt[1], wo3 = wo2 + wt[2];
+  const double to = (w == 0 ? 0.0 : w == 1 ? wo1 : w == 2 ? wo2 : wo3) + ex;
+#pragma unroll
+  for (uint32_t e = 0; e < kScanPer; ++e) v[scan_slot(kScanPer * t + e)] = to + p[e];
+  __syncthreads();
+  for (uint32_t i = t; i < m; i += 256) L[lo + i] = v[scan_slot(i)];
+  if (t == 0) bt[blockIdx.x] = v[scan_slot(m - 1)];
 }
 
-// Group totals -> sequential exclusive prefix in place, in chunks staged
-// through LDS (one workgroup); *total = the sum of every group.
-constexpr uint32_t kOffChunk = 4096;
-__global__ __launch_bounds__(64) void k_cdf_offsets(double* __restrict__ gt, uint64_t ngroups,
-                                                    double* __restrict__ total, int* __restrict__ err) {
-  __shared__ double v[kOffChunk];
-  __shared__ double carry;
-  if (threadIdx.x == 0) carry = 0.0;
-  for (uint64_t c0 = 0; c0 < ngroups; c0 += kOffChunk) {
-    const uint32_t len = (uint32_t)(ngroups - c0 < kOffChunk ? ngroups - c0 : kOffChunk);
-    for (uint32_t i = threadIdx.x; i < len; i += 64) v[i] = gt[c0 + i];
-    __syncthreads();
-    if (threadIdx.x == 0) carry = lds_seq_scan<false>(v, len, carry);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < len; i += 64) gt[c0 + i] = v[i];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double last = carry;
-    *total = last;
-    if (!(isfinite(last) && last > 0.0)) atomicOr(err, 2);
-  }
-}
-
-__global__ __launch_bounds__(TPB) void k_cdf_add(double* __restrict__ C, uint64_t count,
-                                                 const double* __restrict__ gt) {
-  const uint64_t j = (uint64_t)blockIdx.x * TPB + threadIdx.x;
+// P[j] = O[g] + L[j] for the blocks g >= 1 (block 0's offset is 0).
+__global__ __launch_bounds__(TPB) void k_scan_offsets(double* __restrict__ P, uint64_t count,
+                                                      const double* __restrict__ q) {
+  const uint64_t j = kScanBlock + (uint64_t)blockIdx.x * TPB + threadIdx.x;
   if (j >= count) return;
-  C[j] = gt[j / kGroup] + C[j];
+  P[j] = q[j / kScanBlock - 1] + P[j];
+}
+
+__global__ void k_scan_total(const double* __restrict__ P, uint64_t count, double* __restrict__ total,
+                             int* __restrict__ err) {
+  const double last = P[count - 1];
+  *total = last;
+  if (!(isfinite(last) && last > 0.0)) atomicOr(err, 2);
 }
 
 __device__ __forceinline__ double cdf_u(const double* C, uint64_t j, uint64_t count, double total) {
@@ -2130,13 +2139,32 @@ void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hi
   hipLaunchKernelGGL(k_rng_init, dim3(blocks_for(batch, 64)), dim3(64), 0, st, rng, batch, seed, stream0);
 }
 
+// Canonical scan of `count` bins (stride bstride doubles) into C; tmp holds
+// the block totals of every level (cdf_tmp_entries(count) doubles).
+static void scan_into(const double* x, uint64_t xstride, uint64_t count, double* P, double* tmp, hipStream_t st) {
+  const uint64_t nb = (count + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(k_scan_block, dim3((uint32_t)nb), dim3(256), 0, st, x, xstride, count, P, tmp);
+  if (nb == 1) return;
+  // Block totals scanned in place (their own totals after them), then the
+  // offsets added to every block after the first.
+  scan_into(tmp, 1, nb, tmp, tmp + nb, st);
+  hipLaunchKernelGGL(k_scan_offsets, dim3(blocks_for(count - kScanBlock, TPB)), dim3(TPB), 0, st, P, count,
+                     (const double*)tmp);
+}
+
+uint64_t cdf_tmp_entries(uint64_t count) {
+  uint64_t total = 0;
+  for (uint64_t c = count; c > 1;) {
+    c = (c + kScanBlock - 1) / kScanBlock;
+    total += c;
+  }
+  return total + 1;
+}
+
 void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* gt, double* total, int* err,
                hipStream_t st) {
-  const uint64_t ngroups = (count + kGroup - 1) / kGroup;
-  if (ngroups > 0)
-    hipLaunchKernelGGL(k_cdf_local, dim3((uint32_t)ngroups), dim3(64), 0, st, bins, bstride, count, C, gt);
-  hipLaunchKernelGGL(k_cdf_offsets, dim3(1), dim3(64), 0, st, gt, ngroups, total, err);
-  hipLaunchKernelGGL(k_cdf_add, dim3(blocks_for(count, TPB)), dim3(TPB), 0, st, C, count, (const double*)gt);
+  scan_into(bins, bstride, count, C, gt, st);
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(1), 0, st, (const double*)C, count, total, err);
 }
 
 void sample_cdf(const double* C, uint64_t count, const double* total, uint64_t seed, uint64_t stream0,

@@ -227,9 +227,13 @@ void cond_eval(const uint8_t* cbits, uint64_t nc, uint32_t batch, const Cond* co
 void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hipStream_t st);
 
 // ---- CDF and sampling ----
-// Grouped CDF: C[j] = O[g] + L[j] with L the sequential inclusive prefix in
-// groups of 1024 and O the sequential exclusive prefix of group totals.
-// Writes C (count entries) and total = C[count-1] into *total.
+// Canonical inclusive prefix of `count` bins (stride bstride doubles): blocks
+// of 2048 bins scanned by one workgroup each (8 consecutive bins per thread
+// summed in order, thread totals scanned within 16-lane rows, then rows,
+// then waves, each by fixed additions), block totals scanned recursively and
+// added as block offsets. Writes C (count entries) and total = C[count-1]
+// into *total; group_tmp holds cdf_tmp_entries(count) doubles.
+uint64_t cdf_tmp_entries(uint64_t count);
 void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* group_tmp, double* total,
                int* err, hipStream_t st);
 // Materialised sampling: idx = first j with C[j]/total > r (last -> 1.0).
@@ -270,8 +274,9 @@ void pb_extract_ms(void* out, const void* in, uint64_t half, uint32_t q, const M
 void pb_basis_ms(void* out, const MeasState* ms, uint32_t batch, hipStream_t st);
 void pb_norms(double* out, const void* s, uint64_t count, hipStream_t st);  // |s[i]|^2
 // CDF tables of blocks of at most 1024 amplitudes in one launch: cdf[j] is
-// the sequential inclusive prefix of |amps[i]|^2 (cdf_build's value for a
-// single group, bit for bit) and totals[slot] its last entry.
+// the sequential inclusive prefix of |amps[i]|^2 (blocks this small keep the
+// sequential order of the P-block restatement; larger blocks take
+// cdf_build's block scan) and totals[slot] its last entry.
 struct PbSmallTable {
   const void* amps;
   double* cdf;

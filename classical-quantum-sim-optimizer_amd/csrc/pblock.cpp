@@ -810,7 +810,7 @@ class Pool {
       }
       auto norms = std::make_shared<DeviceMem>(8 * b.size);
       auto cdf = std::make_shared<DeviceMem>(8 * b.size);
-      auto gt = std::make_shared<DeviceMem>(8 * ((b.size + 1023) / 1024 + 1));
+      auto gt = std::make_shared<DeviceMem>(8 * k::cdf_tmp_entries(b.size));
       k::pb_norms(norms->as<double>(), b.amps, b.size, st);
       k::cdf_build(norms->as<double>(), 1, b.size, cdf->as<double>(), gt->as<double>(), totals.as<double>() + t,
                    errs.as<int>(), st);

@@ -120,6 +120,8 @@ def _load():
     lib.or_fidelity.restype = D
     lib.or_fidelity.argtypes = [P, P, ctypes.c_int]
     lib.or_set_threads.argtypes = [ctypes.c_int]
+    lib.or_cdf.restype = D
+    lib.or_cdf.argtypes = [P, U64, P]
     lib.or_batch_sample.restype = ctypes.c_int
     lib.or_batch_sample.argtypes = [P, ctypes.c_int, U64, U64, ctypes.c_int, P]
     return lib
@@ -362,6 +364,15 @@ def terminal_trajectories(prefix: np.ndarray, measures: Sequence[Tuple[int, int]
 def ptree(x: np.ndarray) -> float:
     x = np.ascontiguousarray(x, dtype=np.float64)
     return lib().or_ptree(x.ctypes.data, x.size)
+
+
+def cdf(x: np.ndarray) -> np.ndarray:
+    """The canonical inclusive prefix the samplers use."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(max(1, x.size), dtype=np.float64)
+    if x.size:
+        lib().or_cdf(x.ctypes.data, x.size, out.ctypes.data)
+    return out[: x.size]
 
 
 def batch_sample(state: np.ndarray, shots: int, seed: int, block: int) -> np.ndarray:

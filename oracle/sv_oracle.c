@@ -732,29 +732,79 @@ void or_marginal(const double* s, int n, const uint64_t* qubits, int k, double* 
   seg_sums(s, n, map, n - k, out);
 }
 
-/* Grouped CDF: C[j] = O[g] + L[j]; returns total = C[count-1]. */
-static double cdf_build(const double* bins, uint64_t count, double* Cc) {
-  const uint64_t G = 1024, ng = (count + G - 1) / G;
-  double* gt = (double*)malloc(sizeof(double) * ng);
-  for (uint64_t g = 0; g < ng; ++g) {
+/* Canonical inclusive prefix (the engine's block scan, kernels.hip
+ * k_scan_block / scan_into): blocks of 2048 bins; in a block, thread t sums
+ * its 8 bins in order (p), the 256 thread totals are scanned in rows of 16
+ * lanes by Hillis-Steele steps d = 1, 2, 4, 8 (lane i adds lane i-d of its
+ * row), rows are offset by the sequential prefix of the row totals, waves by
+ * the sequential prefix of the wave totals, and L = (wave offset + previous
+ * lane's row-scanned value) + p. Block totals (last outputs) are scanned the
+ * same way and added to every later block. Returns total = C[count-1]. */
+#define SCAN_BLOCK 2048
+static void block_scan(const double* y, uint64_t ystride, uint32_t m, double* out) {
+  double p[256][8], s[256], ex[256];
+  for (int t = 0; t < 256; ++t) {
     double acc = 0.0;
-    for (uint64_t j = g * G; j < count && j < (g + 1) * G; ++j) {
-      acc = acc + bins[j];
-      Cc[j] = acc;
+    for (int e = 0; e < 8; ++e) {
+      uint32_t i = 8 * t + e;
+      double v = i < m ? y[(uint64_t)i * ystride] : 0.0;
+      acc = e == 0 ? v : acc + v;
+      p[t][e] = acc;
     }
-    gt[g] = acc;
+    s[t] = acc;
   }
-  double off = 0.0, last = 0.0;
-  for (uint64_t g = 0; g < ng; ++g) {
-    double tg = gt[g];
-    gt[g] = off;
-    last = off + tg;
-    off = last;
+  for (int w = 0; w < 4; ++w) {
+    double* r = s + 64 * w;
+    for (int d = 1; d <= 8; d *= 2) {
+      double prev[64];
+      memcpy(prev, r, sizeof prev);
+      for (int i = 0; i < 64; ++i)
+        if (i % 16 >= d) r[i] = prev[i] + prev[i - d];
+    }
+    double ro1 = r[15], ro2 = ro1 + r[31], ro3 = ro2 + r[47];
+    double ro[4] = {0.0, ro1, ro2, ro3};
+    for (int i = 0; i < 64; ++i) r[i] = ro[i / 16] + r[i];
+    for (int i = 0; i < 64; ++i) ex[64 * w + i] = i == 0 ? 0.0 : r[i - 1];
   }
-  for (uint64_t j = 0; j < count; ++j) Cc[j] = gt[j / G] + Cc[j];
-  free(gt);
-  return last;
+  double wo1 = s[63], wo2 = wo1 + s[127], wo3 = wo2 + s[191];
+  double wo[4] = {0.0, wo1, wo2, wo3};
+  for (int t = 0; t < 256; ++t) {
+    double to = wo[t / 64] + ex[t];
+    for (int e = 0; e < 8; ++e) {
+      uint32_t i = 8 * t + e;
+      if (i < m) out[i] = to + p[t][e];
+    }
+  }
 }
+
+static void scan_into(const double* x, uint64_t xstride, uint64_t count, double* P) {
+  const uint64_t nb = (count + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  double* bt = (double*)malloc(sizeof(double) * nb);
+  double* tmp = (double*)malloc(sizeof(double) * SCAN_BLOCK);
+  for (uint64_t g = 0; g < nb; ++g) {
+    const uint64_t lo = g * SCAN_BLOCK;
+    const uint32_t m = (uint32_t)(count - lo < SCAN_BLOCK ? count - lo : SCAN_BLOCK);
+    block_scan(x + lo * xstride, xstride, m, tmp);
+    memcpy(P + lo, tmp, sizeof(double) * m);
+    bt[g] = tmp[m - 1];
+  }
+  free(tmp);
+  if (nb > 1) {
+    double* q = (double*)malloc(sizeof(double) * nb);
+    scan_into(bt, 1, nb, q);
+    for (uint64_t j = SCAN_BLOCK; j < count; ++j) P[j] = q[j / SCAN_BLOCK - 1] + P[j];
+    free(q);
+  }
+  free(bt);
+}
+
+static double cdf_build(const double* bins, uint64_t count, double* Cc) {
+  scan_into(bins, 1, count, Cc);
+  return Cc[count - 1];
+}
+
+/* The canonical prefix of x[0..count) (tests pin it). */
+double or_cdf(const double* x, uint64_t count, double* out) { return cdf_build(x, count, out); }
 
 static uint64_t partition(const double* Cc, uint64_t count, double total, double r) {
   uint64_t lo = 0, hi = count;

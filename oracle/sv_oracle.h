@@ -16,8 +16,9 @@
  *   - every FP64 sum of probabilities is the pairwise tree over the index
  *     range (engine.rs:379-391 uses a rayon reduction, shots.rs:139-142 a
  *     sequential sum);
- *   - CDFs are grouped sequential prefixes (groups of 1024) plus a
- *     sequential prefix of group totals;
+ *   - CDFs are the engine's block scan (blocks of 2048 bins: per-thread
+ *     sequential runs of 8, row-local Hillis-Steele steps, sequential row and
+ *     wave offsets, block totals scanned recursively; see block_scan);
  *   - random draws are Philox4x32-10 (key = seed, counter = (draw, stream))
  *     instead of ChaCha8 (rand_chacha 0.3.1).
  * Complex arithmetic is written out in num-complex order and the file is
@@ -124,6 +125,8 @@ void or_marginal(const double* s, int n, const uint64_t* qubits, int k, double* 
 /* Sample indices over a marginal exactly as the engine does. */
 int or_sample_marginal(const double* s, int n, const uint64_t* qubits, int k, uint64_t shots, uint64_t seed,
                        uint64_t* out);
+/* The canonical CDF (inclusive prefix) of x[0..count); returns its last entry. */
+double or_cdf(const double* x, uint64_t count, double* out);
 /* |<a|b>|^2 with the pairwise tree. */
 double or_fidelity(const double* a, const double* b, int n);
 

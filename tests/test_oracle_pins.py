@@ -399,3 +399,37 @@ def test_terminal_trajectories_match_oracle_run_shots():
     prefix, _ = oracle.simulate({**circ, "instructions": unitary, "cregs": {}}, 1)
     want, _ = oracle.run_shots(circ, 300, 19, sample_terminal=False)
     assert oracle.terminal_trajectories(prefix, measures, 4, 300, 19) == want
+
+
+def test_canonical_cdf_exact_on_dyadic_bins_at_every_level():
+    # Sums of multiples of 2^-20 below 2^33 are exact in FP64, so the block
+    # scan (any addition order) must equal the exact integer prefix, across
+    # one block, two levels (2048 < N <= 2048^2) and a partial last block.
+    rng = np.random.default_rng(5)
+    for count in (1, 7, 2048, 2049, 6000, (1 << 13) + 5, 1 << 16):
+        k = rng.integers(0, 1 << 10, size=count)
+        x = k.astype(np.float64) * 2.0 ** -20
+        want = np.cumsum(k).astype(np.float64) * 2.0 ** -20
+        got = oracle.cdf(x)
+        assert np.array_equal(got, want), count
+
+
+def test_canonical_cdf_is_the_block_scan_not_a_sequential_sum():
+    # Bin 0 = 1 and 2047 bins of 2^-53: a sequential sum never moves off 1.0
+    # (each 1 + 2^-53 rounds back to 1); in the block scan thread 0 loses its
+    # seven small bins the same way, but the other 255 threads sum theirs to
+    # 2^-50 each before they reach 1.0, so the total is exactly 1 + 255 * 2^-50.
+    x = np.full(2048, 2.0 ** -53)
+    x[0] = 1.0
+    c = oracle.cdf(x)
+    assert math.fsum(x[:1]) == 1.0 and sum(x.tolist()) == 1.0
+    assert c[-1] == 1.0 + 255 * 2.0 ** -50
+    assert c[7] == 1.0 and c[15] == 1.0 + 2.0 ** -50
+    # Non-decreasing, last entry = total, within the FP64 error bound of the
+    # exact sum on random bins at three levels.
+    rng = np.random.default_rng(8)
+    for count in (3000, 5 << 20):
+        x = rng.random(count) ** 3
+        c = oracle.cdf(x)
+        assert np.all(np.diff(c) >= 0)
+        assert abs(c[-1] - math.fsum(x)) <= 64 * 2.0 ** -53 * math.fsum(x)
