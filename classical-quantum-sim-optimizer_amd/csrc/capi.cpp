@@ -493,6 +493,16 @@ int dq_pb_simulate_shots(const dq_pb_opts* o, const char* json, size_t len, uint
   });
 }
 
+int dq_pb_last_shots_stats(uint64_t* out4) {
+  return guard([&] {
+    const dq::pb::ShotsStats st = dq::pb::last_shots_stats();
+    out4[0] = st.lockstep_shots;
+    out4[1] = st.lone_shots;
+    out4[2] = st.peak_batch_bytes;
+    out4[3] = st.batch_size;
+  });
+}
+
 void dq_pb_result_free(dq_pb_result* r) { delete r; }
 
 int dq_pb_result_json(const dq_pb_result* r, int32_t what, char** out) {

@@ -3054,7 +3054,8 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
                      "the cause is not identified yet)");
   if (opt.workers < 0 || opt.workers > 16) throw ValueErr("workers must be 0 (tuned) or 1..16");
   if (opt.max_fused < 0 || opt.max_fused > 4) throw ValueErr("batched fusion width must be 0 (tuned) or 1..4");
-  if (opt.tile != 0 && (opt.tile < kTileMin || opt.tile > kTileMax)) throw ValueErr("tile_qubits must be 0 or in 10..12");
+  if (opt.tile != -1 && opt.tile != 0 && (opt.tile < kTileMin || opt.tile > kTileMax))
+    throw ValueErr("tile_qubits must be -1 (tuned), 0 or in 10..12");
   if (opt.prec != 8 && opt.prec != 16) throw ValueErr("precision must be complex64 or complex128");
   auto out = std::make_unique<BatchOutcome>();
   ParsedBatch pb = parse_batch(circuits, shots, out.get());
@@ -3063,7 +3064,8 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
   su.shots = &shots;
   su.seeds = &seeds;
   su.prec = opt.prec;
-  su.tile = opt.tile;
+  const bool tune_tile = opt.tile == -1;
+  su.tile = tune_tile ? kTileDefault : opt.tile;
   HIPCHK(hipGetDevice(&su.device));
   su.qubits.resize(nc);
   uint64_t total_shots = 0, total_amps = 0;
@@ -3097,20 +3099,25 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
 
   int workers = opt.workers ? opt.workers : 4;
   int fused = opt.max_fused ? opt.max_fused : 3;
-  if (opt.autotune && nc > 0 && (!opt.workers || !opt.max_fused)) {
+  if (opt.autotune && nc > 0 && (!opt.workers || !opt.max_fused || tune_tile)) {
     // Calibration subset: evenly spaced circuits of the cost order, run once
-    // untimed (plans, arenas and code objects warm), then timed per candidate.
+    // untimed (plans, arenas and code objects warm), then timed per candidate:
+    // (workers, fusion width) at the default tile, then the tile shape (LDS
+    // tile sweeps of 10, 11 or 12 qubits, or none) at the chosen pair.
     std::vector<uint32_t> sub;
     const size_t want = std::min<size_t>(nc, 96);
     for (size_t i = 0; i < want; ++i) sub.push_back(order[(i * nc) / want]);
     run_stream_pool(su, sub, 4, fused, nullptr);
     double best = 1e300;
+    auto name = [](int w, int f, int t) {
+      return "workers=" + std::to_string(w) + ",k<=" + std::to_string(f) + ",tile=" + std::to_string(t);
+    };
     for (int w : {2, 4, 8}) {
       if (opt.workers && w != opt.workers) continue;
       for (int f = 2; f <= 4; ++f) {
         if (opt.max_fused && f != opt.max_fused) continue;
         const double ms = run_stream_pool(su, sub, w, f, nullptr);
-        out->tuning.push_back({"workers=" + std::to_string(w) + ",k<=" + std::to_string(f), ms});
+        out->tuning.push_back({name(w, f, su.tile), ms});
         if (ms < best) {
           best = ms;
           workers = w;
@@ -3118,11 +3125,24 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
         }
       }
     }
+    if (tune_tile) {
+      int best_tile = su.tile;
+      for (int t : {0, 10, 11}) {
+        su.tile = t;
+        const double ms = run_stream_pool(su, sub, workers, fused, nullptr);
+        out->tuning.push_back({name(workers, fused, t), ms});
+        if (ms < best) {
+          best = ms;
+          best_tile = t;
+        }
+      }
+      su.tile = best_tile;
+    }
   }
   out->total_ms = run_stream_pool(su, order, workers, fused, out.get());
   out->workers = workers;
   out->fused = fused;
-  out->tile = opt.tile;
+  out->tile = su.tile;
   out->streams = (uint32_t)workers;
   return out;
 }

@@ -232,7 +232,9 @@ void rng_init(RngState* rng, uint32_t batch, uint64_t seed, uint64_t stream0, hi
 // summed in order, thread totals scanned within 16-lane rows, then rows,
 // then waves, each by fixed additions), block totals scanned recursively and
 // added as block offsets. Writes C (count entries) and total = C[count-1]
-// into *total; group_tmp holds cdf_tmp_entries(count) doubles.
+// into *total; group_tmp holds cdf_tmp_entries(count) doubles. With stride 1,
+// C may be the bins themselves (each 2048-bin block is read whole before it
+// is written).
 uint64_t cdf_tmp_entries(uint64_t count);
 void cdf_build(const double* bins, uint64_t bstride, uint64_t count, double* C, double* group_tmp, double* total,
                int* err, hipStream_t st);

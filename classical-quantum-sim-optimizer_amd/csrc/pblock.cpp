@@ -808,13 +808,14 @@ class Pool {
         tabs[t] = {cdf, b.size, (uint32_t)b.nq(), 0};
         continue;
       }
-      auto norms = std::make_shared<DeviceMem>(8 * b.size);
+      // |a|^2 then its CDF in the same buffer (the block scan reads each
+      // 2048-entry block into LDS before it writes the block back), so the
+      // table stays within the 8 bytes per amplitude the budget counts.
       auto cdf = std::make_shared<DeviceMem>(8 * b.size);
       auto gt = std::make_shared<DeviceMem>(8 * k::cdf_tmp_entries(b.size));
-      k::pb_norms(norms->as<double>(), b.amps, b.size, st);
-      k::cdf_build(norms->as<double>(), 1, b.size, cdf->as<double>(), gt->as<double>(), totals.as<double>() + t,
+      k::pb_norms(cdf->as<double>(), b.amps, b.size, st);
+      k::cdf_build(cdf->as<double>(), 1, b.size, cdf->as<double>(), gt->as<double>(), totals.as<double>() + t,
                    errs.as<int>(), st);
-      keep.push_back(norms);
       keep.push_back(cdf);
       keep.push_back(gt);
       tabs[t] = {cdf->get(), b.size, (uint32_t)b.nq(), 0};
@@ -891,6 +892,11 @@ class Pool {
 // start. Shot s draws from Philox stream s in both places, so every count is
 // the one a lone trajectory gives. Slots that left keep evolving unitarily
 // (or are flagged off) and are never read.
+// A lockstep batch that would exceed the call's memory budget as a whole.
+struct BatchTooLarge : std::exception {
+  const char* what() const noexcept override { return "lockstep batch exceeds the memory budget"; }
+};
+
 class BatchPool {
  public:
   using Mask = std::vector<char>;  // per slot: the current op applies
@@ -903,17 +909,26 @@ class BatchPool {
   Mask alive;
   std::vector<std::map<uint64_t, int>> cbits;
   Options opt;
+  uint64_t total_budget;   // bytes for all B slots together
+  uint64_t peak_total = 0;  // largest B x slot bytes accounted
   hipStream_t st;
   DeviceMem rng, ms, err, red_a, red_b, flags;
   std::vector<std::shared_ptr<DeviceMem>> mats;
   std::map<const Op*, const void*> mat_of;
   std::vector<std::shared_ptr<DeviceMem>> graveyard;
-  std::vector<std::shared_ptr<std::vector<int>>> host_keep;  // sources of queued uploads
+  std::vector<std::shared_ptr<void>> host_keep;  // sources of queued uploads
+  void keep_host(std::shared_ptr<void> p) { host_keep.push_back(std::move(p)); }
 
-  // Every slot starts as `base`; slot b is shot stream0 + b.
-  BatchPool(const Pool& base, uint32_t batch, const Options& o, uint64_t seed, uint64_t stream0)
-      : B(batch), positions(base.positions), free(base.free), live(base.live), alive(batch, 1), cbits(batch),
-        opt(o), st(base.st), mats(base.mats), mat_of(base.mat_of) {
+  // Every slot starts as `base`; slot b is shot streams[b] (its Philox
+  // stream). Each slot keeps the per-shot budget o.memory (the reference's
+  // per-worker limit, whose errors the lone runs reproduce); the batch as a
+  // whole keeps within `total` (BatchTooLarge otherwise: the caller retries
+  // with fewer slots).
+  BatchPool(const Pool& base, const std::vector<uint64_t>& streams, const Options& o, uint64_t total, uint64_t seed)
+      : B((uint32_t)streams.size()), positions(base.positions), free(base.free), live(base.live),
+        alive(streams.size(), 1), cbits(streams.size()), opt(o), total_budget(total), st(base.st), mats(base.mats),
+        mat_of(base.mat_of) {
+    account(0);
     const uint64_t n = positions.size();
     for (auto& b : base.blocks) {
       if (!b) {
@@ -932,7 +947,10 @@ class BatchPool {
     flags.alloc(sizeof(int) * B);
     PBCHK(hipMemsetAsync(err.get(), 0, sizeof(int), st));
     PBCHK(hipMemsetAsync(ms.get(), 0, sizeof(k::MeasState) * B, st));
-    k::rng_init(rng.as<k::RngState>(), B, seed, stream0, st);
+    auto init = std::make_shared<std::vector<k::RngState>>(B);
+    for (uint32_t b = 0; b < B; ++b) (*init)[b] = k::RngState{seed, streams[b], 0};
+    PBCHK(hipMemcpyAsync(rng.get(), init->data(), sizeof(k::RngState) * B, hipMemcpyHostToDevice, st));
+    keep_host(init);
   }
 
   void sync() {
@@ -943,8 +961,16 @@ class BatchPool {
   uint64_t wires() const { return positions.size(); }
   int8_t& kn(uint32_t s, uint64_t q) { return known[(size_t)s * wires() + q]; }
   uint64_t bytes() const { return live * 16 + wires() * 256; }
-  void reserve(uint64_t extra) const {
+  // The whole batch's bytes with `extra` more per slot; BatchTooLarge beyond
+  // total_budget.
+  void account(uint64_t extra) {
+    const uint64_t slot = add_checked(bytes(), extra);
+    if (slot > UINT64_MAX / B || slot * B > total_budget) throw BatchTooLarge();
+    peak_total = std::max(peak_total, slot * B);
+  }
+  void reserve(uint64_t extra) {
     if (add_checked(bytes(), extra) > opt.memory) fail("P-block working allocation exceeds max_memory_mb");
+    account(extra);
   }
   std::unique_ptr<Block> new_block(uint64_t size, std::vector<uint64_t> qubits) const {
     auto b = std::make_unique<Block>();
@@ -1502,7 +1528,11 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
 }
 
 // simulator.rs:119-322
+thread_local ShotsStats g_shots_stats;
+ShotsStats last_shots_stats() { return g_shots_stats; }
+
 Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t shots, bool profile) {
+  g_shots_stats = ShotsStats();
   const double start = now_s();
   Input in = prepare(json, len, o.memory);
   const double pre = now_s() - start;
@@ -1602,38 +1632,73 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
       zero = std::make_unique<Pool>(n, wopt, stream.s);
     }
     const Pool& base = kept ? *kept : *zero;
-    // Every slot stays within the worker budget, so a batch holds at most
-    // 32 GiB of blocks.
-    const uint64_t cap = std::max<uint64_t>(1, (32ull << 30) / std::max<uint64_t>(wopt.memory, 1));
-    const uint64_t bmax = std::min<uint64_t>(1024, cap);
-    for (uint64_t s0 = 0; s0 < shots; s0 += bmax) {
-      const uint32_t bsz = (uint32_t)std::min<uint64_t>(bmax, shots - s0);
-      std::vector<std::map<uint64_t, int>> done;
-      BatchPool::Mask alive;
-      try {
-        BatchPool bp(base, bsz, wopt, seed, s0);
-        for (size_t i = start_op; i < ops.size() && bp.any(bp.alive); ++i) {
-          BatchPool::Mask m = bp.alive;
-          run_batch(ops[i], bp, m);
+    // The slots of a batch share the call's budget (less a retained
+    // prefix): a batch that would outgrow it is retried with half the slots.
+    // Shots a batch evicts (their structural decisions differ from the
+    // batch's) run again from the start in the next round's batches, where
+    // the shots that took the same path stay together; after kRounds rounds,
+    // or after an error, the rest run alone.
+    constexpr int kRounds = 64;
+    const uint64_t total = options.memory - reserved;
+    uint64_t bmax = 1024;
+    std::vector<uint64_t> pending(shots);
+    for (uint64_t s = 0; s < shots; ++s) pending[s] = s;
+    bool failed = false;
+    for (int round = 0; round < kRounds && !pending.empty() && !failed; ++round) {
+      std::vector<uint64_t> evicted;
+      for (size_t p0 = 0; p0 < pending.size();) {
+        const size_t bsz = std::min<size_t>(bmax, pending.size() - p0);
+        const std::vector<uint64_t> streams(pending.begin() + p0, pending.begin() + p0 + bsz);
+        std::vector<std::map<uint64_t, int>> done;
+        BatchPool::Mask alive;
+        try {
+          BatchPool bp(base, streams, wopt, total, seed);
+          for (size_t i = start_op; i < ops.size() && bp.any(bp.alive); ++i) {
+            BatchPool::Mask m = bp.alive;
+            run_batch(ops[i], bp, m);
+          }
+          bp.sync();
+          done = std::move(bp.cbits);
+          alive = std::move(bp.alive);
+          g_shots_stats.peak_batch_bytes = std::max(g_shots_stats.peak_batch_bytes, bp.peak_total + reserved);
+          g_shots_stats.batch_size = std::max<uint64_t>(g_shots_stats.batch_size, bsz);
+        } catch (const BatchTooLarge&) {
+          if (bsz > 1) {
+            bmax = std::max<uint64_t>(1, bsz / 2);
+            continue;
+          }
+          failed = true;
+        } catch (const std::exception&) {
+          failed = true;
         }
-        bp.sync();
-        done = std::move(bp.cbits);
-        alive = std::move(bp.alive);
-      } catch (const std::exception&) {
-        // The lone runs raise the error of the first failing shot, as the
-        // per-shot path does: this and every later shot run alone.
-        for (uint64_t s = s0; s < shots; ++s) todo.push_back(s);
-        break;
+        if (failed) {
+          // The lone runs raise the error of the first failing shot, as the
+          // per-shot path does: this and every later shot run alone.
+          for (size_t q = p0; q < pending.size(); ++q) todo.push_back(pending[q]);
+          for (uint64_t s : evicted) todo.push_back(s);
+          evicted.clear();
+          break;
+        }
+        for (size_t b = 0; b < bsz; ++b) {
+          if (alive[b]) {
+            counts[format_cbits(done[b], nc)] += 1;
+            ++g_shots_stats.lockstep_shots;
+          } else {
+            evicted.push_back(streams[b]);
+          }
+        }
+        p0 += bsz;
       }
-      for (uint32_t s = 0; s < bsz; ++s) {
-        if (alive[s]) counts[format_cbits(done[s], nc)] += 1;
-        else todo.push_back(s0 + s);
-      }
+      pending = std::move(evicted);
     }
+    if (!failed)
+      for (uint64_t s : pending) todo.push_back(s);
+    std::sort(todo.begin(), todo.end());
   } else {
     for (uint64_t s = 0; s < shots; ++s) todo.push_back(s);
   }
   std::unique_ptr<Pool> pool;
+  g_shots_stats.lone_shots += todo.size();
   for (uint64_t s : todo) {
     if (!pool) {
       if (kept) {

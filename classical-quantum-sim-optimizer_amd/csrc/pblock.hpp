@@ -62,6 +62,15 @@ struct Result {
 std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len);
 Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t shots, bool profile);
 
+// How the calling thread's last simulate_shots ran its trajectory shots:
+// completed in lockstep batches, run alone (evicted from a batch, or every
+// shot when batches are off), and the largest batch's accounted bytes
+// (slots x the slot's bytes at its peak, plus a retained prefix).
+struct ShotsStats {
+  uint64_t lockstep_shots = 0, lone_shots = 0, peak_batch_bytes = 0, batch_size = 0;
+};
+ShotsStats last_shots_stats();
+
 // Result queries (result.rs). Query qubits are physical labels; a null list
 // means every qubit, highest label first.
 std::vector<std::vector<uint64_t>> block_qubits(const Result& r);

@@ -55,6 +55,7 @@ def _bind():
         "dq_pb_probabilities": (ctypes.c_int, [P, P, U64, I32, ctypes.POINTER(P)]),
         "dq_pb_counts": (ctypes.c_int, [P, U64, P, U64, I32, U64, _core._COUNT_CB, P]),
         "dq_pb_expectation": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(D)]),
+        "dq_pb_last_shots_stats": (ctypes.c_int, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(_lib, name)
@@ -306,6 +307,17 @@ class PBlockSimulator:
         _check(_lib.dq_pb_simulate_shots(ctypes.byref(self._o), data, len(data), shots, 1 if profile else 0, tab.cb,
                                          tab.user))
         return tab.result()
+
+
+def last_shots_stats() -> Dict[str, int]:
+    """How this thread's last PBlockSimulator.simulate_shots ran its
+    trajectory shots (no reference counterpart): shots completed in lockstep
+    batches, shots run alone, the largest batch's accounted bytes (never above
+    max_memory_mb) and its slot count."""
+    out = (ctypes.c_uint64 * 4)()
+    _check(_lib.dq_pb_last_shots_stats(out))
+    return {"lockstep_shots": int(out[0]), "lone_shots": int(out[1]), "peak_batch_bytes": int(out[2]),
+            "batch_size": int(out[3])}
 
 
 _PB_OPTIONS = ("max_memory_mb", "max_block_qubits", "max_parallel_shots", "sample_terminal", "split_separable",

@@ -74,11 +74,13 @@ class BatchResult:
 
 
 def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = None, base_seed: int = 0,
-              workers: int = 0, max_fused_qubits: int = 0, autotune: bool = True, tile_qubits: int = 12,
+              workers: int = 0, max_fused_qubits: int = 0, autotune: bool = True, tile_qubits: Optional[int] = None,
               precision: str = "complex64", keep_states: bool = False, persistent: bool = False) -> BatchResult:
     """Run circuits as one batch. ``shots`` is an int or a list; ``seeds``
-    defaults to base_seed + index. ``workers`` / ``max_fused_qubits`` of 0 are
-    chosen by timing candidates on a sample when ``autotune`` is set."""
+    defaults to base_seed + index. ``workers`` / ``max_fused_qubits`` of 0 and
+    ``tile_qubits`` of None (LDS tile sweeps of 10, 11 or 12 qubits, or none)
+    are chosen by timing candidates on a sample when ``autotune`` is set; an
+    untuned tile is 12."""
     if precision not in _PREC:
         raise ValueError("precision must be 'complex64' or 'complex128'")
     texts = [_circuit_json(c) for c in circuits]
@@ -90,7 +92,8 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
     lens = (ctypes.c_size_t * max(1, n))(*[len(t) for t in texts])
     sh = (ctypes.c_uint64 * max(1, n))(*shots_l)
     sd = (ctypes.c_uint64 * max(1, n))(*seeds_l)
-    o = _BatchOpts(workers, max_fused_qubits, 1 if autotune else 0, tile_qubits, _PREC[precision],
+    o = _BatchOpts(workers, max_fused_qubits, 1 if autotune else 0, -1 if tile_qubits is None else int(tile_qubits),
+                   _PREC[precision],
                    1 if keep_states else 0, 1 if persistent else 0)
     h = ctypes.c_void_p()
     _check(_lib.dq_batch_run(ptrs, lens, n, sh, sd, ctypes.byref(o), ctypes.byref(h)))

@@ -214,6 +214,13 @@ int dq_pb_simulate(const dq_pb_opts* o, const char* json, size_t len, dq_pb_resu
 int dq_pb_simulate_shots(const dq_pb_opts* o, const char* json, size_t len, uint64_t shots, int32_t profile,
                          dq_count_cb cb, void* user);
 void dq_pb_result_free(dq_pb_result* r);
+/* How the calling thread's last dq_pb_simulate_shots ran its trajectory
+ * shots (no reference counterpart; diagnostics of the lockstep batches):
+ * out4 = {shots completed in lockstep batches, shots run alone (evicted
+ * from a batch, or all of them when batches are off), the largest batch's
+ * accounted bytes (slots x a slot's peak bytes, plus a retained prefix;
+ * never above max_memory_mb), the largest batch's slot count}. */
+int dq_pb_last_shots_stats(uint64_t* out4);
 /* JSON text of a result field; free with dq_string_free. what: 0 physical
  * qubits, 1 classical bits [[cbit, value]...], 2 block qubits (physical
  * labels), 3 diagnostics, 4 profile (diagnostics + total_time; "null" when
@@ -272,7 +279,8 @@ typedef struct dq_batch_opts {
   int32_t workers;          /* host threads, one HIP stream each: 1..16; 0 = tuned */
   int32_t max_fused_qubits; /* 1..4; 0 = tuned */
   int32_t autotune;         /* time candidate (workers, width) pairs on a sample */
-  int32_t tile_qubits;      /* LDS tile sweeps of each circuit's plan: 0 = off, 10..12 */
+  int32_t tile_qubits;      /* LDS tile sweeps of each circuit's plan: 0 = off, 10..12;
+                               -1 = tuned with autotune (10, 11, 12 or off), else 12 */
   int32_t precision;        /* DQ_PREC_C64 or DQ_PREC_C128 */
   int32_t dump_states;      /* keep final states for dq_batch_state_c128 */
   int32_t persistent;       /* 1 = the persistent kernel (disabled: DQ_ERUNTIME) */
@@ -298,7 +306,7 @@ int dq_batch_info(const dq_batch* b, double* total_ms, int32_t* workers, int32_t
 /* Final state of circuit i as complex128 (needs dump_states). */
 int dq_batch_state_c128(const dq_batch* b, uint64_t i, void* out, uint64_t len);
 /* Autotuning record: the number of candidates timed, and candidate i's name
- * ("workers=W,k<=F") and wall ms on the calibration sample. */
+ * ("workers=W,k<=F,tile=T") and wall ms on the calibration sample. */
 uint64_t dq_batch_tuning_count(const dq_batch* b);
 int dq_batch_tuning_entry(const dq_batch* b, uint64_t i, char* name, uint64_t cap, double* ms);
 /* Qubit count of circuit i (0 when i is out of range). */

@@ -89,12 +89,14 @@ def test_batch_autotune_and_latency():
     cs = circuits(40, 4, 14, dmin=4, dmax=12, base=9)
     res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True)
     print("tuned", res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
-    assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4
-    assert len(res.tuning) == 9 and all(ms > 0 for ms in res.tuning.values())
-    assert f"workers={res.workers},k<={res.max_fused_qubits}" in res.tuning
+    assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4 and res.tile_qubits in (0, 10, 11, 12)
+    # 9 (workers, width) pairs at the default tile, then 3 more tile shapes.
+    assert len(res.tuning) == 12 and all(ms > 0 for ms in res.tuning.values())
+    assert f"workers={res.workers},k<={res.max_fused_qubits},tile={res.tile_qubits}" in res.tuning
+    assert {int(k.rsplit("=", 1)[1]) for k in res.tuning} == {0, 10, 11, 12}
     assert res.wall_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
     for i, c in enumerate(cs):
-        assert res.counts(i) == per_circuit_counts(c, 64, 3 + i, res.max_fused_qubits, 12), i
+        assert res.counts(i) == per_circuit_counts(c, 64, 3 + i, res.max_fused_qubits, res.tile_qubits), i
 
 
 @pytest.mark.skip(reason="the persistent batched kernel stalled (no progress for 180 s) on its first GPU run in "

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from dqsim import PBlockSimulator
+from dqsim._pblock import last_shots_stats
 from dqsim.workloads import Circuit
 from oracle import pblock as opb
 from test_gpu_parity import standard_gate_cases
@@ -94,16 +95,58 @@ def test_lockstep_trajectories_match_oracle_and_lone_runs(trial, tmp_path, monke
     # Dynamic circuits whose shots part ways in structure (controls on
     # measured wires, conditional gates that merge blocks, resets): the
     # lockstep batches give the oracle's counts draw for draw, and so do the
-    # lone per-shot runs (a profiled call runs every shot on its own pool)
-    # and batches of two slots (a large per-worker budget shrinks them).
+    # lone per-shot runs (a profiled call runs every shot on its own pool).
+    # Shots whose decisions differ leave the batch and run alone; most stay.
     c = dynamic_circuit(6, 40, 3100 + trial)
     c = Circuit(6, c.data["instructions"] + [inst("measure", qubit=q, cbit=q) for q in range(6)], 6)
     seed = 41 + trial
     print(f"[pblock-oracle] lockstep trial={trial}", flush=True)
     g = PBlockSimulator(seed=seed, sample_terminal=False).simulate_shots(c, 120)
+    st = last_shots_stats()
+    print(f"[pblock-oracle] lockstep trial={trial} {st}", flush=True)
+    assert st["lockstep_shots"] + st["lone_shots"] == 120 and st["lockstep_shots"] >= 60
     o, _ = opb.simulate_shots(c.data, 120, seed, sample_terminal=False)
     assert g == o
     small = PBlockSimulator(seed=seed, sample_terminal=False, max_memory_mb=16384, max_parallel_shots=1)
     assert small.simulate_shots(c, 120) == g
     monkeypatch.chdir(tmp_path)
     assert PBlockSimulator(seed=seed, sample_terminal=False).simulate_shots(c, 120, profile=True) == g
+
+
+def entangled_trajectory_circuit(n, seed):
+    """A GHZ chain and a U3 layer over n wires (one block of n qubits), then
+    every wire measured: with sample_terminal=False every shot runs the same
+    structure, so all of them can stay in lockstep."""
+    rng = np.random.default_rng(seed)
+    ops = [inst("h", qubit=0)] + [inst("cx", control=q, target=q + 1) for q in range(n - 1)]
+    ops += [inst("u3", qubit=q, theta=float(t), phi=float(p), lam=float(l))
+            for q, (t, p, l) in enumerate(rng.uniform(0, 3, size=(n, 3)))]
+    return Circuit(n, ops + [inst("measure", qubit=q, cbit=q) for q in range(n)], n)
+
+
+def test_uniform_trajectories_stay_in_lockstep():
+    c = entangled_trajectory_circuit(8, 3)
+    g = PBlockSimulator(seed=7, sample_terminal=False).simulate_shots(c, 200)
+    st = last_shots_stats()
+    assert st == {**st, "lockstep_shots": 200, "lone_shots": 0, "batch_size": 200}, st
+    o, _ = opb.simulate_shots(c.data, 200, 7, sample_terminal=False)
+    assert g == o
+
+
+def test_lockstep_batches_stay_within_the_budget():
+    # 14 wires in one block: a slot needs 2^14 x 16 B = 256 KiB (plus the
+    # reference's 256 B per wire), so a 1 MB budget holds at most 3 slots at
+    # once; the batch shrinks to fit and the counts are unchanged.
+    n = 14
+    c = entangled_trajectory_circuit(n, 5)
+    roomy = PBlockSimulator(seed=9, sample_terminal=False, max_memory_mb=4096).simulate_shots(c, 48)
+    big = last_shots_stats()
+    tight = PBlockSimulator(seed=9, sample_terminal=False, max_memory_mb=1).simulate_shots(c, 48)
+    small = last_shots_stats()
+    print(f"[pblock-budget] roomy {big} tight {small}", flush=True)
+    assert tight == roomy
+    assert big["batch_size"] == 48 and big["lockstep_shots"] == 48
+    assert small["peak_batch_bytes"] <= 1 << 20 and 1 <= small["batch_size"] <= 3
+    assert small["lockstep_shots"] + small["lone_shots"] == 48
+    o, _ = opb.simulate_shots(c.data, 48, 9, sample_terminal=False, memory_mb=4096)
+    assert roomy == o
