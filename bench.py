@@ -3,10 +3,13 @@ engine on one MI355X.
 
 Workload (one step): evolve a 32-qubit, depth-40 random circuit (Haar SU(2) on
 every qubit plus CZ/CX on a random perfect matching per layer, 1,920 gates,
-seed 1234) from |0...0> in complex64 with greedy block fusion (blocks of <= 3
-qubits by default) grouped into LDS tile sweeps (12-qubit tiles). The same
-circuit is also run once on the plain complex128 path (one launch per compiled
-gate, reference arithmetic) for the speed-up and the accuracy check.
+seed 1234) from |0...0> in complex64 with fused blocks of <= 5 qubits grouped
+into LDS tile sweeps (12-qubit tiles, out of place), the block width and tile
+rule chosen by the learned planner (shipped weights; --fusion greedy fuses as
+far as the width allows). The same circuit also runs under the greedy planner
+at that width and at 3 qubits, and once on the plain complex128 path (one
+launch per compiled gate, reference arithmetic) for the speed-up and the
+accuracy check.
 
 Prints one JSON line. `value` = G * 2^n / seconds per step.
 """
@@ -80,22 +83,23 @@ def profiled_mfma_busy(kernel, workload):
 
 
 def mfma_flops_per_step(circ, precision, fusion, max_fused, tile, out_of_place):
-    """FP32 matrix-core flops of one step from the launch plan: a k-qubit
-    block pass is 8 * 2^k * 2^n (one complex MAC = 8 real flops); 1-qubit
-    blocks run padded to 2 qubits in tile sweeps."""
+    """FP32 matrix-core flops of one step from the launch plan, as (tile
+    sweeps, one-block launches): a k-qubit block pass is 8 * 2^k * 2^n (one
+    complex MAC = 8 real flops); 1-qubit blocks run padded to 2 qubits in
+    tile sweeps. Also returns the plan's learned choices."""
     from dqsim import _core
 
     plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile,
                            out_of_place=out_of_place)
     n = plan["num_qubits"]
-    flops = 0.0
+    tile_flops = fused = 0.0
     for op in plan["launches"]:
         if op["kind"] == "tile" and not op.get("exact"):
             for b in op["blocks"]:
-                flops += 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
+                tile_flops += 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
         elif op["kind"] == "fused_mfma":
-            flops += 8.0 * (1 << len(op["qubits"])) * float(1 << n)
-    return flops
+            fused += 8.0 * (1 << len(op["qubits"])) * float(1 << n)
+    return tile_flops, fused, plan["learned_choices"]
 
 
 def device_copy_gbs(nbytes=4 << 30, reps=5):
@@ -245,11 +249,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--max-fused", type=int, default=3)
-    ap.add_argument("--fusion", default="greedy")
+    ap.add_argument("--max-fused", type=int, default=5,
+                    help="widest fused block (tile sweeps take up to 4 qubits)")
+    ap.add_argument("--fusion", default="learned", choices=["none", "greedy", "learned"],
+                    help="fusion planner (learned: the shipped weights choose block width and tile rule)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the timing of the other fusion planners on the same circuit")
     ap.add_argument("--precision", default="complex64", choices=["complex64", "complex128"],
                     help="precision of the timed evolution (complex128 with --fusion none times the exact sweeps)")
     ap.add_argument("--tile", type=int, default=12, help="LDS tile qubits of fused sweeps (0 = one launch per block)")
+    ap.add_argument("--pinned", action="store_true",
+                    help="out-of-place sweeps keep qubits 0..3 in every tile (default: only in a run's first and "
+                         "last tile)")
     ap.add_argument("--relabel", action="store_true",
                     help="relabelled tile sweeps (measured slower here; default: qubits 0..3 pinned in every tile)")
     ap.add_argument("--sweep-mode", type=int, default=None, choices=[0, 1, 2, 3, 4],
@@ -298,6 +309,8 @@ def main():
     from dqsim import workloads
 
     dqsim._core.set_sweep_relabel(args.relabel)
+    dqsim._core.set_sweep_pinned(args.pinned)
+    pinned = args.pinned
     if args.sweep_mode is not None:
         dqsim._core.set_sweep_mode(args.sweep_mode)
     sweep_mode = dqsim._core.set_sweep_mode(1)
@@ -331,12 +344,13 @@ def main():
     agg = per_kind(rows_all)
     dominant = max(agg.items(), key=lambda kv: kv[1]["ms"])
     dname, d = dominant
-    achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9
     total_bytes = sum(a["bytes"] for a in agg.values()) / args.steps
     value = gates * float(1 << n) / (ms_per_step / 1e3)
 
-    mflops = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused, args.tile, sweep_mode != 0)
-    mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "fused_mfma")) / args.steps
+    tile_flops, fused_flops, choices = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused,
+                                                           args.tile, sweep_mode != 0)
+    mflops = tile_flops + fused_flops
+    mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "xsweep", "fused_mfma")) / args.steps
     mfma = None
     if mflops > 0 and mfma_ms > 0:
         ach = mflops / (mfma_ms / 1e3) / 1e12
@@ -344,11 +358,37 @@ def main():
                 "measured_peak_tflops": mfma_peak_tflops, "frac_of_datasheet": ach / MFMA_FP32_PEAK_TFLOPS,
                 "frac_of_measured_peak": ach / mfma_peak_tflops, "flops_per_step": mflops,
                 "busy_frac_profiled": None}
+    # The same workload under the other fusion planners (greedy and learned
+    # at this block width, and greedy at 3 qubits): all step times go into
+    # the line.
+    planner_cmp = None
+    if args.fusion in ("greedy", "learned") and args.precision == "complex64" and not args.no_compare:
+        per = {f"{args.fusion}_k{args.max_fused}": ms_per_step}
+        for fusion, k in (("greedy", args.max_fused), ("learned", args.max_fused), ("greedy", 3)):
+            key = f"{fusion}_k{k}"
+            if key in per:
+                continue
+            alt = dqsim.StatevectorSimulator(seed=1, precision=args.precision, fusion=fusion, max_fused_qubits=k,
+                                             max_memory_mb=mem_mb, tile_qubits=args.tile)
+            for _ in range(max(1, args.warmup)):
+                alt.evolve_timed(circ, repeats=1, record=False)
+            ta = time.perf_counter()
+            for _ in range(args.steps):
+                alt.evolve_timed(circ, repeats=1, record=False)
+            per[key] = (time.perf_counter() - ta) * 1e3 / args.steps
+            del alt
+        lk, gk = f"learned_k{args.max_fused}", f"greedy_k{args.max_fused}"
+        choice = dqsim._core.plan_json(circ, precision=args.precision, fusion="learned", max_fused_qubits=args.max_fused,
+                                       tile_qubits=args.tile, out_of_place=sweep_mode != 0)["learned_choices"]
+        planner_cmp = {"ms_per_step": per, "learned_speedup_vs_greedy_same_width": per[gk] / per[lk],
+                       "learned_speedup_vs_greedy_k3": per["greedy_k3"] / per[lk], "learned_choices": choice,
+                       "weights": "shipped planner_weights.npy (engine-recorded timings, trained on the GPU)"}
+        log(f"[bench] planner: {per}, learned choice {choice}")
     extra = {"launches_per_step": launches, "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                             for kk, vv in v.items()} for k, v in agg.items()},
              "device_ms_per_step": sum(dev_ms) / len(dev_ms),
              "hbm_gbs_whole_step": total_bytes / (sum(dev_ms) / len(dev_ms) / 1e3) / 1e9,
-             "device_copy_gbs": copy_gbs}
+             "device_copy_gbs": copy_gbs, "planner": planner_cmp}
 
     if not args.no_plain:
         plain = dqsim.StatevectorSimulator(seed=1, precision="complex128", fusion="none", max_memory_mb=mem_mb,
@@ -384,12 +424,41 @@ def main():
             "value": gates * float(1 << n) / (ems / 1e3), "ms": ems, "wall_s": ewall, "launches": elaunch,
             "speedup_vs_plain_complex128": pms / ems, "max_abs_diff_vs_plain": emax}
 
-    workload_key = (f"random_su2_n{n}_d{depth}_k{args.max_fused}_t{args.tile}"
-                    if args.precision == "complex64" and args.fusion == "greedy"
+    # The plan that ran: tile rule and block width (the learned planner's
+    # choice under --fusion learned).
+    rule, kk = ("pinned" if pinned else "free"), args.max_fused
+    if choices:
+        rule, kk = ("pinned" if choices[0]["pinned"] else "free"), choices[0]["k"]
+    workload_key = (f"random_su2_n{n}_d{depth}_k{kk}_t{args.tile}_{rule}"
+                    if args.precision == "complex64" and args.fusion in ("greedy", "learned")
                     else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_k{args.max_fused}_t{args.tile}")
     if mfma is not None:
         mfma["busy_frac_profiled"] = profiled_mfma_busy(KERNEL_NAMES.get(dname, dname), workload_key)
         extra["mfma"] = mfma
+    # Roofline of the dominant kernel per launch: algorithmic HBM bytes and
+    # (for tile sweeps) algorithmic FP32 matrix-core flops over its measured
+    # average launch time; the bound is the resource whose floor (bytes at
+    # 8 TB/s, flops at 157.3 TFLOP/s) is higher.
+    launch_ms = d["ms"] / d["launches"]
+    bytes_pl = d["bytes"] / d["launches"]
+    hbm = {"achieved": bytes_pl / (launch_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "floor_ms": bytes_pl / (HBM_PEAK_GBS * 1e9) * 1e3}
+    hbm["frac"] = hbm["achieved"] / HBM_PEAK_GBS
+    hbm["frac_of_device_copy"] = hbm["achieved"] / copy_gbs
+    kname = KERNEL_NAMES.get(dname, dname)
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": hbm["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": hbm["frac"], "traffic": profiled_traffic(kname, workload_key),
+                "algorithmic_bytes_per_launch": bytes_pl, "launch_ms": launch_ms, "hbm": hbm}
+    if dname in ("xsweep", "tile") and tile_flops > 0:
+        flops_pl = tile_flops * args.steps / d["launches"]
+        mf = {"achieved": flops_pl / (launch_ms / 1e3) / 1e12, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "floor_ms": flops_pl / (MFMA_FP32_PEAK_TFLOPS * 1e12) * 1e3, "flops_per_launch": flops_pl,
+              "intensity_flop_per_byte": flops_pl / bytes_pl}
+        mf["frac"] = mf["achieved"] / MFMA_FP32_PEAK_TFLOPS
+        roofline["mfma"] = mf
+        if mf["floor_ms"] > hbm["floor_ms"]:
+            roofline.update({"bound": "mfma", "achieved": mf["achieved"], "peak": MFMA_FP32_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": mf["frac"]})
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_qubits, args.cpu_depth)
@@ -410,12 +479,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
-                   "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dname, dname), "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "frac_of_device_copy": achieved / copy_gbs,
-                     "traffic": profiled_traffic(KERNEL_NAMES.get(dname, dname), workload_key),
-                     "algorithmic_bytes_per_launch": d["bytes"] / d["launches"]},
+                   "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode,
+                   "tile_rule": rule, "block_width": kk, "parallelism": "single GPU"},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": extra,
     }

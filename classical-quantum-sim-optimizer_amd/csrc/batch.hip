@@ -197,7 +197,7 @@ int set_lds_limit(int block, size_t bytes) {
   return (int)e;
 }
 
-int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads) {
+int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads, int* private_bytes) {
   hipFuncAttributes fa{};
   hipError_t e = hipFuncGetAttributes(&fa, kernel_for(block));
   if (e != hipSuccess) {
@@ -207,7 +207,14 @@ int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads
   *max_dynamic_lds = fa.maxDynamicSharedSizeBytes;
   *static_lds = (int)fa.sharedSizeBytes;
   *max_threads = fa.maxThreadsPerBlock;
+  *private_bytes = (int)fa.localSizeBytes;
   return 0;
+}
+
+int occupancy(int block, size_t lds, int* blocks_per_cu) {
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, kernel_for(block), block, lds);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return (int)e;
 }
 
 }  // namespace bk

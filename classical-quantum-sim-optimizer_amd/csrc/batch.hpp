@@ -27,8 +27,11 @@ struct BatchOp {
 size_t lds_bytes(int lmax, int block);
 // hipFuncSetAttribute(MaxDynamicSharedMemorySize); returns the hipError_t.
 int set_lds_limit(int block, size_t bytes);
-// hipFuncGetAttributes of the kernel instance for `block`; returns hipError_t.
-int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads);
+// hipFuncGetAttributes of the kernel instance for `block` (private_bytes: the
+// per-lane private segment, its call stack included); returns hipError_t.
+int func_info(int block, int* max_dynamic_lds, int* static_lds, int* max_threads, int* private_bytes);
+// hipOccupancyMaxActiveBlocksPerMultiprocessor at `lds` dynamic LDS bytes.
+int occupancy(int block, size_t lds, int* blocks_per_cu);
 
 }  // namespace bk
 }  // namespace dq

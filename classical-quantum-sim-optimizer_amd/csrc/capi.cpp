@@ -310,6 +310,7 @@ void dq_string_free(char* s) { std::free(s); }
 int dq_device_count(void) { return dq::device_count(); }
 
 int dq_set_sweep_relabel(int32_t on) { return dq::set_sweep_relabel(on != 0) ? 1 : 0; }
+int dq_set_sweep_pinned(int32_t on) { return dq::set_sweep_pinned(on != 0) ? 1 : 0; }
 
 int dq_set_sweep_mode(int32_t mode) { return dq::set_sweep_mode(mode); }
 
@@ -431,10 +432,12 @@ uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i) {
 
 void dq_batch_free(dq_batch* b) { delete b; }
 
-int dq_batch_device_info(int32_t block, int32_t* info7) {
+int dq_batch_device_info(int32_t block, int32_t* info11) {
   return guard([&] {
-    dq::batch_device_info(block, &info7[0], &info7[1], &info7[2], &info7[3], &info7[4], &info7[5]);
-    info7[6] = dq::device_count();
+    if (block != 256 && block != 512 && block != 1024) throw std::invalid_argument("block must be 256, 512 or 1024");
+    static_assert(sizeof(int32_t) == sizeof(int), "int32_t is int");
+    dq::batch_device_info(block, (int*)info11);
+    info11[6] = dq::device_count();
   });
 }
 

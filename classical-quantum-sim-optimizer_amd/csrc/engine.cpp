@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <functional>
 #include <map>
@@ -266,12 +267,14 @@ int device_count() {
 // Relabelled tile sweeps (emit_sweeps, emit_exact_sweeps).
 std::atomic<bool> g_sweep_relabel{false};
 bool set_sweep_relabel(bool on) { return g_sweep_relabel.exchange(on); }
+std::atomic<bool> g_sweep_pinned{false};
+bool set_sweep_pinned(bool on) { return g_sweep_pinned.exchange(on); }
 
 // Complex64 fused sweeps when the budget holds a second state: 0 in place
 // (k_tile), out of place (k_xsweep) with 1 persistent 256-thread, 2 one-shot
 // 256-thread, 3 one-shot 512-thread, 4 persistent 512-thread workgroups.
 std::atomic<int> g_sweep_mode{2};
-int set_sweep_mode(int mode) { return g_sweep_mode.exchange(mode < 0 || mode > 4 ? 2 : mode); }
+int set_sweep_mode(int mode) { return mode < 0 || mode > 4 ? -1 : g_sweep_mode.exchange(mode); }
 
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   int dev = 0, cus = 0;
@@ -427,6 +430,9 @@ struct Plan {
   std::vector<uint8_t> blob;  // matrices and condition chains
   DeviceMem dev;
   bool uses_alt = false;  // out-of-place sweeps: the executor needs a second state buffer
+  // Learned choices of out-of-place sweep runs: 100 * (qubits 0..3 in every
+  // tile) + block width, one per unitary run that the model planned.
+  std::vector<int> learned_choices;
   size_t push(const void* p, size_t bytes) {
     size_t off = (blob.size() + 255) & ~(size_t)255;
     blob.resize(off + bytes);
@@ -1215,8 +1221,68 @@ void emit_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t en
 // holds local[j]), read from the layout `in_pos` (qubit -> bit position;
 // contiguous when local bit j sits at position j) and written to the layout
 // `out_pos`, with `blocks` applied in LDS.
+// Cost-model features of an out-of-place sweep launch (the learned planner's
+// inputs; they replace tile_blob's in-place sweep features):
+//   0 1 (out-of-place sweep)       1 3-qubit blocks with a tile bit < 4, /16
+//   2 2-qubit blocks with a tile bit < 6, /16   3 2-qubit blocks, /16
+//   4 gates/32                     5 gather (first sweep of a run)
+//   6 destination positions 0..3 stored from tile bits >= 4, /4
+//   7 lowest destination positions (from 0) stored from tile bits, /12
+//   8 n/34                         9 512-thread workgroups
+//  10 persistent workgroups       11 blob bytes/32768
+//  12 1 (tile sweep)              13 L/12
+//  14 3-qubit blocks, /16         15 4-qubit blocks, /16
+// The matrix-core block routines spread a wave's lanes over the lowest tile
+// bits outside the block (16 lanes for 3 qubits, 64 for 2), so blocks on low
+// bits read LDS in strided slots; so does a store whose lowest destination
+// positions come from high tile bits.
+void xsweep_features(DevOp& d, const std::vector<FusedBlock*>& blocks, const uint32_t* local, int L, int n) {
+  const k::XSweepArgs& a = d.xargs;
+  std::vector<int> pos(64, -1);
+  for (int j = 0; j < L; ++j) pos[local[j]] = j;
+  int n3 = 0, n3lo = 0, n2 = 0, n2lo = 0, n4 = 0;
+  for (const FusedBlock* b : blocks) {
+    int lo = 99;
+    for (uint32_t q : b->qubits) lo = std::min(lo, pos[q]);
+    const size_t k = b->qubits.size();
+    if (k <= 2) {
+      ++n2;
+      n2lo += lo < 6;
+    } else if (k == 3) {
+      ++n3;
+      n3lo += lo < 4;
+    } else {
+      ++n4;
+    }
+  }
+  int strided = 0, run = 0;
+  for (int j = 0; j < L; ++j) {
+    if (a.wpos[j] < 4 && a.wlocal[j] >= 4) ++strided;
+    if (a.wpos[j] == (uint32_t)run) ++run;
+  }
+  float* f = d.feat;
+  std::fill(f, f + mlp::kFeatures, 0.f);
+  f[0] = 1.f;
+  f[1] = n3lo / 16.f;
+  f[2] = n2lo / 16.f;
+  f[3] = n2 / 16.f;
+  f[4] = d.gates / 32.f;
+  f[5] = (float)a.gather;
+  f[6] = strided / 4.f;
+  f[7] = run / 12.f;
+  f[8] = n / 34.f;
+  f[9] = a.threads == 512 ? 1.f : 0.f;
+  f[10] = d.persist ? 1.f : 0.f;
+  f[11] = 16.f * a.blob16 / 32768.f;
+  f[12] = 1.f;
+  f[13] = L / 12.f;
+  f[14] = n3 / 16.f;
+  f[15] = n4 / 16.f;
+}
+
 void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* local, int L, const uint32_t* in_pos,
-                  const uint32_t* out_pos, int n, const std::vector<Op>& ops, bool persist, int threads) {
+                  const uint32_t* out_pos, int n, const std::vector<Op>& ops, bool persist, int threads,
+                  bool dry = false) {
   DevOp d;
   d.kind = D_XSWEEP;
   d.category = 1;
@@ -1265,13 +1331,26 @@ void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* 
   d.touched.assign(local, local + L);
   std::sort(d.touched.begin(), d.touched.end());
   d.tile_lq.assign(local, local + L);
-  const TileBlob tbl = tile_blob(plan, d, blocks, local, L, ops, n);
-  a.nblocks = tbl.nblocks;
-  a.kmax = tbl.kmax;
-  a.blob16 = tbl.blob16;
-  d.tile_off = tbl.off;
+  if (dry) {
+    // Cost-model features only: no matrices, nothing in the plan's blob.
+    size_t bytes = 0;
+    for (const FusedBlock* b : blocks) {
+      d.gates += (int)b->ops.size();
+      bytes += sizeof(k::TileBlock) + tile_matrix_bytes((int)b->qubits.size());
+      a.kmax = std::max<uint32_t>(a.kmax, (uint32_t)std::max<size_t>(2, b->qubits.size()));
+    }
+    a.nblocks = (uint32_t)blocks.size();
+    a.blob16 = (uint32_t)(bytes / 16);
+  } else {
+    const TileBlob tbl = tile_blob(plan, d, blocks, local, L, ops, n);
+    a.nblocks = tbl.nblocks;
+    a.kmax = tbl.kmax;
+    a.blob16 = tbl.blob16;
+    d.tile_off = tbl.off;
+  }
   d.kq = L;
   d.low = 0;
+  xsweep_features(d, blocks, local, L, n);
   plan.uses_alt = true;
   plan.ops.push_back(std::move(d));
 }
@@ -1281,21 +1360,42 @@ void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* 
 // write them to the other, so the bit layout (qubit -> position) may change
 // at every sweep: each sweep writes its tile so that the next sweep's qubits
 // sit at positions 0..L-1, whose tiles are then single contiguous runs of
-// 2^L amplitudes. The anchor qubits 0..3 stay at positions 0..3 throughout
-// (every tile holds them, as in emit_sweeps): a store then moves runs of at
-// least 16 amplitudes (128 bytes), longer by the qubits that the next sweep
-// keeps (those go right above the anchor), and its LDS reads stay free of
-// bank conflicts. A run starts and ends in the identity layout: its first
-// sweep gathers its tile (runs of 16 amplitudes), its last stores to the
-// qubits' own positions. Grouping of blocks into sweeps follows emit_sweeps;
-// the tile is padded with the qubits the next blocks use first, which keeps
-// consecutive sweeps overlapping.
-void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int prec, int n, int L, int kt,
-                  const MergeScorer& scorer, bool persist, int threads) {
+// 2^L amplitudes. Positions 0..3 between two sweeps hold 4 qubits that both
+// tiles share (the first 4 kept qubits in the earlier sweep's tile order):
+// a store then moves runs of at least 16 amplitudes (128 bytes), longer by
+// the further kept qubits (those go right above them). A run starts and ends
+// in the identity layout, so its first and last tiles hold qubits 0..3: the
+// first sweep gathers its tile (runs of 16 amplitudes), the last stores to
+// the qubits' own positions. With `pinned`, qubits 0..3 are in every tile
+// (as in emit_sweeps), which keeps every store's LDS reads in 128-byte runs
+// but leaves L - 4 tile qubits for gates; otherwise a tile only has to share
+// 4 qubits with its neighbours, and the shared qubits are taken from the
+// earlier tile's lowest 4 positions where possible (then its store reads LDS
+// in the same 128-byte runs). Blocks are grouped into sweeps greedily (as in
+// emit_sweeps); the tile is padded with the qubits the next blocks use first,
+// which keeps consecutive sweeps overlapping.
+struct XGroup {
+  std::vector<int> blocks;  // pre-fused blocks in execution order
+  uint64_t q = 0;           // tile qubits (exactly L)
+  int standalone = -1;      // a block run as its own launch in the identity layout
+};
+// Sweeps of a unitary run: the run's ops pre-fused into blocks of <= 2
+// qubits, and those blocks grouped into tiles (independent of the block
+// width the sweeps fuse to afterwards).
+struct XPartition {
+  std::vector<FusedBlock> pre;
+  std::vector<XGroup> groups;
+};
+constexpr size_t kXTileBlocksMax = 64;
+
+XPartition group_xsweeps(std::vector<FusedBlock> pre_blocks, int n, int L, bool pinned) {
   constexpr size_t kWindow = 512;
-  constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = 64;
+  constexpr size_t kTileBlocksMax = kXTileBlocksMax;
+  constexpr int kShared = 4;
+  XPartition part;
   const uint64_t anchor = (1ull << std::min(4, n)) - 1;
-  std::vector<FusedBlock> pre = greedy_fuse(ops, begin, end, 2);
+  std::vector<FusedBlock>& pre = part.pre;
+  pre = std::move(pre_blocks);
   std::vector<uint64_t> mask(pre.size());
   for (size_t b = 0; b < pre.size(); ++b) mask[b] = qubit_mask(pre[b].qubits);
   auto tileable = [&](size_t b) { return !pre[b].qubits.empty() && pre[b].qubits.size() <= 4; };
@@ -1304,10 +1404,17 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
   std::vector<char> taken(pre.size(), 0);
   std::vector<int> mark(pre.size(), -1);
   int trial_no = 0;
-  auto trial = [&](size_t head, int seed, uint64_t& smask) {
+  // A tile that must hold `m` fits when m, plus the qubits still needed to
+  // share kShared with the previous tile `prev` (0: the run's first tile),
+  // is at most L qubits.
+  auto fits = [&](uint64_t m, uint64_t prev) {
+    const int need = prev ? std::max(0, kShared - __builtin_popcountll(m & prev)) : 0;
+    return __builtin_popcountll(m) + need <= L;
+  };
+  auto trial = [&](size_t head, int seed, uint64_t base, uint64_t prev, uint64_t& smask) {
     const int gen = ++trial_no;
     std::vector<int> take;
-    smask = anchor;
+    smask = base;
     if (seed >= 0) {
       mark[seed] = gen;
       take.push_back(seed);
@@ -1321,7 +1428,7 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
         const int b = rem[r];
         if (taken[b] || mark[b] == gen) continue;
         ++seen;
-        if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(smask | mask[b]) <= L) {
+        if (!(mask[b] & blocked) && tileable((size_t)b) && fits(smask | mask[b], prev)) {
           const int grow = __builtin_popcountll(mask[b] & ~smask);
           if (grow < best_grow) {
             best = b;
@@ -1338,12 +1445,46 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
     }
     return take;
   };
-  struct Group {
-    std::vector<int> blocks;  // pre-fused blocks in execution order
-    uint64_t q = 0;           // tile qubits (exactly L)
-    int standalone = -1;      // a block run as its own launch in the identity layout
+  // The best of a trial from the head and trials seeded by each of the first
+  // kStarts blocks that could open the sweep.
+  auto select = [&](size_t head, uint64_t base, uint64_t prev, uint64_t& smask) {
+    constexpr int kStarts = 8;
+    std::vector<int> sweep = trial(head, -1, base, prev, smask);
+    uint64_t blocked = 0;
+    int starts = 0;
+    for (size_t r = head; r < rem.size() && starts < kStarts; ++r) {
+      const int b = rem[r];
+      if (taken[b]) continue;
+      if (!(mask[b] & blocked) && tileable((size_t)b) && fits(base | mask[b], prev)) {
+        ++starts;
+        uint64_t m2 = 0;
+        std::vector<int> t = trial(head, b, base, prev, m2);
+        if (t.size() > sweep.size()) {
+          sweep = std::move(t);
+          smask = m2;
+        }
+      }
+      blocked |= mask[b];
+    }
+    return sweep;
   };
-  std::vector<Group> groups;
+  // Whether taking `sweep` leaves the run without a next tileable block.
+  auto ends_run = [&](size_t head, const std::vector<int>& sweep) {
+    const int gen = ++trial_no;
+    for (int b : sweep) mark[b] = gen;
+    for (size_t r = head; r < rem.size(); ++r) {
+      const int b = rem[r];
+      if (taken[b] || mark[b] == gen) continue;
+      return !tileable((size_t)b);
+    }
+    return true;
+  };
+  using Group = XGroup;
+  std::vector<Group>& groups = part.groups;
+  // The previous tile of the current run (0 at a run's start) and its local
+  // order (tile position -> qubit), as the emission below lays them out.
+  uint64_t prev = 0;
+  std::vector<uint32_t> prev_order;
   size_t head = 0;
   while (head < rem.size()) {
     const int h = rem[head];
@@ -1353,31 +1494,18 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
       groups.push_back(g);
       taken[h] = 1;
       ++head;
+      prev = 0;
+      prev_order.clear();
       continue;
     }
-    constexpr int kStarts = 8;
+    uint64_t base = pinned || !prev ? anchor : 0;
     uint64_t smask = 0;
-    std::vector<int> sweep = trial(head, -1, smask);
-    {
-      uint64_t blocked = 0;
-      int starts = 0;
-      for (size_t r = head; r < rem.size() && starts < kStarts; ++r) {
-        const int b = rem[r];
-        if (taken[b]) continue;
-        if (!(mask[b] & blocked) && tileable((size_t)b) && __builtin_popcountll(anchor | mask[b]) <= L) {
-          ++starts;
-          uint64_t m2 = 0;
-          std::vector<int> t = trial(head, b, m2);
-          if (t.size() > sweep.size()) {
-            sweep = std::move(t);
-            smask = m2;
-          }
-        }
-        blocked |= mask[b];
-      }
+    std::vector<int> sweep = select(head, base, prev, smask);
+    if ((smask & anchor) != anchor && ends_run(head, sweep)) {
+      base |= anchor;  // the run's last tile stores to the identity layout
+      sweep = select(head, base, prev, smask);
     }
     for (int b : sweep) taken[b] = 1;
-    // Pad the tile with the qubits the next blocks use first.
     std::vector<uint32_t> first(64, UINT32_MAX);
     {
       size_t seen = 0;
@@ -1392,40 +1520,69 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
     std::vector<uint32_t> cand;
     for (int q = 0; q < n; ++q) cand.push_back((uint32_t)q);
     std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return first[a] < first[b]; });
+    if (prev) {
+      // Shared qubits: the previous tile's lowest positions first, then its
+      // other qubits in order of next use.
+      for (size_t j = 0; j < prev_order.size() && j < (size_t)kShared; ++j)
+        if (__builtin_popcountll(smask & prev) < kShared) smask |= 1ull << prev_order[j];
+      for (uint32_t q : cand)
+        if (__builtin_popcountll(smask & prev) < kShared && ((prev >> q) & 1)) smask |= 1ull << q;
+    }
+    // Pad the tile with the qubits the next blocks use first.
     for (uint32_t q : cand) {
       if (__builtin_popcountll(smask) >= L) break;
       smask |= 1ull << q;
     }
+    if (__builtin_popcountll(smask) != L || (prev && __builtin_popcountll(smask & prev) < kShared))
+      throw RuntimeErr("internal: out-of-place sweep tile");
+    std::vector<uint32_t> order;
+    for (uint32_t q : prev_order)
+      if ((smask >> q) & 1) order.push_back(q);
+    for (int q = 0; q < n; ++q)
+      if (((smask >> q) & 1) && !((prev >> q) & 1)) order.push_back((uint32_t)q);
+    prev = smask;
+    prev_order = std::move(order);
     Group g;
     g.blocks = std::move(sweep);
     g.q = smask;
     groups.push_back(std::move(g));
     while (head < rem.size() && taken[rem[head]]) ++head;
   }
+  return part;
+}
+
+// Launches of a partition's sweeps with blocks of <= kt qubits. `dry` makes
+// the cost-model features only (no matrices, nothing in the plan's blob) and
+// leaves out the standalone blocks, which every candidate shares.
+void emit_xpartition(Plan& plan, XPartition& part, const std::vector<Op>& ops, int prec, int n, int L, int kt,
+                     const MergeScorer& scorer, bool persist, int threads, bool dry) {
+  constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = kXTileBlocksMax;
+  std::vector<FusedBlock>& pre = part.pre;
+  const std::vector<XGroup>& groups = part.groups;
   // Layouts and launches, one run of sweeps at a time.
   std::vector<uint32_t> ident(64);
   for (uint32_t q = 0; q < 64; ++q) ident[q] = q;
   size_t gi = 0;
   while (gi < groups.size()) {
     if (groups[gi].standalone >= 0) {
-      emit_block(plan, pre[groups[gi].standalone], ops, prec, n);
+      if (!dry) emit_block(plan, pre[groups[gi].standalone], ops, prec, n);
       ++gi;
       continue;
     }
     size_t gj = gi;
     while (gj < groups.size() && groups[gj].standalone < 0) ++gj;
     // Sweep gi reads the identity layout: local order = its qubits ascending
-    // (the anchor first).
+    // (qubits 0..3 first).
     std::vector<uint32_t> cur_pos = ident, order;
     for (int q = 0; q < n; ++q)
       if ((groups[gi].q >> q) & 1) order.push_back((uint32_t)q);
     for (size_t s = gi; s < gj; ++s) {
-      const Group& g = groups[s];
+      const XGroup& g = groups[s];
       std::vector<uint32_t> next_pos = ident, next_order;
       if (s + 1 < gj) {
         const uint64_t nq = groups[s + 1].q;
         for (uint32_t q : order)
-          if ((nq >> q) & 1) next_order.push_back(q);  // kept qubits, anchor first, in this sweep's order
+          if ((nq >> q) & 1) next_order.push_back(q);  // kept qubits, in this sweep's order
         for (int q = 0; q < n; ++q)
           if (((nq >> q) & 1) && !((g.q >> q) & 1)) next_order.push_back((uint32_t)q);
         if ((int)next_order.size() != L) throw RuntimeErr("internal: out-of-place sweep tile size");
@@ -1453,7 +1610,7 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
         for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
         const bool last = j >= blocks.size();
         emit_xlaunch(plan, bp, order.data(), L, cur_pos.data(), last ? next_pos.data() : cur_pos.data(), n, ops,
-                     persist, threads);
+                     persist, threads, dry);
         i = j;
       } while (i < blocks.size());
       cur_pos = next_pos;
@@ -1461,6 +1618,12 @@ void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t e
     }
     gi = gj;
   }
+}
+
+void emit_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int prec, int n, int L, int kt,
+                  const MergeScorer& scorer, bool persist, int threads, bool pinned) {
+  XPartition part = group_xsweeps(greedy_fuse(ops, begin, end, 2), n, L, pinned);
+  emit_xpartition(plan, part, ops, prec, n, L, kt, scorer, persist, threads, false);
 }
 
 // Exact complex128 sweeps: maximal runs of consecutive compiled ops (program
@@ -1808,7 +1971,72 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         return planner_score(a, b, o, n);
       };
     if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
-      if (fusion == FUSE_LEARNED && !g_planner.empty()) {
+      if (permute) {
+        const bool persist = mode == 1 || mode == 4;
+        const int threads = mode >= 3 && tile >= 11 ? 512 : 256;
+        const int def_k = std::max(1, std::min(max_fused, 4));
+        const bool def_pin = g_sweep_pinned;
+        int best_k = def_k;
+        bool best_pin = def_pin;
+        if (fusion == FUSE_LEARNED && !g_planner.empty()) {
+          // Choose (tile rule, block width) by the model's predicted total
+          // time; keep the default (the process-wide tile rule, blocks of up
+          // to max_fused qubits) unless another is predicted >= 3 % faster.
+          // Host threads: the two tile rules' groupings, then one dry
+          // emission per (rule, width) candidate.
+          const std::vector<FusedBlock> pre = greedy_fuse(ops, i, j, 2);
+          XPartition parts[2];
+          double pred[2][5] = {};
+          std::exception_ptr err[2][5];
+          auto run_all = [&](std::vector<std::pair<int, int>> jobs, const std::function<void(int, int)>& fn) {
+            std::vector<std::thread> th;
+            for (size_t t = 1; t < jobs.size(); ++t)
+              th.emplace_back([&, t] {
+                try {
+                  fn(jobs[t].first, jobs[t].second);
+                } catch (...) {
+                  err[jobs[t].first][jobs[t].second] = std::current_exception();
+                }
+              });
+            try {
+              fn(jobs[0].first, jobs[0].second);
+            } catch (...) {
+              err[jobs[0].first][jobs[0].second] = std::current_exception();
+            }
+            for (std::thread& t : th) t.join();
+            for (auto& row : err)
+              for (const std::exception_ptr& e : row)
+                if (e) std::rethrow_exception(e);
+          };
+          run_all({{0, 0}, {1, 0}}, [&](int pc, int) { parts[pc] = group_xsweeps(pre, n, tile, pc == 1); });
+          std::vector<std::pair<int, int>> cands;
+          for (int pc = 0; pc < 2; ++pc)
+            for (int kt = std::min(2, def_k); kt <= def_k; ++kt) cands.push_back({pc, kt});
+          run_all(cands, [&](int pc, int kt) {
+            Plan cand;  // a dry emission only reads the partition, so candidates share it
+            emit_xpartition(cand, parts[pc], ops, prec, n, tile, kt, nullptr, persist, threads, true);
+            for (const DevOp& d : cand.ops) pred[pc][kt] += predicted_ms(d, n);
+          });
+          const double def_t = pred[def_pin ? 1 : 0][def_k];
+          double best = def_t;
+          for (int pc = 0; pc < 2; ++pc)
+            for (int kt = std::min(2, def_k); kt <= def_k; ++kt)
+              if (pred[pc][kt] < best) {
+                best = pred[pc][kt];
+                best_k = kt;
+                best_pin = pc == 1;
+              }
+          if (best > 0.97 * def_t) {
+            best_k = def_k;
+            best_pin = def_pin;
+          }
+          plan.learned_choices.push_back((best_pin ? 100 : 0) + best_k);
+          emit_xpartition(plan, parts[best_pin ? 1 : 0], ops, prec, n, tile, best_k, nullptr, persist, threads,
+                          false);
+        } else {
+          emit_xsweeps(plan, ops, i, j, prec, n, tile, best_k, nullptr, persist, threads, best_pin);
+        }
+      } else if (fusion == FUSE_LEARNED && !g_planner.empty()) {
         // Choose (block width, tile size) by the model's predicted total
         // time; leave the default (blocks <= 3 qubits, the largest tile) only
         // for a predicted gain of at least 3 %.
@@ -1835,9 +2063,6 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
           best_l = def_l;
         }
         emit_sweeps(plan, ops, i, j, prec, n, best_l, best_k, nullptr);
-      } else if (permute) {
-        emit_xsweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer, mode == 1 || mode == 4,
-                     mode >= 3 && tile >= 11 ? 512 : 256);
       } else {
         emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
       }
@@ -2666,7 +2891,11 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
   std::ostringstream js;
   js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
      << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions)
-     << ", \"blob_bytes\": " << plan.blob.size() << ", \"launches\": [";
+     << ", \"blob_bytes\": " << plan.blob.size() << ", \"learned_choices\": [";
+  for (size_t i = 0; i < plan.learned_choices.size(); ++i)
+    js << (i ? ", " : "") << "{\"pinned\": " << (plan.learned_choices[i] >= 100 ? "true" : "false")
+       << ", \"k\": " << plan.learned_choices[i] % 100 << "}";
+  js << "], \"launches\": [";
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     const DevOp& d = plan.ops[i];
     js << (i ? ", " : "") << "{\"kind\": \"" << names[d.kind] << "\", \"k\": " << d.kq
@@ -2677,6 +2906,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
     for (size_t t = 0; t < d.touched.size(); ++t) js << (t ? ", " : "") << d.touched[t];
     js << "], \"ops\": [";
     for (size_t t = 0; t < d.src.size(); ++t) js << (t ? ", " : "") << d.src[t];
+    js << "], \"features\": [";
+    for (int t = 0; t < mlp::kFeatures; ++t) js << (t ? ", " : "") << json_num(d.feat[t]);
     js << "]";
     if (d.kind == D_TILE) {
       js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"chains\": " << d.chains
@@ -2802,15 +3033,18 @@ int device_attr(hipDeviceAttribute_t a) {
 
 }  // namespace
 
-void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_dynamic, int* static_lds,
-                       int* max_threads, int* lmax) {
-  *max_lds_per_block = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
+void batch_device_info(int block, int* info) {
+  const int max_lds = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
   int l = 0;
-  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= *max_lds_per_block) ++l;
-  *lmax = l;
-  *set_rc = bk::set_lds_limit(block, bk::lds_bytes(l, block));
-  *max_dynamic = *static_lds = *max_threads = -1;
-  bk::func_info(block, max_dynamic, static_lds, max_threads);
+  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= max_lds) ++l;
+  info[0] = max_lds;
+  info[5] = l;
+  info[1] = bk::set_lds_limit(block, bk::lds_bytes(l, block));
+  info[2] = info[3] = info[4] = info[7] = -1;
+  bk::func_info(block, &info[2], &info[3], &info[4], &info[7]);
+  info[8] = device_attr(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor);
+  info[10] = -1;
+  info[9] = bk::occupancy(block, bk::lds_bytes(l, block), &info[10]);
 }
 
 namespace {

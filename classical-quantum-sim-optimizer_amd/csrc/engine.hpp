@@ -190,15 +190,18 @@ Counts batch_counts(const BatchOutcome& b, uint64_t i);
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch);
 // Relabelled tile sweeps on or off (process-wide); returns the previous setting.
 bool set_sweep_relabel(bool on);
+// Out-of-place sweeps: qubits 0..3 in every tile (true) or only in each
+// run's first and last tile (false, the default); returns the previous value.
+bool set_sweep_pinned(bool on);
 // Host-only text dump of the program the persistent batched kernel would
 // upload (circuits, launch order, fused ops, FP32 matrices; circuits above
 // `lmax` qubits use an HBM slot of the printed size).
 std::string batch_program_text(const std::vector<std::pair<const char*, size_t>>& circuits,
                                const std::vector<uint64_t>& shots, const std::vector<uint64_t>& seeds, int max_fused,
                                int lmax);
-// Device and kernel limits relevant to the batched launcher (no launch).
-void batch_device_info(int block, int* max_lds_per_block, int* set_rc, int* max_dynamic, int* static_lds,
-                       int* max_threads, int* lmax);
+// Device and kernel limits relevant to the batched launcher (no launch);
+// fills info[0..5] and info[7..10] in the order of dq_batch_device_info.
+void batch_device_info(int block, int* info);
 
 // Learned fusion planner weights (2 hidden layers of 64, FP32). Null clears.
 // Host forward pass of the planner MLP (packed weights, one feature vector).

@@ -867,15 +867,24 @@ struct XSweepRead {
   }
 };
 
-template <int L, bool K4, bool PERSIST, int NTH>
+// GBLOB: the blob (block descriptors and matrices) is read from global
+// memory, where every workgroup of the launch finds it in cache, instead of
+// being copied into LDS behind the tile; used for blobs large enough to cost
+// resident workgroups (see xsweep_blob_in_lds).
+template <int L, bool K4, bool PERSIST, int NTH, bool GBLOB>
 __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, float2* __restrict__ dst,
                                                 uint64_t stride, uint64_t ntiles, XSweepArgs a,
                                                 const float4* __restrict__ blob) {
   extern __shared__ float4 lds4[];
   constexpr int PER = (1 << L) / NTH;
   float2* tile = (float2*)lds4;
-  const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L) / 2);
-  for (uint32_t i = threadIdx.x; i < a.blob16; i += NTH) lds4[(1 << L) / 2 + i] = blob[i];
+  const unsigned char* meta;
+  if (GBLOB) {
+    meta = (const unsigned char*)blob;
+  } else {
+    meta = (const unsigned char*)(lds4 + (1 << L) / 2);
+    for (uint32_t i = threadIdx.x; i < a.blob16; i += NTH) lds4[(1 << L) / 2 + i] = blob[i];
+  }
   const TileBlock* blocks = (const TileBlock*)meta;
   const float2* sb = src + (uint64_t)blockIdx.y * stride;
   float2* db = dst + (uint64_t)blockIdx.y * stride;
@@ -2023,7 +2032,16 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
 #undef L
 }
 
-size_t xsweep_lds_bytes(const XSweepArgs& a) { return ((size_t)8 << a.L) + (size_t)16 * a.blob16; }
+// The blob goes into LDS while it adds at most a quarter of the tile's bytes
+// (8 KiB for 12-qubit tiles: four 256-thread workgroups still fit a CU's
+// 160 KiB); a larger one would cost resident workgroups and is read from
+// global memory. Measured on 32-qubit sweeps: LDS is faster for the few
+// blocks of a random-circuit sweep (17.4 vs 18.1 ms), global for sweeps of
+// 10+ blocks (41.8 vs 49.8 ms).
+bool xsweep_blob_in_lds(const XSweepArgs& a) { return (size_t)16 * a.blob16 <= ((size_t)8 << a.L) / 4; }
+size_t xsweep_lds_bytes(const XSweepArgs& a) {
+  return ((size_t)8 << a.L) + (xsweep_blob_in_lds(a) ? (size_t)16 * a.blob16 : 0);
+}
 
 void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
             const void* blob, bool persist, hipStream_t st) {
@@ -2034,11 +2052,14 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = xsweep_lds_bytes(a);
   dim3 g((uint32_t)(persist ? tile_grid(ntiles) : ntiles), batch);
-#define L(LL, K4, P, NT)                                                                                          \
-  hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \
+  const bool gb = !xsweep_blob_in_lds(a);
+#define LG(LL, K4, P, NT, G)                                                                                         \
+  hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT, G>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \
                      ntiles, a, (const float4*)blob)
+#define L(LL, K4, P, NT) \
+  if (gb) LG(LL, K4, P, NT, true); else LG(LL, K4, P, NT, false)
 #define LP(LL, K4, NT) \
-  if (persist) L(LL, K4, true, NT); else L(LL, K4, false, NT)
+  if (persist) { L(LL, K4, true, NT); } else { L(LL, K4, false, NT); }
 #define LT(LL, K4) \
   if (a.threads == 512) { LP(LL, K4, 512); } else { LP(LL, K4, 256); }
   const bool k4 = a.kmax >= 4;
@@ -2050,6 +2071,7 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
 #undef LT
 #undef LP
 #undef L
+#undef LG
 }
 
 void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob,

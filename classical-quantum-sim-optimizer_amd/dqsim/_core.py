@@ -155,6 +155,7 @@ def _load():
         "dq_device_count": (ctypes.c_int, []),
         "dq_mfma_fp32_peak": (ctypes.c_int, [I32, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_set_sweep_relabel": (ctypes.c_int, [I32]),
+        "dq_set_sweep_pinned": (ctypes.c_int, [I32]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
@@ -427,6 +428,11 @@ class StatevectorSimulator:
             raise ValueError(f"fusion must be one of {sorted(_FUSIONS)}")
         o.precision = _PRECISIONS[precision]
         o.fusion = _FUSIONS[fusion]
+        if fusion == "learned" and not _planner_weights_set[0]:
+            from . import planner  # the shipped weights, once per process
+
+            planner.install_default()
+            _planner_weights_set[0] = True
         o.max_fused_qubits = min(_as_u64(max_fused_qubits, "max_fused_qubits"), 99)
         o.tile_qubits = min(_as_u64(tile_qubits, "tile_qubits"), 99)
         h = ctypes.c_void_p()
@@ -578,14 +584,31 @@ def set_sweep_relabel(on: bool) -> bool:
 
 
 def set_sweep_mode(mode: int) -> int:
-    """Sweep mode of complex64 fused plans for this process: 0 in place,
-    1 out of place with persistent workgroups (default), 2 out of place with
-    one workgroup per tile. Out-of-place sweeps run only when a call's budget
-    holds a second state. Returns the previous mode."""
-    return int(_lib.dq_set_sweep_mode(int(mode)))
+    """Sweep mode of complex64 fused plans for this process: 0 in place; out
+    of place with 1 persistent 256-thread workgroups, 2 one 256-thread
+    workgroup per tile (default), 3 one 512-thread workgroup per tile, 4
+    persistent 512-thread workgroups. Out-of-place sweeps run only when a
+    call's budget holds a second state. Returns the previous mode."""
+    prev = int(_lib.dq_set_sweep_mode(int(mode)))
+    if prev < 0:
+        raise ValueError(f"sweep mode must be 0..4, got {mode}")
+    return prev
+
+
+def set_sweep_pinned(on: bool) -> bool:
+    """Out-of-place sweeps keep qubits 0..3 in every tile (True) or only in
+    the first and last tile of a run, consecutive tiles sharing 4 qubits
+    (False, the default: more tile qubits for gates, fewer sweeps). Returns
+    the previous setting."""
+    return bool(_lib.dq_set_sweep_pinned(1 if on else 0))
+
+
+_planner_weights_set = [False]  # weights installed or cleared in this process
 
 
 def set_planner_weights(weights) -> None:
+    """Weights of the learned planner for this process (None: greedy)."""
+    _planner_weights_set[0] = True
     if weights is None:
         _check(_lib.dq_set_planner_weights(None, 0))
         return

@@ -123,10 +123,13 @@ def program_text(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] 
 
 def device_info(block: int = 256) -> Dict[str, int]:
     """Launch limits of the batched kernel for ``block`` threads (no launch):
-    device LDS per block, the planned LDS qubit count, whether the kernel
-    accepted that dynamic LDS size, and the kernel's own attributes."""
-    info = (ctypes.c_int32 * 7)()
+    device LDS per block and per CU, the planned LDS qubit count, whether the
+    kernel accepted that dynamic LDS size, the kernel's own attributes (its
+    per-lane private segment included) and the runtime's count of resident
+    workgroups per CU at that LDS size."""
+    info = (ctypes.c_int32 * 11)()
     _check(_lib.dq_batch_device_info(block, info))
     keys = ("max_lds_per_block", "set_attribute_rc", "kernel_max_dynamic_lds", "kernel_static_lds",
-            "kernel_max_threads", "lds_qubits", "device_count")
+            "kernel_max_threads", "lds_qubits", "device_count", "kernel_private_bytes", "lds_per_cu",
+            "occupancy_rc", "blocks_per_cu")
     return dict(zip(keys, (int(v) for v in info)))

@@ -1,19 +1,42 @@
 """Learned fusion planner.
 
 A 16 -> 64 -> 64 -> 1 FP32 MLP predicts the device time of one launch (log of
-milliseconds per 2^24 amplitudes) from the launch's features:
+milliseconds per 2^24 amplitudes) from the launch's features. A one-block
+launch and an in-place tile sweep use
 
    0 k/5              1 lowest qubit/34      2 highest qubit/34    3 qubits < 4, /4
    4 ops folded/32    5 diagonal-op share    6 Clifford-op share   7 all ops diagonal
    8 n/34             9 k >= 4 (MFMA path)  10 span/34            11 share of amplitudes touched
   12 tile sweep      13 tile qubits/12      14 3-qubit-path blocks/16  15 4-qubit blocks/16
 
-(0-11 describe a one-block launch and are zero for a tile sweep; 12-15 are
-zero for a one-block launch.) With ``fusion="learned"`` and weights
-installed, complex64 plans with LDS tile sweeps are chosen among candidate
-(block width, tile size) pairs by the model's predicted total time; plans
-without sweeps ask the model before every merge and merge blocks a and b only
-when t(a) + t(b) > t(a+b). Without weights both fall back to plain greedy.
+(0-11 describe a one-block launch and are zero for an in-place tile sweep
+except 4 and 8; 12-15 are zero for a one-block launch). An out-of-place
+(permuting) tile sweep, the complex64 default when the budget holds a second
+state, uses
+
+   0 1 (out of place)  1 3-qubit blocks on a tile bit < 4, /16
+   2 2-qubit blocks on a tile bit < 6, /16      3 2-qubit blocks/16
+   4 gates/32          5 gather (first sweep of a run)
+   6 destination positions 0..3 stored from tile bits >= 4, /4
+   7 lowest destination positions stored from tile bits, /12
+   8 n/34              9 512-thread workgroups 10 persistent workgroups
+  11 blob bytes/32768 12 1 (tile sweep)       13 tile qubits/12
+  14 3-qubit blocks/16 15 4-qubit blocks/16
+
+With ``fusion="learned"`` and weights installed:
+  * out-of-place sweeps: the plan is chosen among candidate (tile rule, block
+    width) pairs -- qubits 0..3 pinned in every tile or only shared between
+    neighbouring tiles, blocks of 2..max_fused (<= 4) qubits -- by the
+    model's predicted total time; the default (the process-wide tile rule,
+    blocks of max_fused qubits) is kept unless another pair is predicted at
+    least 3 % faster;
+  * in-place sweeps: the same among (block width, tile size) pairs;
+  * plans without sweeps ask the model before every merge and merge blocks a
+    and b only when t(a) + t(b) > t(a+b).
+Without weights all three fall back to plain greedy. ``StatevectorSimulator``
+with ``fusion="learned"`` installs the weights shipped with the package
+(``planner_weights.npy``, trained by ``python -m dqsim.planner``) unless
+weights were installed (or cleared) explicitly.
 
 Training data are the engine's own per-launch HIP-event timings
 (``StatevectorSimulator.evolve_timed``) on seeded random circuits, and the
@@ -23,9 +46,14 @@ network is trained on the GPU by the engine's forward/backward/Adam kernels
 
 from __future__ import annotations
 
+import argparse
 import ctypes
+import json
 import math
-from typing import Iterable, List, Optional, Sequence, Tuple
+import os
+import sys
+import time
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -86,12 +114,91 @@ def install(w: Optional[np.ndarray]) -> None:
     _core.set_planner_weights(w)
 
 
+WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_weights.npy")
+
+
+def load_default(path: str = WEIGHTS_FILE) -> Optional[np.ndarray]:
+    """The shipped weights (None if the file is absent)."""
+    if not os.path.exists(path):
+        return None
+    w = np.load(path, allow_pickle=False)
+    if w.dtype != np.float32 or w.shape != (N_PARAMS,):
+        raise ValueError(f"{path}: expected {N_PARAMS} float32 weights, got {w.dtype} {w.shape}")
+    return w
+
+
+def install_default(path: str = WEIGHTS_FILE) -> bool:
+    """Install the shipped weights; False (greedy stays) if there are none."""
+    w = load_default(path)
+    if w is None:
+        return False
+    install(w)
+    return True
+
+
 def launch_records(circuit, max_fused_qubits: int, precision: str = "complex64", fusion: str = "greedy",
-                   tile_qubits: int = 0):
+                   tile_qubits: int = 0, out_of_place: bool = True):
+    """Per-launch timings of one evolution. out_of_place=False gives the call
+    a budget that holds one state only, so its tile sweeps run in place."""
+    if out_of_place:
+        mem = 1 << 20
+    else:
+        state_mb = ((1 << circuit.num_qubits) * (8 if precision == "complex64" else 16)) >> 20
+        mem = state_mb + state_mb // 2 + 64
     sim = _core.StatevectorSimulator(seed=1, precision=precision, fusion=fusion, max_fused_qubits=max_fused_qubits,
-                                     max_memory_mb=1 << 20, tile_qubits=tile_qubits)
+                                     max_memory_mb=mem, tile_qubits=tile_qubits)
     rows, launches, gates, total_ms, _ = sim.evolve_timed(circuit, repeats=1, record=True)
     return rows, total_ms
+
+
+def sweep_dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4), seed: int = 0, log=None,
+                  block_share: float = 0.1, in_place_share: float = 0.15, time_limit_s: Optional[float] = None):
+    """(X, y, info): features and log(ms per 2^24 amplitudes) of every launch
+    of the circuits, complex64. Each circuit is evolved once: with
+    probability block_share one launch per block (width 2..5), with
+    in_place_share in-place 12-qubit tile sweeps, otherwise out-of-place
+    12-qubit sweeps under a tile rule (pinned or shared) and block width drawn
+    at random -- the choices the learned planner makes. Collection stops
+    after time_limit_s seconds if given."""
+    rng = np.random.default_rng(seed)
+    X: List[List[float]] = []
+    y: List[float] = []
+    info = {"circuits": 0, "block_plans": 0, "in_place_plans": 0, "out_of_place_plans": 0, "qubits": {}}
+    t0 = time.perf_counter()
+    last = t0
+    for i, c in enumerate(circuits):
+        n = c.num_qubits
+        u = float(rng.random())
+        k = int(widths[int(rng.integers(len(widths)))])
+        pinned = bool(rng.integers(2))
+        if u < block_share:
+            rows, _ = launch_records(c, int(rng.integers(2, 6)), tile_qubits=0)
+            info["block_plans"] += 1
+        elif u < block_share + in_place_share:
+            rows, _ = launch_records(c, k, tile_qubits=12, out_of_place=False)
+            info["in_place_plans"] += 1
+        else:
+            prev = _core.set_sweep_pinned(pinned)
+            try:
+                rows, _ = launch_records(c, k, tile_qubits=12)
+            finally:
+                _core.set_sweep_pinned(prev)
+            info["out_of_place_plans"] += 1
+        info["circuits"] += 1
+        info["qubits"][str(n)] = info["qubits"].get(str(n), 0) + 1
+        for r in rows:
+            if r["ms"] <= 0:
+                continue
+            X.append(r["feat"])
+            y.append(math.log(r["ms"] * 2.0 ** (24 - n)))
+        now = time.perf_counter()
+        if log is not None and (now - last > 30 or i == 0):
+            last = now
+            log(f"[planner] dataset: circuit {i + 1} ({n} qubits), {len(y)} launches, {now - t0:.0f} s")
+        if time_limit_s is not None and now - t0 > time_limit_s:
+            break
+    info["collect_s"] = time.perf_counter() - t0
+    return np.asarray(X, dtype=np.float32), np.asarray(y, dtype=np.float32), info
 
 
 def dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4, 5), seed: int = 0, log=None,
@@ -142,3 +249,71 @@ def training_circuits(count: int, nmin: int = 24, nmax: int = 28, dmin: int = 20
         n = int(rng.integers(nmin, nmax + 1))
         d = int(rng.integers(dmin, dmax + 1))
         yield family_circuit(families[i % len(families)], n, d, seed * 1_000_003 + i)
+
+
+# Share of training circuits per qubit range: the time of a launch per 2^24
+# amplitudes is what the model learns, so most circuits are small (fast to
+# record) and the range the planner is evaluated on (28-33) is still covered.
+SIZE_MIX = ((24, 27, 0.62), (28, 30, 0.25), (31, 32, 0.10), (33, 33, 0.03))
+
+
+def mixed_size_circuits(count: int, seed: int, dmin: int = 20, dmax: int = 80,
+                        mix: Sequence[Tuple[int, int, float]] = SIZE_MIX, families: Sequence[str] = FAMILIES):
+    """Seeded training circuits: qubit count drawn from `mix`, depth uniform in
+    [dmin, dmax], families in rotation."""
+    rng = np.random.default_rng(seed)
+    p = np.array([m[2] for m in mix], dtype=np.float64)
+    p /= p.sum()
+    for i in range(count):
+        lo, hi, _ = mix[int(rng.choice(len(mix), p=p))]
+        n = int(rng.integers(lo, hi + 1))
+        d = int(rng.integers(dmin, dmax + 1))
+        yield family_circuit(families[i % len(families)], n, d, seed * 1_000_003 + i)
+
+
+def train_and_save(out_dir: str, circuits: int = 4000, seed: int = 11, epochs: int = 60, time_limit_s: float = 600,
+                   log=print) -> Dict:
+    """Record engine timings, train the MLP on the GPU, and write
+    planner_weights.npy, planner_weights.json (how it was made) and
+    planner_dataset.npz (the samples) into out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+    X, y, info = sweep_dataset(mixed_size_circuits(circuits, seed), seed=seed, log=log, time_limit_s=time_limit_s)
+    log(f"[planner] {info['circuits']} circuits, {len(y)} launches in {info['collect_s']:.0f} s")
+    np.savez_compressed(os.path.join(out_dir, "planner_dataset.npz"), X=X, y=y)
+    perm = np.random.default_rng(seed).permutation(len(y))
+    cut = int(0.9 * len(y))
+    tr, te = perm[:cut], perm[cut:]
+    t0 = time.perf_counter()
+    w, curve = train(X[tr], y[tr], epochs=epochs, batch=256, lr=2e-3, seed=seed)
+    train_s = time.perf_counter() - t0
+    pred = predict(w, X[te])
+    meta = dict(info)
+    meta.update({"seed": seed, "size_mix": [list(m) for m in SIZE_MIX], "families": list(FAMILIES),
+                 "depth_range": [20, 80], "train_samples": int(len(tr)), "test_samples": int(len(te)),
+                 "epochs": epochs, "train_s": train_s, "final_train_loss": float(curve[-1]),
+                 "test_mse": float(np.mean((pred - y[te]) ** 2)),
+                 "test_mse_predict_mean": float(np.mean((y[te] - y[tr].mean()) ** 2))})
+    np.save(os.path.join(out_dir, "planner_weights.npy"), w)
+    with open(os.path.join(out_dir, "planner_weights.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    log(f"[planner] trained in {train_s:.0f} s: test mse {meta['test_mse']:.4f} "
+        f"(predicting the mean: {meta['test_mse_predict_mean']:.4f})")
+    return meta
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="Train the learned planner on engine-recorded launch timings.")
+    ap.add_argument("--out", required=True, help="directory for planner_weights.npy/.json and the dataset")
+    ap.add_argument("--circuits", type=int, default=4000)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--time-limit", type=float, default=600, help="seconds of timing collection at most")
+    a = ap.parse_args(argv)
+    meta = train_and_save(a.out, a.circuits, a.seed, a.epochs, a.time_limit,
+                          log=lambda m: print(m, flush=True))
+    print(json.dumps({k: v for k, v in meta.items() if k != "qubits"}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -15,14 +15,16 @@
   batch_loop_report .. the same circuits through the per-circuit API, one
                        simulate_shots call after another (the baseline the
                        batched runner is measured against).
-  planner_report ..... learned fusion planner trained on engine-recorded launch
-                       timings vs the greedy fuser on held-out circuits (one
-                       launch per fused block, the plans the model times).
+  planner_report ..... the shipped learned planner (trained on engine-recorded
+                       launch timings) vs the greedy fuser on held-out
+                       28-33-qubit circuits, out-of-place tile sweeps.
 """
 
 from __future__ import annotations
 
+import json
 import math
+import os
 import time
 from typing import Dict, List
 
@@ -122,74 +124,72 @@ def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, lo
     return out
 
 
-def planner_report(train_circuits: int = 1200, train_n=(24, 30), eval_n=(28, 30), seed: int = 5, epochs: int = 300,
-                   log=print) -> dict:
-    """Train the planner MLP on engine-recorded launch timings (per-block and
-    tile plans of four circuit families), then on held-out circuits compare
-    (a) tile plans: greedy default (blocks <= 3 qubits, 12-qubit tiles) vs the
-    learned choice of (block width <= 4, tile size); (b) one launch per block:
-    greedy k <= 4 / k <= 5 vs learned merges at k <= 5. Accuracy of every plan
-    against the exact complex128 state."""
-    t0 = time.perf_counter()
-    X, y = planner.dataset(planner.training_circuits(train_circuits, nmin=train_n[0], nmax=train_n[1], dmin=20,
-                                                     dmax=80, seed=seed), seed=seed, log=log)
-    collect_s = time.perf_counter() - t0
-    perm = np.random.default_rng(seed).permutation(len(y))
-    cut = int(0.9 * len(y))
-    tr, te = perm[:cut], perm[cut:]
-    t0 = time.perf_counter()
-    w, curve = planner.train(X[tr], y[tr], epochs=epochs, batch=256, lr=2e-3, seed=seed)
-    train_s = time.perf_counter() - t0
-    pred = planner.predict(w, X[te])
-    test_mse = float(np.mean((pred - y[te]) ** 2))
-    base_mse = float(np.mean((y[te] - y[tr].mean()) ** 2))
+PLANNER_HELD_OUT = (("su2", 28, 40), ("brickwork", 29, 40), ("qft", 30, 0), ("mixed", 31, 40), ("su2", 32, 40),
+                    ("brickwork", 32, 60), ("qft", 33, 0))
 
-    def timed(c, n, **kw):
-        mem = (1 << n) * 16 * 3 // (1 << 20) + 4096
-        sim = _core.StatevectorSimulator(seed=1, precision="complex64", max_memory_mb=mem, **kw)
-        sim.evolve_timed(c, record=False)  # warm-up
-        _, launches, _, ms, res = sim.evolve_timed(c, record=False, keep=True)
-        return ms, launches, res
 
+def planner_report(held_out=PLANNER_HELD_OUT, repeats: int = 3, log=print) -> dict:
+    """The shipped planner weights (planner_weights.npy, trained by
+    ``python -m dqsim.planner`` on engine-recorded launch timings) against the
+    greedy planner on held-out complex64 circuits (seeds 20_000 + n, never in
+    training), out-of-place tile sweeps: greedy with blocks of <= 5 qubits
+    (fused as far as the tiles allow), greedy with <= 3 (the hand-tuned
+    width), and the learned choice of (tile rule, block width) at <= 5. Wall
+    time per evolution (plan + upload + sweeps), median of `repeats`; max
+    amplitude error of each plan against the complex128 path up to 32 qubits."""
+    from . import planner
+
+    if not planner.install_default():
+        raise RuntimeError("planner_weights.npy is missing")
+    meta_path = planner.WEIGHTS_FILE[:-4] + ".json"
+    meta = json.load(open(meta_path)) if os.path.exists(meta_path) else {}
+    configs = (("greedy_k5", "greedy", 5), ("greedy_k3", "greedy", 3), ("learned_k5", "learned", 5))
     rows = []
-    for fam in planner.FAMILIES:
-        for n in eval_n:
-            c = planner.family_circuit(fam, n, 40, 10_000 + n)
-            mem = (1 << n) * 16 * 3 // (1 << 20) + 4096
-            ref = _core.StatevectorSimulator(seed=1, max_memory_mb=mem)
-            _, _, _, _, rres = ref.evolve_timed(c, record=False, keep=True)
-            row = {"family": fam, "qubits": n}
-            configs = [("greedy_tiles_k3", {"fusion": "greedy", "max_fused_qubits": 3}),
-                       ("learned_tiles_k4", {"fusion": "learned", "max_fused_qubits": 4})]
-            if fam == "su2":
-                configs += [("greedy_blocks_k4", {"fusion": "greedy", "max_fused_qubits": 4, "tile_qubits": 0}),
-                            ("greedy_blocks_k5", {"fusion": "greedy", "max_fused_qubits": 5, "tile_qubits": 0}),
-                            ("learned_blocks_k5", {"fusion": "learned", "max_fused_qubits": 5, "tile_qubits": 0})]
-            for name, kw in configs:
-                if kw["fusion"] == "learned":
-                    planner.install(w)
-                try:
-                    ms, launches, res = timed(c, n, **kw)
-                    err, _ = res.compare(rres)
+    try:
+        for fam, n, depth in held_out:
+            c = planner.family_circuit(fam, n, depth, 20_000 + n)
+            mem = (1 << n) * 8 * 3 // (1 << 20) + 4096
+            ref = None
+            if n <= 32:
+                exact = _core.StatevectorSimulator(seed=1, precision="complex128", max_memory_mb=mem * 2)
+                ref = exact.evolve_timed(c, record=False, keep=True)[4]
+            row = {"family": fam, "qubits": n, "depth": depth,
+                   "learned_choice": _core.plan_json(c, precision="complex64", fusion="learned", max_fused_qubits=5,
+                                                     out_of_place=True)["learned_choices"]}
+            for name, fusion, k in configs:
+                sim = _core.StatevectorSimulator(seed=1, precision="complex64", fusion=fusion, max_fused_qubits=k,
+                                                 max_memory_mb=mem)
+                sim.evolve_timed(c, record=False)  # warm-up
+                walls = []
+                for _ in range(repeats):
+                    t = time.perf_counter()
+                    sim.evolve_timed(c, record=False)
+                    walls.append((time.perf_counter() - t) * 1e3)
+                err = None
+                if ref is not None:
+                    res = sim.evolve_timed(c, record=False, keep=True)[4]
+                    err, _ = res.compare(ref)
                     del res
-                finally:
-                    planner.install(None)
-                row[name] = {"ms": ms, "launches": launches, "max_err": err}
-            del rres
+                row[name] = {"ms": float(np.median(walls)), "max_err": err}
+                del sim
+            del ref
+            row["learned_vs_greedy_k5"] = row["greedy_k5"]["ms"] / row["learned_k5"]["ms"]
+            row["learned_vs_greedy_k3"] = row["greedy_k3"]["ms"] / row["learned_k5"]["ms"]
             rows.append(row)
             log(f"[planner] {row}")
-    tot = {k: sum(r[k]["ms"] for r in rows) for k in ("greedy_tiles_k3", "learned_tiles_k4")}
-    blk = {k: sum(r[k]["ms"] for r in rows if k in r) for k in ("greedy_blocks_k4", "greedy_blocks_k5",
-                                                                "learned_blocks_k5")}
-    out = {"train_samples": int(len(tr)), "test_samples": int(len(te)), "collect_s": collect_s, "train_s": train_s,
-           "final_train_loss": float(curve[-1]), "test_mse": test_mse, "test_mse_predict_mean": base_mse,
-           "eval": rows, "total_ms_tiles": tot, "total_ms_blocks": blk,
-           "learned_vs_greedy_tiles": tot["greedy_tiles_k3"] / tot["learned_tiles_k4"],
-           "learned_vs_best_greedy_blocks": min(blk["greedy_blocks_k4"], blk["greedy_blocks_k5"]) /
-           blk["learned_blocks_k5"],
-           "max_err": max(v["max_err"] for r in rows for v in r.values() if isinstance(v, dict))}
-    log(f"[planner] tiles {tot} gain {out['learned_vs_greedy_tiles']:.3f}; blocks {blk} gain "
-        f"{out['learned_vs_best_greedy_blocks']:.3f}")
+    finally:
+        planner.install(None)
+    tot = {name: sum(r[name]["ms"] for r in rows) for name, _, _ in configs}
+    out = {"weights": {k: meta.get(k) for k in ("circuits", "seed", "size_mix", "families", "depth_range",
+                                                 "train_samples", "test_samples", "collect_s", "train_s",
+                                                 "test_mse", "test_mse_predict_mean")},
+           "eval": rows, "total_ms": tot,
+           "learned_vs_greedy_k5": tot["greedy_k5"] / tot["learned_k5"],
+           "learned_vs_greedy_k3": tot["greedy_k3"] / tot["learned_k5"],
+           "max_err": max((v["max_err"] or 0.0) for r in rows for v in r.values()
+                          if isinstance(v, dict) and "max_err" in v)}
+    log(f"[planner] totals {tot}: learned vs greedy k<=5 {out['learned_vs_greedy_k5']:.3f}, "
+        f"vs greedy k<=3 {out['learned_vs_greedy_k3']:.3f}")
     return out
 
 

@@ -251,11 +251,21 @@ int dq_set_sweep_relabel(int32_t on);
  * second state, its tile sweeps may run out of place: each sweep reads its
  * tiles from one state buffer as contiguous runs and writes them to the other
  * in the bit layout the next sweep reads contiguously; the last sweep of a
- * run restores the qubits' own positions. 0 = always in place, 1 = out of
- * place with persistent workgroups (default), 2 = out of place with one
- * workgroup per tile. Results stay within complex64 rounding of each other
- * (blocks may group differently). Returns the previous mode. */
+ * run restores the qubits' own positions. 0 = always in place; out of place
+ * with 1 = persistent 256-thread workgroups, 2 = one 256-thread workgroup per
+ * tile (default), 3 = one 512-thread workgroup per tile (tiles of 11-12
+ * qubits; 256 threads otherwise), 4 = persistent 512-thread workgroups.
+ * Results stay within complex64 rounding of each other (blocks may group
+ * differently). Returns the previous mode; -1 for a mode outside 0..4. */
 int dq_set_sweep_mode(int32_t mode);
+
+/* Process-wide tile rule of out-of-place sweeps (no counterpart in the
+ * reference: a tuning and A/B switch). on != 0 keeps qubits 0..3 in every
+ * tile; 0 (default) needs them only in the first and last tile of a run of
+ * sweeps and otherwise asks consecutive tiles to share 4 qubits, which leaves
+ * more tile qubits for gates and so fewer sweeps. Results stay within
+ * complex64 rounding of each other. Returns the previous setting. */
+int dq_set_sweep_pinned(int32_t on);
 
 /* Number of visible GPUs (0 on a host without one). */
 int dq_device_count(void);
@@ -315,11 +325,13 @@ void dq_batch_free(dq_batch* b);
 /* The persistent batched kernel (one workgroup per circuit from an atomic
  * queue, launched through a hipGraph) is disabled: it stalled on its first
  * GPU run with no cause shown. The two calls below only inspect it.
- * Launch limits of its kernel instance for `block` threads (no launch):
- * info7 = {device max LDS per block, hipFuncSetAttribute rc for the planned
- * LDS size, kernel max dynamic LDS, kernel static LDS, kernel max threads,
- * planned LDS qubits, device count}. */
-int dq_batch_device_info(int32_t block, int32_t* info7);
+ * Launch limits of its kernel instance for `block` threads (256, 512 or
+ * 1024; no launch): info11 = {device max LDS per block, hipFuncSetAttribute
+ * rc for the planned LDS size, kernel max dynamic LDS, kernel static LDS,
+ * kernel max threads, planned LDS qubits, device count, kernel private
+ * bytes per lane, device LDS per CU, occupancy query rc, resident
+ * workgroups per CU at the planned LDS size}. */
+int dq_batch_device_info(int32_t block, int32_t* info11);
 /* Host-only text dump of the program the persistent kernel would take for these
  * circuits at fusion width `max_fused` and `lds_qubits` LDS-resident qubits
  * (header line: circuits, ops, matrix floats, total shots, HBM slot

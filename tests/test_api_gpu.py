@@ -241,6 +241,8 @@ def test_batched_kernel_launch_limits_are_readable():
         print(block, info, flush=True)
         assert info["device_count"] >= 1 and info["max_lds_per_block"] >= 65536
         assert 8 <= info["lds_qubits"] <= 15 and info["kernel_max_threads"] >= block
+        assert info["lds_per_cu"] >= info["max_lds_per_block"] and info["kernel_private_bytes"] >= 0
+        assert info["occupancy_rc"] == 0 and info["blocks_per_cu"] >= 0
 
 
 def test_fp32_matrix_core_microbenchmark_runs():

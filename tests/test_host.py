@@ -635,24 +635,53 @@ def _xsweep_move(state, launch, n, L):
     return out
 
 
+@pytest.fixture(params=[True, False], ids=["pinned", "free"])
+def sweep_pinned(request):
+    prev = _core.set_sweep_pinned(request.param)
+    yield request.param
+    _core.set_sweep_pinned(prev)
+
+
 @pytest.mark.parametrize("kind,n,L,k", [("su2", 16, 12, 3), ("su2", 18, 12, 4), ("su2", 17, 11, 3),
-                                        ("mixed", 17, 12, 3), ("qft", 16, 10, 2), ("su2", 20, 12, 2)])
-def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k):
+                                        ("mixed", 17, 12, 3), ("qft", 16, 10, 2), ("su2", 20, 12, 2),
+                                        ("brickwork", 22, 12, 3)])
+def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k, sweep_pinned):
     # Tag every amplitude with its index and push the tags through the
     # launches' read/write maps: each launch's tile qubits must sit where the
     # current layout holds them, the blocks must act on tile qubits, and a run
-    # of sweeps must end with every amplitude back at its own index.
+    # of sweeps must end with every amplitude back at its own index. Pinned
+    # tiles hold qubits 0..3 always; free tiles hold them at a run's first and
+    # last sweep and share at least 4 qubits with the previous tile.
     c = {"su2": lambda: workloads.random_su2(n=n, depth=10, seed=n + k),
          "mixed": lambda: workloads.mixed(n, 5, seed=n),
-         "qft": lambda: workloads.qft(n, seed=2, measure=False)[0]}[kind]()
+         "qft": lambda: workloads.qft(n, seed=2, measure=False)[0],
+         "brickwork": lambda: workloads.brickwork(n=n, layers=12, seed=n, measure=False)}[kind]()
     plan = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=L,
                            out_of_place=True)
     launches = plan["launches"]
+    runs, cur = [], []
+    for x in launches:
+        if x.get("out_of_place"):
+            cur.append(x)
+        elif cur:
+            runs.append(cur)
+            cur = []
+    if cur:
+        runs.append(cur)
+    for run in runs:
+        assert {0, 1, 2, 3} <= set(run[0]["local_qubits"]) and {0, 1, 2, 3} <= set(run[-1]["local_qubits"])
+        for a, b in zip(run, run[1:]):
+            assert len(set(a["local_qubits"]) & set(b["local_qubits"])) >= 4
+    if not sweep_pinned and kind in ("su2", "brickwork") and n >= 18:
+        # Free tiles leave qubits 0..3 out of some sweep.
+        assert any(not {0, 1, 2, 3} <= set(x["local_qubits"]) for run in runs for x in run)
     assert any(x.get("out_of_place") for x in launches)
-    # Each run of sweeps (between standalone launches) gathers its first tile only.
+    # Each run of sweeps (between standalone launches) may gather its first
+    # tile only, and does unless that tile is qubits 0..L-1.
     for i, x in enumerate(launches):
         if x.get("out_of_place"):
-            assert x["gather"] == (i == 0 or not launches[i - 1].get("out_of_place")), i
+            first = i == 0 or not launches[i - 1].get("out_of_place")
+            assert x["gather"] == (first and x["load_positions"] != list(range(L))), i
     tags = np.arange(1 << n, dtype=np.uint64)
     pos = list(range(n))  # qubit -> bit position
     for x in plan["launches"]:
@@ -662,7 +691,8 @@ def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k):
         lq = x["local_qubits"]
         assert [pos[q] for q in lq] == x["load_positions"]
         assert all(set(b["qubits"]) <= set(lq) for b in x["blocks"])
-        assert {0, 1, 2, 3} <= set(lq)
+        if sweep_pinned:
+            assert {0, 1, 2, 3} <= set(lq)
         tags = _xsweep_move(tags, x, n, L)
         free = [q for q in sorted(range(n), key=lambda q: pos[q]) if q not in lq]
         new = [None] * n
@@ -682,3 +712,84 @@ def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k):
     # Same ops as the in-place plan, each once.
     ops = sorted(o for x in plan["launches"] for o in x["ops"])
     assert ops == list(range(plan["compiled_ops"]))
+
+
+@pytest.mark.parametrize("kind,n,k", [("su2", 20, 3), ("brickwork", 22, 4), ("mixed", 18, 2)])
+def test_out_of_place_sweep_features_match_their_launch(kind, n, k, sweep_pinned):
+    # The learned planner's inputs for an out-of-place sweep, recomputed from
+    # the launch's own layout and blocks.
+    c = {"su2": lambda: workloads.random_su2(n=n, depth=12, seed=n),
+         "brickwork": lambda: workloads.brickwork(n=n, layers=10, seed=n, measure=False),
+         "mixed": lambda: workloads.mixed(n, 5, seed=n)}[kind]()
+    plan = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=k, tile_qubits=12,
+                           out_of_place=True)
+    assert plan["learned_choices"] == []
+    seen = 0
+    for x in plan["launches"]:
+        if not x.get("out_of_place"):
+            continue
+        seen += 1
+        f = np.array(x["features"])
+        pos = {q: j for j, q in enumerate(x["local_qubits"])}
+        sizes = [len(b["qubits"]) for b in x["blocks"]]
+        low = [min(pos[q] for q in b["qubits"]) for b in x["blocks"]]
+        sp = x["store_positions"]
+        run = 0
+        while run in sp:
+            run += 1
+        want = np.zeros(16)
+        want[0] = 1
+        want[1] = sum(1 for s, lo in zip(sizes, low) if s == 3 and lo < 4) / 16
+        want[2] = sum(1 for s, lo in zip(sizes, low) if s <= 2 and lo < 6) / 16
+        want[3] = sum(1 for s in sizes if s <= 2) / 16
+        want[4] = x["gates"] / 32
+        want[5] = 1.0 if x["gather"] else 0.0
+        want[6] = sum(1 for j, p in enumerate(sp) if p < 4 and j >= 4) / 4
+        want[7] = run / 12
+        want[8] = n / 34
+        want[11] = f[11]  # blob bytes: checked for range below
+        want[12] = 1
+        want[13] = 1
+        want[14] = sum(1 for s in sizes if s == 3) / 16
+        want[15] = sum(1 for s in sizes if s == 4) / 16
+        assert np.allclose(f, want, atol=1e-6), (f, want)
+        assert 0 < f[11] <= 1
+    assert seen > 0
+
+
+def test_learned_choice_selects_one_of_the_greedy_plans():
+    # With weights installed, the out-of-place plan of fusion="learned" is
+    # the greedy plan of the (tile rule, block width) it reports choosing.
+    from dqsim import planner
+    rng = np.random.default_rng(4)
+    prev = _core.set_sweep_pinned(False)
+    try:
+        for trial in range(4):
+            w = planner.load_default() if trial == 0 else None
+            if w is None:
+                w = rng.normal(0, 0.3, planner.N_PARAMS).astype(np.float32)
+            planner.install(w)
+            c = workloads.random_su2(n=22, depth=16, seed=trial)
+            lp = _core.plan_json(c, precision="complex64", fusion="learned", max_fused_qubits=3,
+                                 out_of_place=True)
+            assert len(lp["learned_choices"]) == 1
+            ch = lp["learned_choices"][0]
+            assert ch["k"] in (2, 3)
+            planner.install(None)
+            _core.set_sweep_pinned(ch["pinned"])
+            gp = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=ch["k"],
+                                 out_of_place=True)
+            _core.set_sweep_pinned(False)
+            assert [x["blocks"] for x in lp["launches"]] == [x["blocks"] for x in gp["launches"]]
+            assert [x["store_positions"] for x in lp["launches"]] == [x["store_positions"] for x in gp["launches"]]
+    finally:
+        planner.install(None)
+        _core.set_sweep_pinned(prev)
+
+
+def test_shipped_planner_weights_load():
+    from dqsim import planner
+    w = planner.load_default()
+    assert w is not None, "planner_weights.npy is missing from the package"
+    assert w.dtype == np.float32 and w.shape == (planner.N_PARAMS,) and np.isfinite(w).all()
+    assert planner.load_default("/nonexistent/planner_weights.npy") is None
