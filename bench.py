@@ -429,9 +429,13 @@ def main():
     rule, kk = ("pinned" if pinned else "free"), args.max_fused
     if choices:
         rule, kk = ("pinned" if choices[0]["pinned"] else "free"), choices[0]["k"]
+    if args.tile == 0:
+        rule = "blocks"
+    elif sweep_mode == 0:
+        rule = "inplace"
     workload_key = (f"random_su2_n{n}_d{depth}_k{kk}_t{args.tile}_{rule}"
                     if args.precision == "complex64" and args.fusion in ("greedy", "learned")
-                    else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_k{args.max_fused}_t{args.tile}")
+                    else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_t{args.tile}")
     if mfma is not None:
         mfma["busy_frac_profiled"] = profiled_mfma_busy(KERNEL_NAMES.get(dname, dname), workload_key)
         extra["mfma"] = mfma
