@@ -31,6 +31,8 @@ def build(kind, n):
         return workloads.random_su2(n=n, depth=12, seed=n)
     if kind == "qft":
         return workloads.qft(n, seed=3, measure=False)[0]
+    if kind == "brickwork":
+        return workloads.brickwork(n=n, layers=12, seed=n, measure=False)
     return mixed_circuit(n, 6, seed=n)
 
 
@@ -122,3 +124,30 @@ def test_sweep_modes_match_oracle(mode, kind, n, L, k):
             assert np.abs(got - want).max() < 1e-5, budget
     finally:
         dqsim._core.set_sweep_mode(old)
+
+
+@pytest.mark.parametrize("pinned", [False, True], ids=["free", "pinned"])
+@pytest.mark.parametrize("kind,n,L,k", [("su2", 20, 12, 3), ("su2", 19, 11, 4), ("qft", 18, 12, 2),
+                                        ("mixed", 18, 12, 3), ("brickwork", 20, 12, 3), ("brickwork", 21, 12, 4)],
+                         ids=lambda v: str(v))
+def test_tile_rules_match_oracle(pinned, kind, n, L, k):
+    # Out-of-place sweeps under both tile rules (qubits 0..3 in every tile,
+    # or only in a run's first and last tile); with room for one state only,
+    # the same call sweeps in place.
+    print(f"[tile-rule] pinned={pinned} {kind} n={n} L={L} k={k}", flush=True)
+    circ = build(kind, n)
+    want = oracle.simulate(circ.data, 1)[0]
+    prev = dqsim._core.set_sweep_pinned(pinned)
+    try:
+        plan = dqsim._core.plan_json(circ, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                     tile_qubits=L, out_of_place=True)
+        xs = [x for x in plan["launches"] if x.get("out_of_place")]
+        assert xs and all(len(x["local_qubits"]) == L for x in xs)
+        state_mb = (1 << n) * 8 >> 20
+        for budget in (max(1, state_mb) + 1, 4 * state_mb + 64):
+            sim = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy", max_fused_qubits=k,
+                                             tile_qubits=L, max_memory_mb=budget)
+            got = sim.simulate(circ).statevector
+            assert np.abs(got - want).max() < 1e-5, budget
+    finally:
+        dqsim._core.set_sweep_pinned(prev)
