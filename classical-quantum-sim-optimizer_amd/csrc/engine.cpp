@@ -3189,8 +3189,16 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
         }
         const uint32_t c = set[j];
         const double t0 = now_s();
+        // Sweep out of place exactly when a simulate_shots call under the
+        // default max_memory_mb would (its budget then holds the state, the
+        // classical bits, the sampling table and a second state), so the
+        // circuit's counts equal that call's.
+        const Parsed& pc = su.pb->p[c];
+        const uint64_t sb = state_layout(pc.n, su.prec).bytes, budget = Options{}.budget;
+        const uint64_t used = sb + pc.nc + cdf_bytes(su.qubits[c].size());
+        const bool permute = used <= budget && sb <= budget - used;
         auto plan = std::make_unique<Plan>(
-            build_plan(su.pb->p[c].ops, 0, su.pb->pe[c], su.prec, FUSE_GREEDY, fused, (int)su.pb->p[c].n, su.tile));
+            build_plan(pc.ops, 0, su.pb->pe[c], su.prec, FUSE_GREEDY, fused, (int)pc.n, su.tile, permute));
         {
           std::lock_guard<std::mutex> lk(mu);
           plans[j] = std::move(plan);
@@ -3233,7 +3241,7 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
           const uint64_t shots = (*su.shots)[c], seed = (*su.seeds)[c];
           plan->upload(st.s);
           Exec ex(su.prec, n, p.nc, 1, st.s);
-          ex.cbase = plan->base();
+          ex.prepare(*plan);  // second state buffer (from the worker's arena) for out-of-place sweeps
           k::init_zero(su.prec, ex.sp, ex.stride, 1, n, st.s);
           k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
           for (const DevOp& d : plan->ops) ex.run(d, 1);

@@ -135,8 +135,11 @@ int dq_sv_simulate(dq_sv* sim, const char* json, size_t len, dq_result** out);
  * from |0...0> into the caller's device buffer `state` of `state_len` = 2^n
  * amplitudes in the simulator's precision (complex128 or complex64,
  * interleaved re/im), with the launch plan and arithmetic of dq_sv_simulate
- * on the same options, so the state equals that call's statevector bit for
- * bit. Work runs on `stream` (a hipStream_t; NULL = the null stream) and the
+ * on the same options when that call keeps one state buffer (complex64 tile
+ * sweeps in place: sweep mode 0, or a max_memory_mb that holds no second
+ * state), so the state equals that call's statevector bit for bit; against
+ * dq_sv_simulate's out-of-place sweeps it differs only by FP32 rounding.
+ * Work runs on `stream` (a hipStream_t; NULL = the null stream) and the
  * call returns when it has finished. The only other device memory used is a
  * static per-GPU scratch array of DQ_NOALLOC_SCRATCH_BYTES for the plan's
  * constants; a circuit whose constants need more is rejected with DQ_EVALUE
