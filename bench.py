@@ -6,10 +6,12 @@ every qubit plus CZ/CX on a random perfect matching per layer, 1,920 gates,
 seed 1234) from |0...0> in complex64 with fused blocks of <= 5 qubits grouped
 into LDS tile sweeps (12-qubit tiles, out of place), the block width and tile
 rule chosen by the learned planner (shipped weights; --fusion greedy fuses as
-far as the width allows). The same circuit also runs under the greedy planner
-at that width and at 3 qubits, and once on the plain complex128 path (one
-launch per compiled gate, reference arithmetic) for the speed-up and the
-accuracy check.
+far as the width allows); 3-qubit blocks run in split precision (FP32
+operands as two FP16 parts on the FP16 matrix core, FP32 accumulation;
+--block-arith 0: FP32 matrix cores). The same circuit also runs under the
+greedy planner at that width and at 3 qubits, with FP32 matrix-core blocks,
+and once on the plain complex128 path (one launch per compiled gate,
+reference arithmetic) for the speed-up and the accuracy check.
 
 Prints one JSON line. `value` = G * 2^n / seconds per step.
 """
@@ -31,6 +33,7 @@ KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag":
                 "tile": "k_tile", "tile128": "k_tile128", "xsweep": "k_xsweep"}
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_FP16_PEAK_TFLOPS = 2500.0  # dense FP16 matrix-core peak (split-precision blocks run there)
 
 
 def log(msg):
@@ -83,23 +86,29 @@ def profiled_mfma_busy(kernel, workload):
 
 
 def mfma_flops_per_step(circ, precision, fusion, max_fused, tile, out_of_place):
-    """FP32 matrix-core flops of one step from the launch plan, as (tile
-    sweeps, one-block launches): a k-qubit block pass is 8 * 2^k * 2^n (one
-    complex MAC = 8 real flops); 1-qubit blocks run padded to 2 qubits in
-    tile sweeps. Also returns the plan's learned choices."""
+    """Matrix-core flops of one step from the launch plan, as (tile sweeps,
+    of which split precision, one-block launches): a k-qubit block pass is
+    8 * 2^k * 2^n algorithmic flops (one complex MAC = 8 real flops); 1-qubit
+    blocks run padded to 2 qubits in tile sweeps. A split-precision 3-qubit
+    block issues 4x its algorithmic flops on the FP16 matrix core (two
+    16x16x32 f16 MFMAs of K = 32 where the FP32 path has K = 16 in four
+    16x16x4 f32 ones). Also returns the plan's learned choices."""
     from dqsim import _core
 
     plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile,
                            out_of_place=out_of_place)
     n = plan["num_qubits"]
-    tile_flops = fused = 0.0
+    tile_flops = split_flops = fused = 0.0
     for op in plan["launches"]:
         if op["kind"] == "tile" and not op.get("exact"):
             for b in op["blocks"]:
-                tile_flops += 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
+                f = 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
+                tile_flops += f
+                if b.get("split"):
+                    split_flops += f
         elif op["kind"] == "fused_mfma":
             fused += 8.0 * (1 << len(op["qubits"])) * float(1 << n)
-    return tile_flops, fused, plan["learned_choices"]
+    return tile_flops, split_flops, fused, plan["learned_choices"]
 
 
 def device_copy_gbs(nbytes=4 << 30, reps=5):
@@ -267,6 +276,9 @@ def main():
                     help="complex64 fused sweeps: 0 in place; out of place with 1 persistent 256-thread, "
                          "2 one-shot 256-thread, 3 one-shot 512-thread, 4 persistent 512-thread workgroups "
                          "(default: the library's)")
+    ap.add_argument("--block-arith", type=int, default=None, choices=[0, 1],
+                    help="3-qubit blocks of complex64 sweeps: 0 FP32 matrix cores, 1 split precision (FP16 parts, "
+                         "FP32 accumulation) (default: the library's)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=24)
@@ -313,6 +325,10 @@ def main():
     pinned = args.pinned
     if args.sweep_mode is not None:
         dqsim._core.set_sweep_mode(args.sweep_mode)
+    if args.block_arith is not None:
+        dqsim._core.set_block_arith(args.block_arith)
+    block_arith = dqsim._core.set_block_arith(0)
+    dqsim._core.set_block_arith(block_arith)
     sweep_mode = dqsim._core.set_sweep_mode(1)
     dqsim._core.set_sweep_mode(sweep_mode)
     copy_gbs = device_copy_gbs()
@@ -347,7 +363,7 @@ def main():
     total_bytes = sum(a["bytes"] for a in agg.values()) / args.steps
     value = gates * float(1 << n) / (ms_per_step / 1e3)
 
-    tile_flops, fused_flops, choices = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused,
+    tile_flops, split_flops, fused_flops, choices = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused,
                                                            args.tile, sweep_mode != 0)
     mflops = tile_flops + fused_flops
     mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "xsweep", "fused_mfma")) / args.steps
@@ -357,7 +373,7 @@ def main():
         mfma = {"achieved_tflops": ach, "datasheet_peak_tflops": MFMA_FP32_PEAK_TFLOPS,
                 "measured_peak_tflops": mfma_peak_tflops, "frac_of_datasheet": ach / MFMA_FP32_PEAK_TFLOPS,
                 "frac_of_measured_peak": ach / mfma_peak_tflops, "flops_per_step": mflops,
-                "busy_frac_profiled": None}
+                "split_precision_flops_per_step": split_flops, "busy_frac_profiled": None}
     # The same workload under the other fusion planners (greedy and learned
     # at this block width, and greedy at 3 qubits): all step times go into
     # the line.
@@ -377,12 +393,29 @@ def main():
                 alt.evolve_timed(circ, repeats=1, record=False)
             per[key] = (time.perf_counter() - ta) * 1e3 / args.steps
             del alt
+        if block_arith == 1:
+            # The same plan with 3-qubit blocks on the FP32 matrix cores.
+            dqsim._core.set_block_arith(0)
+            alt = dqsim.StatevectorSimulator(seed=1, precision=args.precision, fusion=args.fusion,
+                                             max_fused_qubits=args.max_fused, max_memory_mb=mem_mb,
+                                             tile_qubits=args.tile)
+            for _ in range(max(1, args.warmup)):
+                alt.evolve_timed(circ, repeats=1, record=False)
+            ta = time.perf_counter()
+            for _ in range(args.steps):
+                alt.evolve_timed(circ, repeats=1, record=False)
+            per[f"{args.fusion}_k{args.max_fused}_fp32_blocks"] = (time.perf_counter() - ta) * 1e3 / args.steps
+            del alt
+            dqsim._core.set_block_arith(1)
         lk, gk = f"learned_k{args.max_fused}", f"greedy_k{args.max_fused}"
         choice = dqsim._core.plan_json(circ, precision=args.precision, fusion="learned", max_fused_qubits=args.max_fused,
                                        tile_qubits=args.tile, out_of_place=sweep_mode != 0)["learned_choices"]
         planner_cmp = {"ms_per_step": per, "learned_speedup_vs_greedy_same_width": per[gk] / per[lk],
                        "learned_speedup_vs_greedy_k3": per["greedy_k3"] / per[lk], "learned_choices": choice,
                        "weights": "shipped planner_weights.npy (engine-recorded timings, trained on the GPU)"}
+        fk = f"{args.fusion}_k{args.max_fused}_fp32_blocks"
+        if fk in per:
+            planner_cmp["split_blocks_speedup_vs_fp32_blocks"] = per[fk] / per[f"{args.fusion}_k{args.max_fused}"]
         log(f"[bench] planner: {per}, learned choice {choice}")
     extra = {"launches_per_step": launches, "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                             for kk, vv in v.items()} for k, v in agg.items()},
@@ -433,7 +466,7 @@ def main():
         rule = "blocks"
     elif sweep_mode == 0:
         rule = "inplace"
-    workload_key = (f"random_su2_n{n}_d{depth}_k{kk}_t{args.tile}_{rule}"
+    workload_key = (f"random_su2_n{n}_d{depth}_k{kk}_t{args.tile}_{rule}" + ("_split" if split_flops > 0 else "")
                     if args.precision == "complex64" and args.fusion in ("greedy", "learned")
                     else f"random_su2_n{n}_d{depth}_{args.precision}_{args.fusion}_t{args.tile}")
     if mfma is not None:
@@ -454,9 +487,15 @@ def main():
                 "frac": hbm["frac"], "traffic": profiled_traffic(kname, workload_key),
                 "algorithmic_bytes_per_launch": bytes_pl, "launch_ms": launch_ms, "hbm": hbm}
     if dname in ("xsweep", "tile") and tile_flops > 0:
+        # Algorithmic flops per launch; the floor prices FP32 blocks at the
+        # FP32 matrix-core peak and split-precision blocks' issued FP16 flops
+        # (4x algorithmic) at the FP16 peak.
         flops_pl = tile_flops * args.steps / d["launches"]
+        split_pl = split_flops * args.steps / d["launches"]
+        floor_ms = ((flops_pl - split_pl) / (MFMA_FP32_PEAK_TFLOPS * 1e12)
+                    + 4.0 * split_pl / (MFMA_FP16_PEAK_TFLOPS * 1e12)) * 1e3
         mf = {"achieved": flops_pl / (launch_ms / 1e3) / 1e12, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-              "floor_ms": flops_pl / (MFMA_FP32_PEAK_TFLOPS * 1e12) * 1e3, "flops_per_launch": flops_pl,
+              "floor_ms": floor_ms, "flops_per_launch": flops_pl, "split_precision_flops_per_launch": split_pl,
               "intensity_flop_per_byte": flops_pl / bytes_pl}
         mf["frac"] = mf["achieved"] / MFMA_FP32_PEAK_TFLOPS
         roofline["mfma"] = mf
@@ -483,7 +522,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
-                   "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode,
+                   "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode, "block_arith": block_arith,
                    "tile_rule": rule, "block_width": kk, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -314,6 +314,8 @@ int dq_set_sweep_pinned(int32_t on) { return dq::set_sweep_pinned(on != 0) ? 1 :
 
 int dq_set_sweep_mode(int32_t mode) { return dq::set_sweep_mode(mode); }
 
+int dq_set_block_arith(int32_t mode) { return dq::set_block_arith(mode); }
+
 int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch) {
   return guard([&] { *tflops = dq::mfma_fp32_peak_tflops(reps, ms_per_launch); });
 }

@@ -276,6 +276,14 @@ bool set_sweep_pinned(bool on) { return g_sweep_pinned.exchange(on); }
 std::atomic<int> g_sweep_mode{2};
 int set_sweep_mode(int mode) { return mode < 0 || mode > 4 ? -1 : g_sweep_mode.exchange(mode); }
 
+// Arithmetic of 3-qubit blocks in complex64 tile sweeps: 0 FP32 matrix cores
+// (16x16x4 f32), 1 split precision (each FP32 operand as two FP16 parts on
+// 16x16x32 f16, all four partial products accumulated in FP32; the default:
+// a quarter of the matrix-core cycles at about 1.1-1.5x the FP32 path's
+// amplitude error).
+std::atomic<int> g_block_arith{1};
+int set_block_arith(int mode) { return mode < 0 || mode > 1 ? -1 : g_block_arith.exchange(mode); }
+
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   int dev = 0, cus = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -355,7 +363,7 @@ struct DevOp {
   k::TileArgs targs{};
   size_t tile_off = 0;
   std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
-  std::vector<int> tile_mfma;
+  std::vector<int> tile_mfma;  // block kind of each tile block (0: no matrix core; 4: split precision)
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
   int chains = 0;      // exact sweeps: register chains / single-op passes
   int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
@@ -665,13 +673,58 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
 // MFMA, kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA,
 // kind 2 (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
 // (kind 0, a complex FP32 matrix, serves no tile block.)
-int tile_kind(int k) { return k <= 2 ? 3 : k == 3 ? 1 : 2; }
+int tile_kind(int k, bool split = false) { return k <= 2 ? 3 : k == 3 ? (split ? 4 : 1) : 2; }
+
+// IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520).
+uint16_t half_bits(double x) {
+  const uint16_t sign = std::signbit(x) ? 0x8000 : 0;
+  const double a = std::fabs(x);
+  if (a == 0.0) return sign;
+  int e = 0;
+  std::frexp(a, &e);                     // a in [2^(e-1), 2^e)
+  int ex = std::max(e - 1, -14);         // exponent of the leading bit; subnormals share -14
+  double r = std::nearbyint(std::ldexp(a, 10 - ex));  // significand in units of the spacing 2^(ex-10)
+  if (r >= 2048.0) {                     // rounding carried into the next binade
+    r = 1024.0;
+    ++ex;
+  }
+  if (ex > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
+  if (r < 1024.0) return (uint16_t)(sign | (uint16_t)r);  // subnormal (ex = -14)
+  return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)(r - 1024.0));
+}
+double half_value(uint16_t h) {
+  const int ex = (h >> 10) & 31, m = h & 1023;
+  const double v = ex == 0 ? std::ldexp((double)m, -24) : std::ldexp(1024.0 + m, ex - 25);
+  return (h & 0x8000) ? -v : v;
+}
 size_t tile_matrix_bytes(int k) {
   const size_t d = (size_t)1 << std::max(k, 2);
   return 4 * (2 * d) * (2 * d);
 }
-std::vector<float> tile_matrix(const std::vector<cplx>& m, int k) {
-  const int kind = tile_kind(k);
+std::vector<float> tile_matrix(const std::vector<cplx>& m, int k, bool split = false) {
+  const int kind = tile_kind(k, split);
+  if (kind == 4) {
+    // Split precision: per lane l (row l % 16, u = l / 16), dword s of
+    // [s][lane] packs two FP16 values (low half first): s = 0, 1 the high
+    // parts Eh of the real- and imaginary-part columns of input 2u + s, s =
+    // 2, 3 the low parts El = E - Eh of the same, with E the embedding of the
+    // FP32 path (row 2c+p: output c, part p) taken from the FP64 matrix.
+    std::vector<float> v(256, 0.f);
+    for (int s = 0; s < 4; ++s)
+      for (int l = 0; l < 64; ++l) {
+        const int row = l % 16, c = row >> 1, p = row & 1, cin = 2 * (l / 16) + (s & 1);
+        const cplx x = m[(size_t)c * 8 + cin];
+        const double ev[2] = {p == 0 ? x.re : x.im, p == 0 ? -x.im : x.re};  // input part 0 (real), 1 (imaginary)
+        uint32_t word = 0;
+        for (int q = 0; q < 2; ++q) {
+          const uint16_t hi = half_bits(ev[q]);
+          const uint16_t part = s < 2 ? hi : half_bits(ev[q] - half_value(hi));
+          word |= (uint32_t)part << (16 * q);
+        }
+        std::memcpy(&v[(size_t)s * 64 + l], &word, 4);
+      }
+    return v;
+  }
   if (kind == 3) {
     // E[2c+p][2c'+s]: output c part p, input c' part s (0 real, 1 imaginary).
     // Stored as A fragments: [i][h][kk] = E[4 h + i][kk].
@@ -956,9 +1009,10 @@ void restore_layout(BitLayout& lay, int L, int F, const std::function<void(const
 struct TileBlob {
   size_t off = 0;
   uint32_t blob16 = 0, nblocks = 0, kmax = 0;
+  bool split = false;  // some block runs in split precision (the tile is held scaled)
 };
 TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const uint32_t* cq, int L,
-                   const std::vector<Op>& ops, int n) {
+                   const std::vector<Op>& ops, int n, bool split) {
   TileBlob out;
   std::vector<int> local(64, -1);
   for (int b = 0; b < L; ++b) local[cq[b]] = b;
@@ -982,9 +1036,10 @@ TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const
     if (b->mat.empty()) build_block_matrix(*b, ops);
     k::TileBlock t{};
     t.k = (uint32_t)b->qubits.size();
-    t.kind = (uint32_t)tile_kind((int)t.k);
+    t.kind = (uint32_t)tile_kind((int)t.k, split);
     t.mat_off = (uint32_t)(head + mats.size() * sizeof(float));
-    std::vector<float> m = tile_matrix(b->mat, (int)t.k);
+    std::vector<float> m = tile_matrix(b->mat, (int)t.k, split);
+    out.split |= t.kind == 4;
     mats.insert(mats.end(), m.begin(), m.end());
     while (mats.size() % 4) mats.push_back(0.f);  // 16-byte alignment
     for (uint32_t i = 0; i < t.k; ++i) {
@@ -997,7 +1052,7 @@ TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const
     d.gates += (int)b->ops.size();
     d.src.insert(d.src.end(), b->ops.begin(), b->ops.end());
     d.tile_blocks.push_back(orig->qubits);
-    d.tile_mfma.push_back((int)(t.kind != 0));
+    d.tile_mfma.push_back((int)t.kind);
   }
   std::vector<uint8_t> blob(head + mats.size() * sizeof(float));
   std::memcpy(blob.data(), tb.data(), head);
@@ -1036,7 +1091,8 @@ void emit_tile(Plan& plan, std::vector<FusedBlock*>& blocks, const TileMap& tm, 
   std::sort(d.touched.begin(), d.touched.end());
   d.tile_lq.assign(tm.lq, tm.lq + L);
   tm.to_args(d.targs);
-  const TileBlob tbl = tile_blob(plan, d, blocks, tm.cq, L, ops, n);  // LDS order (after the lane swaps)
+  const TileBlob tbl = tile_blob(plan, d, blocks, tm.cq, L, ops, n, g_block_arith.load() == 1);  // LDS order
+  d.targs.split = tbl.split ? 1 : 0;
   d.targs.nblocks = tbl.nblocks;
   d.targs.kmax = tbl.kmax;
   d.targs.blob16 = tbl.blob16;
@@ -1342,7 +1398,9 @@ void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* 
     a.nblocks = (uint32_t)blocks.size();
     a.blob16 = (uint32_t)(bytes / 16);
   } else {
-    const TileBlob tbl = tile_blob(plan, d, blocks, local, L, ops, n);
+    // Split precision runs one workgroup per tile (not in persistent launches).
+    const TileBlob tbl = tile_blob(plan, d, blocks, local, L, ops, n, g_block_arith.load() == 1 && !persist);
+    a.split = tbl.split ? 1 : 0;
     a.nblocks = tbl.nblocks;
     a.kmax = tbl.kmax;
     a.blob16 = tbl.blob16;
@@ -2926,7 +2984,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       list("swaps", d.targs.sbit, d.targs.nswap);
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
-        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";
+        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false")
+           << ", \"split\": " << (d.tile_mfma[b] == 4 ? "true" : "false") << ", \"qubits\": [";
         for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
         js << "]}";
       }
@@ -2952,7 +3011,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       list("tile_store_positions", a.wtile, a.ntb);
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
-        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false") << ", \"qubits\": [";
+        js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false")
+           << ", \"split\": " << (d.tile_mfma[b] == 4 ? "true" : "false") << ", \"qubits\": [";
         for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
         js << "]}";
       }

@@ -155,6 +155,9 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
 // (0 in place, 1 out of place with persistent workgroups, 2 out of place with
 // one workgroup per tile); returns the previous mode.
 int set_sweep_mode(int mode);
+// Arithmetic of 3-qubit blocks in complex64 tile sweeps (0 FP32 matrix cores,
+// 1 split precision); returns the previous mode, -1 for an invalid one.
+int set_block_arith(int mode);
 
 int device_count();
 

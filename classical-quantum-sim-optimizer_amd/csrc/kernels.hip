@@ -430,9 +430,16 @@ __device__ __forceinline__ void tile_lane_swaps(float2 (&r)[PER], const TileArgs
   for (int i = 0; i < PER; ++i) r[i] = make_float2(x[i], y[i]);
 }
 
+// Tiles of split-precision sweeps sit in LDS scaled by 2^14: amplitudes of a
+// normalised state (|a| <= 1) then map into [2^-14, 2^14] for every |a| >=
+// 2^-28, FP16's normal range, and stay below its 65504 limit with room for
+// FP32 rounding of the norm. Powers of two scale exactly.
+constexpr float kSplitScale = 16384.f, kSplitUnscale = 1.f / 16384.f;
+__device__ __forceinline__ float2 split_scale(float2 v) { return make_float2(v.x * kSplitScale, v.y * kSplitScale); }
+
 // Store of a tile from LDS: local bit i goes to bit position opos[i] (equal
 // to spos[i] unless the sweep relabels).
-template <int L, typename V>
+template <int L, typename V, bool SPLIT = false>
 __device__ __forceinline__ void tile_store(V* __restrict__ st, const V* tile, uint64_t base, const TileArgs& a) {
   constexpr int PER = (1 << L) / TPB;
   uint64_t olo = 0;
@@ -451,7 +458,12 @@ __device__ __forceinline__ void tile_store(V* __restrict__ st, const V* tile, ui
 #pragma unroll
     for (int bit = 0; bit < L - 8; ++bit)
       if ((i >> bit) & 1) o |= om[bit];
-    st[base | olo | o] = tile[threadIdx.x + TPB * i];
+    if constexpr (SPLIT) {
+      const float2 v = tile[threadIdx.x + TPB * i];
+      st[base | olo | o] = make_float2(v.x * kSplitUnscale, v.y * kSplitUnscale);
+    } else {
+      st[base | olo | o] = tile[threadIdx.x + TPB * i];
+    }
   }
 }
 
@@ -540,6 +552,98 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
     for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], x1[j].x, acc[j], 0, 0, 0);
 #pragma unroll
     for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[j].y, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
+      tile[bs[j] ^ o1] = make_float2(acc[j][2], acc[j][3]);
+    }
+  }
+}
+
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values split into FP16 parts, a = ha + la and b = hb + lb to about
+// 2^-23 relative: h packs (ha, hb) rounded to nearest even, l packs the
+// residuals a - ha, b - hb, each formed exactly by one fused multiply-add
+// (a * one - ha with the FP16 operand widened exactly: v_fma_mix_f32) and
+// rounded to nearest even. `one` is 1.0 hidden from constant folding (see
+// split_one), which keeps the multiply-add a single mixed-precision
+// instruction; the compiler then also inserts the wait states its writes need
+// before the matrix core reads them. Inputs are tile values held scaled by
+// kSplitScale, so they stay in FP16's normal range.
+__device__ __forceinline__ void split_f16(float a, float b, float one, uint32_t& h, uint32_t& l) {
+#pragma clang fp contract(fast)
+  const halfx2 hv = __builtin_convertvector((floatx2){a, b}, halfx2);
+  const floatx2 r = {a * one - (float)hv[0], b * one - (float)hv[1]};
+  h = __builtin_bit_cast(uint32_t, hv);
+  l = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, halfx2));
+}
+// 1.0 in a scalar register, opaque to constant folding (an empty asm
+// statement: no instruction is emitted).
+__device__ __forceinline__ float split_one() {
+  float one = 1.0f;
+  asm volatile("" : "+s"(one));
+  return one;
+}
+
+// k = 3 in split precision on v_mfma_f32_16x16x32_f16 (A lane l: A[l%16][8u
+// + j], B lane l: B[8u + j][l%16], u = l/16, j = 0..7; D as for 16x16x4).
+// Lane l holds inputs 2u and 2u+1 of group l%16 whole, as for the FP32 path,
+// and feeds its 8 K slots with their FP16 parts: slots 0-1 high parts of
+// input 2u (real, imaginary), 2-3 its low parts, 4-7 the same for input
+// 2u+1. The embedding E (rows and columns as for the FP32 path) is split
+// the same way, E = Eh + El, and two MFMAs accumulate in FP32
+//   D = [Eh Eh] [xh; xl] + [El El] [xh; xl] = (Eh + El)(xh + xl),
+// all four partial products, so the result matches an FP32 contraction to
+// about 2^-22 relative per element: a quarter of the 16x16x4 path's matrix
+// cycles for six VALU operations per lane and group tile. `e` holds per lane
+// l four packed FP16 pairs: e[64 s + l], s = 0, 1: Eh of inputs 2u + s (real
+// part column, imaginary part column) for row l%16; s = 2, 3: the same of El.
+template <int L, int NTH = TPB>
+__device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock& b, const uint32_t* e) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, u = lane >> 4;
+  uint32_t f[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) f[s] = e[64 * s + lane];
+  const uint32_t fh[4] = {f[0], f[0], f[1], f[1]}, fl[4] = {f[2], f[2], f[3], f[3]};
+  const halfx8 ah = __builtin_bit_cast(halfx8, fh), al = __builtin_bit_cast(halfx8, fl);
+  const float one = split_one();
+  uint32_t srt[3];
+  uniform_bits<3>(b, srt);
+  const uint32_t lane_part = insert_zeros32(col, srt, 3);
+  const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
+  constexpr int NT = (1 << L) / 8 / 16;  // 16-group tiles of the block
+  constexpr int PERW = NT / (NTH / 64);
+  constexpr int U = PERW >= 4 ? 4 : PERW;
+  static_assert(NT % (NTH / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
+  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
+  uint32_t base = insert_zeros32(w * 16, srt, 3);
+#pragma unroll
+  for (int it = 0; it < PERW; it += U) {
+    uint32_t bs[U];
+    float2 x0[U], x1[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      bs[j] = base;
+      base = ((base | mask) + step) & ~mask;
+      x0[j] = tile[bs[j] ^ o0];
+      x1[j] = tile[bs[j] ^ o1];
+    }
+    floatx4 acc[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      uint32_t wv[4];
+      split_f16(x0[j].x, x0[j].y, one, wv[0], wv[1]);
+      split_f16(x1[j].x, x1[j].y, one, wv[2], wv[3]);
+      const halfx8 bv = __builtin_bit_cast(halfx8, wv);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bv, floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bv, acc[j], 0, 0, 0);
+    }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
@@ -681,7 +785,9 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, const TileBlock&
 
 // K4: the sweep holds 4-qubit blocks (the 32x32x2 path is compiled in only
 // then, keeping the register count of 3-qubit sweeps at four workgroups/CU).
-template <int L, bool K4>
+// SPLIT: 3-qubit blocks run in split precision (kind 4) on a tile held scaled
+// by kSplitScale in LDS.
+template <int L, bool K4, bool SPLIT>
 __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t stride, uint64_t ntiles, TileArgs a,
                                               const float4* __restrict__ blob) {
   extern __shared__ float4 lds4[];
@@ -730,7 +836,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
     for (int i = 0; i < PER; ++i) r[i] = st[b0 | lo | hi(i)];
     tile_lane_swaps(r, a);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = SPLIT ? split_scale(r[i]) : r[i];
   }
   __syncthreads();  // the first tile and the blob are in LDS
   for (; tau < ntiles; tau += gridDim.x) {
@@ -746,16 +852,17 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       const TileBlock& b = blocks[bi];
       const float* m = (const float*)(meta + b.mat_off);
       if (b.kind == 3) tile_block_mfma4<L>(tile, b, m);
+      else if (SPLIT && b.kind == 4) tile_block_mfma16h<L>(tile, b, (const uint32_t*)m);
       else if (!K4 || b.kind == 1) tile_block_mfma16<L>(tile, b, m);
       else tile_block_mfma32<L>(tile, b, m);
       __syncthreads();
     }
-    tile_store<L>(st, tile, base, a);
+    tile_store<L, float2, SPLIT>(st, tile, base, a);
     if (more) {
       tile_lane_swaps(r, a);
       __syncthreads();  // every wave has read this tile out of LDS
 #pragma unroll
-      for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = r[i];
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = SPLIT ? split_scale(r[i]) : r[i];
     }
     __syncthreads();
   }
@@ -772,13 +879,14 @@ __device__ __forceinline__ uint64_t scatter_bits(uint64_t v, const uint32_t* pos
 }
 
 // The blocks of one sweep applied to the LDS tile, a barrier after each.
-template <int L, bool K4, int NTH>
+template <int L, bool K4, int NTH, bool SPLIT>
 __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock* blocks, const unsigned char* meta,
                                                   uint32_t nblocks) {
   for (uint32_t bi = 0; bi < nblocks; ++bi) {
     const TileBlock& b = blocks[bi];
     const float* m = (const float*)(meta + b.mat_off);
     if (b.kind == 3) tile_block_mfma4<L, NTH>(tile, b, m);
+    else if (SPLIT && b.kind == 4) tile_block_mfma16h<L, NTH>(tile, b, (const uint32_t*)m);
     else if (!K4 || b.kind == 1) tile_block_mfma16<L, NTH>(tile, b, m);
     else tile_block_mfma32<L, NTH>(tile, b, m);
     __syncthreads();
@@ -789,7 +897,7 @@ __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock*
 // reads local index lo_l | hi_l(i) and writes destination offset lo_w | hi_w(i)
 // (plus the tile's base), so each wave's stores cover runs of consecutive
 // addresses as long as the low destination positions are tile bits.
-template <int L, int NTH>
+template <int L, int NTH, bool SPLIT>
 __device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const float2* tile, uint64_t wbase,
                                              const XSweepArgs& a) {
   constexpr int PER = (1 << L) / NTH;
@@ -822,7 +930,12 @@ __device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const flo
         l |= hl[bit];
         w |= hw[bit];
       }
-    dst[w] = tile[l];
+    if constexpr (SPLIT) {
+      const float2 v = tile[l];
+      dst[w] = make_float2(v.x * kSplitUnscale, v.y * kSplitUnscale);
+    } else {
+      dst[w] = tile[l];
+    }
   }
 }
 
@@ -871,7 +984,7 @@ struct XSweepRead {
 // memory, where every workgroup of the launch finds it in cache, instead of
 // being copied into LDS behind the tile; used for blobs large enough to cost
 // resident workgroups (see xsweep_blob_in_lds).
-template <int L, bool K4, bool PERSIST, int NTH, bool GBLOB>
+template <int L, bool K4, bool PERSIST, int NTH, bool GBLOB, bool SPLIT>
 __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, float2* __restrict__ dst,
                                                 uint64_t stride, uint64_t ntiles, XSweepArgs a,
                                                 const float4* __restrict__ blob) {
@@ -896,10 +1009,10 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
     __syncthreads();  // the tile and the blob are in LDS
-    tile_apply_blocks<L, K4, NTH>(tile, blocks, meta, a.nblocks);
-    xsweep_store<L, NTH>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
+    tile_apply_blocks<L, K4, NTH, SPLIT>(tile, blocks, meta, a.nblocks);
+    xsweep_store<L, NTH, SPLIT>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
     return;
   }
   // Persistent: the next tile's loads are in flight while this tile's blocks
@@ -909,7 +1022,7 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
   }
   __syncthreads();
   for (; tau < ntiles; tau += gridDim.x) {
@@ -920,12 +1033,12 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
 #pragma unroll
       for (int i = 0; i < PER; ++i) r[i] = sb[nb | rd.off(i)];
     }
-    tile_apply_blocks<L, K4, NTH>(tile, blocks, meta, a.nblocks);
-    xsweep_store<L, NTH>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
+    tile_apply_blocks<L, K4, NTH, SPLIT>(tile, blocks, meta, a.nblocks);
+    xsweep_store<L, NTH, SPLIT>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
     if (more) {
       __syncthreads();  // every wave has read this tile out of LDS
 #pragma unroll
-      for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = r[i];
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
     }
     __syncthreads();
   }
@@ -2021,8 +2134,10 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = tile_lds_bytes(a);
   dim3 g((uint32_t)tile_grid(ntiles), batch);
+#define LS(LL, K4, S) \
+  hipLaunchKernelGGL((k_tile<LL, K4, S>), g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
 #define L(LL, K4) \
-  hipLaunchKernelGGL((k_tile<LL, K4>), g, dim3(TPB), lds, st, (float2*)s, stride, ntiles, a, (const float4*)blob)
+  if (a.split) LS(LL, K4, true); else LS(LL, K4, false)
   const bool k4 = a.kmax >= 4;
   switch (a.L) {
     case 10: if (k4) L(10, true); else L(10, false); break;
@@ -2030,6 +2145,7 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
     default: if (k4) L(12, true); else L(12, false); break;
   }
 #undef L
+#undef LS
 }
 
 // The blob goes into LDS while it adds at most a quarter of the tile's bytes
@@ -2052,10 +2168,13 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = xsweep_lds_bytes(a);
   dim3 g((uint32_t)(persist ? tile_grid(ntiles) : ntiles), batch);
+  if (a.split && persist) throw std::runtime_error("xsweep: split-precision sweeps run one workgroup per tile");
   const bool gb = !xsweep_blob_in_lds(a);
-#define LG(LL, K4, P, NT, G)                                                                                         \
-  hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT, G>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \
+#define LGS(LL, K4, P, NT, G, S)                                                                                        \
+  hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT, G, S>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \
                      ntiles, a, (const float4*)blob)
+#define LG(LL, K4, P, NT, G) \
+  if (!P && a.split) LGS(LL, K4, false, NT, G, true); else LGS(LL, K4, P, NT, G, false)
 #define L(LL, K4, P, NT) \
   if (gb) LG(LL, K4, P, NT, true); else LG(LL, K4, P, NT, false)
 #define LP(LL, K4, NT) \
@@ -2072,6 +2191,7 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
 #undef LP
 #undef L
 #undef LG
+#undef LGS
 }
 
 void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs& a, const void* blob,

@@ -82,7 +82,8 @@ bool fused_block_mfma(void* s, uint64_t stride, uint32_t batch, int n, int k, co
 struct TileBlock {
   uint32_t k;        // 1..5 qubits
   uint32_t kind;     // 3: 8x8 real embedding on 4x4x1 (k <= 2); 1: permuted 16x16
-                     // embedding (k = 3); 2: 2^(k+1) real embedding (k = 4, 5)
+                     // embedding (k = 3); 2: 2^(k+1) real embedding (k = 4, 5);
+                     // 4: the k = 3 embedding split into FP16 parts for 16x16x32 f16
   uint32_t mat_off;  // byte offset of the matrix in the blob
   uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
   uint32_t srt[5];   // the same local bits sorted ascending
@@ -106,7 +107,7 @@ struct TileArgs {
   uint32_t srt[12];   // the tile's bit positions sorted ascending
   uint32_t nswap;     // lane swaps between the load and LDS (0..2)
   uint32_t sbit[2];   // lane bit of swap s (< F)
-  uint32_t pad;
+  uint32_t split;     // 1: 3-qubit blocks in split precision (kind 4), the tile held scaled by 2^14 in LDS
 };
 // Exact complex128 sweep (no fusion): the blob holds nops TileOp descriptors,
 // then each op's FP64 data (16-byte aligned). Ops are applied in program
@@ -175,7 +176,8 @@ struct XSweepArgs {
   uint32_t gather;     // 0: contiguous read; 1: read positions rpos
   uint32_t ntb;        // tile-index bits (n - L)
   uint32_t threads;    // workgroup size: 256, or 512 for L >= 11
-  uint32_t pad[3];
+  uint32_t split;      // 1: 3-qubit blocks in split precision (kind 4), the tile held scaled by 2^14 in LDS
+  uint32_t pad[2];
   uint32_t rpos[12];   // gather: source position of local bit j
   uint32_t rsrt[12];   // gather: rpos sorted ascending
   uint32_t wlocal[12]; // store index bit j -> local bit

@@ -156,6 +156,7 @@ def _load():
         "dq_mfma_fp32_peak": (ctypes.c_int, [I32, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "dq_set_sweep_relabel": (ctypes.c_int, [I32]),
         "dq_set_sweep_pinned": (ctypes.c_int, [I32]),
+        "dq_set_block_arith": (ctypes.c_int, [I32]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
@@ -601,6 +602,17 @@ def set_sweep_pinned(on: bool) -> bool:
     (False, the default: more tile qubits for gates, fewer sweeps). Returns
     the previous setting."""
     return bool(_lib.dq_set_sweep_pinned(1 if on else 0))
+
+
+def set_block_arith(mode: int) -> int:
+    """Arithmetic of 3-qubit blocks in complex64 tile sweeps for this
+    process: 0 FP32 matrix cores, 1 split precision (FP32 operands as two
+    FP16 parts on the 16x16x32 f16 matrix core, FP32 accumulation; the
+    default). Returns the previous mode."""
+    prev = int(_lib.dq_set_block_arith(int(mode)))
+    if prev < 0:
+        raise ValueError(f"block arithmetic must be 0 or 1, got {mode}")
+    return prev
 
 
 _planner_weights_set = [False]  # weights installed or cleared in this process

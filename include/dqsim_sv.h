@@ -270,6 +270,18 @@ int dq_set_sweep_mode(int32_t mode);
  * complex64 rounding of each other. Returns the previous setting. */
 int dq_set_sweep_pinned(int32_t on);
 
+/* Process-wide arithmetic of 3-qubit blocks in complex64 tile sweeps (no
+ * counterpart in the reference: a tuning and A/B switch). 0 = FP32 matrix
+ * cores (v_mfma_f32_16x16x4f32); 1 = split precision: every FP32 operand
+ * (tile amplitude, embedding entry) as a sum of two FP16 parts on
+ * v_mfma_f32_16x16x32_f16, all four partial products accumulated in FP32
+ * (about 2^-22 relative per element, a quarter of the matrix-core cycles;
+ * the default). Persistent out-of-place sweeps (sweep modes 1 and 4) stay on
+ * FP32. Results
+ * stay within complex64 rounding of each other. Returns the previous mode;
+ * -1 for a mode outside 0..1. */
+int dq_set_block_arith(int32_t mode);
+
 /* Number of visible GPUs (0 on a host without one). */
 int dq_device_count(void);
 
