@@ -363,7 +363,7 @@ struct DevOp {
   k::TileArgs targs{};
   size_t tile_off = 0;
   std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
-  std::vector<int> tile_mfma;  // block kind of each tile block (0: no matrix core; 4: split precision)
+  std::vector<int> tile_mfma;  // block kind of each tile block (0: no matrix core; 4, 5: split precision)
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
   int chains = 0;      // exact sweeps: register chains / single-op passes
   int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
@@ -673,7 +673,9 @@ uint64_t qubit_mask(const std::vector<uint32_t>& q) {
 // MFMA, kind 1 (k = 3) permuted 16x16 real embedding for the 16x16x4 MFMA,
 // kind 2 (k = 4) the real embedding [[Ur, -Ui], [Ui, Ur]] for the 32x32x2 MFMA.
 // (kind 0, a complex FP32 matrix, serves no tile block.)
-int tile_kind(int k, bool split = false) { return k <= 2 ? 3 : k == 3 ? (split ? 4 : 1) : 2; }
+int tile_kind(int k, bool split = false) {
+  return k <= 2 ? 3 : k == 3 ? (split ? 4 : 1) : (k == 4 && split) ? 5 : 2;
+}
 
 // IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520).
 uint16_t half_bits(double x) {
@@ -722,6 +724,33 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k, bool split = f
           word |= (uint32_t)part << (16 * q);
         }
         std::memcpy(&v[(size_t)s * 64 + l], &word, 4);
+      }
+    return v;
+  }
+  if (kind == 5) {
+    // Split precision, k = 4: the embedding E = [[Ur, -Ui], [Ui, Ur]] of the
+    // FP32 path (rows: real parts of outputs 0..15, then imaginary parts;
+    // columns: real parts of inputs, then imaginary parts) taken from the
+    // FP64 matrix. Per (t, lane l), row l % 32, h = l / 32, inputs a = 4t +
+    // h and b = 4t + 2 + h: dwords [4 (64 t + l) + w] pack (real-part column,
+    // imaginary-part column) of Eh for a (w = 0) and b (w = 1), of El = E -
+    // Eh for a (w = 2) and b (w = 3), low half first.
+    std::vector<float> v(1024, 0.f);
+    for (int t = 0; t < 4; ++t)
+      for (int l = 0; l < 64; ++l) {
+        const int row = l % 32, c = row % 16, p = row / 16, h = l / 32;
+        for (int w = 0; w < 4; ++w) {
+          const int cin = 4 * t + 2 * (w & 1) + h;
+          const cplx x = m[(size_t)c * 16 + cin];
+          const double ev[2] = {p == 0 ? x.re : x.im, p == 0 ? -x.im : x.re};
+          uint32_t word = 0;
+          for (int q = 0; q < 2; ++q) {
+            const uint16_t hi = half_bits(ev[q]);
+            const uint16_t part = w < 2 ? hi : half_bits(ev[q] - half_value(hi));
+            word |= (uint32_t)part << (16 * q);
+          }
+          std::memcpy(&v[(size_t)4 * (64 * t + l) + w], &word, 4);
+        }
       }
     return v;
   }
@@ -1039,7 +1068,7 @@ TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const
     t.kind = (uint32_t)tile_kind((int)t.k, split);
     t.mat_off = (uint32_t)(head + mats.size() * sizeof(float));
     std::vector<float> m = tile_matrix(b->mat, (int)t.k, split);
-    out.split |= t.kind == 4;
+    out.split |= t.kind == 4 || t.kind == 5;
     mats.insert(mats.end(), m.begin(), m.end());
     while (mats.size() % 4) mats.push_back(0.f);  // 16-byte alignment
     for (uint32_t i = 0; i < t.k; ++i) {
@@ -2985,7 +3014,7 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false")
-           << ", \"split\": " << (d.tile_mfma[b] == 4 ? "true" : "false") << ", \"qubits\": [";
+           << ", \"split\": " << (d.tile_mfma[b] >= 4 ? "true" : "false") << ", \"qubits\": [";
         for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
         js << "]}";
       }
@@ -3012,7 +3041,7 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       js << ", \"blocks\": [";
       for (size_t b = 0; b < d.tile_blocks.size(); ++b) {
         js << (b ? ", " : "") << "{\"mfma\": " << (d.tile_mfma[b] ? "true" : "false")
-           << ", \"split\": " << (d.tile_mfma[b] == 4 ? "true" : "false") << ", \"qubits\": [";
+           << ", \"split\": " << (d.tile_mfma[b] >= 4 ? "true" : "false") << ", \"qubits\": [";
         for (size_t t = 0; t < d.tile_blocks[b].size(); ++t) js << (t ? ", " : "") << d.tile_blocks[b][t];
         js << "]}";
       }

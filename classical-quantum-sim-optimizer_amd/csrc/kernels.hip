@@ -783,6 +783,81 @@ __device__ __forceinline__ void tile_block_mfma32(float2* tile, const TileBlock&
   }
 }
 
+// k = 4 in split precision on v_mfma_f32_32x32x16_f16 (A lane l: A[l%32][8h
+// + j], B lane l: B[8h + j][l%32], h = l/32, j = 0..7; D as for 32x32x2).
+// Group tiles, inputs, outputs and the embedding's rows and columns are those
+// of tile_block_mfma32: lane half h holds inputs 2i + h (i = 0..7) of group
+// l%32 whole. Instruction t (0..3) takes the lane's inputs i = 2t and 2t+1 as
+// slots 0-1 high parts (real, imaginary) of input 2(2t) + h, 2-3 its low
+// parts, 4-7 the same for input 2(2t+1) + h; per t the A fragments repeat the
+// embedding columns of those inputs over their high and low slots, once with
+// Eh and once with El, so the eight MFMAs accumulate all four partial
+// products (Eh + El)(xh + xl) in FP32. `e` holds per (t, lane) four dwords
+// e[4 (64 t + l) + w]: w = 0, 1 the packed Eh pairs (real-, imaginary-part
+// column) of the two inputs, w = 2, 3 the El pairs. The tile is held scaled by
+// kSplitScale.
+template <int L, int NTH = TPB>
+__device__ __forceinline__ void tile_block_mfma32h(float2* tile, const TileBlock& b, const uint32_t* e) {
+  constexpr int K = 4, NIN = 8, NOUT = 8;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  uint32_t srt[K];
+  uniform_bits<K>(b, srt);
+  const uint32_t lane_part = insert_zeros32(j, srt, K);
+  uint32_t offs[NIN], outs[NOUT];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) offs[i] = local_offset<K>(b, 2 * i + h) ^ lane_part;
+#pragma unroll
+  for (int r = 0; r < NOUT; ++r) outs[r] = local_offset<K>(b, 8 * (r / 4) + 4 * h + (r % 4)) ^ lane_part;
+  constexpr int NT = (1 << L) / 16 / 32;
+  constexpr int PERW = (NT + NTH / 64 - 1) / (NTH / 64);
+  constexpr int U = PERW >= 2 ? 2 : 1;
+  static_assert(PERW % U == 0, "32-group tiles per wave split evenly");
+  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]) | (1u << srt[3]);
+  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t step = insert_zeros32(32 * (NTH / 64), srt, K);
+  uint32_t base = insert_zeros32(w * 32, srt, K);
+  if (NT < NTH / 64 && (int)w >= NT) return;  // wave-uniform: no 32-group tile for this wave
+  const float one = split_one();
+  const uint4* e4 = (const uint4*)e;
+#pragma unroll
+  for (int it = 0; it < PERW; it += U) {
+    uint32_t bs[U];
+    float2 v[U][NIN];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bs[u] = base;
+      base = ((base | mask) + step) & ~mask;
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) v[u][i] = tile[bs[u] ^ offs[i]];
+    }
+    floatx16 acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[u][r] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint4 f = e4[64 * t + lane];
+      const uint32_t fh[4] = {f.x, f.x, f.y, f.y}, fl[4] = {f.z, f.z, f.w, f.w};
+      const halfx8 ah = __builtin_bit_cast(halfx8, fh), al = __builtin_bit_cast(halfx8, fl);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint32_t wv[4];
+        split_f16(v[u][2 * t].x, v[u][2 * t].y, one, wv[0], wv[1]);
+        split_f16(v[u][2 * t + 1].x, v[u][2 * t + 1].y, one, wv[2], wv[3]);
+        const halfx8 bv = __builtin_bit_cast(halfx8, wv);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bv, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bv, acc[u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < NOUT; ++r) tile[bs[u] ^ outs[r]] = make_float2(acc[u][r], acc[u][r + 8]);
+  }
+}
+
 // K4: the sweep holds 4-qubit blocks (the 32x32x2 path is compiled in only
 // then, keeping the register count of 3-qubit sweeps at four workgroups/CU).
 // SPLIT: 3-qubit blocks run in split precision (kind 4) on a tile held scaled
@@ -853,6 +928,7 @@ __global__ __launch_bounds__(TPB) void k_tile(float2* __restrict__ s, uint64_t s
       const float* m = (const float*)(meta + b.mat_off);
       if (b.kind == 3) tile_block_mfma4<L>(tile, b, m);
       else if (SPLIT && b.kind == 4) tile_block_mfma16h<L>(tile, b, (const uint32_t*)m);
+      else if (K4 && SPLIT && b.kind == 5) tile_block_mfma32h<L>(tile, b, (const uint32_t*)m);
       else if (!K4 || b.kind == 1) tile_block_mfma16<L>(tile, b, m);
       else tile_block_mfma32<L>(tile, b, m);
       __syncthreads();
@@ -887,6 +963,7 @@ __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock*
     const float* m = (const float*)(meta + b.mat_off);
     if (b.kind == 3) tile_block_mfma4<L, NTH>(tile, b, m);
     else if (SPLIT && b.kind == 4) tile_block_mfma16h<L, NTH>(tile, b, (const uint32_t*)m);
+    else if (K4 && SPLIT && b.kind == 5) tile_block_mfma32h<L, NTH>(tile, b, (const uint32_t*)m);
     else if (!K4 || b.kind == 1) tile_block_mfma16<L, NTH>(tile, b, m);
     else tile_block_mfma32<L, NTH>(tile, b, m);
     __syncthreads();

@@ -83,7 +83,8 @@ struct TileBlock {
   uint32_t k;        // 1..5 qubits
   uint32_t kind;     // 3: 8x8 real embedding on 4x4x1 (k <= 2); 1: permuted 16x16
                      // embedding (k = 3); 2: 2^(k+1) real embedding (k = 4, 5);
-                     // 4: the k = 3 embedding split into FP16 parts for 16x16x32 f16
+                     // 4: the k = 3 embedding split into FP16 parts for 16x16x32 f16;
+                     // 5: the k = 4 embedding split into FP16 parts for 32x32x16 f16
   uint32_t mat_off;  // byte offset of the matrix in the blob
   uint32_t loc[5];   // local bit of each block qubit, loc[0] = MSB of the block index
   uint32_t srt[5];   // the same local bits sorted ascending

@@ -73,3 +73,25 @@ def test_split_mode_switch_and_default():
         _core.set_block_arith(prev)
     with pytest.raises(ValueError):
         _core.set_block_arith(2)
+
+
+# 4-qubit blocks in split precision (32x32x16 f16) next to 3-qubit ones, in
+# place and out of place, at every tile size.
+CASES4 = [("su2", 18, 12, 4, 2), ("su2", 20, 12, 4, 0), ("mixed", 18, 11, 4, 2), ("su2", 17, 10, 4, 2),
+          ("qft", 18, 12, 4, 0), ("brickwork", 22, 12, 4, 2)]
+
+
+@pytest.mark.parametrize("kind,n,L,k,mode", CASES4, ids=lambda v: str(v))
+def test_split_4q_blocks_match_oracle_within_fp32_error(kind, n, L, k, mode):
+    print(f"[split4] {kind} n={n} L={L} k={k} mode={mode}", flush=True)
+    circ = build(kind, n)
+    want = oracle.simulate(circ.data, 1)[0]
+    split, plan = evolve(circ, k, L, mode, 1)
+    fp32, _ = evolve(circ, k, L, mode, 0)
+    blocks = [b for x in plan["launches"] if x["kind"] == "tile" for b in x["blocks"]]
+    assert any(len(b["qubits"]) == 4 and b["split"] for b in blocks)
+    err_split = np.abs(split - want).max()
+    err_fp32 = np.abs(fp32 - want).max()
+    print(f"[split4] max err split {err_split:.3e}, fp32 {err_fp32:.3e}", flush=True)
+    assert err_split < 1e-5
+    assert err_split <= 4 * err_fp32 + 1e-12
