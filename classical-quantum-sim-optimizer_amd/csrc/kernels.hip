@@ -530,16 +530,20 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
   uint32_t base = insert_zeros32(w * 16, srt, 3);
+  // Byte addresses: the lane's part plus a wave-uniform group part (scalar) on
+  // disjoint bits.
+  unsigned char* p0 = (unsigned char*)(tile + o0);
+  unsigned char* p1 = (unsigned char*)(tile + o1);
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
     uint32_t bs[U];
     float2 x0[U], x1[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      bs[j] = base;
+      bs[j] = __builtin_amdgcn_readfirstlane(base) * 8u;
       base = ((base | mask) + step) & ~mask;
-      x0[j] = tile[bs[j] ^ o0];
-      x1[j] = tile[bs[j] ^ o1];
+      x0[j] = *(const float2*)(p0 + bs[j]);
+      x1[j] = *(const float2*)(p1 + bs[j]);
     }
     floatx4 acc[U];
 #pragma unroll
@@ -554,8 +558,8 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
     for (int j = 0; j < U; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], x1[j].y, acc[j], 0, 0, 0);
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
-      tile[bs[j] ^ o1] = make_float2(acc[j][2], acc[j][3]);
+      *(float2*)(p0 + bs[j]) = make_float2(acc[j][0], acc[j][1]);
+      *(float2*)(p1 + bs[j]) = make_float2(acc[j][2], acc[j][3]);
     }
   }
 }
@@ -567,18 +571,20 @@ typedef float floatx2 __attribute__((ext_vector_type(2)));
 // Two FP32 values split into FP16 parts, a = ha + la and b = hb + lb to about
 // 2^-23 relative: h packs (ha, hb) rounded to nearest even, l packs the
 // residuals a - ha, b - hb, each formed exactly by one fused multiply-add
-// (a * one - ha with the FP16 operand widened exactly: v_fma_mix_f32) and
-// rounded to nearest even. `one` is 1.0 hidden from constant folding (see
-// split_one), which keeps the multiply-add a single mixed-precision
-// instruction; the compiler then also inserts the wait states its writes need
-// before the matrix core reads them. Inputs are tile values held scaled by
-// kSplitScale, so they stay in FP16's normal range.
+// (a * one - ha with the FP16 operand widened exactly) and rounded to
+// nearest even into the low or high half of l: three instructions per pair
+// (v_cvt_pk_f16_f32, v_fma_mixlo_f16, v_fma_mixhi_f16). `one` is 1.0 hidden
+// from constant folding (see split_one), which keeps each residual a single
+// mixed-precision multiply-add; written in C, the compiler also inserts the
+// wait states its writes need before the matrix core reads them. Inputs are
+// tile values held scaled by kSplitScale, so they stay in FP16's normal range.
 __device__ __forceinline__ void split_f16(float a, float b, float one, uint32_t& h, uint32_t& l) {
 #pragma clang fp contract(fast)
   const halfx2 hv = __builtin_convertvector((floatx2){a, b}, halfx2);
-  const floatx2 r = {a * one - (float)hv[0], b * one - (float)hv[1]};
+  const _Float16 la = (_Float16)(a * one - (float)hv[0]);
+  const _Float16 lb = (_Float16)(b * one - (float)hv[1]);
   h = __builtin_bit_cast(uint32_t, hv);
-  l = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, halfx2));
+  l = __builtin_bit_cast(uint32_t, (halfx2){la, lb});
 }
 // 1.0 in a scalar register, opaque to constant folding (an empty asm
 // statement: no instruction is emitted).
@@ -623,16 +629,21 @@ __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
   uint32_t base = insert_zeros32(w * 16, srt, 3);
+  // Byte addresses: the lane's part (o0, o1 on the block's bits and the lane's
+  // group bits) plus a wave-uniform group part on the other bits (scalar);
+  // the parts share no bit, so adding them equals OR-ing the indices.
+  unsigned char* p0 = (unsigned char*)(tile + o0);
+  unsigned char* p1 = (unsigned char*)(tile + o1);
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
     uint32_t bs[U];
     float2 x0[U], x1[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      bs[j] = base;
+      bs[j] = __builtin_amdgcn_readfirstlane(base) * 8u;
       base = ((base | mask) + step) & ~mask;
-      x0[j] = tile[bs[j] ^ o0];
-      x1[j] = tile[bs[j] ^ o1];
+      x0[j] = *(const float2*)(p0 + bs[j]);
+      x1[j] = *(const float2*)(p1 + bs[j]);
     }
     floatx4 acc[U];
 #pragma unroll
@@ -646,8 +657,8 @@ __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      tile[bs[j] ^ o0] = make_float2(acc[j][0], acc[j][1]);
-      tile[bs[j] ^ o1] = make_float2(acc[j][2], acc[j][3]);
+      *(float2*)(p0 + bs[j]) = make_float2(acc[j][0], acc[j][1]);
+      *(float2*)(p1 + bs[j]) = make_float2(acc[j][2], acc[j][3]);
     }
   }
 }
@@ -689,16 +700,21 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
   const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
   const uint32_t step = insert_zeros32(64 * (NTH / 64), srt, 2);
   uint32_t base = insert_zeros32(w * 64, srt, 2);
+  // Byte addresses: the lane's part plus a wave-uniform group part (scalar) on
+  // disjoint bits.
+  unsigned char* pj[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) pj[j] = (unsigned char*)(tile + off[j]);
 #pragma unroll
   for (int it = 0; it < PERW; it += U) {
     uint32_t bs[U];
     float2 x[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      bs[u] = base;
+      bs[u] = __builtin_amdgcn_readfirstlane(base) * 8u;
       base = ((base | mask) + step) & ~mask;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) x[u][j] = tile[bs[u] ^ off[j]];
+      for (int j = 0; j < 4; ++j) x[u][j] = *(const float2*)(pj[j] + bs[u]);
     }
     floatx4 lo[U], hi[U];
 #pragma unroll
@@ -717,10 +733,10 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      tile[bs[u] ^ off[0]] = make_float2(lo[u][0], lo[u][1]);
-      tile[bs[u] ^ off[1]] = make_float2(lo[u][2], lo[u][3]);
-      tile[bs[u] ^ off[2]] = make_float2(hi[u][0], hi[u][1]);
-      tile[bs[u] ^ off[3]] = make_float2(hi[u][2], hi[u][3]);
+      *(float2*)(pj[0] + bs[u]) = make_float2(lo[u][0], lo[u][1]);
+      *(float2*)(pj[1] + bs[u]) = make_float2(lo[u][2], lo[u][3]);
+      *(float2*)(pj[2] + bs[u]) = make_float2(hi[u][0], hi[u][1]);
+      *(float2*)(pj[3] + bs[u]) = make_float2(hi[u][2], hi[u][3]);
     }
   }
 }
