@@ -362,6 +362,7 @@ int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, c
       opt.prec = o->precision == DQ_PREC_C128 ? 16 : 8;
       opt.dump_states = o->dump_states != 0;
       opt.persistent = o->persistent != 0;
+      opt.lds_qubits = o->lds_qubits;
     }
     auto* b = new dq_batch;
     try {
@@ -434,9 +435,17 @@ uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i) {
 
 void dq_batch_free(dq_batch* b) { delete b; }
 
+int dq_batch_lds_info(const dq_batch* b, uint64_t* count, double* ms) {
+  return guard([&] {
+    if (!b || !count || !ms) throw std::invalid_argument("null argument");
+    *count = b->b->lds_count;
+    *ms = b->b->lds_ms;
+  });
+}
+
 int dq_batch_device_info(int32_t block, int32_t* info11) {
   return guard([&] {
-    if (block != 256 && block != 512 && block != 1024) throw std::invalid_argument("block must be 256, 512 or 1024");
+    if (block != 256) throw std::invalid_argument("block must be 256");
     static_assert(sizeof(int32_t) == sizeof(int), "int32_t is int");
     dq::batch_device_info(block, (int*)info11);
     info11[6] = dq::device_count();

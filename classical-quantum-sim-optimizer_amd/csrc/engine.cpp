@@ -3123,17 +3123,16 @@ int device_attr(hipDeviceAttribute_t a) {
 }  // namespace
 
 void batch_device_info(int block, int* info) {
+  if (block != 256) throw ValueErr("the single-launch batched kernel runs 256-thread workgroups");
   const int max_lds = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock);
-  int l = 0;
-  while (l < 20 && (int)bk::lds_bytes(l + 1, block) + 64 <= max_lds) ++l;
   info[0] = max_lds;
-  info[5] = l;
-  info[1] = bk::set_lds_limit(block, bk::lds_bytes(l, block));
+  info[5] = bk::kBatchLdsMax;
+  info[1] = bk::lds_bytes(bk::kBatchLdsMax) <= 65536 ? 0 : -1;  // no attribute call needed within 64 KiB
   info[2] = info[3] = info[4] = info[7] = -1;
-  bk::func_info(block, &info[2], &info[3], &info[4], &info[7]);
+  bk::func_info(&info[2], &info[3], &info[4], &info[7]);
   info[8] = device_attr(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor);
   info[10] = -1;
-  info[9] = bk::occupancy(block, bk::lds_bytes(l, block), &info[10]);
+  info[9] = bk::occupancy(bk::lds_bytes(bk::kBatchLdsMax), &info[10]);
 }
 
 namespace {
@@ -3373,6 +3372,130 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
   if (err) std::rethrow_exception(err);
   return (now_s() - t0) * 1e3;
 }
+// The single launch: every circuit of `set` on one workgroup of
+// k_batch_lds, captured in a hipGraph and launched on a stream of its own, so
+// it runs while the stream pool works through the other circuits.
+struct LdsBatch {
+  Stream st;
+  DeviceMem circ, list, ops, mats, outd, times, dump, dump_off;
+  hipGraphExec_t exec = nullptr;
+  Event ev0, ev1;
+  std::vector<uint32_t> set;
+  BatchProgram prog;
+  std::vector<uint64_t> doff;
+
+  // Returns the circuits of `s` it leaves to the pool: those with a fused
+  // block wider than 4 qubits (a single compiled op can be, e.g. c4x).
+  std::vector<uint32_t> start(const ParsedBatch& pb, const std::vector<uint64_t>& shots,
+                              const std::vector<uint64_t>& seeds, const std::vector<uint32_t>& s, int fused,
+                              BatchOutcome* out, int device) {
+    HIPCHK(hipSetDevice(device));
+    std::vector<uint32_t> rest;
+    prog = build_batch(pb, shots, seeds, fused, s);
+    for (size_t j = 0; j < s.size(); ++j) {
+      bool ok = true;
+      for (uint32_t o = 0; o < prog.circ[j].nops; ++o) ok = ok && prog.ops[prog.circ[j].op_off + o].k <= 4;
+      (ok ? set : rest).push_back(s[j]);
+    }
+    if (!rest.empty()) prog = build_batch(pb, shots, seeds, fused, set);
+    if (set.empty()) return rest;
+    int lds_n = 1;
+    for (size_t j = 0; j < set.size(); ++j) {
+      bk::BatchCircuit& c = prog.circ[j];
+      if ((int)c.n > bk::kBatchLdsMax) throw RuntimeErr("internal: circuit too large for the single launch");
+      lds_n = std::max(lds_n, (int)c.n);
+      c.out_off = out->offsets[set[j]];  // outcomes land in the batch-wide order
+    }
+    const bool keep = !out->state_off.empty();
+    if (keep) {
+      uint64_t acc = 0;
+      for (size_t j = 0; j < set.size(); ++j) {
+        doff.push_back(acc);
+        acc += 1ull << prog.circ[j].n;
+      }
+      doff.push_back(acc);
+    }
+    auto up = [&](DeviceMem& m, const void* p, size_t bytes) {
+      m.alloc(std::max<size_t>(bytes, 16));
+      if (bytes) HIPCHK(hipMemcpyAsync(m.get(), p, bytes, hipMemcpyHostToDevice, st.s));
+    };
+    up(circ, prog.circ.data(), prog.circ.size() * sizeof(bk::BatchCircuit));
+    up(list, prog.order.data(), prog.order.size() * sizeof(uint32_t));  // heaviest circuits first
+    up(ops, prog.ops.data(), prog.ops.size() * sizeof(bk::BatchOp));
+    up(mats, prog.mats.data(), prog.mats.size() * sizeof(float));
+    outd.alloc(4 * std::max<uint64_t>(1, out->idx.size()));
+    times.alloc(16 * set.size());
+    if (keep) {
+      up(dump_off, doff.data(), doff.size() * sizeof(uint64_t));
+      dump.alloc(8 * std::max<uint64_t>(1, doff.back()));
+    }
+    bk::LdsLaunch a{};
+    a.circ = circ.as<bk::BatchCircuit>();
+    a.list = list.as<uint32_t>();
+    a.count = (uint32_t)set.size();
+    a.lds_n = lds_n;
+    a.ops = ops.as<bk::BatchOp>();
+    a.mats = mats.as<float>();
+    a.out = outd.as<uint32_t>();
+    a.times = times.as<uint64_t>();
+    a.dump = keep ? dump.as<float2>() : nullptr;
+    a.dump_off = keep ? dump_off.as<uint64_t>() : nullptr;
+    // One kernel node; the uploads above precede the graph on the stream.
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st.s, hipStreamCaptureModeThreadLocal));
+    try {
+      bk::launch_lds(a, st.s);
+    } catch (...) {
+      (void)hipStreamEndCapture(st.s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st.s, &g));
+    const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ie);
+    HIPCHK(hipEventRecord(ev0.e, st.s));
+    HIPCHK(hipGraphLaunch(exec, st.s));
+    HIPCHK(hipEventRecord(ev1.e, st.s));
+    return rest;
+  }
+
+  void finish(BatchOutcome* out) {
+    if (set.empty()) return;
+    HIPCHK(hipStreamSynchronize(st.s));
+    HIPCHK(hipGetLastError());
+    out->lds_ms = elapsed_ms(ev0, ev1);
+    std::vector<uint32_t> got(out->idx.size());
+    if (!got.empty()) HIPCHK(hipMemcpy(got.data(), outd.get(), 4 * got.size(), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> tm(2 * set.size());
+    HIPCHK(hipMemcpy(tm.data(), times.get(), 16 * set.size(), hipMemcpyDeviceToHost));
+    const double khz = (double)device_attr(hipDeviceAttributeWallClockRate);
+    for (size_t j = 0; j < set.size(); ++j) {
+      const uint32_t c = set[j];
+      const uint64_t off = out->offsets[c], ns = prog.circ[j].shots;
+      std::copy(got.begin() + off, got.begin() + off + ns, out->idx.begin() + off);
+      out->latency_us[c] = (double)(tm[2 * j + 1] - tm[2 * j]) / khz * 1e3;
+    }
+    if (!doff.empty()) {
+      std::vector<float> f(2 * doff.back());
+      HIPCHK(hipMemcpy(f.data(), dump.get(), 8 * doff.back(), hipMemcpyDeviceToHost));
+      for (size_t j = 0; j < set.size(); ++j) {
+        const uint64_t len = 1ull << prog.circ[j].n;
+        double* dst = out->states.data() + 2 * out->state_off[set[j]];
+        for (uint64_t q = 0; q < 2 * len; ++q) dst[q] = (double)f[2 * doff[j] + q];
+      }
+    }
+    HIPCHK(hipGraphExecDestroy(exec));
+    exec = nullptr;
+  }
+
+  ~LdsBatch() {
+    if (exec) {
+      (void)hipStreamSynchronize(st.s);
+      (void)hipGraphExecDestroy(exec);
+    }
+  }
+};
 }  // namespace
 
 std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,
@@ -3382,7 +3505,9 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
   if (shots.size() != nc || seeds.size() != nc) throw ValueErr("shots and seeds need one entry per circuit");
   if (opt.persistent)
     throw RuntimeErr("the persistent batched kernel is disabled in this build (it stalled on its first GPU run; "
-                     "the cause is not identified yet)");
+                     "the cause is not identified yet); the single-launch kernel (lds_qubits) replaces it");
+  if (opt.lds_qubits < 0 || opt.lds_qubits > bk::kBatchLdsMax)
+    throw ValueErr("lds_qubits must be 0 (off) or 1.." + std::to_string(bk::kBatchLdsMax));
   if (opt.workers < 0 || opt.workers > 16) throw ValueErr("workers must be 0 (tuned) or 1..16");
   if (opt.max_fused < 0 || opt.max_fused > 4) throw ValueErr("batched fusion width must be 0 (tuned) or 1..4");
   if (opt.tile != -1 && opt.tile != 0 && (opt.tile < kTileMin || opt.tile > kTileMax))
@@ -3428,16 +3553,32 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
   auto cost = [&](uint32_t i) { return std::ldexp((double)(pb.pe[i] + 8), (int)pb.p[i].n); };
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
 
+  // Circuits for the single launch: complex64, at most lds_qubits qubits,
+  // every qubit measured (the full state's index is then the outcome), at
+  // most 2^32 - 1 shots. The rest run on the stream pool.
+  std::vector<uint32_t> lds_set, pool_set;
+  for (uint32_t i : order) {
+    const Parsed& p = pb.p[i];
+    bool full = su.qubits[i].size() == p.n;
+    for (size_t q = 0; full && q < su.qubits[i].size(); ++q) full = su.qubits[i][q] == p.n - 1 - q;
+    if (opt.lds_qubits > 0 && opt.prec == 8 && (int)p.n <= opt.lds_qubits && full && shots[i] <= 0xFFFFFFFFull)
+      lds_set.push_back(i);
+    else
+      pool_set.push_back(i);
+  }
+  const size_t npool = pool_set.size();
+
   int workers = opt.workers ? opt.workers : 4;
   int fused = opt.max_fused ? opt.max_fused : 3;
-  if (opt.autotune && nc > 0 && (!opt.workers || !opt.max_fused || tune_tile)) {
-    // Calibration subset: evenly spaced circuits of the cost order, run once
-    // untimed (plans, arenas and code objects warm), then timed per candidate:
-    // (workers, fusion width) at the default tile, then the tile shape (LDS
-    // tile sweeps of 10, 11 or 12 qubits, or none) at the chosen pair.
+  if (opt.autotune && npool > 0 && (!opt.workers || !opt.max_fused || tune_tile)) {
+    // Calibration subset: evenly spaced circuits of the pool's cost order, run
+    // once untimed (plans, arenas and code objects warm), then timed per
+    // candidate: (workers, fusion width) at the default tile, then the tile
+    // shape (LDS tile sweeps of 10, 11 or 12 qubits, or none) at the chosen
+    // pair.
     std::vector<uint32_t> sub;
-    const size_t want = std::min<size_t>(nc, 96);
-    for (size_t i = 0; i < want; ++i) sub.push_back(order[(i * nc) / want]);
+    const size_t want = std::min<size_t>(npool, 96);
+    for (size_t i = 0; i < want; ++i) sub.push_back(pool_set[(i * npool) / want]);
     run_stream_pool(su, sub, 4, fused, nullptr);
     double best = 1e300;
     auto name = [](int w, int f, int t) {
@@ -3470,11 +3611,21 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
       su.tile = best_tile;
     }
   }
-  out->total_ms = run_stream_pool(su, order, workers, fused, out.get());
+  const double t0 = now_s();
+  LdsBatch lds;
+  if (!lds_set.empty()) {
+    const std::vector<uint32_t> rest =
+        lds.start(pb, shots, seeds, lds_set, std::max(1, std::min(fused, 4)), out.get(), su.device);
+    pool_set.insert(pool_set.end(), rest.begin(), rest.end());
+  }
+  if (!pool_set.empty()) run_stream_pool(su, pool_set, workers, fused, out.get());
+  lds.finish(out.get());
+  out->total_ms = (now_s() - t0) * 1e3;
   out->workers = workers;
   out->fused = fused;
   out->tile = su.tile;
-  out->streams = (uint32_t)workers;
+  out->streams = (uint32_t)workers + (lds.set.empty() ? 0u : 1u);
+  out->lds_count = lds.set.size();
   return out;
 }
 

@@ -161,7 +161,7 @@ int set_block_arith(int mode);
 
 int device_count();
 
-// ---- batched small circuits (persistent kernel, batch.hip) ----
+// ---- batched small circuits (stream pool + single-launch kernel, batch.hip) ----
 struct BatchOptions {
   int workers = 0;          // host threads, one HIP stream each; 0 = tuned (default 4)
   int max_fused = 0;        // fusion width 1..4; 0 = tuned (default 3)
@@ -169,7 +169,11 @@ struct BatchOptions {
   int tile = kTileDefault;  // LDS tile qubits of each circuit's plan (0 = off)
   int prec = 8;             // bytes per amplitude: 8 complex64, 16 complex128
   bool dump_states = false;
-  bool persistent = false;  // the persistent kernel: disabled, raises
+  bool persistent = false;  // the persistent kernel: removed, raises
+  // Single launch: complex64 circuits of up to this many qubits (<= 12) that
+  // measure every qubit run one workgroup each, all in one hipGraph launch
+  // concurrent with the pool; 0 = every circuit on the pool.
+  int lds_qubits = 12;
 };
 struct BatchOutcome {
   std::vector<uint64_t> nq, nc;             // per circuit
@@ -182,6 +186,8 @@ struct BatchOutcome {
   double total_ms = 0;                      // wall time of the run (tuning excluded)
   int workers = 0, fused = 0, tile = 0;
   uint32_t streams = 0;
+  uint64_t lds_count = 0;                   // circuits run by the single launch
+  double lds_ms = 0;                        // its device time (graph launch to completion)
   std::vector<std::pair<std::string, double>> tuning;  // candidate -> ms on the sample
 };
 std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*, size_t>>& circuits,

@@ -90,6 +90,7 @@ class _BatchOpts(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("dump_states", ctypes.c_int32),
         ("persistent", ctypes.c_int32),
+        ("lds_qubits", ctypes.c_int32),
     ]
 
 
@@ -163,6 +164,7 @@ def _load():
         "dq_set_planner_weights": (ctypes.c_int, [P, U64]),
         "dq_batch_run": (ctypes.c_int, [P, P, U64, P, P, ctypes.POINTER(_BatchOpts), ctypes.POINTER(P)]),
         "dq_batch_counts": (ctypes.c_int, [P, U64, _COUNT_CB, P]),
+        "dq_batch_lds_info": (ctypes.c_int, [P, ctypes.POINTER(U64), ctypes.POINTER(D)]),
         "dq_batch_indices": (ctypes.c_int, [P, U64, P, U64]),
         "dq_batch_info": (ctypes.c_int, [P, ctypes.POINTER(D), ctypes.POINTER(I32), ctypes.POINTER(I32),
                                          ctypes.POINTER(I32), ctypes.POINTER(ctypes.c_uint32), P, U64]),

@@ -6,9 +6,16 @@ terminal measurements; each gets its own shot count and seed. A pool of host
 threads, each with its own HIP stream and stream-ordered device arena (no
 allocation or device-wide synchronisation per circuit), takes the circuits
 largest first; each circuit follows ``simulate_shots``' terminal-sampling
-path with its own seed, so its counts equal that call's. The persistent
-single-launch kernel is kept for inspection only (``persistent=True``
-raises): it stalled on its first GPU run and no cause has been shown.
+path with its own seed, so its counts equal that call's.
+
+Small complex64 circuits (up to ``lds_qubits`` qubits, default 12, every qubit
+measured) instead run in one single-launch kernel captured in a hipGraph, one
+workgroup per circuit with its whole state in LDS, concurrently with the pool:
+their final states match the FP64 oracle within complex64 rounding and their
+outcomes are the chunked sampler's draws on those states (restated by
+``oracle.batch_sample``), not the per-circuit call's kernels. The earlier
+persistent kernel is removed (``persistent=True`` raises): it stalled on its
+first GPU run and no cause has been shown.
 """
 
 from __future__ import annotations
@@ -43,6 +50,10 @@ class BatchResult:
         self.tile_qubits = tile.value
         self.streams = streams.value
         self.latency_us = lat[:ncirc]
+        cnt, lms = ctypes.c_uint64(), ctypes.c_double()
+        _check(_lib.dq_batch_lds_info(handle, ctypes.byref(cnt), ctypes.byref(lms)))
+        self.lds_circuits = cnt.value  # circuits run by the single launch
+        self.lds_ms = lms.value        # its device time
         self.tuning = {}
         name, ms = ctypes.create_string_buffer(64), ctypes.c_double()
         for i in range(int(_lib.dq_batch_tuning_count(handle))):
@@ -75,12 +86,14 @@ class BatchResult:
 
 def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = None, base_seed: int = 0,
               workers: int = 0, max_fused_qubits: int = 0, autotune: bool = True, tile_qubits: Optional[int] = None,
-              precision: str = "complex64", keep_states: bool = False, persistent: bool = False) -> BatchResult:
+              precision: str = "complex64", keep_states: bool = False, persistent: bool = False,
+              lds_qubits: int = 12) -> BatchResult:
     """Run circuits as one batch. ``shots`` is an int or a list; ``seeds``
     defaults to base_seed + index. ``workers`` / ``max_fused_qubits`` of 0 and
     ``tile_qubits`` of None (LDS tile sweeps of 10, 11 or 12 qubits, or none)
     are chosen by timing candidates on a sample when ``autotune`` is set; an
-    untuned tile is 12."""
+    untuned tile is 12. Complex64 circuits of up to ``lds_qubits`` qubits
+    (0..12; 0 = none) that measure every qubit run in the single launch."""
     if precision not in _PREC:
         raise ValueError("precision must be 'complex64' or 'complex128'")
     texts = [_circuit_json(c) for c in circuits]
@@ -94,7 +107,7 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
     sd = (ctypes.c_uint64 * max(1, n))(*seeds_l)
     o = _BatchOpts(workers, max_fused_qubits, 1 if autotune else 0, -1 if tile_qubits is None else int(tile_qubits),
                    _PREC[precision],
-                   1 if keep_states else 0, 1 if persistent else 0)
+                   1 if keep_states else 0, 1 if persistent else 0, int(lds_qubits))
     h = ctypes.c_void_p()
     _check(_lib.dq_batch_run(ptrs, lens, n, sh, sd, ctypes.byref(o), ctypes.byref(h)))
     nq = [int(_lib.dq_batch_num_qubits(h, i)) for i in range(n)]
@@ -122,11 +135,11 @@ def program_text(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] 
 
 
 def device_info(block: int = 256) -> Dict[str, int]:
-    """Launch limits of the batched kernel for ``block`` threads (no launch):
-    device LDS per block and per CU, the planned LDS qubit count, whether the
-    kernel accepted that dynamic LDS size, the kernel's own attributes (its
-    per-lane private segment included) and the runtime's count of resident
-    workgroups per CU at that LDS size."""
+    """Launch limits of the single-launch batched kernel (256 threads; no
+    launch): device LDS per block and per CU, the largest LDS-resident
+    circuit, whether its LDS request stays within the default limit (0), the
+    kernel's own attributes (its per-lane private segment included) and the
+    runtime's count of resident workgroups per CU at that LDS size."""
     info = (ctypes.c_int32 * 11)()
     _check(_lib.dq_batch_device_info(block, info))
     keys = ("max_lds_per_block", "set_attribute_rc", "kernel_max_dynamic_lds", "kernel_static_lds",

@@ -98,13 +98,22 @@ def precision_report(n64: int = 30, n128: int = 33, shots: int = 4096, samples: 
 
 def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, log=print) -> dict:
     """The batched workload (4096 circuits of 10-22 qubits, depth 10-60, 1024
-    shots each, complex64) through the stream-pool runner with autotuned
-    workers and fusion width. `wall_ms` covers parsing, planning, evolution
-    and sampling of every circuit (the tuning sample is timed separately)."""
+    shots each, complex64): circuits of up to 12 qubits in the single-launch
+    kernel (one hipGraph launch, one workgroup per circuit), the rest on the
+    stream-pool runner with autotuned workers, fusion width and tile shape,
+    both at once. `wall_ms` covers parsing, planning, evolution and sampling
+    of every circuit (the tuning sample is timed separately). The same
+    circuits also run with every circuit on the pool (`pool_only`)."""
     t0 = time.perf_counter()
     circs = [workloads.batch_circuit(i, base_seed=base_seed) for i in range(count)]
     gen_s = time.perf_counter() - t0
     work = sum(workloads.gate_count(c.data["instructions"]) * float(1 << c.num_qubits) for c in circs)
+    pool = batch.run_batch(circs, shots=shots, base_seed=base_seed, autotune=True, lds_qubits=0)
+    pool_only = {"wall_ms": pool.wall_ms, "circuits_per_s": count / (pool.wall_ms / 1e3),
+                 "latency_us_p50": float(np.percentile(pool.latency_us, 50)),
+                 "latency_us_p99": float(np.percentile(pool.latency_us, 99)), "workers": pool.workers,
+                 "max_fused_qubits": pool.max_fused_qubits, "tile_qubits": pool.tile_qubits}
+    del pool
     t0 = time.perf_counter()
     res = batch.run_batch(circs, shots=shots, base_seed=base_seed, autotune=True)
     wall = time.perf_counter() - t0
@@ -118,8 +127,11 @@ def batch_report(count: int = 4096, shots: int = 1024, base_seed: int = 2024, lo
            "latency_us_p99": float(np.percentile(lat, 99)), "workers": res.workers,
            "max_fused_qubits": res.max_fused_qubits, "tile_qubits": res.tile_qubits, "streams": res.streams,
            "tuning_ms_on_sample": res.tuning, "host_wall_s_incl_json_and_tuning": wall, "generate_s": gen_s,
-           "path": "stream-pool runner: one host thread + HIP stream + device arena per worker, simulate_shots "
-                   "terminal-sampling path per circuit"}
+           "single_launch_circuits": res.lds_circuits, "single_launch_device_ms": res.lds_ms,
+           "pool_only": pool_only, "speedup_vs_pool_only": pool_only["wall_ms"] / res.wall_ms,
+           "path": "circuits of <= 12 qubits: single-launch kernel (one hipGraph launch, one workgroup per circuit, "
+                   "state in LDS); the rest: stream-pool runner (one host thread + HIP stream + device arena per "
+                   "worker, simulate_shots terminal-sampling path per circuit), concurrently"}
     log(f"[batch] {out}")
     return out
 

@@ -308,18 +308,26 @@ typedef struct dq_batch_opts {
                                -1 = tuned with autotune (10, 11, 12 or off), else 12 */
   int32_t precision;        /* DQ_PREC_C64 or DQ_PREC_C128 */
   int32_t dump_states;      /* keep final states for dq_batch_state_c128 */
-  int32_t persistent;       /* 1 = the persistent kernel (disabled: DQ_ERUNTIME) */
+  int32_t persistent;       /* 1 = the persistent kernel (removed: DQ_ERUNTIME) */
+  int32_t lds_qubits;       /* single launch for complex64 circuits of up to this
+                               many qubits (0..12) that measure every qubit; 0 = off */
 } dq_batch_opts;
 /* Run `ncirc` circuits (terminal measurements only) with shots[i] shots and
  * seed seeds[i] each. A pool of host threads, each with its own HIP stream
  * and stream-ordered device arena, takes circuits largest first; each circuit
  * follows dq_sv_simulate_shots' terminal-sampling path (greedy fusion of
  * width max_fused_qubits, the given tile and precision, seed seeds[i]), so
- * its counts equal those of that call. */
+ * its counts equal those of that call. Complex64 circuits of up to
+ * lds_qubits qubits that measure every qubit run instead in one hipGraph
+ * launch of the single-launch kernel (one workgroup per circuit, the state in
+ * LDS, FP32 block arithmetic, the chunked FP64 sampler with Philox draws),
+ * concurrently with the pool. */
 int dq_batch_run(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
                  const uint64_t* seeds, const dq_batch_opts* o, dq_batch** out);
 /* Counts of circuit i (same format as simulate_shots). */
 int dq_batch_counts(const dq_batch* b, uint64_t i, dq_count_cb cb, void* user);
+/* Circuits run by the single launch and its device milliseconds. */
+int dq_batch_lds_info(const dq_batch* b, uint64_t* count, double* ms);
 /* Sampled outcomes of circuit i over its measured qubits, the highest
  * measured qubit as the most significant bit (len = shots[i]). */
 int dq_batch_indices(const dq_batch* b, uint64_t i, uint32_t* out, uint64_t len);
@@ -337,21 +345,18 @@ int dq_batch_tuning_entry(const dq_batch* b, uint64_t i, char* name, uint64_t ca
 /* Qubit count of circuit i (0 when i is out of range). */
 uint32_t dq_batch_num_qubits(const dq_batch* b, uint64_t i);
 void dq_batch_free(dq_batch* b);
-/* The persistent batched kernel (one workgroup per circuit from an atomic
- * queue, launched through a hipGraph) is disabled: it stalled on its first
- * GPU run with no cause shown. The two calls below only inspect it.
- * Launch limits of its kernel instance for `block` threads (256, 512 or
- * 1024; no launch): info11 = {device max LDS per block, hipFuncSetAttribute
- * rc for the planned LDS size, kernel max dynamic LDS, kernel static LDS,
- * kernel max threads, planned LDS qubits, device count, kernel private
- * bytes per lane, device LDS per CU, occupancy query rc, resident
- * workgroups per CU at the planned LDS size}. */
+/* Launch limits of the single-launch batched kernel (block must be 256; no
+ * launch): info11 = {device max LDS per block, 0 when the largest LDS request
+ * stays within the default 64 KiB, kernel max dynamic LDS, kernel static LDS,
+ * kernel max threads, largest LDS-resident circuit (qubits), device count,
+ * kernel private bytes per lane, device LDS per CU, occupancy query rc,
+ * resident workgroups per CU at the largest LDS request}. */
 int dq_batch_device_info(int32_t block, int32_t* info11);
-/* Host-only text dump of the program the persistent kernel would take for these
- * circuits at fusion width `max_fused` and `lds_qubits` LDS-resident qubits
- * (header line: circuits, ops, matrix floats, total shots, HBM slot
- * amplitudes; then one line per circuit, the launch order, one line per op,
- * the FP32 matrices). Free it with dq_string_free. */
+/* Host-only text dump of the batched program for these circuits at fusion
+ * width `max_fused` (header line: circuits, ops, matrix floats, total shots,
+ * the largest state above `lds_qubits` qubits in amplitudes; then one line
+ * per circuit, the launch order, one line per op, the FP32 matrices). Free it
+ * with dq_string_free. */
 int dq_batch_program_text(const char* const* jsons, const size_t* lens, uint64_t ncirc, const uint64_t* shots,
                           const uint64_t* seeds, int32_t max_fused, int32_t lds_qubits, char** out);
 

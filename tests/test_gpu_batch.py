@@ -4,10 +4,16 @@ The stream-pool runner (one host thread, HIP stream and device arena per
 worker) runs every circuit through simulate_shots' terminal-sampling path with
 the circuit's own seed, so its counts must equal those of the per-circuit call
 exactly, for any worker count; final states are pinned by the FP64 oracle.
+Those tests set lds_qubits=0 so that every circuit takes the pool.
 
-The persistent single-launch kernel stays disabled and its test skipped: it
-stalled (no progress for 180 s) on its first GPU run and no cause has been
-shown (a CPU emulation of its logic on that test's program stayed in bounds and
+The single-launch kernel (one workgroup per circuit, state in LDS, one
+hipGraph launch) runs complex64 circuits of up to 12 qubits that measure every
+qubit: their states are pinned by the FP64 oracle and their outcomes, draw for
+draw, by the oracle's restatement of its chunked sampler on those states.
+
+The earlier persistent kernel is removed and its test skipped: it stalled (no
+progress for 180 s) on its first GPU run and no cause has been shown (a CPU
+emulation of its logic on that test's program stayed in bounds and
 reproduced the oracle)."""
 
 import numpy as np
@@ -36,7 +42,9 @@ def test_batch_counts_equal_per_circuit_calls(workers, k):
     # 3..12 qubits (one launch per block) and 16..18 qubits (LDS tile sweeps).
     cs = circuits(10, 3, 12) + circuits(4, 16, 18, base=5)
     print("batch", workers, k, [c.num_qubits for c in cs], flush=True)
-    res = batch.run_batch(cs, shots=300, base_seed=11, workers=workers, max_fused_qubits=k, autotune=False)
+    res = batch.run_batch(cs, shots=300, base_seed=11, workers=workers, max_fused_qubits=k, autotune=False,
+                          lds_qubits=0)
+    assert res.lds_circuits == 0
     assert res.workers == workers and res.max_fused_qubits == k and res.tile_qubits == 12
     for i, c in enumerate(cs):
         got = res.counts(i)
@@ -47,7 +55,7 @@ def test_batch_counts_equal_per_circuit_calls(workers, k):
 def test_batch_states_match_oracle_and_indices_match_counts():
     cs = circuits(8, 3, 12) + circuits(2, 16, 17, base=5)
     res = batch.run_batch(cs, shots=200, base_seed=3, workers=4, max_fused_qubits=3, autotune=False,
-                          keep_states=True)
+                          keep_states=True, lds_qubits=0)
     for i, c in enumerate(cs):
         n = c.num_qubits
         unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
@@ -79,7 +87,7 @@ def test_batch_zero_shots_and_unmeasured_circuit():
     measured = circuits(2, 3, 6, base=2)
     bare = Circuit(3, [{"kind": "h", "qubit": 0}, {"kind": "cx", "control": 0, "target": 2}], 0)
     res = batch.run_batch(measured + [bare], shots=[0, 50, 40], base_seed=1, workers=2, max_fused_qubits=3,
-                          autotune=False)
+                          autotune=False, lds_qubits=0)
     assert res.counts(0) == {}
     assert sum(res.counts(1).values()) == 50
     assert res.counts(2) == {"": 40}
@@ -87,7 +95,7 @@ def test_batch_zero_shots_and_unmeasured_circuit():
 
 def test_batch_autotune_and_latency():
     cs = circuits(40, 4, 14, dmin=4, dmax=12, base=9)
-    res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True)
+    res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True, lds_qubits=0)
     print("tuned", res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
     assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4 and res.tile_qubits in (0, 10, 11, 12)
     # 9 (workers, width) pairs at the default tile, then 3 more tile shapes.
@@ -97,6 +105,47 @@ def test_batch_autotune_and_latency():
     assert res.wall_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()
     for i, c in enumerate(cs):
         assert res.counts(i) == per_circuit_counts(c, 64, 3 + i, res.max_fused_qubits, res.tile_qubits), i
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_single_launch_states_and_samples_match_oracle(k):
+    # 3..12 qubits in the single launch, 16..17 on the pool, both in one call.
+    cs = circuits(14, 3, 12, base=31) + circuits(2, 16, 17, base=5)
+    print("[batch-lds]", k, [c.num_qubits for c in cs], flush=True)
+    res = batch.run_batch(cs, shots=300, base_seed=11, workers=2, max_fused_qubits=k, autotune=False,
+                          keep_states=True)
+    assert res.lds_circuits == 14 and res.lds_ms > 0
+    for i, c in enumerate(cs):
+        n = c.num_qubits
+        unitary = [x for x in c.data["instructions"] if x["kind"] != "measure"]
+        want, _ = oracle.simulate(Circuit(n, unitary).data, 1)
+        got = res.statevector(i)
+        assert np.abs(got - want).max() < 1e-5, (i, n)
+        if n <= 12:
+            idx = oracle.batch_sample(got, 300, 11 + i, 256)
+            assert np.array_equal(res.indices(i).astype(np.uint64), idx), i
+            assert res.latency_us[i] > 0
+        else:
+            assert res.counts(i) == per_circuit_counts(c, 300, 11 + i, k, 12), i
+
+
+def test_single_launch_edge_cases():
+    # Zero shots, one qubit, a circuit that measures only some qubits (pool),
+    # and a different shot count per circuit.
+    one = Circuit(1, [{"kind": "h", "qubit": 0}, {"kind": "measure", "qubit": 0, "cbit": 0}], 1)
+    part = Circuit(4, [{"kind": "h", "qubit": 0}, {"kind": "cx", "control": 0, "target": 3},
+                       {"kind": "measure", "qubit": 3, "cbit": 0}], 1)
+    cs = [one, part] + circuits(3, 5, 9, base=44)
+    shots = [257, 40, 0, 1, 1000]
+    res = batch.run_batch(cs, shots=shots, base_seed=2, workers=1, max_fused_qubits=3, autotune=False,
+                          keep_states=True)
+    assert res.lds_circuits == 4
+    assert sum(res.counts(0).values()) == 257 and set(res.counts(0)) <= {"0", "1"}
+    assert res.counts(1) == per_circuit_counts(part, 40, 3, 3, 12)
+    assert res.counts(2) == {}
+    for i in (0, 3, 4):
+        idx = oracle.batch_sample(res.statevector(i), shots[i], 2 + i, 256)
+        assert np.array_equal(res.indices(i).astype(np.uint64), idx), i
 
 
 @pytest.mark.skip(reason="the persistent batched kernel stalled (no progress for 180 s) on its first GPU run in "

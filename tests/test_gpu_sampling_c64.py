@@ -130,7 +130,7 @@ def test_batch_indices():
     cs += [workloads.batch_circuit(i, base_seed=5, nmin=16, nmax=17, dmin=2, dmax=8) for i in range(2)]
     print("[c64] batch indices", [c.num_qubits for c in cs], flush=True)
     res = batch.run_batch(cs, shots=300, base_seed=9, workers=3, max_fused_qubits=3, autotune=False,
-                          keep_states=True)
+                          keep_states=True, lds_qubits=0)
     for i, c in enumerate(cs):
         n = c.num_qubits
         st = res.statevector(i)
