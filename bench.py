@@ -279,6 +279,9 @@ def main():
     ap.add_argument("--block-arith", type=int, default=None, choices=[0, 1],
                     help="3-qubit blocks of complex64 sweeps: 0 FP32 matrix cores, 1 split precision (FP16 parts, "
                          "FP32 accumulation) (default: the library's)")
+    ap.add_argument("--no-zero-start", action="store_true",
+                    help="initialise the state and read it in the first sweeps (default: plans that open with "
+                         "out-of-place sweeps start from a synthesised |0...0> and skip reading all-zero tiles)")
     ap.add_argument("--no-plain", action="store_true", help="skip the complex128 plain comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-qubits", type=int, default=24)
@@ -329,6 +332,8 @@ def main():
         dqsim._core.set_block_arith(args.block_arith)
     block_arith = dqsim._core.set_block_arith(0)
     dqsim._core.set_block_arith(block_arith)
+    if args.no_zero_start:
+        dqsim._core.set_zero_start(False)
     sweep_mode = dqsim._core.set_sweep_mode(1)
     dqsim._core.set_sweep_mode(sweep_mode)
     copy_gbs = device_copy_gbs()
@@ -523,6 +528,7 @@ def main():
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
                    "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode, "block_arith": block_arith,
+                   "zero_start": not args.no_zero_start,
                    "tile_rule": rule, "block_width": kk, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

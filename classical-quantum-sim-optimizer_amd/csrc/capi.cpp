@@ -316,6 +316,15 @@ int dq_set_sweep_mode(int32_t mode) { return dq::set_sweep_mode(mode); }
 
 int dq_set_block_arith(int32_t mode) { return dq::set_block_arith(mode); }
 
+int dq_set_zero_start(int32_t on) { return dq::set_zero_start(on != 0) ? 1 : 0; }
+
+int dq_split_f16(const double* x, uint64_t n, uint16_t* hi, uint16_t* lo) {
+  return guard([&] {
+    if (n && (!x || !hi || !lo)) throw std::invalid_argument("null argument");
+    dq::split_f16_host(x, n, hi, lo);
+  });
+}
+
 int dq_mfma_fp32_peak(int32_t reps, double* tflops, double* ms_per_launch) {
   return guard([&] { *tflops = dq::mfma_fp32_peak_tflops(reps, ms_per_launch); });
 }

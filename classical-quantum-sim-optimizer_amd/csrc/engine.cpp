@@ -284,6 +284,12 @@ int set_sweep_mode(int mode) { return mode < 0 || mode > 4 ? -1 : g_sweep_mode.e
 std::atomic<int> g_block_arith{1};
 int set_block_arith(int mode) { return mode < 0 || mode > 1 ? -1 : g_block_arith.exchange(mode); }
 
+// Plans that open with out-of-place sweeps start from a synthesised |0...0>
+// and skip the reads of all-zero tiles (Exec::start_zero); off: initialise
+// the state and read it.
+std::atomic<bool> g_zero_start{true};
+bool set_zero_start(bool on) { return g_zero_start.exchange(on); }
+
 double mfma_fp32_peak_tflops(int reps, double* ms_per_launch) {
   int dev = 0, cus = 0;
   HIPCHK(hipGetDevice(&dev));
@@ -364,6 +370,7 @@ struct DevOp {
   size_t tile_off = 0;
   std::vector<std::vector<uint32_t>> tile_blocks;  // qubits of each block (plan description)
   std::vector<int> tile_mfma;  // block kind of each tile block (0: no matrix core; 4, 5: split precision)
+  std::vector<uint32_t> xtile_q;  // out-of-place sweeps: the qubit of each tile-index bit
   bool exact = false;  // complex128 sweep of unfused ops (k_tile128)
   int chains = 0;      // exact sweeps: register chains / single-op passes
   int fixed_chains = 0;  // exact sweeps: chains run by compiled-signature code
@@ -677,27 +684,46 @@ int tile_kind(int k, bool split = false) {
   return k <= 2 ? 3 : k == 3 ? (split ? 4 : 1) : (k == 4 && split) ? 5 : 2;
 }
 
-// IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520).
+// IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520),
+// from the bits of the double in integer arithmetic (exact, no libm calls).
 uint16_t half_bits(double x) {
-  const uint16_t sign = std::signbit(x) ? 0x8000 : 0;
-  const double a = std::fabs(x);
-  if (a == 0.0) return sign;
-  int e = 0;
-  std::frexp(a, &e);                     // a in [2^(e-1), 2^e)
-  int ex = std::max(e - 1, -14);         // exponent of the leading bit; subnormals share -14
-  double r = std::nearbyint(std::ldexp(a, 10 - ex));  // significand in units of the spacing 2^(ex-10)
-  if (r >= 2048.0) {                     // rounding carried into the next binade
-    r = 1024.0;
+  uint64_t u;
+  std::memcpy(&u, &x, 8);
+  const uint16_t sign = (uint16_t)((u >> 48) & 0x8000);
+  const int be = (int)((u >> 52) & 0x7FF);
+  if (be == 0) return sign;  // zero or a double subnormal: rounds to zero
+  const int e = be - 1023;   // |x| in [2^e, 2^(e+1))
+  if (e < -25) return sign;  // below half the smallest FP16 subnormal
+  if (e > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
+  const uint64_t m = (u & ((1ull << 52) - 1)) | (1ull << 52);  // |x| = m * 2^(e - 52)
+  int ex = std::max(e, -14);                                   // FP16 exponent; subnormals share -14
+  const int shift = 42 + (ex - e);                             // to units of 2^(ex - 10)
+  uint64_t q = m >> shift;
+  const uint64_t rem = m & ((1ull << shift) - 1), half = 1ull << (shift - 1);
+  if (rem > half || (rem == half && (q & 1))) ++q;
+  if (q >= 2048) {  // rounding carried into the next binade
+    q = 1024;
     ++ex;
+    if (ex > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
   }
-  if (ex > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
-  if (r < 1024.0) return (uint16_t)(sign | (uint16_t)r);  // subnormal (ex = -14)
-  return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)(r - 1024.0));
+  if (q < 1024) return (uint16_t)(sign | (uint16_t)q);  // subnormal (ex = -14)
+  return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)(q - 1024));
 }
 double half_value(uint16_t h) {
   const int ex = (h >> 10) & 31, m = h & 1023;
-  const double v = ex == 0 ? std::ldexp((double)m, -24) : std::ldexp(1024.0 + m, ex - 25);
+  double v;
+  if (ex == 0) {
+    v = (double)m * 0x1p-24;
+  } else {
+    const uint64_t u = ((uint64_t)(ex - 15 + 1023) << 52) | ((uint64_t)m << 42);
+    std::memcpy(&v, &u, 8);
+  }
   return (h & 0x8000) ? -v : v;
+}
+// FP16 split of an FP64 value: hi = RNE(x), lo = RNE(x - hi).
+void half_split(double x, uint16_t& hi, uint16_t& lo) {
+  hi = half_bits(x);
+  lo = half_bits(x - half_value(hi));
 }
 size_t tile_matrix_bytes(int k) {
   const size_t d = (size_t)1 << std::max(k, 2);
@@ -712,18 +738,20 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k, bool split = f
     // 2, 3 the low parts El = E - Eh of the same, with E the embedding of the
     // FP32 path (row 2c+p: output c, part p) taken from the FP64 matrix.
     std::vector<float> v(256, 0.f);
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 2; ++s)
       for (int l = 0; l < 64; ++l) {
-        const int row = l % 16, c = row >> 1, p = row & 1, cin = 2 * (l / 16) + (s & 1);
+        const int row = l % 16, c = row >> 1, p = row & 1, cin = 2 * (l / 16) + s;
         const cplx x = m[(size_t)c * 8 + cin];
         const double ev[2] = {p == 0 ? x.re : x.im, p == 0 ? -x.im : x.re};  // input part 0 (real), 1 (imaginary)
-        uint32_t word = 0;
+        uint32_t wh = 0, wl = 0;
         for (int q = 0; q < 2; ++q) {
-          const uint16_t hi = half_bits(ev[q]);
-          const uint16_t part = s < 2 ? hi : half_bits(ev[q] - half_value(hi));
-          word |= (uint32_t)part << (16 * q);
+          uint16_t hi, lo;
+          half_split(ev[q], hi, lo);
+          wh |= (uint32_t)hi << (16 * q);
+          wl |= (uint32_t)lo << (16 * q);
         }
-        std::memcpy(&v[(size_t)s * 64 + l], &word, 4);
+        std::memcpy(&v[(size_t)s * 64 + l], &wh, 4);
+        std::memcpy(&v[(size_t)(s + 2) * 64 + l], &wl, 4);
       }
     return v;
   }
@@ -739,17 +767,19 @@ std::vector<float> tile_matrix(const std::vector<cplx>& m, int k, bool split = f
     for (int t = 0; t < 4; ++t)
       for (int l = 0; l < 64; ++l) {
         const int row = l % 32, c = row % 16, p = row / 16, h = l / 32;
-        for (int w = 0; w < 4; ++w) {
-          const int cin = 4 * t + 2 * (w & 1) + h;
+        for (int w = 0; w < 2; ++w) {
+          const int cin = 4 * t + 2 * w + h;
           const cplx x = m[(size_t)c * 16 + cin];
           const double ev[2] = {p == 0 ? x.re : x.im, p == 0 ? -x.im : x.re};
-          uint32_t word = 0;
+          uint32_t wh = 0, wl = 0;
           for (int q = 0; q < 2; ++q) {
-            const uint16_t hi = half_bits(ev[q]);
-            const uint16_t part = w < 2 ? hi : half_bits(ev[q] - half_value(hi));
-            word |= (uint32_t)part << (16 * q);
+            uint16_t hi, lo;
+            half_split(ev[q], hi, lo);
+            wh |= (uint32_t)hi << (16 * q);
+            wl |= (uint32_t)lo << (16 * q);
           }
-          std::memcpy(&v[(size_t)4 * (64 * t + l) + w], &word, 4);
+          std::memcpy(&v[(size_t)4 * (64 * t + l) + w], &wh, 4);
+          std::memcpy(&v[(size_t)4 * (64 * t + l) + w + 2], &wl, 4);
         }
       }
     return v;
@@ -1401,7 +1431,10 @@ void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* 
   // Tile-index bit k: the k-th lowest source position outside the tile.
   uint32_t k = 0;
   for (int p = 0; p < n; ++p)
-    if (!src_local[p]) a.wtile[k++] = out_pos[in_at[p]];
+    if (!src_local[p]) {
+      d.xtile_q.push_back((uint32_t)in_at[p]);
+      a.wtile[k++] = out_pos[in_at[p]];
+    }
   // Both maps must be bijections onto positions 0..n-1 (the kernel's reads
   // and writes then stay inside the state and cover it once).
   uint64_t rseen = 0, wseen = 0;
@@ -2099,10 +2132,25 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
           std::vector<std::pair<int, int>> cands;
           for (int pc = 0; pc < 2; ++pc)
             for (int kt = std::min(2, def_k); kt <= def_k; ++kt) cands.push_back({pc, kt});
+          // While nothing has been emitted yet, every candidate is emitted in
+          // full (matrices included) on its own thread, into its own plan
+          // and from its own copy of the partition, and the chosen one
+          // becomes the plan: no second emission after the choice. Later
+          // unitary runs emit dry candidates (features only, sharing the
+          // partition) and then the chosen one. Both give the same
+          // features, so the same choice.
+          const bool whole = plan.ops.empty() && plan.blob.empty();
+          std::unique_ptr<Plan> full[2][5];
           run_all(cands, [&](int pc, int kt) {
-            Plan cand;  // a dry emission only reads the partition, so candidates share it
-            emit_xpartition(cand, parts[pc], ops, prec, n, tile, kt, nullptr, persist, threads, true);
+            Plan cand;
+            if (whole) {
+              XPartition mine = parts[pc];
+              emit_xpartition(cand, mine, ops, prec, n, tile, kt, nullptr, persist, threads, false);
+            } else {
+              emit_xpartition(cand, parts[pc], ops, prec, n, tile, kt, nullptr, persist, threads, true);
+            }
             for (const DevOp& d : cand.ops) pred[pc][kt] += predicted_ms(d, n);
+            if (whole) full[pc][kt] = std::make_unique<Plan>(std::move(cand));
           });
           const double def_t = pred[def_pin ? 1 : 0][def_k];
           double best = def_t;
@@ -2118,8 +2166,15 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
             best_pin = def_pin;
           }
           plan.learned_choices.push_back((best_pin ? 100 : 0) + best_k);
-          emit_xpartition(plan, parts[best_pin ? 1 : 0], ops, prec, n, tile, best_k, nullptr, persist, threads,
-                          false);
+          if (whole) {
+            Plan& c = *full[best_pin ? 1 : 0][best_k];
+            plan.ops = std::move(c.ops);
+            plan.blob = std::move(c.blob);
+            plan.uses_alt = plan.uses_alt || c.uses_alt;
+          } else {
+            emit_xpartition(plan, parts[best_pin ? 1 : 0], ops, prec, n, tile, best_k, nullptr, persist, threads,
+                            false);
+          }
         } else {
           emit_xsweeps(plan, ops, i, j, prec, n, tile, best_k, nullptr, persist, threads, best_pin);
         }
@@ -2203,6 +2258,44 @@ struct Exec {
   void* alt = nullptr;    // the other buffer (out-of-place sweeps write here, then the two trade roles)
   const uint8_t* cbase = nullptr;
   bool mfma_enabled = true;
+  bool zero_source = false;         // the next out-of-place sweep reads |0...0> (nothing was written)
+  std::vector<uint64_t> zero_masks;  // zero-tile masks of the leading out-of-place sweeps (run order)
+  size_t next_sweep = 0;
+
+  // Every slot to |0...0> before a plan runs from its start. When the plan
+  // opens with one-shot out-of-place sweeps, nothing is written here: the
+  // first sweep synthesises its source (tile 0 holds amplitude 1 at local
+  // index 0, every other tile stores zeros), and while some qubit has not
+  // yet been in any tile it is 0 in every amplitude, so the next sweeps'
+  // tiles whose index sets it store zeros without reading (zero_mask). The
+  // result is the same bit for bit: every block maps a zero tile to zeros.
+  void start_zero(const Plan& plan) {
+    zero_masks.clear();
+    next_sweep = 0;
+    const DevOp* first = nullptr;
+    for (const DevOp& d : plan.ops)
+      if (d.kind != D_NOOP) {
+        first = &d;
+        break;
+      }
+    zero_source = g_zero_start.load() && first && first->kind == D_XSWEEP && !first->persist &&
+                  first->conds.empty() && alt != nullptr;
+    if (!zero_source) {
+      k::init_zero(prec, sp, stride, batch, n, st);
+      return;
+    }
+    uint64_t support = 0;  // qubits that may be non-zero in some amplitude
+    for (const DevOp& d : plan.ops) {
+      if (d.kind == D_NOOP) continue;
+      if (d.kind != D_XSWEEP || d.persist || !d.conds.empty()) break;
+      uint64_t mask = 0;
+      for (size_t b = 0; b < d.xtile_q.size(); ++b)
+        if (!((support >> d.xtile_q[b]) & 1)) mask |= 1ull << b;
+      if (mask == 0) break;
+      zero_masks.push_back(mask);
+      for (uint32_t q : d.tile_lq) support |= 1ull << q;
+    }
+  }
 
   // Plan constants, and the second state buffer when the plan sweeps out of
   // place (only for states the executor owns).
@@ -2296,11 +2389,17 @@ struct Exec {
     void* s = sp;
     if (run_unitary(d, prec, s, stride, nb, n, cbase, f, mfma_enabled, st)) return;
     switch (d.kind) {
-      case D_XSWEEP:
+      case D_XSWEEP: {
         if (!d.conds.empty() || alt == nullptr) throw RuntimeErr("internal: out-of-place sweep without its buffers");
-        k::xsweep(sp, alt, stride, nb, n, d.xargs, cbase + d.tile_off, d.persist, st);
+        k::XSweepArgs a = d.xargs;
+        a.from_zero = zero_source ? 1u : 0u;
+        a.zero_mask = next_sweep < zero_masks.size() ? zero_masks[next_sweep] : 0;
+        zero_source = false;
+        ++next_sweep;
+        k::xsweep(sp, alt, stride, nb, n, a, cbase + d.tile_off, d.persist, st);
         std::swap(sp, alt);
         break;
+      }
       case D_MEAS: measure(d.qubit, d.cbit, f, nb); break;
       case D_RESET: {
         measure(d.qubit, -1, f, nb);
@@ -2528,7 +2627,7 @@ std::unique_ptr<Result> simulate(const Options& o, const char* json, size_t len)
   plan.upload(st.s);
   Exec ex(o.prec, (int)p.n, p.nc, 1, st.s);
   ex.prepare(plan);
-  k::init_zero(o.prec, ex.sp, ex.stride, 1, (int)p.n, st.s);
+  ex.start_zero(plan);
   k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
   auto res = std::make_unique<Result>();
   res->prec = o.prec;
@@ -2685,7 +2784,7 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
       plan.upload(st.s);
       Exec ex(o.prec, n, nc, 1, st.s);
       ex.prepare(plan);
-      k::init_zero(o.prec, ex.sp, ex.stride, 1, n, st.s);
+      ex.start_zero(plan);
       k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
       for (const DevOp& d : plan.ops) ex.run(d, 1);
       std::vector<uint64_t> idx = sample_indices(ex, qubits, shots, seed);
@@ -2729,7 +2828,7 @@ Counts simulate_shots(const Options& o, const char* json, size_t len, uint64_t s
     pplan.upload(st.s);
     Exec pex(o.prec, n, nc, 1, st.s);
     pex.prepare(pplan);
-    k::init_zero(o.prec, pex.sp, pex.stride, 1, n, st.s);
+    pex.start_zero(pplan);
     for (const DevOp& d : pplan.ops) pex.run(d, 1);
     st.sync();
     pex.settle();
@@ -2920,7 +3019,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
   stats.gates = count_gates(p.c.instructions);
   stats.launches = plan.ops.size();
   for (int rep = 0; rep < std::max(1, repeats); ++rep) {
-    k::init_zero(o.prec, ex.sp, ex.stride, 1, n, st.s);
+    ex.start_zero(plan);
     const bool rec = record && rep == std::max(1, repeats) - 1;
     Event t0, t1;
     std::vector<Event> evs;
@@ -3330,7 +3429,7 @@ double run_stream_pool(const BatchSetup& su, const std::vector<uint32_t>& set, i
           plan->upload(st.s);
           Exec ex(su.prec, n, p.nc, 1, st.s);
           ex.prepare(*plan);  // second state buffer (from the worker's arena) for out-of-place sweeps
-          k::init_zero(su.prec, ex.sp, ex.stride, 1, n, st.s);
+          ex.start_zero(*plan);
           k::rng_init(ex.rng.as<k::RngState>(), 1, seed, 0, st.s);
           for (const DevOp& d : plan->ops) ex.run(d, 1);
           std::vector<uint64_t> ix;
@@ -3794,6 +3893,10 @@ void mlp_train(float* w, const float* X, const float* T, uint64_t n, int epochs,
   HIPCHK(hipMemcpyAsync(w, d.w.get(), 4 * mlp::kMlpParams, hipMemcpyDeviceToHost, st.s));
   st.sync();
   HIPCHK(hipGetLastError());
+}
+
+void split_f16_host(const double* x, uint64_t n, uint16_t* hi, uint16_t* lo) {
+  for (uint64_t i = 0; i < n; ++i) half_split(x[i], hi[i], lo[i]);
 }
 
 }  // namespace dq

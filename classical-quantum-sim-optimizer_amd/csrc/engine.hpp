@@ -158,6 +158,12 @@ int set_sweep_mode(int mode);
 // Arithmetic of 3-qubit blocks in complex64 tile sweeps (0 FP32 matrix cores,
 // 1 split precision); returns the previous mode, -1 for an invalid one.
 int set_block_arith(int mode);
+// Zero-start of plans that open with out-of-place sweeps on or off
+// (process-wide); returns the previous setting.
+bool set_zero_start(bool on);
+// The FP16 split of split-precision block matrices: hi = RNE(x), lo =
+// RNE(x - hi), per element (|x| < 65520).
+void split_f16_host(const double* x, uint64_t n, uint16_t* hi, uint16_t* lo);
 
 int device_count();
 

@@ -990,7 +990,7 @@ __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock*
 // reads local index lo_l | hi_l(i) and writes destination offset lo_w | hi_w(i)
 // (plus the tile's base), so each wave's stores cover runs of consecutive
 // addresses as long as the low destination positions are tile bits.
-template <int L, int NTH, bool SPLIT>
+template <int L, int NTH, bool SPLIT, bool ZERO = false>
 __device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const float2* tile, uint64_t wbase,
                                              const XSweepArgs& a) {
   constexpr int PER = (1 << L) / NTH;
@@ -1023,7 +1023,9 @@ __device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const flo
         l |= hl[bit];
         w |= hw[bit];
       }
-    if constexpr (SPLIT) {
+    if constexpr (ZERO) {
+      dst[w] = make_float2(0.f, 0.f);
+    } else if constexpr (SPLIT) {
       const float2 v = tile[l];
       dst[w] = make_float2(v.x * kSplitUnscale, v.y * kSplitUnscale);
     } else {
@@ -1084,6 +1086,15 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
   extern __shared__ float4 lds4[];
   constexpr int PER = (1 << L) / NTH;
   float2* tile = (float2*)lds4;
+  if (!PERSIST && (blockIdx.x & a.zero_mask) != 0) {
+    // A tile of zeros (a qubit fixed to 1 by its index is 0 in every
+    // amplitude of the source, e.g. every tile but tile 0 of |0...0>): every
+    // block maps it to zeros, so it only stores zeros (no read, no blob copy,
+    // no block work).
+    xsweep_store<L, NTH, SPLIT, true>(dst + (uint64_t)blockIdx.y * stride, tile,
+                                      scatter_bits(blockIdx.x, a.wtile, a.ntb), a);
+    return;
+  }
   const unsigned char* meta;
   if (GBLOB) {
     meta = (const unsigned char*)blob;
@@ -1098,11 +1109,18 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
   float2 r[PER];
   uint64_t tau = blockIdx.x;
   if (!PERSIST) {
-    const uint64_t b0 = rd.base(tau, a);
+    if (a.from_zero) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
+      for (int i = 0; i < PER; ++i)
+        tile[threadIdx.x + NTH * i] =
+            make_float2(threadIdx.x == 0 && i == 0 ? (SPLIT ? kSplitScale : 1.f) : 0.f, 0.f);
+    } else {
+      const uint64_t b0 = rd.base(tau, a);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
+      for (int i = 0; i < PER; ++i) r[i] = sb[b0 | rd.off(i)];
+#pragma unroll
+      for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
+    }
     __syncthreads();  // the tile and the blob are in LDS
     tile_apply_blocks<L, K4, NTH, SPLIT>(tile, blocks, meta, a.nblocks);
     xsweep_store<L, NTH, SPLIT>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
@@ -2261,7 +2279,8 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = xsweep_lds_bytes(a);
   dim3 g((uint32_t)(persist ? tile_grid(ntiles) : ntiles), batch);
-  if (a.split && persist) throw std::runtime_error("xsweep: split-precision sweeps run one workgroup per tile");
+  if ((a.split || a.from_zero) && persist)
+    throw std::runtime_error("xsweep: split-precision sweeps and sweeps from |0...0> run one workgroup per tile");
   const bool gb = !xsweep_blob_in_lds(a);
 #define LGS(LL, K4, P, NT, G, S)                                                                                        \
   hipLaunchKernelGGL((k_xsweep<LL, K4, P, NT, G, S>), g, dim3(NT), lds, st, (const float2*)src, (float2*)dst, stride, \

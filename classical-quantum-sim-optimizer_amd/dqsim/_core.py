@@ -158,6 +158,8 @@ def _load():
         "dq_set_sweep_relabel": (ctypes.c_int, [I32]),
         "dq_set_sweep_pinned": (ctypes.c_int, [I32]),
         "dq_set_block_arith": (ctypes.c_int, [I32]),
+        "dq_set_zero_start": (ctypes.c_int, [I32]),
+        "dq_split_f16": (ctypes.c_int, [P, U64, P, P]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
                                               ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(D),
                                               ctypes.POINTER(P)]),
@@ -615,6 +617,14 @@ def set_block_arith(mode: int) -> int:
     if prev < 0:
         raise ValueError(f"block arithmetic must be 0 or 1, got {mode}")
     return prev
+
+
+def set_zero_start(on: bool) -> bool:
+    """Plans that open with out-of-place sweeps start from a synthesised
+    |0...0> and skip reading all-zero tiles (True, the default), or
+    initialise the state and read it (False); results are identical.
+    Returns the previous setting."""
+    return bool(_lib.dq_set_zero_start(1 if on else 0))
 
 
 _planner_weights_set = [False]  # weights installed or cleared in this process

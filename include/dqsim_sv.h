@@ -282,6 +282,21 @@ int dq_set_sweep_pinned(int32_t on);
  * -1 for a mode outside 0..1. */
 int dq_set_block_arith(int32_t mode);
 
+/* Process-wide zero start (no counterpart in the reference: a tuning and A/B
+ * switch). on != 0 (default): a plan that opens with one-shot out-of-place
+ * sweeps starts from a synthesised |0...0> (no initialisation, no read by
+ * the first sweep) and, while some qubit has not yet been in a tile, its
+ * sweeps store the tiles that are all zeros without reading them. 0:
+ * initialise the state and read it. Results are identical bit for bit.
+ * Returns the previous setting. */
+int dq_set_zero_start(int32_t on);
+
+/* The host-side FP16 split of split-precision block matrices (no reference
+ * counterpart; exposed for tests): hi[i] = x[i] rounded to the nearest FP16
+ * (ties to even), lo[i] = x[i] - value(hi[i]) rounded the same way, as
+ * IEEE binary16 bit patterns. |x[i]| >= 65520 is DQ_ERUNTIME. */
+int dq_split_f16(const double* x, uint64_t n, uint16_t* hi, uint16_t* lo);
+
 /* Number of visible GPUs (0 on a host without one). */
 int dq_device_count(void);
 

@@ -793,3 +793,26 @@ def test_shipped_planner_weights_load():
     assert w is not None, "planner_weights.npy is missing from the package"
     assert w.dtype == np.float32 and w.shape == (planner.N_PARAMS,) and np.isfinite(w).all()
     assert planner.load_default("/nonexistent/planner_weights.npy") is None
+
+
+def test_fp16_split_of_block_matrices_matches_ieee_rounding():
+    # Split-precision block matrices are written as FP16 pairs hi + lo of the
+    # FP64 embedding. hi must be IEEE binary16 round-to-nearest-even of x
+    # (numpy's float16 conversion is that rounding) and lo the same rounding
+    # of the exact residual x - hi; together they carry about 22 bits.
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-1, 1, 20000), rng.uniform(-1e-4, 1e-4, 5000), rng.uniform(-1e-7, 1e-7, 2000),
+                        np.array([0.0, -0.0, 2.0 ** -25, -2.0 ** -25, 2.0 ** -24, 3 * 2.0 ** -26, 2.0 ** -14,
+                                  65504.0, 1.0, -1.0, 0.5 + 2.0 ** -12, 0.5 + 3 * 2.0 ** -12, 1 / 3])])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    hi = np.empty(x.size, dtype=np.uint16)
+    lo = np.empty(x.size, dtype=np.uint16)
+    _core._check(_core._lib.dq_split_f16(x.ctypes.data, x.size, hi.ctypes.data, lo.ctypes.data))
+    assert np.array_equal(hi, x.astype(np.float16).view(np.uint16))
+    hv = hi.view(np.float16).astype(np.float64)
+    assert np.array_equal(lo, (x - hv).astype(np.float16).view(np.uint16))
+    recon = hv + lo.view(np.float16).astype(np.float64)
+    big = np.abs(x) > 2.0 ** -4
+    assert np.all(np.abs(recon - x)[big] <= np.abs(x[big]) * 2.0 ** -21)
+    with pytest.raises(RuntimeError):
+        _core._check(_core._lib.dq_split_f16(np.array([70000.0]).ctypes.data, 1, hi.ctypes.data, lo.ctypes.data))
