@@ -92,7 +92,10 @@ def mfma_flops_per_step(circ, precision, fusion, max_fused, tile, out_of_place):
     blocks run padded to 2 qubits in tile sweeps. A split-precision 3-qubit
     block issues 4x its algorithmic flops on the FP16 matrix core (two
     16x16x32 f16 MFMAs of K = 32 where the FP32 path has K = 16 in four
-    16x16x4 f32 ones). Also returns the plan's learned choices."""
+    16x16x4 f32 ones). A sweep's flops count the tiles it computes (under a
+    zero start the leading sweeps store all-zero tiles without running their
+    blocks: the plan's work_fraction). Also returns the plan's learned
+    choices."""
     from dqsim import _core
 
     plan = _core.plan_json(circ, precision=precision, fusion=fusion, max_fused_qubits=max_fused, tile_qubits=tile,
@@ -102,7 +105,7 @@ def mfma_flops_per_step(circ, precision, fusion, max_fused, tile, out_of_place):
     for op in plan["launches"]:
         if op["kind"] == "tile" and not op.get("exact"):
             for b in op["blocks"]:
-                f = 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n)
+                f = 8.0 * (1 << max(2, len(b["qubits"]))) * float(1 << n) * op.get("work_fraction", 1.0)
                 tile_flops += f
                 if b.get("split"):
                     split_flops += f
@@ -352,26 +355,29 @@ def main():
         t = time.perf_counter()
         fused.evolve_timed(circ, repeats=1, record=False)
         log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s")
-    rows_all, dev_ms = [], []
+    # The timed steps run as a user's call does (one event pair around the
+    # whole evolution); one more step afterwards records an event around
+    # every launch for the per-kernel figures.
+    dev_ms = []
     t0 = time.perf_counter()
     for s in range(args.steps):
-        rows, launches, g, total_ms, _ = fused.evolve_timed(circ, repeats=1, record=True)
-        rows_all += rows
+        _, launches, g, total_ms, _ = fused.evolve_timed(circ, repeats=1, record=False)
         dev_ms.append(total_ms)
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    rows_all, launches, g, rec_ms, _ = fused.evolve_timed(circ, repeats=1, record=True)
     log(f"[bench] fused: {ms_per_step:.1f} ms/step wall, device {sum(dev_ms) / len(dev_ms):.1f} ms, "
-        f"{launches} launches")
+        f"{launches} launches (recorded step: device {rec_ms:.1f} ms)")
     agg = per_kind(rows_all)
     dominant = max(agg.items(), key=lambda kv: kv[1]["ms"])
     dname, d = dominant
-    total_bytes = sum(a["bytes"] for a in agg.values()) / args.steps
+    total_bytes = sum(a["bytes"] for a in agg.values())
     value = gates * float(1 << n) / (ms_per_step / 1e3)
 
     tile_flops, split_flops, fused_flops, choices = mfma_flops_per_step(circ, args.precision, args.fusion, args.max_fused,
                                                            args.tile, sweep_mode != 0)
     mflops = tile_flops + fused_flops
-    mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "xsweep", "fused_mfma")) / args.steps
+    mfma_ms = sum(a["ms"] for k, a in agg.items() if k in ("tile", "xsweep", "fused_mfma"))
     mfma = None
     if mflops > 0 and mfma_ms > 0:
         ach = mflops / (mfma_ms / 1e3) / 1e12
@@ -495,8 +501,8 @@ def main():
         # Algorithmic flops per launch; the floor prices FP32 blocks at the
         # FP32 matrix-core peak and split-precision blocks' issued FP16 flops
         # (4x algorithmic) at the FP16 peak.
-        flops_pl = tile_flops * args.steps / d["launches"]
-        split_pl = split_flops * args.steps / d["launches"]
+        flops_pl = tile_flops / d["launches"]
+        split_pl = split_flops / d["launches"]
         floor_ms = ((flops_pl - split_pl) / (MFMA_FP32_PEAK_TFLOPS * 1e12)
                     + 4.0 * split_pl / (MFMA_FP16_PEAK_TFLOPS * 1e12)) * 1e3
         mf = {"achieved": flops_pl / (launch_ms / 1e3) / 1e12, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -2100,8 +2100,9 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         bool best_pin = def_pin;
         if (fusion == FUSE_LEARNED && !g_planner.empty()) {
           // Choose (tile rule, block width) by the model's predicted total
-          // time; keep the default (the process-wide tile rule, blocks of up
-          // to max_fused qubits) unless another is predicted >= 3 % faster.
+          // time: the candidate predicted fastest (the default, the
+          // process-wide tile rule with blocks of up to max_fused qubits, on
+          // a tie).
           // Host threads: the two tile rules' groupings, then one dry
           // emission per (rule, width) candidate.
           const std::vector<FusedBlock> pre = greedy_fuse(ops, i, j, 2);
@@ -2152,8 +2153,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
             for (const DevOp& d : cand.ops) pred[pc][kt] += predicted_ms(d, n);
             if (whole) full[pc][kt] = std::make_unique<Plan>(std::move(cand));
           });
-          const double def_t = pred[def_pin ? 1 : 0][def_k];
-          double best = def_t;
+          double best = pred[def_pin ? 1 : 0][def_k];
           for (int pc = 0; pc < 2; ++pc)
             for (int kt = std::min(2, def_k); kt <= def_k; ++kt)
               if (pred[pc][kt] < best) {
@@ -2161,10 +2161,6 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
                 best_k = kt;
                 best_pin = pc == 1;
               }
-          if (best > 0.97 * def_t) {
-            best_k = def_k;
-            best_pin = def_pin;
-          }
           plan.learned_choices.push_back((best_pin ? 100 : 0) + best_k);
           if (whole) {
             Plan& c = *full[best_pin ? 1 : 0][best_k];
@@ -2247,6 +2243,47 @@ bool run_unitary(const DevOp& d, int prec, void* s, uint64_t stride, uint32_t nb
   }
 }
 
+// Zero start of a plan that runs from |0...0> with a second state buffer:
+// whether its first launch is a one-shot out-of-place sweep that synthesises
+// its source, and (masks) the zero-tile masks of its leading sweeps in run
+// order. While some qubit has not yet been in any tile it is 0 in every
+// amplitude, so a sweep's tiles whose index sets such a qubit hold zeros
+// (bit b of a mask: tile-index bit b is such a qubit).
+bool zero_start_masks(const Plan& plan, std::vector<uint64_t>* masks) {
+  masks->clear();
+  const DevOp* first = nullptr;
+  for (const DevOp& d : plan.ops)
+    if (d.kind != D_NOOP) {
+      first = &d;
+      break;
+    }
+  if (!g_zero_start.load() || !first || first->kind != D_XSWEEP || first->persist || !first->conds.empty())
+    return false;
+  uint64_t support = 0;  // qubits that may be non-zero in some amplitude
+  for (const DevOp& d : plan.ops) {
+    if (d.kind == D_NOOP) continue;
+    if (d.kind != D_XSWEEP || d.persist || !d.conds.empty()) break;
+    uint64_t mask = 0;
+    for (size_t b = 0; b < d.xtile_q.size(); ++b)
+      if (!((support >> d.xtile_q[b]) & 1)) mask |= 1ull << b;
+    if (mask == 0) break;
+    masks->push_back(mask);
+    for (uint32_t q : d.tile_lq) support |= 1ull << q;
+  }
+  return true;
+}
+
+// Fractions of out-of-place sweep number i (run order) under a zero start:
+// of the source read from HBM (0 for the synthesised first source), and of
+// the tiles whose blocks run (the others store zeros).
+double sweep_read_fraction(bool started, const std::vector<uint64_t>& masks, size_t i) {
+  if (i == 0 && started) return 0.0;
+  return i < masks.size() ? std::ldexp(1.0, -__builtin_popcountll(masks[i])) : 1.0;
+}
+double sweep_work_fraction(const std::vector<uint64_t>& masks, size_t i) {
+  return i < masks.size() ? std::ldexp(1.0, -__builtin_popcountll(masks[i])) : 1.0;
+}
+
 struct Exec {
   int prec, n;
   uint64_t nc, ncs, stride, c1;
@@ -2259,8 +2296,10 @@ struct Exec {
   const uint8_t* cbase = nullptr;
   bool mfma_enabled = true;
   bool zero_source = false;         // the next out-of-place sweep reads |0...0> (nothing was written)
+  bool zero_started = false;        // the last start_zero left the state unwritten
   std::vector<uint64_t> zero_masks;  // zero-tile masks of the leading out-of-place sweeps (run order)
   size_t next_sweep = 0;
+
 
   // Every slot to |0...0> before a plan runs from its start. When the plan
   // opens with one-shot out-of-place sweeps, nothing is written here: the
@@ -2270,32 +2309,13 @@ struct Exec {
   // tiles whose index sets it store zeros without reading (zero_mask). The
   // result is the same bit for bit: every block maps a zero tile to zeros.
   void start_zero(const Plan& plan) {
-    zero_masks.clear();
     next_sweep = 0;
-    const DevOp* first = nullptr;
-    for (const DevOp& d : plan.ops)
-      if (d.kind != D_NOOP) {
-        first = &d;
-        break;
-      }
-    zero_source = g_zero_start.load() && first && first->kind == D_XSWEEP && !first->persist &&
-                  first->conds.empty() && alt != nullptr;
-    if (!zero_source) {
-      k::init_zero(prec, sp, stride, batch, n, st);
-      return;
-    }
-    uint64_t support = 0;  // qubits that may be non-zero in some amplitude
-    for (const DevOp& d : plan.ops) {
-      if (d.kind == D_NOOP) continue;
-      if (d.kind != D_XSWEEP || d.persist || !d.conds.empty()) break;
-      uint64_t mask = 0;
-      for (size_t b = 0; b < d.xtile_q.size(); ++b)
-        if (!((support >> d.xtile_q[b]) & 1)) mask |= 1ull << b;
-      if (mask == 0) break;
-      zero_masks.push_back(mask);
-      for (uint32_t q : d.tile_lq) support |= 1ull << q;
-    }
+    zero_source = zero_start_masks(plan, &zero_masks) && alt != nullptr;
+    if (!zero_source) zero_masks.clear();
+    zero_started = zero_source;
+    if (!zero_source) k::init_zero(prec, sp, stride, batch, n, st);
   }
+
 
   // Plan constants, and the second state buffer when the plan sweeps out of
   // place (only for states the executor owns).
@@ -3035,6 +3055,7 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
     stats.total_ms = elapsed_ms(t0, t1);
     if (rec) {
       stats.ops.clear();
+      size_t sweep = 0;
       for (size_t i = 0; i < plan.ops.size(); ++i) {
         const DevOp& d = plan.ops[i];
         OpTiming t;
@@ -3044,6 +3065,11 @@ EvolveStats evolve_timed(const Options& o, const char* json, size_t len, int rep
         t.gates = d.gates;
         t.ms = elapsed_ms(evs[i], evs[i + 1]);
         t.bytes = op_bytes(d, n, o.prec);
+        if (d.kind == D_XSWEEP) {
+          // One state written, the part of the source that was read.
+          t.bytes = (1.0 + sweep_read_fraction(ex.zero_started, ex.zero_masks, sweep)) * o.prec * std::ldexp(1.0, n);
+          ++sweep;
+        }
         std::copy(d.feat, d.feat + mlp::kFeatures, t.feat);
         stats.ops.push_back(t);
       }
@@ -3081,13 +3107,27 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
   for (size_t i = 0; i < plan.learned_choices.size(); ++i)
     js << (i ? ", " : "") << "{\"pinned\": " << (plan.learned_choices[i] >= 100 ? "true" : "false")
        << ", \"k\": " << plan.learned_choices[i] % 100 << "}";
-  js << "], \"launches\": [";
+  // A run from |0...0> (zero start): each out-of-place sweep's fractions of
+  // the source read and of the tiles computed; bytes count what is read.
+  std::vector<uint64_t> zmasks;
+  const bool zstart = zero_start_masks(plan, &zmasks) && plan.uses_alt;
+  if (!zstart) zmasks.clear();
+  size_t sweep = 0;
+  js << "], \"zero_start\": " << (zstart ? "true" : "false") << ", \"launches\": [";
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     const DevOp& d = plan.ops[i];
+    double bytes = op_bytes(d, (int)p.n, prec), readf = 1.0, workf = 1.0;
+    if (d.kind == D_XSWEEP) {
+      readf = sweep_read_fraction(zstart, zmasks, sweep);
+      workf = sweep_work_fraction(zmasks, sweep);
+      bytes = (1.0 + readf) * prec * std::ldexp(1.0, (int)p.n);
+      ++sweep;
+    }
     js << (i ? ", " : "") << "{\"kind\": \"" << names[d.kind] << "\", \"k\": " << d.kq
        << ", \"gates\": " << d.gates << ", \"conditional\": " << (d.conds.empty() ? "false" : "true")
        << ", \"mfma\": " << (d.mfma ? "true" : "false") << ", \"entries\": " << d.pat.size()
-       << ", \"category\": " << d.category << ", \"bytes\": " << json_num(op_bytes(d, (int)p.n, prec))
+       << ", \"category\": " << d.category << ", \"bytes\": " << json_num(bytes)
+       << ", \"read_fraction\": " << json_num(readf) << ", \"work_fraction\": " << json_num(workf)
        << ", \"qubits\": [";
     for (size_t t = 0; t < d.touched.size(); ++t) js << (t ? ", " : "") << d.touched[t];
     js << "], \"ops\": [";

@@ -27,10 +27,11 @@ With ``fusion="learned"`` and weights installed:
   * out-of-place sweeps: the plan is chosen among candidate (tile rule, block
     width) pairs -- qubits 0..3 pinned in every tile or only shared between
     neighbouring tiles, blocks of 2..max_fused (<= 4) qubits -- by the
-    model's predicted total time; the default (the process-wide tile rule,
-    blocks of max_fused qubits) is kept unless another pair is predicted at
-    least 3 % faster;
-  * in-place sweeps: the same among (block width, tile size) pairs;
+    model's predicted total time, the pair predicted fastest (the default,
+    the process-wide tile rule with blocks of max_fused qubits, on a tie);
+  * in-place sweeps: among (block width, tile size) pairs, the default
+    (blocks <= 3 qubits, the largest tile) unless another pair is predicted
+    at least 3 % faster;
   * plans without sweeps ask the model before every merge and merge blocks a
     and b only when t(a) + t(b) > t(a+b).
 Without weights all three fall back to plain greedy. ``StatevectorSimulator``
