@@ -159,8 +159,8 @@ int dq_sv_create(const dq_sv_opts* o, dq_sv** out) {
       throw dq::ValueErr("precision must be complex128 or complex64");
     if (o->fusion < DQ_FUSE_NONE || o->fusion > DQ_FUSE_LEARNED) throw dq::ValueErr("unknown fusion mode");
     if (o->max_fused_qubits < 1 || o->max_fused_qubits > 5) throw dq::ValueErr("max_fused_qubits must be in 1..5");
-    if (o->tile_qubits != 0 && (o->tile_qubits < dq::kTileMin || o->tile_qubits > dq::kTileMax))
-      throw dq::ValueErr("tile_qubits must be 0 or in 10..12");
+    if (o->tile_qubits != 0 && (o->tile_qubits < dq::kTileMin || o->tile_qubits > dq::kXTileMax))
+      throw dq::ValueErr("tile_qubits must be 0 or in 10..13");
     auto* s = new dq_sv;
     s->o.has_seed = o->has_seed != 0;
     s->o.seed = o->seed;
@@ -281,8 +281,8 @@ const char* dq_last_error(void) { return g_err.c_str(); }
 int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,
                  int32_t tile_qubits, int32_t out_of_place, char** out) {
   return guard([&] {
-    if (tile_qubits != 0 && (tile_qubits < dq::kTileMin || tile_qubits > dq::kTileMax))
-      throw dq::ValueErr("tile_qubits must be 0 or in 10..12");
+    if (tile_qubits != 0 && (tile_qubits < dq::kTileMin || tile_qubits > dq::kXTileMax))
+      throw dq::ValueErr("tile_qubits must be 0 or in 10..13");
     std::string s = dq::plan_description(json, len, precision, fusion, max_fused, tile_qubits, out_of_place != 0);
     char* p = (char*)std::malloc(s.size() + 1);
     if (!p) throw std::bad_alloc();

@@ -2090,10 +2090,11 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
       scorer = [n](const FusedBlock& a, const FusedBlock& b, const std::vector<Op>& o) {
         return planner_score(a, b, o, n);
       };
-    if (prec == 8 && tile >= kTileMin && tile <= kTileMax && n >= tile + 4) {
+    if (prec == 8 && tile >= kTileMin && tile <= kXTileMax && n >= tile + 4) {
       if (permute) {
-        const bool persist = mode == 1 || mode == 4;
-        const int threads = mode >= 3 && tile >= 11 ? 512 : 256;
+        // 13-qubit tiles: 512 threads, one workgroup per tile.
+        const bool persist = tile <= kTileMax && (mode == 1 || mode == 4);
+        const int threads = tile > kTileMax || (mode >= 3 && tile >= 11) ? 512 : 256;
         const int def_k = std::max(1, std::min(max_fused, 4));
         const bool def_pin = g_sweep_pinned;
         int best_k = def_k;
@@ -2178,11 +2179,11 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         // Choose (block width, tile size) by the model's predicted total
         // time; leave the default (blocks <= 3 qubits, the largest tile) only
         // for a predicted gain of at least 3 %.
-        const int def_k = std::min(max_fused, 3), def_l = tile;
+        const int def_k = std::min(max_fused, 3), def_l = std::min(tile, kTileMax);
         int best_k = def_k, best_l = def_l;
         double best = 1e300, def_t = 1e300;
         for (int kt = 2; kt <= std::max(2, std::min(max_fused, 4)); ++kt)
-          for (int l = kTileMin; l <= tile; ++l) {
+          for (int l = kTileMin; l <= def_l; ++l) {
             if (n < l + 4) continue;
             Plan cand;
             const int k = std::min(kt, max_fused);
@@ -2202,7 +2203,7 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
         }
         emit_sweeps(plan, ops, i, j, prec, n, best_l, best_k, nullptr);
       } else {
-        emit_sweeps(plan, ops, i, j, prec, n, tile, max_fused, scorer);
+        emit_sweeps(plan, ops, i, j, prec, n, std::min(tile, kTileMax), max_fused, scorer);
       }
     } else {
       std::vector<FusedBlock> blocks = greedy_fuse(ops, i, j, std::max(1, std::min(max_fused, 5)), scorer);
@@ -3169,7 +3170,7 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
         for (uint32_t t = 0; t < cnt; ++t) js << (t ? ", " : "") << v[t];
         js << "]";
       };
-      uint32_t store[12] = {};
+      uint32_t store[kXTileMax] = {};
       for (uint32_t j = 0; j < a.L; ++j) store[a.wlocal[j]] = a.wpos[j];
       js << ", \"exact\": false, \"out_of_place\": true, \"gather\": " << (a.gather ? "true" : "false")
          << ", \"persist\": " << (d.persist ? "true" : "false");

@@ -38,6 +38,9 @@ enum Fusion { FUSE_NONE = 0, FUSE_GREEDY = 1, FUSE_LEARNED = 2 };
 
 // LDS tile sweeps (complex64 with fusion): tile sizes the kernel is built for.
 constexpr int kTileMin = 10, kTileMax = 12, kTileDefault = 12;
+// Out-of-place complex64 sweeps also take 13-qubit tiles (the whole 64 KiB
+// LDS; in-place sweeps of such a plan use 12).
+constexpr int kXTileMax = 13;
 
 struct Options {
   bool has_seed = false;

@@ -2265,17 +2265,22 @@ void tile_sweep(void* s, uint64_t stride, uint32_t batch, int n, const TileArgs&
 // global memory. Measured on 32-qubit sweeps: LDS is faster for the few
 // blocks of a random-circuit sweep (17.4 vs 18.1 ms), global for sweeps of
 // 10+ blocks (41.8 vs 49.8 ms).
-bool xsweep_blob_in_lds(const XSweepArgs& a) { return (size_t)16 * a.blob16 <= ((size_t)8 << a.L) / 4; }
+// 13-qubit tiles take the whole default 64 KiB of LDS: their blob stays in
+// global memory.
+bool xsweep_blob_in_lds(const XSweepArgs& a) {
+  return a.L <= 12 && (size_t)16 * a.blob16 <= ((size_t)8 << a.L) / 4;
+}
 size_t xsweep_lds_bytes(const XSweepArgs& a) {
   return ((size_t)8 << a.L) + (xsweep_blob_in_lds(a) ? (size_t)16 * a.blob16 : 0);
 }
 
 void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
             const void* blob, bool persist, hipStream_t st) {
-  if (a.L < 10 || a.L > 12 || (int)a.L > n || a.ntb != (uint32_t)(n - (int)a.L))
+  if (a.L < 10 || a.L > 13 || (int)a.L > n || a.ntb != (uint32_t)(n - (int)a.L))
     throw std::runtime_error("xsweep: tile size does not match the state");
-  if (a.threads != 256 && !(a.threads == 512 && a.L >= 11))
-    throw std::runtime_error("xsweep: 256 threads, or 512 for tiles of 11-12 qubits");
+  if (a.L == 13 ? (a.threads != 512 || persist) : (a.threads != 256 && !(a.threads == 512 && a.L >= 11)))
+    throw std::runtime_error("xsweep: 256 threads, or 512 for tiles of 11-12 qubits; 13-qubit tiles: 512 "
+                             "threads, one workgroup per tile");
   const uint64_t ntiles = 1ull << (n - (int)a.L);
   const size_t lds = xsweep_lds_bytes(a);
   dim3 g((uint32_t)(persist ? tile_grid(ntiles) : ntiles), batch);
@@ -2297,7 +2302,8 @@ void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, 
   switch (a.L) {
     case 10: if (k4) { LP(10, true, 256); } else { LP(10, false, 256); } break;
     case 11: if (k4) { LT(11, true); } else { LT(11, false); } break;
-    default: if (k4) { LT(12, true); } else { LT(12, false); } break;
+    case 12: if (k4) { LT(12, true); } else { LT(12, false); } break;
+    default: if (k4) { LG(13, true, false, 512, true); } else { LG(13, false, false, 512, true); } break;
   }
 #undef LT
 #undef LP

@@ -176,15 +176,15 @@ struct XSweepArgs {
   uint32_t kmax;       // widest block (3 or 4)
   uint32_t gather;     // 0: contiguous read; 1: read positions rpos
   uint32_t ntb;        // tile-index bits (n - L)
-  uint32_t threads;    // workgroup size: 256, or 512 for L >= 11
+  uint32_t threads;    // workgroup size: 256, or 512 for L >= 11 (L = 13: 512 only)
   uint32_t split;      // 1: 3-qubit blocks in split precision (kind 4), the tile held scaled by 2^14 in LDS
   uint32_t from_zero;  // 1: the source is |0...0> (one-shot launches): nothing is read, tile 0 starts as
                        // amplitude 1 at local index 0, every other tile stores zeros without block work
   uint32_t pad;
-  uint32_t rpos[12];   // gather: source position of local bit j
-  uint32_t rsrt[12];   // gather: rpos sorted ascending
-  uint32_t wlocal[12]; // store index bit j -> local bit
-  uint32_t wpos[12];   // store index bit j -> destination position
+  uint32_t rpos[13];   // gather: source position of local bit j
+  uint32_t rsrt[13];   // gather: rpos sorted ascending
+  uint32_t wlocal[13]; // store index bit j -> local bit
+  uint32_t wpos[13];   // store index bit j -> destination position
   uint32_t wtile[52];  // tile-index bit k -> destination position
   uint64_t zero_mask;  // one-shot launches: tiles whose index has one of these bits set hold zeros (their
                        // qubit is still 0 in every amplitude of the source): they store zeros without reading

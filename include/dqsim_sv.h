@@ -67,7 +67,9 @@ typedef struct dq_sv_opts {
   int32_t max_fused_qubits; /* 1..5, default 4 */
   int32_t tile_qubits;      /* complex64 + fusion: fused blocks are grouped into
                                sweeps over 2^tile_qubits-amplitude LDS tiles
-                               (10..12, default 12); 0 = one launch per block */
+                               (10..13, default 12; 13 only for out-of-place
+                               sweeps, in-place ones use 12); 0 = one launch
+                               per block */
 } dq_sv_opts;
 
 /* SimulationProfile fields (simulator.rs:43-59). */

@@ -23,6 +23,9 @@ CASES = [
     ("su2", 17, 11, 4), ("su2", 18, 12, 4), ("su2", 20, 12, 5), ("su2", 20, 12, 3),
     ("qft", 16, 12, 4), ("mixed", 17, 12, 3), ("mixed", 18, 11, 5), ("mixed", 16, 10, 4),
     ("su2", 24, 12, 3), ("mixed", 22, 12, 3),
+    # 13-qubit tiles (out-of-place sweeps only: 512 threads, the whole 64 KiB of LDS)
+    ("su2", 17, 13, 3), ("su2", 20, 13, 4), ("su2", 22, 13, 2), ("qft", 17, 13, 4), ("mixed", 18, 13, 3),
+    ("mixed", 19, 13, 5),
 ]
 
 

@@ -644,7 +644,7 @@ def sweep_pinned(request):
 
 @pytest.mark.parametrize("kind,n,L,k", [("su2", 16, 12, 3), ("su2", 18, 12, 4), ("su2", 17, 11, 3),
                                         ("mixed", 17, 12, 3), ("qft", 16, 10, 2), ("su2", 20, 12, 2),
-                                        ("brickwork", 22, 12, 3)])
+                                        ("brickwork", 22, 12, 3), ("su2", 18, 13, 3), ("mixed", 19, 13, 4)])
 def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k, sweep_pinned):
     # Tag every amplitude with its index and push the tags through the
     # launches' read/write maps: each launch's tile qubits must sit where the
@@ -712,6 +712,26 @@ def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k, sweep_pinn
     # Same ops as the in-place plan, each once.
     ops = sorted(o for x in plan["launches"] for o in x["ops"])
     assert ops == list(range(plan["compiled_ops"]))
+
+
+def test_thirteen_qubit_tiles_are_out_of_place_only():
+    # 13-qubit tiles fill the 64 KiB of LDS: out-of-place sweeps take them
+    # (512 threads, one workgroup per tile), in-place sweeps of the same
+    # request use 12-qubit tiles; 14 is rejected.
+    c = workloads.random_su2(n=20, depth=10, seed=4)
+    xp = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=3, tile_qubits=13,
+                         out_of_place=True)
+    xs = [x for x in xp["launches"] if x.get("out_of_place")]
+    assert xs and all(x["k"] == 13 and len(x["local_qubits"]) == 13 and not x["persist"] for x in xs)
+    assert all(x["features"][9] == 1.0 for x in xs)  # 512-thread workgroups
+    ip = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=3, tile_qubits=13,
+                         out_of_place=False)
+    tiles = [x for x in ip["launches"] if x["kind"] == "tile"]
+    assert tiles and all(x["k"] == 12 and not x.get("out_of_place") for x in tiles)
+    with pytest.raises(ValueError):
+        _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=3, tile_qubits=14)
+    with pytest.raises(ValueError):
+        dqsim.StatevectorSimulator(precision="complex64", fusion="greedy", tile_qubits=14)
 
 
 @pytest.mark.parametrize("kind,n,k", [("su2", 20, 3), ("brickwork", 22, 4), ("mixed", 18, 2)])
