@@ -472,7 +472,7 @@ def test_tile_sweeps_off_and_limits():
     small = workloads.random_su2(n=15, depth=6, seed=1)
     assert not any(x["kind"] == "tile"
                    for x in _core.plan_json(small, precision="complex64", fusion="greedy")["launches"])
-    for bad in (9, 13, 64):
+    for bad in (9, 14, 64):
         with pytest.raises(ValueError):
             _core.plan_json(c, precision="complex64", fusion="greedy", tile_qubits=bad)
         with pytest.raises(ValueError):
