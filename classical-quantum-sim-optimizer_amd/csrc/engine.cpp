@@ -684,8 +684,11 @@ int tile_kind(int k, bool split = false) {
   return k <= 2 ? 3 : k == 3 ? (split ? 4 : 1) : (k == 4 && split) ? 5 : 2;
 }
 
-// IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520),
-// from the bits of the double in integer arithmetic (exact, no libm calls).
+// IEEE binary16 encoding of x rounded to nearest, ties to even (|x| < 65520).
+// The rounding is one FP64 add and subtract of 1.5 * 2^52 on x scaled so
+// that FP16's last place is 1 (scaling by a power of two is exact, and the
+// add rounds to an integer, to nearest even, since |x * s| < 2^12); the
+// encoding is then read off the bits of the rounded double.
 uint16_t half_bits(double x) {
   uint64_t u;
   std::memcpy(&u, &x, 8);
@@ -695,19 +698,20 @@ uint16_t half_bits(double x) {
   const int e = be - 1023;   // |x| in [2^e, 2^(e+1))
   if (e < -25) return sign;  // below half the smallest FP16 subnormal
   if (e > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
-  const uint64_t m = (u & ((1ull << 52) - 1)) | (1ull << 52);  // |x| = m * 2^(e - 52)
-  int ex = std::max(e, -14);                                   // FP16 exponent; subnormals share -14
-  const int shift = 42 + (ex - e);                             // to units of 2^(ex - 10)
-  uint64_t q = m >> shift;
-  const uint64_t rem = m & ((1ull << shift) - 1), half = 1ull << (shift - 1);
-  if (rem > half || (rem == half && (q & 1))) ++q;
-  if (q >= 2048) {  // rounding carried into the next binade
-    q = 1024;
-    ++ex;
-    if (ex > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
+  const int ex = e < -14 ? -14 : e;  // FP16 exponent; subnormals share -14
+  double s;                          // 2^(10 - ex)
+  const uint64_t su = (uint64_t)(1023 + 10 - ex) << 52;
+  std::memcpy(&s, &su, 8);
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  const double a = x < 0 ? -x : x;
+  const double q = (a * s + magic) - magic;  // RNE(|x| / 2^(ex - 10)), an integer < 2^12
+  const uint32_t qi = (uint32_t)q;
+  if (qi < 1024) return (uint16_t)(sign | (uint16_t)qi);  // subnormal (ex = -14), or zero
+  if (qi >= 2048) {                                        // rounding carried into the next binade
+    if (ex + 1 > 15) throw RuntimeErr("internal: FP16 overflow in a split-precision matrix");
+    return (uint16_t)(sign | (uint16_t)((ex + 16) << 10));
   }
-  if (q < 1024) return (uint16_t)(sign | (uint16_t)q);  // subnormal (ex = -14)
-  return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)(q - 1024));
+  return (uint16_t)(sign | (uint16_t)((ex + 15) << 10) | (uint16_t)(qi - 1024));
 }
 double half_value(uint16_t h) {
   const int ex = (h >> 10) & 31, m = h & 1023;
