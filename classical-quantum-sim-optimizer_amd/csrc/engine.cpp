@@ -1677,19 +1677,29 @@ XPartition group_xsweeps(std::vector<FusedBlock> pre_blocks, int n, int L, bool 
 
 // Launches of a partition's sweeps with blocks of <= kt qubits. `dry` makes
 // the cost-model features only (no matrices, nothing in the plan's blob) and
-// leaves out the standalone blocks, which every candidate shares.
+// leaves out the standalone blocks, which every candidate shares. The
+// layouts are laid out first (each sweep's store layout is the next one's
+// read layout); the sweeps' blocks and launches are then independent, and a
+// large plan builds them on host threads, each sweep into a plan of its own
+// whose blob is appended in sweep order afterwards.
 void emit_xpartition(Plan& plan, XPartition& part, const std::vector<Op>& ops, int prec, int n, int L, int kt,
                      const MergeScorer& scorer, bool persist, int threads, bool dry) {
   constexpr size_t kTileBlobMax = 32768, kTileBlocksMax = kXTileBlocksMax;
   std::vector<FusedBlock>& pre = part.pre;
   const std::vector<XGroup>& groups = part.groups;
-  // Layouts and launches, one run of sweeps at a time.
+  struct Job {
+    size_t s;
+    std::vector<uint32_t> order, cur_pos, next_pos;
+    Plan out;
+  };
   std::vector<uint32_t> ident(64);
   for (uint32_t q = 0; q < 64; ++q) ident[q] = q;
-  size_t gi = 0;
+  size_t nsweeps = 0;
+  for (const XGroup& g : groups) nsweeps += g.standalone < 0;
+  std::vector<Job> jobs(nsweeps);
+  size_t gi = 0, ji = 0;
   while (gi < groups.size()) {
     if (groups[gi].standalone >= 0) {
-      if (!dry) emit_block(plan, pre[groups[gi].standalone], ops, prec, n);
       ++gi;
       continue;
     }
@@ -1715,32 +1725,76 @@ void emit_xpartition(Plan& plan, XPartition& part, const std::vector<Op>& ops, i
         for (int q = 0; q < n; ++q)
           if (!((nq >> q) & 1)) next_pos[q] = p++;
       }
-      // The sweep's ops fused again into blocks of <= kt qubits, split where
-      // the blob would exceed its LDS budget; only the last launch changes
-      // the layout.
-      std::vector<int> opsorder;
-      for (int b : g.blocks) opsorder.insert(opsorder.end(), pre[b].ops.begin(), pre[b].ops.end());
-      std::vector<FusedBlock> blocks = greedy_fuse_list(ops, opsorder, std::max(1, std::min(kt, 4)), scorer);
-      size_t i = 0;
-      do {
-        size_t blob = 0, j = i;
-        while (j < blocks.size() && j - i < kTileBlocksMax) {
-          const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
-          if (blob + bb > kTileBlobMax) break;
-          blob += bb;
-          ++j;
-        }
-        std::vector<FusedBlock*> bp;
-        for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
-        const bool last = j >= blocks.size();
-        emit_xlaunch(plan, bp, order.data(), L, cur_pos.data(), last ? next_pos.data() : cur_pos.data(), n, ops,
-                     persist, threads, dry);
-        i = j;
-      } while (i < blocks.size());
-      cur_pos = next_pos;
-      order = next_order;
+      Job& jb = jobs[ji++];
+      jb.s = s;
+      jb.order = order;
+      jb.cur_pos = cur_pos;
+      jb.next_pos = next_pos;
+      cur_pos = std::move(next_pos);
+      order = std::move(next_order);
     }
     gi = gj;
+  }
+  // One sweep: its ops fused again into blocks of <= kt qubits, split into
+  // launches where the blob would exceed its LDS budget; only the last
+  // launch changes the layout.
+  auto build = [&](Job& jb) {
+    std::vector<int> opsorder;
+    for (int b : groups[jb.s].blocks) opsorder.insert(opsorder.end(), pre[b].ops.begin(), pre[b].ops.end());
+    std::vector<FusedBlock> blocks = greedy_fuse_list(ops, opsorder, std::max(1, std::min(kt, 4)), scorer);
+    size_t i = 0;
+    do {
+      size_t blob = 0, j = i;
+      while (j < blocks.size() && j - i < kTileBlocksMax) {
+        const size_t bb = sizeof(k::TileBlock) + tile_matrix_bytes((int)blocks[j].qubits.size());
+        if (blob + bb > kTileBlobMax) break;
+        blob += bb;
+        ++j;
+      }
+      std::vector<FusedBlock*> bp;
+      for (size_t b = i; b < j; ++b) bp.push_back(&blocks[b]);
+      const bool last = j >= blocks.size();
+      emit_xlaunch(jb.out, bp, jb.order.data(), L, jb.cur_pos.data(), last ? jb.next_pos.data() : jb.cur_pos.data(),
+                   n, ops, persist, threads, dry);
+      i = j;
+    } while (i < blocks.size());
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t nth = !dry && n >= 24 && jobs.size() >= 8 ? std::min<size_t>({hw ? (size_t)hw : 1, 8, jobs.size()}) : 1;
+  if (nth <= 1) {
+    for (Job& jb : jobs) build(jb);
+  } else {
+    std::atomic<size_t> next{0};
+    std::vector<std::exception_ptr> errs(nth);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nth; ++t)
+      th.emplace_back([&, t] {
+        try {
+          for (size_t x; (x = next++) < jobs.size();) build(jobs[x]);
+        } catch (...) {
+          errs[t] = std::current_exception();
+        }
+      });
+    for (std::thread& t : th) t.join();
+    for (const std::exception_ptr& e : errs)
+      if (e) std::rethrow_exception(e);
+  }
+  // Into the plan in group order: standalone blocks emitted here, each
+  // sweep's launches appended with their blob offsets moved by where the
+  // sweep's blob lands (both 256-byte aligned).
+  ji = 0;
+  for (const XGroup& g : groups) {
+    if (g.standalone >= 0) {
+      if (!dry) emit_block(plan, pre[g.standalone], ops, prec, n);
+      continue;
+    }
+    Job& jb = jobs[ji++];
+    const size_t base = jb.out.blob.empty() ? 0 : plan.push(jb.out.blob.data(), jb.out.blob.size());
+    for (DevOp& d : jb.out.ops) {
+      d.tile_off += base;
+      plan.ops.push_back(std::move(d));
+    }
+    plan.uses_alt = plan.uses_alt || jb.out.uses_alt;
   }
 }
 
@@ -2138,25 +2192,12 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
           std::vector<std::pair<int, int>> cands;
           for (int pc = 0; pc < 2; ++pc)
             for (int kt = std::min(2, def_k); kt <= def_k; ++kt) cands.push_back({pc, kt});
-          // While nothing has been emitted yet, every candidate is emitted in
-          // full (matrices included) on its own thread, into its own plan
-          // and from its own copy of the partition, and the chosen one
-          // becomes the plan: no second emission after the choice. Later
-          // unitary runs emit dry candidates (features only, sharing the
-          // partition) and then the chosen one. Both give the same
-          // features, so the same choice.
-          const bool whole = plan.ops.empty() && plan.blob.empty();
-          std::unique_ptr<Plan> full[2][5];
+          // Dry emissions (features only; candidates share the partition),
+          // then the chosen one emitted in full.
           run_all(cands, [&](int pc, int kt) {
             Plan cand;
-            if (whole) {
-              XPartition mine = parts[pc];
-              emit_xpartition(cand, mine, ops, prec, n, tile, kt, nullptr, persist, threads, false);
-            } else {
-              emit_xpartition(cand, parts[pc], ops, prec, n, tile, kt, nullptr, persist, threads, true);
-            }
+            emit_xpartition(cand, parts[pc], ops, prec, n, tile, kt, nullptr, persist, threads, true);
             for (const DevOp& d : cand.ops) pred[pc][kt] += predicted_ms(d, n);
-            if (whole) full[pc][kt] = std::make_unique<Plan>(std::move(cand));
           });
           double best = pred[def_pin ? 1 : 0][def_k];
           for (int pc = 0; pc < 2; ++pc)
@@ -2167,15 +2208,8 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
                 best_pin = pc == 1;
               }
           plan.learned_choices.push_back((best_pin ? 100 : 0) + best_k);
-          if (whole) {
-            Plan& c = *full[best_pin ? 1 : 0][best_k];
-            plan.ops = std::move(c.ops);
-            plan.blob = std::move(c.blob);
-            plan.uses_alt = plan.uses_alt || c.uses_alt;
-          } else {
-            emit_xpartition(plan, parts[best_pin ? 1 : 0], ops, prec, n, tile, best_k, nullptr, persist, threads,
-                            false);
-          }
+          emit_xpartition(plan, parts[best_pin ? 1 : 0], ops, prec, n, tile, best_k, nullptr, persist, threads,
+                          false);
         } else {
           emit_xsweeps(plan, ops, i, j, prec, n, tile, best_k, nullptr, persist, threads, best_pin);
         }
@@ -3108,7 +3142,13 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
   std::ostringstream js;
   js << "{\"num_qubits\": " << p.n << ", \"num_cbits\": " << p.nc << ", \"compiled_ops\": " << p.ops.size()
      << ", \"prefix_ops\": " << pe << ", \"gates\": " << count_gates(p.c.instructions)
-     << ", \"blob_bytes\": " << plan.blob.size() << ", \"learned_choices\": [";
+     << ", \"blob_bytes\": " << plan.blob.size() << ", \"blob_hash\": \"" << [&] {
+          uint64_t h = 1469598103934665603ull;  // FNV-1a
+          for (uint8_t c : plan.blob) h = (h ^ c) * 1099511628211ull;
+          std::ostringstream hs;
+          hs << std::hex << h;
+          return hs.str();
+        }() << "\", \"learned_choices\": [";
   for (size_t i = 0; i < plan.learned_choices.size(); ++i)
     js << (i ? ", " : "") << "{\"pinned\": " << (plan.learned_choices[i] >= 100 ? "true" : "false")
        << ", \"k\": " << plan.learned_choices[i] % 100 << "}";

@@ -185,7 +185,9 @@ const char* dq_last_error(void);
 /* ---- no reference counterpart: host-only inspection and measurement ---- */
 
 /* Parse, validate, compile and fuse without touching the GPU; writes a JSON
- * description of the launch plan. out_of_place != 0 describes the plan of a
+ * description of the launch plan (launches, blob size and FNV-1a hash of the
+ * plan's matrix blob, learned choices; per out-of-place sweep the zero-start
+ * read_fraction and work_fraction). out_of_place != 0 describes the plan of a
  * call whose budget holds a second state (complex64 fused sweeps may then run
  * out of place, see dq_set_sweep_mode). Free it with dq_string_free. */
 int dq_plan_json(const char* json, size_t len, int32_t precision, int32_t fusion, int32_t max_fused,

@@ -714,6 +714,34 @@ def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k, sweep_pinn
     assert ops == list(range(plan["compiled_ops"]))
 
 
+def test_threaded_sweep_emission_is_deterministic():
+    # Plans of 24 qubits and up build their sweeps' blocks on host threads and
+    # append them in sweep order: the launches and the matrix blob (hash) are
+    # the same on every call, and the learned plan is the greedy plan of the
+    # width it chose, blob included.
+    from dqsim import planner
+    c = workloads.random_su2(n=26, depth=24, seed=8)
+    runs = [_core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=3, tile_qubits=12,
+                            out_of_place=True) for _ in range(4)]
+    assert sum(1 for x in runs[0]["launches"] if x.get("out_of_place")) >= 8
+    assert all(r == runs[0] for r in runs[1:])
+    assert runs[0]["blob_bytes"] > 0 and len(runs[0]["blob_hash"]) > 0
+    planner.install(planner.load_default())
+    try:
+        lp = _core.plan_json(c, precision="complex64", fusion="learned", max_fused_qubits=5, tile_qubits=12,
+                             out_of_place=True)
+    finally:
+        planner.install(None)
+    (choice,) = lp["learned_choices"]
+    prev = _core.set_sweep_pinned(choice["pinned"])
+    try:
+        gp = _core.plan_json(c, precision="complex64", fusion="greedy", max_fused_qubits=choice["k"],
+                             tile_qubits=12, out_of_place=True)
+    finally:
+        _core.set_sweep_pinned(prev)
+    assert gp["launches"] == lp["launches"] and gp["blob_hash"] == lp["blob_hash"]
+
+
 def test_thirteen_qubit_tiles_are_out_of_place_only():
     # 13-qubit tiles fill the 64 KiB of LDS: out-of-place sweeps take them
     # (512 threads, one workgroup per tile), in-place sweeps of the same
