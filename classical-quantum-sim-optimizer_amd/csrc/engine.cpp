@@ -3692,7 +3692,7 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
                      "the cause is not identified yet); the single-launch kernel (lds_qubits) replaces it");
   if (opt.lds_qubits < 0 || opt.lds_qubits > bk::kBatchLdsMax)
     throw ValueErr("lds_qubits must be 0 (off) or 1.." + std::to_string(bk::kBatchLdsMax));
-  if (opt.workers < 0 || opt.workers > 16) throw ValueErr("workers must be 0 (tuned) or 1..16");
+  if (opt.workers < -1 || opt.workers > 16) throw ValueErr("workers must be 0 (4), -1 (tuned) or 1..16");
   if (opt.max_fused < 0 || opt.max_fused > 4) throw ValueErr("batched fusion width must be 0 (tuned) or 1..4");
   if (opt.tile != -1 && opt.tile != 0 && (opt.tile < kTileMin || opt.tile > kTileMax))
     throw ValueErr("tile_qubits must be -1 (tuned), 0 or in 10..12");
@@ -3752,14 +3752,17 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
   }
   const size_t npool = pool_set.size();
 
-  int workers = opt.workers ? opt.workers : 4;
+  int workers = opt.workers > 0 ? opt.workers : 4;
   int fused = opt.max_fused ? opt.max_fused : 3;
-  if (opt.autotune && npool > 0 && (!opt.workers || !opt.max_fused || tune_tile)) {
+  if (opt.autotune && npool > 0 && (opt.workers < 0 || !opt.max_fused || tune_tile)) {
     // Calibration subset: evenly spaced circuits of the pool's cost order, run
     // once untimed (plans, arenas and code objects warm), then timed per
-    // candidate: (workers, fusion width) at the default tile, then the tile
-    // shape (LDS tile sweeps of 10, 11 or 12 qubits, or none) at the chosen
-    // pair.
+    // candidate: the fusion width at the default tile, then the tile shape
+    // (LDS tile sweeps of 10, 11 or 12 qubits, or none) at the chosen width.
+    // The worker count is tuned only when asked for (workers = -1): on a
+    // 96-circuit subset the start-up of each worker weighs more than in the
+    // whole call, and the subset's pick (2 workers) ran 4,096 circuits 1.34x
+    // slower than the default 4.
     std::vector<uint32_t> sub;
     const size_t want = std::min<size_t>(npool, 96);
     for (size_t i = 0; i < want; ++i) sub.push_back(pool_set[(i * npool) / want]);
@@ -3769,7 +3772,7 @@ std::unique_ptr<BatchOutcome> batch_run(const std::vector<std::pair<const char*,
       return "workers=" + std::to_string(w) + ",k<=" + std::to_string(f) + ",tile=" + std::to_string(t);
     };
     for (int w : {2, 4, 8}) {
-      if (opt.workers && w != opt.workers) continue;
+      if (opt.workers > 0 ? w != opt.workers : opt.workers == 0 && w != workers) continue;
       for (int f = 2; f <= 4; ++f) {
         if (opt.max_fused && f != opt.max_fused) continue;
         const double ms = run_stream_pool(su, sub, w, f, nullptr);

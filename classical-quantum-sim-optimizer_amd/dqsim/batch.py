@@ -89,10 +89,12 @@ def run_batch(circuits: Sequence, shots=1024, seeds: Optional[Sequence[int]] = N
               precision: str = "complex64", keep_states: bool = False, persistent: bool = False,
               lds_qubits: int = 12) -> BatchResult:
     """Run circuits as one batch. ``shots`` is an int or a list; ``seeds``
-    defaults to base_seed + index. ``workers`` / ``max_fused_qubits`` of 0 and
+    defaults to base_seed + index. ``max_fused_qubits`` of 0 and
     ``tile_qubits`` of None (LDS tile sweeps of 10, 11 or 12 qubits, or none)
     are chosen by timing candidates on a sample when ``autotune`` is set; an
-    untuned tile is 12. Complex64 circuits of up to ``lds_qubits`` qubits
+    untuned tile is 12. ``workers`` of 0 runs 4 host threads (each with its
+    own HIP stream), -1 times 2, 4 and 8 on the sample as well (a small
+    sample favours fewer workers than the whole call does). Complex64 circuits of up to ``lds_qubits`` qubits
     (0..12; 0 = none) that measure every qubit run in the single launch."""
     if precision not in _PREC:
         raise ValueError("precision must be 'complex64' or 'complex128'")

@@ -320,9 +320,10 @@ int dq_sv_evolve_timed(dq_sv* sim, const char* json, size_t len, int32_t repeats
  * such workloads as a loop of simulate_shots calls) ---- */
 typedef struct dq_batch dq_batch;
 typedef struct dq_batch_opts {
-  int32_t workers;          /* host threads, one HIP stream each: 1..16; 0 = tuned */
+  int32_t workers;          /* host threads, one HIP stream each: 1..16; 0 = 4;
+                               -1 = tuned with autotune (2, 4 or 8) */
   int32_t max_fused_qubits; /* 1..4; 0 = tuned */
-  int32_t autotune;         /* time candidate (workers, width) pairs on a sample */
+  int32_t autotune;         /* time candidate settings on a 96-circuit sample */
   int32_t tile_qubits;      /* LDS tile sweeps of each circuit's plan: 0 = off, 10..12;
                                -1 = tuned with autotune (10, 11, 12 or off), else 12 */
   int32_t precision;        /* DQ_PREC_C64 or DQ_PREC_C128 */

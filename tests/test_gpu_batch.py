@@ -93,13 +93,19 @@ def test_batch_zero_shots_and_unmeasured_circuit():
     assert res.counts(2) == {"": 40}
 
 
-def test_batch_autotune_and_latency():
+@pytest.mark.parametrize("workers", [0, -1])
+def test_batch_autotune_and_latency(workers):
     cs = circuits(40, 4, 14, dmin=4, dmax=12, base=9)
-    res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True, lds_qubits=0)
-    print("tuned", res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
-    assert res.workers in (2, 4, 8) and 2 <= res.max_fused_qubits <= 4 and res.tile_qubits in (0, 10, 11, 12)
-    # 9 (workers, width) pairs at the default tile, then 3 more tile shapes.
-    assert len(res.tuning) == 12 and all(ms > 0 for ms in res.tuning.values())
+    res = batch.run_batch(cs, shots=64, base_seed=3, autotune=True, lds_qubits=0, workers=workers)
+    print("tuned", workers, res.workers, res.max_fused_qubits, res.wall_ms, flush=True)
+    assert 2 <= res.max_fused_qubits <= 4 and res.tile_qubits in (0, 10, 11, 12)
+    if workers == 0:
+        # 4 workers; 3 widths at the default tile, then 3 more tile shapes.
+        assert res.workers == 4 and len(res.tuning) == 6
+    else:
+        # 9 (workers, width) pairs at the default tile, then 3 more tile shapes.
+        assert res.workers in (2, 4, 8) and len(res.tuning) == 12
+    assert all(ms > 0 for ms in res.tuning.values())
     assert f"workers={res.workers},k<={res.max_fused_qubits},tile={res.tile_qubits}" in res.tuning
     assert {int(k.rsplit("=", 1)[1]) for k in res.tuning} == {0, 10, 11, 12}
     assert res.wall_ms > 0 and len(res.latency_us) == 40 and (res.latency_us > 0).all()

@@ -339,6 +339,8 @@ def test_batched_runner_validates_before_touching_the_device():
         batch.run_batch([c], shots=8, persistent=True)
     with pytest.raises(ValueError, match="workers"):
         batch.run_batch([c], shots=8, workers=17)
+    with pytest.raises(ValueError, match="workers"):
+        batch.run_batch([c], shots=8, workers=-2)
     with pytest.raises(ValueError, match="fusion width"):
         batch.run_batch([c], shots=8, max_fused_qubits=5)
     with pytest.raises(ValueError, match="tile_qubits"):
