@@ -1399,10 +1399,12 @@ void xsweep_features(DevOp& d, const std::vector<FusedBlock*>& blocks, const uin
   f[15] = n4 / 16.f;
 }
 
-void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* local, int L, const uint32_t* in_pos,
-                  const uint32_t* out_pos, int n, const std::vector<Op>& ops, bool persist, int threads,
-                  bool dry = false) {
-  DevOp d;
+// The layout of an out-of-place sweep in d.xargs: tile local bit j holds
+// qubit local[j], read from position in_pos[local[j]] and stored at
+// out_pos[local[j]]; the tile-index bits are the source positions outside
+// the tile (d.xtile_q: their qubits), stored at their out_pos.
+void xsweep_layout(DevOp& d, const uint32_t* local, int L, const uint32_t* in_pos, const uint32_t* out_pos, int n,
+                   bool persist, int threads) {
   d.kind = D_XSWEEP;
   d.category = 1;
   d.gates = 0;
@@ -1453,6 +1455,14 @@ void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* 
   d.touched.assign(local, local + L);
   std::sort(d.touched.begin(), d.touched.end());
   d.tile_lq.assign(local, local + L);
+}
+
+void emit_xlaunch(Plan& plan, std::vector<FusedBlock*>& blocks, const uint32_t* local, int L, const uint32_t* in_pos,
+                  const uint32_t* out_pos, int n, const std::vector<Op>& ops, bool persist, int threads,
+                  bool dry = false) {
+  DevOp d;
+  xsweep_layout(d, local, L, in_pos, out_pos, n, persist, threads);
+  k::XSweepArgs& a = d.xargs;
   if (dry) {
     // Cost-model features only: no matrices, nothing in the plan's blob.
     size_t bytes = 0;
@@ -1833,19 +1843,12 @@ size_t exact_op_bytes(const Op& op) {
   return sizeof(k::TileOp) + ((data + 15) & ~(size_t)15);
 }
 
-void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, const TileMap& tm) {
-  DevOp d;
-  d.kind = D_TILE;
-  d.exact = true;
-  d.category = 1;
-  d.gates = 0;
-  const int L = tm.L;
-  std::vector<int> local(64, -1);
-  for (int b = 0; b < L; ++b) local[tm.cq[b]] = b;  // LDS order (after the lane swaps)
-  d.touched.assign(tm.lq, tm.lq + L);
-  std::sort(d.touched.begin(), d.touched.end());
-  d.tile_lq.assign(tm.lq, tm.lq + L);
-  tm.to_args(d.targs);
+// The chains and op records of the exact ops [i, j) on a tile whose local
+// bit of qubit q is local[q], as k_tile128 and k_xsweep128 read them: adds
+// the ops to d's gate count, sources and per-op qubit lists and returns the
+// blob (*nchains chains, *nfixed of them on compiled signatures).
+std::vector<uint8_t> exact_tile_blob(Plan& plan, DevOp& d, const std::vector<Op>& ops, size_t i, size_t j,
+                                     const std::vector<int>& local, uint32_t* nchains, int* nfixed) {
   auto lmask = [&](uint64_t g) {
     uint32_t m = 0;
     for (uint32_t q = 0; q < 64; ++q)
@@ -2034,10 +2037,31 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, co
     std::memcpy(blob.data() + off, dd.data(), dd.size());
     off += dd.size();
   }
+  *nchains = (uint32_t)chains.size();
+  *nfixed = fixed_chains;
+  return blob;
+}
+
+void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, const TileMap& tm) {
+  DevOp d;
+  d.kind = D_TILE;
+  d.exact = true;
+  d.category = 1;
+  d.gates = 0;
+  const int L = tm.L;
+  std::vector<int> local(64, -1);
+  for (int b = 0; b < L; ++b) local[tm.cq[b]] = b;  // LDS order (after the lane swaps)
+  d.touched.assign(tm.lq, tm.lq + L);
+  std::sort(d.touched.begin(), d.touched.end());
+  d.tile_lq.assign(tm.lq, tm.lq + L);
+  tm.to_args(d.targs);
+  uint32_t nchains = 0;
+  int nfixed = 0;
+  const std::vector<uint8_t> blob = exact_tile_blob(plan, d, ops, i, j, local, &nchains, &nfixed);
   d.targs.L = (uint32_t)L;
-  d.targs.nblocks = (uint32_t)chains.size();
-  d.chains = (int)chains.size();
-  d.fixed_chains = fixed_chains;
+  d.targs.nblocks = nchains;
+  d.chains = (int)nchains;
+  d.fixed_chains = nfixed;
   d.targs.blob16 = (uint32_t)(blob.size() / 16);
   d.tile_off = plan.push(blob.data(), blob.size());
   d.kq = L;
@@ -2045,7 +2069,7 @@ void emit_tile128(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, co
   d.feat[4] = d.gates / 32.f;
   d.feat[12] = 1.f;
   d.feat[13] = L / 12.f;
-  d.feat[14] = tops.size() / 16.f;
+  d.feat[14] = d.tile_blocks.size() / 16.f;
   plan.ops.push_back(std::move(d));
 }
 
@@ -2113,6 +2137,150 @@ void emit_exact_sweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, siz
   restore();
 }
 
+// One exact out-of-place sweep: ops [i, j) in program order on the tile
+// whose local bit b holds qubit local[b], read in layout in_pos and stored in
+// layout out_pos.
+void emit_exact_xlaunch(Plan& plan, const std::vector<Op>& ops, size_t i, size_t j, const uint32_t* local, int L,
+                        const uint32_t* in_pos, const uint32_t* out_pos, int n) {
+  DevOp d;
+  xsweep_layout(d, local, L, in_pos, out_pos, n, false, 256);
+  d.exact = true;
+  std::vector<int> lpos(64, -1);
+  for (int b = 0; b < L; ++b) lpos[local[b]] = b;
+  uint32_t nchains = 0;
+  int nfixed = 0;
+  const std::vector<uint8_t> blob = exact_tile_blob(plan, d, ops, i, j, lpos, &nchains, &nfixed);
+  d.xargs.nblocks = nchains;
+  d.xargs.blob16 = (uint32_t)(blob.size() / 16);
+  d.chains = (int)nchains;
+  d.fixed_chains = nfixed;
+  d.tile_off = plan.push(blob.data(), blob.size());
+  d.kq = L;
+  d.low = 0;
+  d.feat[4] = d.gates / 32.f;
+  d.feat[12] = 1.f;
+  d.feat[13] = L / 12.f;
+  d.feat[14] = d.tile_blocks.size() / 16.f;
+  plan.uses_alt = true;
+  plan.ops.push_back(std::move(d));
+}
+
+// Exact complex128 sweeps out of place (a second state buffer): runs of
+// consecutive tileable compiled ops, in program order as in
+// emit_exact_sweeps (each op keeps its per-op kernel's arithmetic), on tiles
+// of any L qubits. Each sweep reads its tile as one contiguous run (a run's
+// first sweep gathers it from the identity layout) and stores it in the
+// layout the next sweep reads; a run's last sweep stores the identity
+// layout. Stores move runs of at least 8 amplitudes (128 bytes): consecutive
+// tiles share at least 3 qubits, and a run's first and last tiles hold
+// qubits 0..2. Ops on more than 5 qubits run as their own launches (with the
+// anchor and the shared qubits they would not fit a tile).
+void emit_exact_xsweeps(Plan& plan, const std::vector<Op>& ops, size_t begin, size_t end, int n, int L) {
+  constexpr size_t kBlobMax = 32768, kOpsMax = 128, kWindow = 512;
+  constexpr int kShared = 3;
+  const uint64_t anchor = 7;
+  auto tileable = [&](size_t o) { return exact_tileable(ops[o]) && op_qubits(ops[o]).size() <= 5; };
+  auto plain = [&](size_t o) {
+    plan.ops.push_back(lower(ops[o], plan, 16));
+    plan.ops.back().touched = op_qubits(ops[o]);
+    block_features(plan.ops.back().touched, {(int)o}, ops, n, plan.ops.back().feat);
+    plan.ops.back().src = {(int)o};
+  };
+  std::vector<uint32_t> ident(64);
+  for (uint32_t q = 0; q < 64; ++q) ident[q] = q;
+  size_t i = begin;
+  while (i < end) {
+    if (!tileable(i)) {
+      plain(i++);
+      continue;
+    }
+    size_t e = i;
+    while (e < end && tileable(e)) ++e;
+    // Sweeps of the run [i, e): op ranges and tiles.
+    struct Sweep {
+      size_t a, b;
+      uint64_t q;
+    };
+    std::vector<Sweep> sw;
+    uint64_t prev = 0;
+    std::vector<uint32_t> prev_order;
+    for (size_t a = i; a < e;) {
+      const uint64_t base = prev ? 0 : anchor;
+      auto fits = [&](uint64_t m, bool last) {
+        m |= base | (last ? anchor : 0);
+        const int need = prev ? std::max(0, kShared - __builtin_popcountll(m & prev)) : 0;
+        return __builtin_popcountll(m) + need <= L;
+      };
+      uint64_t m = qubit_mask(op_qubits(ops[a]));
+      size_t blob = exact_op_bytes(ops[a]), b = a + 1;
+      if (!fits(m, b == e)) throw RuntimeErr("internal: exact out-of-place sweep tile");
+      while (b < e && b - a < kOpsMax) {
+        const uint64_t mb = qubit_mask(op_qubits(ops[b]));
+        const size_t bb = exact_op_bytes(ops[b]);
+        if (blob + bb > kBlobMax || !fits(m | mb, b + 1 == e)) break;
+        m |= mb;
+        blob += bb;
+        ++b;
+      }
+      m |= base | (b == e ? anchor : 0);
+      // Shared qubits: the previous tile's lowest positions first, then its
+      // other qubits in order of next use; then the qubits the next ops use
+      // first.
+      std::vector<uint32_t> first(64, UINT32_MAX);
+      for (size_t o = b; o < e && o < b + kWindow; ++o)
+        for (uint32_t q : op_qubits(ops[o]))
+          if (first[q] == UINT32_MAX) first[q] = (uint32_t)(o - b);
+      std::vector<uint32_t> cand;
+      for (int q = 0; q < n; ++q) cand.push_back((uint32_t)q);
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t x, uint32_t y) { return first[x] < first[y]; });
+      if (prev) {
+        for (size_t t = 0; t < prev_order.size() && t < (size_t)kShared; ++t)
+          if (__builtin_popcountll(m & prev) < kShared) m |= 1ull << prev_order[t];
+        for (uint32_t q : cand)
+          if (__builtin_popcountll(m & prev) < kShared && ((prev >> q) & 1)) m |= 1ull << q;
+      }
+      for (uint32_t q : cand) {
+        if (__builtin_popcountll(m) >= L) break;
+        m |= 1ull << q;
+      }
+      if (__builtin_popcountll(m) != L || (prev && __builtin_popcountll(m & prev) < kShared))
+        throw RuntimeErr("internal: exact out-of-place sweep tile");
+      std::vector<uint32_t> order;
+      for (uint32_t q : prev_order)
+        if ((m >> q) & 1) order.push_back(q);
+      for (int q = 0; q < n; ++q)
+        if (((m >> q) & 1) && !((prev >> q) & 1)) order.push_back((uint32_t)q);
+      sw.push_back({a, b, m});
+      prev = m;
+      prev_order = std::move(order);
+      a = b;
+    }
+    // Layouts (as emit_xpartition lays them out) and launches.
+    std::vector<uint32_t> cur_pos = ident, order;
+    for (int q = 0; q < n; ++q)
+      if ((sw[0].q >> q) & 1) order.push_back((uint32_t)q);
+    for (size_t s = 0; s < sw.size(); ++s) {
+      std::vector<uint32_t> next_pos = ident, next_order;
+      if (s + 1 < sw.size()) {
+        const uint64_t nq = sw[s + 1].q;
+        for (uint32_t q : order)
+          if ((nq >> q) & 1) next_order.push_back(q);
+        for (int q = 0; q < n; ++q)
+          if (((nq >> q) & 1) && !((sw[s].q >> q) & 1)) next_order.push_back((uint32_t)q);
+        if ((int)next_order.size() != L) throw RuntimeErr("internal: exact out-of-place sweep tile size");
+        uint32_t p = 0;
+        for (uint32_t q : next_order) next_pos[q] = p++;
+        for (int q = 0; q < n; ++q)
+          if (!((nq >> q) & 1)) next_pos[q] = p++;
+      }
+      emit_exact_xlaunch(plan, ops, sw[s].a, sw[s].b, order.data(), L, cur_pos.data(), next_pos.data(), n);
+      cur_pos = std::move(next_pos);
+      order = std::move(next_order);
+    }
+    i = e;
+  }
+}
+
 // `permute`: complex64 fused sweeps may run out of place (emit_xsweeps) when
 // the process-wide sweep mode allows it; the caller then gives the executor
 // room for a second state buffer.
@@ -2127,7 +2295,8 @@ Plan build_plan(const std::vector<Op>& ops, size_t begin, size_t end, int prec, 
     if (prec == 16 && fusion == FUSE_NONE && tile >= kTileMin && n >= lc + 4 && ops[i].is_unitary()) {
       size_t j = i;
       while (j < end && ops[j].is_unitary()) ++j;
-      emit_exact_sweeps(plan, ops, i, j, n, lc);
+      if (permute) emit_exact_xsweeps(plan, ops, i, j, n, lc);
+      else emit_exact_sweeps(plan, ops, i, j, n, lc);
       i = j;
       continue;
     }
@@ -2455,7 +2624,8 @@ struct Exec {
         a.zero_mask = next_sweep < zero_masks.size() ? zero_masks[next_sweep] : 0;
         zero_source = false;
         ++next_sweep;
-        k::xsweep(sp, alt, stride, nb, n, a, cbase + d.tile_off, d.persist, st);
+        if (d.exact) k::xsweep128(sp, alt, stride, nb, n, a, cbase + d.tile_off, st);
+        else k::xsweep(sp, alt, stride, nb, n, a, cbase + d.tile_off, d.persist, st);
         std::swap(sp, alt);
         break;
       }
@@ -3216,7 +3386,8 @@ std::string plan_description(const char* json, size_t len, int prec, int fusion,
       };
       uint32_t store[kXTileMax] = {};
       for (uint32_t j = 0; j < a.L; ++j) store[a.wlocal[j]] = a.wpos[j];
-      js << ", \"exact\": false, \"out_of_place\": true, \"gather\": " << (a.gather ? "true" : "false")
+      js << ", \"exact\": " << (d.exact ? "true" : "false") << ", \"out_of_place\": true, \"gather\": "
+         << (a.gather ? "true" : "false")
          << ", \"persist\": " << (d.persist ? "true" : "false");
       list("local_qubits", d.tile_lq.data(), (uint32_t)d.tile_lq.size());
       list("load_positions", a.rpos, a.L);

@@ -989,9 +989,10 @@ __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock*
 // Store of a tile from LDS in output order: store index u = tid + TPB * i
 // reads local index lo_l | hi_l(i) and writes destination offset lo_w | hi_w(i)
 // (plus the tile's base), so each wave's stores cover runs of consecutive
-// addresses as long as the low destination positions are tile bits.
-template <int L, int NTH, bool SPLIT, bool ZERO = false>
-__device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const float2* tile, uint64_t wbase,
+// addresses as long as the low destination positions are tile bits. T is
+// float2 (complex64 tiles) or double2 (exact complex128 tiles, SPLIT false).
+template <int L, int NTH, bool SPLIT, bool ZERO = false, typename T = float2>
+__device__ __forceinline__ void xsweep_store(T* __restrict__ dst, const T* tile, uint64_t wbase,
                                              const XSweepArgs& a) {
   constexpr int PER = (1 << L) / NTH;
   constexpr int LB = NTH == 512 ? 9 : 8;  // thread-index bits
@@ -1024,7 +1025,8 @@ __device__ __forceinline__ void xsweep_store(float2* __restrict__ dst, const flo
         w |= hw[bit];
       }
     if constexpr (ZERO) {
-      dst[w] = make_float2(0.f, 0.f);
+      if constexpr (sizeof(T) == 16) dst[w] = make_double2(0.0, 0.0);
+      else dst[w] = make_float2(0.f, 0.f);
     } else if constexpr (SPLIT) {
       const float2 v = tile[l];
       dst[w] = make_float2(v.x * kSplitUnscale, v.y * kSplitUnscale);
@@ -1559,6 +1561,62 @@ __global__ __launch_bounds__(TPB, 3) void k_tile128(double2* __restrict__ s, uin
     }
     __syncthreads();
   }
+}
+
+// Exact complex128 sweep out of place: one workgroup per 2^L-amplitude tile,
+// read as one contiguous run (or gathered from positions rpos at a run's
+// first sweep), the tile's compiled ops applied in program order with
+// k_tile128's arithmetic (the same chains and per-op code), and the tile
+// stored through the next sweep's layout into the other state buffer. Zero
+// start as in k_xsweep: from_zero synthesises |0...0> (nothing is read);
+// tiles whose index meets zero_mask hold zeros and only store zeros.
+template <int L>
+__global__ __launch_bounds__(TPB, 3) void k_xsweep128(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   uint64_t stride, XSweepArgs a, const float4* __restrict__ blob) {
+  extern __shared__ float4 lds4[];
+  constexpr int PER = (1 << L) / TPB;
+  double2* tile = (double2*)lds4;
+  double2* db = dst + (uint64_t)blockIdx.y * stride;
+  const uint64_t wbase = scatter_bits(blockIdx.x, a.wtile, a.ntb);
+  if ((blockIdx.x & a.zero_mask) != 0) {
+    xsweep_store<L, TPB, false, true, double2>(db, tile, wbase, a);
+    return;
+  }
+  const unsigned char* meta = (const unsigned char*)(lds4 + (1 << L));
+  for (uint32_t i = threadIdx.x; i < a.blob16; i += TPB) lds4[(1 << L) + i] = blob[i];
+  const TileChain* chains = (const TileChain*)meta;
+  const TileOp* ops = (const TileOp*)(meta + sizeof(TileChain) * a.nblocks);
+  if (a.from_zero) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      tile[threadIdx.x + TPB * i] = make_double2(threadIdx.x == 0 && i == 0 ? 1.0 : 0.0, 0.0);
+  } else {
+    const double2* sb = src + (uint64_t)blockIdx.y * stride;
+    const XSweepRead<L, TPB> rd(a);
+    const uint64_t b0 = rd.base(blockIdx.x, a);
+    // Real and imaginary parts in separate register arrays (as in k_tile128).
+    double rx[PER], ry[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const double2 v = sb[b0 | rd.off(i)];
+      rx[i] = v.x;
+      ry[i] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[threadIdx.x + TPB * i] = make_double2(rx[i], ry[i]);
+  }
+  __syncthreads();  // the tile and the blob are in LDS
+  for (uint32_t ci = 0; ci < a.nblocks; ++ci) {
+    const TileChain& ch = chains[ci];
+    const uint32_t nq = uni(ch.nq);
+    if ((nq >> 8) && (nq & 0xFF) == 1) tile128_chain1_fixed<L>(tile, ch, ops, meta);
+    else if (nq >> 8) tile128_chain_sig<L>(tile, ch, nq >> 8, ops, meta);
+    else if (nq == 0) tile128_op<L>(tile, ops[uni(ch.first)], meta);
+    else if (nq == 1) tile128_chain<L, 1>(tile, ch, ops, meta);
+    else tile128_chain<L, 2>(tile, ch, ops, meta);
+    __syncthreads();
+  }
+  xsweep_store<L, TPB, false, false, double2>(db, tile, wbase, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -2321,6 +2379,21 @@ void tile_sweep_c128(void* s, uint64_t stride, uint32_t batch, int n, const Tile
     hipLaunchKernelGGL(k_tile128<10>, g, dim3(TPB), lds, st, (double2*)s, stride, ntiles, a, (const float4*)blob);
   else
     hipLaunchKernelGGL(k_tile128<11>, g, dim3(TPB), lds, st, (double2*)s, stride, ntiles, a, (const float4*)blob);
+}
+
+void xsweep128(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
+               const void* blob, hipStream_t st) {
+  if ((a.L != 10 && a.L != 11) || (int)a.L > n || a.ntb != (uint32_t)(n - (int)a.L) || a.threads != TPB)
+    throw std::runtime_error("xsweep128: 10- or 11-qubit tiles of 256 threads matching the state");
+  const size_t lds = ((size_t)16 << a.L) + (size_t)16 * a.blob16;
+  if (lds > 65536) throw std::runtime_error("xsweep128: tile and blob exceed 64 KiB of LDS");
+  dim3 g((uint32_t)(1ull << (n - (int)a.L)), batch);
+  if (a.L == 10)
+    hipLaunchKernelGGL(k_xsweep128<10>, g, dim3(TPB), lds, st, (const double2*)src, (double2*)dst, stride, a,
+                       (const float4*)blob);
+  else
+    hipLaunchKernelGGL(k_xsweep128<11>, g, dim3(TPB), lds, st, (const double2*)src, (double2*)dst, stride, a,
+                       (const float4*)blob);
 }
 
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st) {

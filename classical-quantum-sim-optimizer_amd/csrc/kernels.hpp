@@ -194,6 +194,11 @@ size_t xsweep_lds_bytes(const XSweepArgs& a);
 // otherwise one workgroup per tile.
 void xsweep(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
             const void* blob_dev, bool persist, hipStream_t st);
+// Exact complex128 out-of-place sweep (k_xsweep128): one 256-thread
+// workgroup per tile of 10 or 11 qubits; a.nblocks chains and the op records
+// in the blob as for tile_sweep_c128; split, kmax and persistence unused.
+void xsweep128(const void* src, void* dst, uint64_t stride, uint32_t batch, int n, const XSweepArgs& a,
+               const void* blob_dev, hipStream_t st);
 
 // ---- state set-up ----
 void init_zero(int prec, void* s, uint64_t stride, uint32_t batch, int n, hipStream_t st);

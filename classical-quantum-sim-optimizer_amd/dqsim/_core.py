@@ -592,8 +592,10 @@ def set_sweep_mode(mode: int) -> int:
     """Sweep mode of complex64 fused plans for this process: 0 in place; out
     of place with 1 persistent 256-thread workgroups, 2 one 256-thread
     workgroup per tile (default), 3 one 512-thread workgroup per tile, 4
-    persistent 512-thread workgroups. Out-of-place sweeps run only when a
-    call's budget holds a second state. Returns the previous mode."""
+    persistent 512-thread workgroups. Exact complex128 sweeps run out of
+    place (one 256-thread workgroup per tile) in every mode but 0. Out-of-
+    place sweeps run only when a call's budget holds a second state. Returns
+    the previous mode."""
     prev = int(_lib.dq_set_sweep_mode(int(mode)))
     if prev < 0:
         raise ValueError(f"sweep mode must be 0..4, got {mode}")

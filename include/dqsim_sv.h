@@ -263,7 +263,11 @@ int dq_set_sweep_relabel(int32_t on);
  * tile (default), 3 = one 512-thread workgroup per tile (tiles of 11-12
  * qubits; 256 threads otherwise), 4 = persistent 512-thread workgroups.
  * Results stay within complex64 rounding of each other (blocks may group
- * differently). Returns the previous mode; -1 for a mode outside 0..4. */
+ * differently). Exact complex128 sweeps (complex128 without fusion) run out
+ * of place, one 256-thread workgroup per tile, in every mode but 0, and stay
+ * bit-identical to one launch per op (consecutive tiles share 3 qubits, so
+ * stores move 128-byte runs). Returns the previous mode; -1 for a mode
+ * outside 0..4. */
 int dq_set_sweep_mode(int32_t mode);
 
 /* Process-wide tile rule of out-of-place sweeps (no counterpart in the

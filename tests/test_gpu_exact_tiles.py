@@ -1,7 +1,9 @@
 """Exact complex128 tile sweeps (the default for complex128 without fusion
 at 15+ qubits): consecutive compiled ops applied in LDS with the per-op
-kernels' arithmetic, in program order. Results must equal the oracle bit for
-bit, like one launch per op."""
+kernels' arithmetic, in program order, out of place between two state
+buffers when the budget holds both (the default budget does at these sizes),
+else in place. Results must equal the oracle bit for bit, like one launch
+per op, in both placements."""
 
 import numpy as np
 import pytest
@@ -17,13 +19,24 @@ pytestmark = pytest.mark.gpu
 
 
 def exact_state(insts, n, tile=12, seed=60):
+    """The state from out-of-place sweeps, checked equal (bit for bit up to
+    the sign of zeros) to the state from in-place sweeps."""
     sim = dqsim.StatevectorSimulator(seed=seed, tile_qubits=tile)
-    return sim.simulate(Circuit(n, insts)).statevector
+    got = sim.simulate(Circuit(n, insts)).statevector
+    prev = dqsim._core.set_sweep_mode(0)
+    try:
+        in_place = sim.simulate(Circuit(n, insts)).statevector
+    finally:
+        dqsim._core.set_sweep_mode(prev)
+    assert np.array_equal(got, in_place)
+    return got
 
 
 def has_exact_tiles(insts, n, cbits=0):
     plan = dqsim._core.plan_json(Circuit(n, insts, cbits=cbits))
-    return any(x["kind"] == "tile" for x in plan["launches"])
+    xplan = dqsim._core.plan_json(Circuit(n, insts, cbits=cbits), out_of_place=True)
+    return (any(x["kind"] == "tile" for x in plan["launches"])
+            and any(x.get("out_of_place") and x["exact"] for x in xplan["launches"]))
 
 
 def test_every_gate_kind_bit_exact_in_sweeps():

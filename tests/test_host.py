@@ -716,6 +716,59 @@ def test_out_of_place_sweeps_move_every_amplitude_home(kind, n, L, k, sweep_pinn
     assert ops == list(range(plan["compiled_ops"]))
 
 
+@pytest.mark.parametrize("kind,n,tile", [("su2", 16, 12), ("mixed", 17, 12), ("qft", 18, 10), ("su2", 20, 12),
+                                         ("brickwork", 19, 10)])
+def test_exact_out_of_place_sweeps_keep_program_order(kind, n, tile):
+    # complex128 without fusion and room for a second state: runs of
+    # consecutive ops (program order, nothing reordered) in out-of-place
+    # sweeps of L = min(tile, 11) qubits; every amplitude ends at its own
+    # index, each sweep's ops act on its tile, consecutive tiles share >= 3
+    # qubits, a run's first and last tiles hold qubits 0..2, and stores move
+    # runs of >= 8 amplitudes (128 bytes).
+    L = min(tile, 11)
+    c = {"su2": lambda: workloads.random_su2(n=n, depth=8, seed=n),
+         "mixed": lambda: workloads.mixed(n, 5, seed=n),
+         "qft": lambda: workloads.qft(n, seed=2, measure=False)[0],
+         "brickwork": lambda: workloads.brickwork(n=n, layers=10, seed=n, measure=False)}[kind]()
+    plan = _core.plan_json(c, precision="complex128", fusion="none", tile_qubits=tile, out_of_place=True)
+    launches = plan["launches"]
+    xs = [x for x in launches if x.get("out_of_place")]
+    assert xs and all(x["exact"] and x["k"] == L for x in xs)
+    order = [o for x in launches for o in x["ops"]]
+    assert order == list(range(plan["compiled_ops"]))  # program order, each op once
+    tags = np.arange(1 << n, dtype=np.uint64)
+    pos = list(range(n))
+    prev = None
+    for i, x in enumerate(launches):
+        if not x.get("out_of_place"):
+            assert pos == list(range(n))
+            prev = None
+            continue
+        lq = x["local_qubits"]
+        assert [pos[q] for q in lq] == x["load_positions"]
+        assert all(set(b["qubits"]) <= set(lq) for b in x["blocks"])
+        if prev is None:
+            assert {0, 1, 2} <= set(lq)
+        else:
+            assert len(set(prev) & set(lq)) >= 3
+        last = i + 1 == len(launches) or not launches[i + 1].get("out_of_place")
+        if last:
+            assert {0, 1, 2} <= set(lq)
+        assert set(range(3)) <= set(x["store_positions"])
+        tags = _xsweep_move(tags, x, n, L)
+        free = [q for q in sorted(range(n), key=lambda q: pos[q]) if q not in lq]
+        new = [None] * n
+        for j, q in enumerate(lq):
+            new[q] = x["store_positions"][j]
+        for kk, q in enumerate(free):
+            new[q] = x["tile_store_positions"][kk]
+        pos = new
+        prev = lq
+    assert pos == list(range(n))
+    assert np.array_equal(tags, np.arange(1 << n, dtype=np.uint64))
+    assert plan["zero_start"] == bool(launches[0].get("out_of_place"))
+
+
 def test_threaded_sweep_emission_is_deterministic():
     # Plans of 24 qubits and up build their sweeps' blocks on host threads and
     # append them in sweep order: the launches and the matrix blob (hash) are
