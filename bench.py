@@ -27,10 +27,10 @@ sys.path.insert(0, os.path.join(ROOT, "classical-quantum-sim-optimizer_amd"))
 sys.path.insert(0, ROOT)
 
 KIND_NAMES = {0: "one", 1: "x", 2: "swap", 3: "diag", 4: "dense", 5: "fused_valu", 6: "measure", 7: "reset",
-              8: "fused_mfma", 9: "tile", 10: "tile128", 11: "xsweep"}
+              8: "fused_mfma", 9: "tile", 10: "tile128", 11: "xsweep", 12: "xsweep128"}
 KERNEL_NAMES = {"one": "k_one", "x": "k_pairswap", "swap": "k_pairswap", "diag": "k_diag", "dense": "k_dense",
                 "fused_valu": "k_fused", "fused_mfma": "k_fused_mfma", "measure": "k_collapse", "reset": "k_collapse",
-                "tile": "k_tile", "tile128": "k_tile128", "xsweep": "k_xsweep"}
+                "tile": "k_tile", "tile128": "k_tile128", "xsweep": "k_xsweep", "xsweep128": "k_xsweep128"}
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_FP16_PEAK_TFLOPS = 2500.0  # dense FP16 matrix-core peak (split-precision blocks run there)

@@ -2663,7 +2663,7 @@ double op_bytes(const DevOp& d, int n, int prec) {
 
 int op_kind_code(const DevOp& d) {
   switch (d.kind) {
-    case D_XSWEEP: return 11;
+    case D_XSWEEP: return d.exact ? 12 : 11;
     case D_ONE: return 0;
     case D_X: return 1;
     case D_SWAP: return 2;

@@ -104,7 +104,8 @@ void dq_counts_free(dq_counts* t);
 typedef struct dq_op_timing {
   int32_t kind;  /* 0 one,1 x,2 swap,3 diag,4 dense,5 fused(VALU),6 measure,7 reset,8 fused(MFMA),
                     9 tile sweep (k = tile qubits, gates = ops applied),
-                    10 exact complex128 tile sweep, 11 out-of-place tile sweep */
+                    10 exact complex128 tile sweep, 11 out-of-place tile sweep,
+                    12 exact complex128 out-of-place tile sweep */
   int32_t k;     /* qubits touched */
   int32_t low;   /* lowest qubit touched */
   int32_t gates; /* compiled ops folded into the launch */
