@@ -159,7 +159,10 @@ def sweep_dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4), seed: i
     probability block_share one launch per block (width 2..5), with
     in_place_share in-place 12-qubit tile sweeps, otherwise out-of-place
     12-qubit sweeps under a tile rule (pinned or shared) and block width drawn
-    at random -- the choices the learned planner makes. Collection stops
+    at random -- the choices the learned planner makes. Sweeps that the zero
+    start shortens (a plan's first sweeps from |0...0>, whose recorded bytes
+    fall below one full read and write of the state) are left out: their
+    time depends on the start state, not on the features. Collection stops
     after time_limit_s seconds if given."""
     rng = np.random.default_rng(seed)
     X: List[List[float]] = []
@@ -187,8 +190,9 @@ def sweep_dataset(circuits: Iterable, widths: Sequence[int] = (2, 3, 4), seed: i
             info["out_of_place_plans"] += 1
         info["circuits"] += 1
         info["qubits"][str(n)] = info["qubits"].get(str(n), 0) + 1
+        full = 2.0 * 8.0 * 2.0 ** n  # one complex64 state read and written
         for r in rows:
-            if r["ms"] <= 0:
+            if r["ms"] <= 0 or (r["kind"] == 11 and r["bytes"] < full):
                 continue
             X.append(r["feat"])
             y.append(math.log(r["ms"] * 2.0 ** (24 - n)))
@@ -272,15 +276,53 @@ def mixed_size_circuits(count: int, seed: int, dmin: int = 20, dmax: int = 80,
         yield family_circuit(families[i % len(families)], n, d, seed * 1_000_003 + i)
 
 
-def train_and_save(out_dir: str, circuits: int = 4000, seed: int = 11, epochs: int = 60, time_limit_s: float = 600,
-                   log=print) -> Dict:
-    """Record engine timings, train the MLP on the GPU, and write
-    planner_weights.npy, planner_weights.json (how it was made) and
-    planner_dataset.npz (the samples) into out_dir."""
+def collect(out_dir: str, circuits: int = 4000, seed: int = 11, time_limit_s: float = 600, log=print) -> Dict:
+    """Record engine timings of seeded training circuits and write them to
+    out_dir/planner_dataset.npz (X, y) with planner_dataset.json (what was
+    recorded)."""
     os.makedirs(out_dir, exist_ok=True)
     X, y, info = sweep_dataset(mixed_size_circuits(circuits, seed), seed=seed, log=log, time_limit_s=time_limit_s)
     log(f"[planner] {info['circuits']} circuits, {len(y)} launches in {info['collect_s']:.0f} s")
     np.savez_compressed(os.path.join(out_dir, "planner_dataset.npz"), X=X, y=y)
+    info = dict(info, seed=seed, samples=int(len(y)))
+    with open(os.path.join(out_dir, "planner_dataset.json"), "w") as f:
+        json.dump(info, f, indent=1)
+    return info
+
+
+def train_and_save(out_dir: str, circuits: int = 4000, seed: int = 11, epochs: int = 60, time_limit_s: float = 600,
+                   log=print, datasets: Sequence[str] = ()) -> Dict:
+    """Record engine timings (unless circuits is 0), add the samples of
+    earlier recordings (`datasets`: planner_dataset.npz files, each with its
+    planner_dataset.json beside it), train the MLP on the GPU, and write
+    planner_weights.npy, planner_weights.json (how it was made) and
+    planner_dataset.npz (this run's samples) into out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+    parts_X, parts_y, runs = [], [], []
+    info = {"circuits": 0, "block_plans": 0, "in_place_plans": 0, "out_of_place_plans": 0, "qubits": {},
+            "collect_s": 0.0}
+    if circuits > 0:
+        X, y, info = sweep_dataset(mixed_size_circuits(circuits, seed), seed=seed, log=log,
+                                   time_limit_s=time_limit_s)
+        log(f"[planner] {info['circuits']} circuits, {len(y)} launches in {info['collect_s']:.0f} s")
+        np.savez_compressed(os.path.join(out_dir, "planner_dataset.npz"), X=X, y=y)
+        parts_X.append(X)
+        parts_y.append(y)
+        runs.append({"seed": seed, "circuits": info["circuits"], "samples": int(len(y))})
+    for path in datasets:
+        with np.load(path) as d:
+            parts_X.append(d["X"].astype(np.float32))
+            parts_y.append(d["y"].astype(np.float32))
+        side = os.path.join(os.path.dirname(path), "planner_dataset.json")
+        extra = json.load(open(side)) if os.path.exists(side) else {}
+        for key in ("circuits", "block_plans", "in_place_plans", "out_of_place_plans", "collect_s"):
+            info[key] = info.get(key, 0) + extra.get(key, 0)
+        for q, c in extra.get("qubits", {}).items():
+            info["qubits"][q] = info["qubits"].get(q, 0) + c
+        runs.append({"seed": extra.get("seed"), "circuits": extra.get("circuits"), "samples": int(len(parts_y[-1]))})
+    if not parts_y:
+        raise ValueError("no samples: record circuits or give datasets")
+    X, y = np.concatenate(parts_X), np.concatenate(parts_y)
     perm = np.random.default_rng(seed).permutation(len(y))
     cut = int(0.9 * len(y))
     tr, te = perm[:cut], perm[cut:]
@@ -289,7 +331,7 @@ def train_and_save(out_dir: str, circuits: int = 4000, seed: int = 11, epochs: i
     train_s = time.perf_counter() - t0
     pred = predict(w, X[te])
     meta = dict(info)
-    meta.update({"seed": seed, "size_mix": [list(m) for m in SIZE_MIX], "families": list(FAMILIES),
+    meta.update({"seed": seed, "recordings": runs, "size_mix": [list(m) for m in SIZE_MIX], "families": list(FAMILIES),
                  "depth_range": [20, 80], "train_samples": int(len(tr)), "test_samples": int(len(te)),
                  "epochs": epochs, "train_s": train_s, "final_train_loss": float(curve[-1]),
                  "test_mse": float(np.mean((pred - y[te]) ** 2)),
@@ -309,9 +351,16 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--time-limit", type=float, default=600, help="seconds of timing collection at most")
+    ap.add_argument("--collect-only", action="store_true", help="record timings into --out, do not train")
+    ap.add_argument("--datasets", nargs="*", default=[],
+                    help="planner_dataset.npz files of earlier recordings to train on as well")
     a = ap.parse_args(argv)
-    meta = train_and_save(a.out, a.circuits, a.seed, a.epochs, a.time_limit,
-                          log=lambda m: print(m, flush=True))
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.collect_only:
+        info = collect(a.out, a.circuits, a.seed, a.time_limit, log=log)
+        print(json.dumps({k: v for k, v in info.items() if k != "qubits"}), flush=True)
+        return 0
+    meta = train_and_save(a.out, a.circuits, a.seed, a.epochs, a.time_limit, log=log, datasets=a.datasets)
     print(json.dumps({k: v for k, v in meta.items() if k != "qubits"}), flush=True)
     return 0
 
