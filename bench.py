@@ -282,6 +282,9 @@ def main():
     ap.add_argument("--block-arith", type=int, default=None, choices=[0, 1],
                     help="3-qubit blocks of complex64 sweeps: 0 FP32 matrix cores, 1 split precision (FP16 parts, "
                          "FP32 accumulation) (default: the library's)")
+    ap.add_argument("--wave-runs", type=int, default=None, choices=[0, 1],
+                    help="256-thread out-of-place sweeps: 1 wave-owned block runs with a barrier after each run, "
+                         "0 a barrier after every block (default: the library's)")
     ap.add_argument("--no-zero-start", action="store_true",
                     help="initialise the state and read it in the first sweeps (default: plans that open with "
                          "out-of-place sweeps start from a synthesised |0...0> and skip reading all-zero tiles)")
@@ -335,6 +338,10 @@ def main():
         dqsim._core.set_block_arith(args.block_arith)
     block_arith = dqsim._core.set_block_arith(0)
     dqsim._core.set_block_arith(block_arith)
+    if args.wave_runs is not None:
+        dqsim._core.set_wave_runs(args.wave_runs)
+    wave_runs = dqsim._core.set_wave_runs(1)
+    dqsim._core.set_wave_runs(wave_runs)
     if args.no_zero_start:
         dqsim._core.set_zero_start(False)
     sweep_mode = dqsim._core.set_sweep_mode(1)
@@ -534,7 +541,7 @@ def main():
         "config": {"workload": f"random_su2 n={n} depth={depth} seed=1234, {args.precision}, {args.fusion} fusion "
                                f"k<={args.max_fused}, LDS tile {args.tile} qubits", "qubits": n, "depth": depth, "gates": gates,
                    "relabelled_sweeps": args.relabel, "sweep_mode": sweep_mode, "block_arith": block_arith,
-                   "zero_start": not args.no_zero_start,
+                   "wave_runs": wave_runs, "zero_start": not args.no_zero_start,
                    "tile_rule": rule, "block_width": kk, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -315,6 +315,7 @@ int dq_set_sweep_pinned(int32_t on) { return dq::set_sweep_pinned(on != 0) ? 1 :
 int dq_set_sweep_mode(int32_t mode) { return dq::set_sweep_mode(mode); }
 
 int dq_set_block_arith(int32_t mode) { return dq::set_block_arith(mode); }
+int dq_set_wave_runs(int32_t mode) { return dq::set_wave_runs(mode); }
 
 int dq_set_zero_start(int32_t on) { return dq::set_zero_start(on != 0) ? 1 : 0; }
 

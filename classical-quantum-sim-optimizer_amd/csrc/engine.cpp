@@ -283,6 +283,10 @@ int set_sweep_mode(int mode) { return mode < 0 || mode > 4 ? -1 : g_sweep_mode.e
 // amplitude error).
 std::atomic<int> g_block_arith{1};
 int set_block_arith(int mode) { return mode < 0 || mode > 1 ? -1 : g_block_arith.exchange(mode); }
+// Blocks of a tile grouped into wave-owned runs (1, the default) or a
+// barrier after every block (0); read by the 256-thread out-of-place sweeps.
+std::atomic<int> g_wave_runs{1};
+int set_wave_runs(int mode) { return mode < 0 || mode > 1 ? -1 : g_wave_runs.exchange(mode); }
 
 // Plans that open with out-of-place sweeps start from a synthesised |0...0>
 // and skip the reads of all-zero tiles (Exec::start_zero); off: initialise
@@ -1074,6 +1078,70 @@ struct TileBlob {
   uint32_t blob16 = 0, nblocks = 0, kmax = 0;
   bool split = false;  // some block runs in split precision (the tile is held scaled)
 };
+// Runs of a tile's blocks for the wave-owned group split of the 256-thread
+// out-of-place sweeps: consecutive blocks of at most 3 qubits share two wave
+// bits (pad[2] = w0 | w1 << 8, w0 < w1; pad[0] = the same two bits with each
+// block's bits below them removed) that no block of the run touches and that
+// lie above every block's lane bits (the 4 lowest bits outside a 3-qubit
+// block, the 6 lowest outside a 2-qubit one: the lanes then address LDS as
+// with the per-block split, bank for bank), the two highest such bits; only
+// the last block of a run has a barrier after it (pad[1] bit 0). A 4-qubit
+// block is a run of its own. With `runs` false every block is a run of one.
+// Other kernels ignore the fields.
+void wave_runs(std::vector<k::TileBlock>& tb, int L, bool runs) {
+  auto lane_top = [](const k::TileBlock& t) {  // highest lane bit of the block
+    uint32_t m = 0;
+    for (uint32_t q = 0; q < t.k; ++q) m |= 1u << t.loc[q];
+    int left = t.k == 3 ? 4 : 6, bit = 0;
+    for (;; ++bit)
+      if (!((m >> bit) & 1u) && --left == 0) return bit;
+  };
+  auto pick = [L](uint32_t used, int above, uint32_t* w) {  // two highest free bits above `above`
+    int c = 0;
+    for (int bit = L - 1; bit > above && c < 2; --bit)
+      if (!((used >> bit) & 1u)) w[c++] = (uint32_t)bit;
+    return c == 2;
+  };
+  size_t i = 0;
+  while (i < tb.size()) {
+    if (tb[i].k > 3) {
+      tb[i].pad[0] = tb[i].pad[2] = 0;
+      tb[i].pad[1] = 1;
+      ++i;
+      continue;
+    }
+    uint32_t used = 0, w[2] = {0, 0};
+    int above = -1;
+    size_t j = i;
+    while (j < tb.size() && tb[j].k <= 3) {
+      uint32_t u = used, cand[2];
+      for (uint32_t q = 0; q < tb[j].k; ++q) u |= 1u << tb[j].loc[q];
+      const int a = std::max(above, lane_top(tb[j]));
+      if (!pick(u, a, cand)) break;
+      used = u;
+      above = a;
+      w[0] = cand[0];
+      w[1] = cand[1];
+      ++j;
+      if (!runs) break;
+    }
+    if (j == i) throw std::logic_error("wave_runs: no two free tile bits above a block's lane bits");
+    const uint32_t w0 = w[1], w1 = w[0];
+    for (size_t m = i; m < j; ++m) {
+      k::TileBlock& t = tb[m];
+      uint32_t c0 = w0, c1 = w1;
+      for (uint32_t q = 0; q < t.k; ++q) {
+        c0 -= t.loc[q] < w0;
+        c1 -= t.loc[q] < w1;
+      }
+      t.pad[0] = c0 | c1 << 8;
+      t.pad[1] = m + 1 == j ? 1u : 0u;
+      t.pad[2] = w0 | w1 << 8;
+    }
+    i = j;
+  }
+}
+
 TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const uint32_t* cq, int L,
                    const std::vector<Op>& ops, int n, bool split) {
   TileBlob out;
@@ -1117,6 +1185,7 @@ TileBlob tile_blob(Plan& plan, DevOp& d, std::vector<FusedBlock*>& blocks, const
     d.tile_blocks.push_back(orig->qubits);
     d.tile_mfma.push_back((int)t.kind);
   }
+  wave_runs(tb, L, g_wave_runs.load() == 1);
   std::vector<uint8_t> blob(head + mats.size() * sizeof(float));
   std::memcpy(blob.data(), tb.data(), head);
   std::memcpy(blob.data() + head, mats.data(), mats.size() * sizeof(float));

@@ -161,6 +161,10 @@ int set_sweep_mode(int mode);
 // Arithmetic of 3-qubit blocks in complex64 tile sweeps (0 FP32 matrix cores,
 // 1 split precision); returns the previous mode, -1 for an invalid one.
 int set_block_arith(int mode);
+// Wave-owned block runs in 256-thread out-of-place sweeps on (1, default) or
+// a barrier after every block (0); returns the previous mode, -1 for an
+// invalid one.
+int set_wave_runs(int mode);
 // Zero-start of plans that open with out-of-place sweeps on or off
 // (process-wide); returns the previous setting.
 bool set_zero_start(bool on);

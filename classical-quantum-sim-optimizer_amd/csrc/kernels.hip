@@ -499,6 +499,53 @@ __device__ __forceinline__ void uniform_bits(const TileBlock& b, uint32_t* srt) 
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Groups of a block in one wave: local index = the lane's part | a
+// wave-uniform part, on disjoint bits. Without WB the wave bits are the two
+// group-index bits above the lane's (wave v takes every fourth row of
+// groups, a different set for every block). With WB (256-thread workgroups
+// of out-of-place sweeps) the block belongs to a run of blocks that leave two
+// tile bits w0 < w1 untouched, and wave v owns the local indices whose bits
+// w0, w1 equal v's two bits for every block of the run: consecutive blocks of
+// a run need no workgroup barrier, since each wave reads back only what it
+// wrote itself and a wave's LDS accesses complete in issue order. The
+// descriptor carries pad[2] = w0 | w1 << 8 and pad[0] = the same two bits
+// with the block's bits below each removed, so zeros inserted at pad[0]'s
+// positions and then at the block's bits land at the block's bits and w0, w1.
+// `lanes` groups are covered per step by the lane part (lane_grp < lanes).
+template <int K, bool WB, int NTH>
+struct GroupWalk {
+  uint32_t lane_part, mask, step, base, wpart;
+  __device__ __forceinline__ GroupWalk(const TileBlock& b, const uint32_t* srt, uint32_t lane_grp, uint32_t lanes) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    mask = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) mask |= 1u << srt[i];
+    if constexpr (WB) {
+      static_assert(NTH == 256, "wave-owned groups take two wave bits");
+      const uint32_t wl = __builtin_amdgcn_readfirstlane(b.pad[2]), wc = __builtin_amdgcn_readfirstlane(b.pad[0]);
+      const uint32_t c[2] = {wc & 255u, wc >> 8};
+      const uint32_t w0 = wl & 255u, w1 = wl >> 8;
+      lane_part = insert_zeros32(insert_zeros32(lane_grp, c, 2), srt, K);
+      step = insert_zeros32(insert_zeros32(lanes, c, 2), srt, K);
+      mask |= (1u << w0) | (1u << w1);
+      base = 0;
+      wpart = ((w & 1u) << w0) | ((w >> 1) << w1);
+    } else {
+      lane_part = insert_zeros32(lane_grp, srt, K);
+      step = insert_zeros32(lanes * (NTH / 64), srt, K);
+      base = insert_zeros32(w * lanes, srt, K);
+      wpart = 0;
+    }
+  }
+  // Wave-uniform part of the next group row (masked-carry stepping: with the
+  // skipped bits set, adding `step` carries over them).
+  __device__ __forceinline__ uint32_t next() {
+    const uint32_t g = __builtin_amdgcn_readfirstlane(base) | wpart;
+    base = ((base | mask) + step) & ~mask;
+    return g;
+  }
+};
+
 // k = 3 on v_mfma_f32_16x16x4f32 (A lane l: A[l%16][l/16]; B lane l:
 // B[l/16][l%16]; D lane l, register r: D[4(l/16)+r][l%16]). The embedding
 // e (16x16, row-major) is permuted so that
@@ -507,16 +554,17 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // so lane l (u = l/16) feeds the four K steps from inputs 2u and 2u+1 of
 // group l%16 and receives outputs 2u and 2u+1 of that group whole.
 // `e` holds the A fragments: e[64 s + l] = embedding[l % 16][4 s + l / 16].
-template <int L, int NTH = TPB>
+template <int L, int NTH = TPB, bool WB = false>
 __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock& b, const float* e) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int col = lane & 15, u = lane >> 4;
   float a[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) a[s] = e[64 * s + lane];
   uint32_t srt[3];
   uniform_bits<3>(b, srt);
-  const uint32_t lane_part = insert_zeros32(col, srt, 3);
+  GroupWalk<3, WB, NTH> gw(b, srt, col, 16);
+  const uint32_t lane_part = gw.lane_part;
   const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
   // 16-group tiles per wave, U of them per iteration with independent MFMA
   // chains interleaved.
@@ -524,12 +572,6 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
   constexpr int PERW = NT / (NTH / 64);   // L >= 10: at least 2 per wave
   constexpr int U = PERW >= 4 ? 4 : PERW;
   static_assert(NT % (NTH / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
-  // Masked-carry stepping: with the block's bits set, adding IZ(step) carries
-  // over them, so base = IZ(mt * 16) advances by 3 scalar operations.
-  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
-  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
-  uint32_t base = insert_zeros32(w * 16, srt, 3);
   // Byte addresses: the lane's part plus a wave-uniform group part (scalar) on
   // disjoint bits.
   unsigned char* p0 = (unsigned char*)(tile + o0);
@@ -540,8 +582,7 @@ __device__ __forceinline__ void tile_block_mfma16(float2* tile, const TileBlock&
     float2 x0[U], x1[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      bs[j] = __builtin_amdgcn_readfirstlane(base) * 8u;
-      base = ((base | mask) + step) & ~mask;
+      bs[j] = gw.next() * 8u;
       x0[j] = *(const float2*)(p0 + bs[j]);
       x1[j] = *(const float2*)(p1 + bs[j]);
     }
@@ -607,9 +648,9 @@ __device__ __forceinline__ float split_one() {
 // cycles for six VALU operations per lane and group tile. `e` holds per lane
 // l four packed FP16 pairs: e[64 s + l], s = 0, 1: Eh of inputs 2u + s (real
 // part column, imaginary part column) for row l%16; s = 2, 3: the same of El.
-template <int L, int NTH = TPB>
+template <int L, int NTH = TPB, bool WB = false>
 __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock& b, const uint32_t* e) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int col = lane & 15, u = lane >> 4;
   uint32_t f[4];
 #pragma unroll
@@ -619,16 +660,13 @@ __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock
   const float one = split_one();
   uint32_t srt[3];
   uniform_bits<3>(b, srt);
-  const uint32_t lane_part = insert_zeros32(col, srt, 3);
+  GroupWalk<3, WB, NTH> gw(b, srt, col, 16);
+  const uint32_t lane_part = gw.lane_part;
   const uint32_t o0 = local_offset<3>(b, 2 * u) ^ lane_part, o1 = local_offset<3>(b, 2 * u + 1) ^ lane_part;
   constexpr int NT = (1 << L) / 8 / 16;  // 16-group tiles of the block
   constexpr int PERW = NT / (NTH / 64);
   constexpr int U = PERW >= 4 ? 4 : PERW;
   static_assert(NT % (NTH / 64) == 0 && PERW % U == 0, "16-group tiles split evenly over the waves");
-  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]) | (1u << srt[2]);
-  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(16 * (NTH / 64), srt, 3);
-  uint32_t base = insert_zeros32(w * 16, srt, 3);
   // Byte addresses: the lane's part (o0, o1 on the block's bits and the lane's
   // group bits) plus a wave-uniform group part on the other bits (scalar);
   // the parts share no bit, so adding them equals OR-ing the indices.
@@ -640,8 +678,7 @@ __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock
     float2 x0[U], x1[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      bs[j] = __builtin_amdgcn_readfirstlane(base) * 8u;
-      base = ((base | mask) + step) & ~mask;
+      bs[j] = gw.next() * 8u;
       x0[j] = *(const float2*)(p0 + bs[j]);
       x1[j] = *(const float2*)(p1 + bs[j]);
     }
@@ -672,9 +709,9 @@ __device__ __forceinline__ void tile_block_mfma16h(float2* tile, const TileBlock
 // row 4h + i of the lane's own group. Two accumulators (h = 0, 1) give the
 // lane its group's 4 outputs: 0.5 MFMA cycles per amplitude, half of a
 // 3-qubit block's. `e` holds the A fragments: e[16 i + 8 h + kk] = E[4h+i][kk].
-template <int L, int NTH = TPB>
+template <int L, int NTH = TPB, bool WB = false>
 __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& b, const float* e) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   float a[16];
   const float4* e4 = (const float4*)(e + 16 * (lane & 3));
 #pragma unroll
@@ -687,7 +724,8 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
   }
   uint32_t srt[2];
   uniform_bits<2>(b, srt);
-  const uint32_t lane_part = insert_zeros32(lane, srt, 2);
+  GroupWalk<2, WB, NTH> gw(b, srt, lane, 64);
+  const uint32_t lane_part = gw.lane_part;
   uint32_t off[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) off[j] = local_offset<2>(b, j) ^ lane_part;
@@ -696,10 +734,6 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
   constexpr int PERW = NR / (NTH / 64);  // L >= 10: at least 1 per wave
   constexpr int U = 1;
   static_assert(NR % (NTH / 64) == 0 && PERW % U == 0, "64-group rows split evenly over the waves");
-  const uint32_t mask = (1u << srt[0]) | (1u << srt[1]);
-  const uint32_t w = __builtin_amdgcn_readfirstlane(wave);
-  const uint32_t step = insert_zeros32(64 * (NTH / 64), srt, 2);
-  uint32_t base = insert_zeros32(w * 64, srt, 2);
   // Byte addresses: the lane's part plus a wave-uniform group part (scalar) on
   // disjoint bits.
   unsigned char* pj[4];
@@ -711,8 +745,7 @@ __device__ __forceinline__ void tile_block_mfma4(float2* tile, const TileBlock& 
     float2 x[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      bs[u] = __builtin_amdgcn_readfirstlane(base) * 8u;
-      base = ((base | mask) + step) & ~mask;
+      bs[u] = gw.next() * 8u;
 #pragma unroll
       for (int j = 0; j < 4; ++j) x[u][j] = *(const float2*)(pj[j] + bs[u]);
     }
@@ -970,19 +1003,22 @@ __device__ __forceinline__ uint64_t scatter_bits(uint64_t v, const uint32_t* pos
   return o;
 }
 
-// The blocks of one sweep applied to the LDS tile, a barrier after each.
-template <int L, bool K4, int NTH, bool SPLIT>
+// The blocks of one sweep applied to the LDS tile. Without WB a barrier
+// follows every block; with WB (wave-owned groups, see GroupWalk) only the
+// last block of each run, as its descriptor's pad[1] bit 0 says (blocks of 4
+// qubits keep the per-block wave split and form runs of their own).
+template <int L, bool K4, int NTH, bool SPLIT, bool WB = false>
 __device__ __forceinline__ void tile_apply_blocks(float2* tile, const TileBlock* blocks, const unsigned char* meta,
                                                   uint32_t nblocks) {
   for (uint32_t bi = 0; bi < nblocks; ++bi) {
     const TileBlock& b = blocks[bi];
     const float* m = (const float*)(meta + b.mat_off);
-    if (b.kind == 3) tile_block_mfma4<L, NTH>(tile, b, m);
-    else if (SPLIT && b.kind == 4) tile_block_mfma16h<L, NTH>(tile, b, (const uint32_t*)m);
+    if (b.kind == 3) tile_block_mfma4<L, NTH, WB>(tile, b, m);
+    else if (SPLIT && b.kind == 4) tile_block_mfma16h<L, NTH, WB>(tile, b, (const uint32_t*)m);
     else if (K4 && SPLIT && b.kind == 5) tile_block_mfma32h<L, NTH>(tile, b, (const uint32_t*)m);
-    else if (!K4 || b.kind == 1) tile_block_mfma16<L, NTH>(tile, b, m);
+    else if (!K4 || b.kind == 1) tile_block_mfma16<L, NTH, WB>(tile, b, m);
     else tile_block_mfma32<L, NTH>(tile, b, m);
-    __syncthreads();
+    if (!WB || (__builtin_amdgcn_readfirstlane(b.pad[1]) & 1u)) __syncthreads();
   }
 }
 
@@ -1124,7 +1160,7 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
       for (int i = 0; i < PER; ++i) tile[threadIdx.x + NTH * i] = SPLIT ? split_scale(r[i]) : r[i];
     }
     __syncthreads();  // the tile and the blob are in LDS
-    tile_apply_blocks<L, K4, NTH, SPLIT>(tile, blocks, meta, a.nblocks);
+    tile_apply_blocks<L, K4, NTH, SPLIT, NTH == 256>(tile, blocks, meta, a.nblocks);
     xsweep_store<L, NTH, SPLIT>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
     return;
   }
@@ -1146,7 +1182,7 @@ __global__ __launch_bounds__(NTH) void k_xsweep(const float2* __restrict__ src, 
 #pragma unroll
       for (int i = 0; i < PER; ++i) r[i] = sb[nb | rd.off(i)];
     }
-    tile_apply_blocks<L, K4, NTH, SPLIT>(tile, blocks, meta, a.nblocks);
+    tile_apply_blocks<L, K4, NTH, SPLIT, NTH == 256>(tile, blocks, meta, a.nblocks);
     xsweep_store<L, NTH, SPLIT>(db, tile, scatter_bits(tau, a.wtile, a.ntb), a);
     if (more) {
       __syncthreads();  // every wave has read this tile out of LDS

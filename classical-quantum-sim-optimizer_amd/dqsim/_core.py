@@ -158,6 +158,7 @@ def _load():
         "dq_set_sweep_relabel": (ctypes.c_int, [I32]),
         "dq_set_sweep_pinned": (ctypes.c_int, [I32]),
         "dq_set_block_arith": (ctypes.c_int, [I32]),
+        "dq_set_wave_runs": (ctypes.c_int, [I32]),
         "dq_set_zero_start": (ctypes.c_int, [I32]),
         "dq_split_f16": (ctypes.c_int, [P, U64, P, P]),
         "dq_sv_evolve_timed": (ctypes.c_int, [P, ctypes.c_char_p, SZ, I32, ctypes.POINTER(OpTiming), U64,
@@ -618,6 +619,18 @@ def set_block_arith(mode: int) -> int:
     prev = int(_lib.dq_set_block_arith(int(mode)))
     if prev < 0:
         raise ValueError(f"block arithmetic must be 0 or 1, got {mode}")
+    return prev
+
+
+def set_wave_runs(mode: int) -> int:
+    """Barriers between the blocks of 256-thread out-of-place complex64
+    sweeps for this process: 1 (the default) groups consecutive blocks that
+    leave two tile bits untouched into runs whose amplitudes each wave owns,
+    with a workgroup barrier only after a run; 0 puts one after every block.
+    The states are bit-identical. Returns the previous mode."""
+    prev = int(_lib.dq_set_wave_runs(int(mode)))
+    if prev < 0:
+        raise ValueError(f"wave runs must be 0 or 1, got {mode}")
     return prev
 
 

@@ -291,6 +291,16 @@ int dq_set_sweep_pinned(int32_t on);
  * -1 for a mode outside 0..1. */
 int dq_set_block_arith(int32_t mode);
 
+/* Process-wide barrier placement between the blocks of 256-thread
+ * out-of-place complex64 sweeps (no counterpart in the reference: a tuning
+ * and A/B switch). 1 (default): consecutive blocks of at most 3 qubits that
+ * leave two tile bits untouched form a run in which each wave owns the
+ * amplitudes whose two such bits equal its index, with a workgroup barrier
+ * only after a run's last block; 0: a barrier after every block. Results are
+ * bit-identical either way. Returns the previous mode; -1 for a mode outside
+ * 0..1. */
+int dq_set_wave_runs(int32_t mode);
+
 /* Process-wide zero start (no counterpart in the reference: a tuning and A/B
  * switch). on != 0 (default): a plan that opens with one-shot out-of-place
  * sweeps starts from a synthesised |0...0> (no initialisation, no read by

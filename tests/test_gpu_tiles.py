@@ -154,3 +154,35 @@ def test_tile_rules_match_oracle(pinned, kind, n, L, k):
             assert np.abs(got - want).max() < 1e-5, budget
     finally:
         dqsim._core.set_sweep_pinned(prev)
+
+
+@pytest.mark.parametrize("mode,arith", [(2, 1), (2, 0), (1, 0), (1, 1), (3, 1)], ids=lambda v: str(v))
+@pytest.mark.parametrize("kind,n,L,k", [("su2", 20, 12, 3), ("su2", 18, 11, 4), ("mixed", 17, 12, 3),
+                                        ("qft", 16, 10, 2), ("brickwork", 20, 12, 5), ("mixed", 18, 10, 3)],
+                         ids=lambda v: str(v))
+def test_wave_runs_bit_identical(mode, arith, kind, n, L, k):
+    # Blocks grouped into wave-owned runs (barriers only after a run) give
+    # the same bits as a barrier after every block: each group goes through
+    # the same operations, only the wave that computes it changes. Sweep
+    # modes 1 and 2 (256-thread out-of-place workgroups) use the runs; mode
+    # 3 (512 threads) keeps a barrier per block either way. Both states are
+    # also within the complex64 tolerance of the oracle.
+    print(f"[wave-runs] mode={mode} arith={arith} {kind} n={n} L={L} k={k}", flush=True)
+    circ = build(kind, n)
+    want = oracle.simulate(circ.data, 1)[0]
+    old_mode, old_arith = dqsim._core.set_sweep_mode(mode), dqsim._core.set_block_arith(arith)
+    states = {}
+    try:
+        for runs in (1, 0):
+            prev = dqsim._core.set_wave_runs(runs)
+            try:
+                sim = dqsim.StatevectorSimulator(seed=1, precision="complex64", fusion="greedy",
+                                                 max_fused_qubits=k, tile_qubits=L)
+                states[runs] = sim.simulate(circ).statevector
+            finally:
+                dqsim._core.set_wave_runs(prev)
+    finally:
+        dqsim._core.set_sweep_mode(old_mode)
+        dqsim._core.set_block_arith(old_arith)
+    assert np.array_equal(states[1], states[0])
+    assert np.abs(states[1] - want).max() < 1e-5
